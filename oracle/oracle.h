@@ -1,0 +1,152 @@
+/*
+ * CPU oracle for the trajectory-ensemble path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load this library, and only as the checker
+ * (or, for cpu_baseline, as the CPU baseline being timed).  The engine under
+ * hawaiihybrid_amd/ never links or calls it.
+ *
+ * It restates, in plain C, the reference CPU algorithm of the path:
+ *   Mersenne Twister (thirdparty/mtwist.c), initial-condition sampling
+ *   (src/hawaii.c:1206-1421, 2969-2993, 3486-3512), Hamilton's equations
+ *   (src/hawaii.c:474-615, 1041-1204), the He-Ar and CO2-Ar surfaces
+ *   (PES-IDS/HeAr.h, PES-IDS/ai_pes_co2ar*, PES-IDS/ai_ids_co2ar*,
+ *   src/angles_handler.cpp:45-193), the per-trajectory correlation function
+ *   (src/hawaii.c:2116-2347), ensemble accumulation (src/hawaii.c:2969-3114),
+ *   CF -> SF (src/hawaii.c:5401-5601) and the rejection-sampler M0
+ *   (src/hawaii.c:1423-1491).
+ *
+ * The one deliberate replacement: SUNDIALS CVODE (BDF, not vendored) is replaced by
+ * the explicit embedded Runge-Kutta 8(5,3) pair of Dormand & Prince with its 7th-order
+ * dense output (Hairer, Norsett & Wanner, "Solving ODEs I", II.5 and II.6: DOP853),
+ * sampled at t_k = k*dt like CVode's CV_NORMAL interpolation (src/trajectory.c:155).
+ */
+#ifndef HH_ORACLE_H
+#define HH_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- MT19937 */
+typedef struct {
+    uint32_t mt[624];
+    int idx;
+} orc_mt;
+
+void     orc_mt_seed(orc_mt *s, uint32_t seed);
+uint32_t orc_mt_u32(orc_mt *s);
+double   orc_mt_drand(orc_mt *s);
+void     orc_mt_fill(uint32_t seed, uint32_t *out, size_t n);
+
+/* ---------------------------------------------------------------- systems */
+enum { ORC_SYS_HEAR = 0, ORC_SYS_CO2AR = 1 };
+
+typedef struct {
+    int system;      /* ORC_SYS_* */
+    double mu;       /* reduced mass, a.u. */
+    double II[2];    /* linear-rotor inertia (II[0] == II[1]); unused for He-Ar */
+} orc_system;
+
+int    orc_qp_size(const orc_system *s);
+double orc_pes(const orc_system *s, const double *q);
+void   orc_dpes(const orc_system *s, const double *q, double *dvdq);
+void   orc_dipole(const orc_system *s, const double *q, double mu[3]);
+void   orc_rhs(const orc_system *s, const double *y, double *ydot);
+double orc_kinetic(const orc_system *s, const double *y);
+double orc_hamiltonian(const orc_system *s, const double *y);
+
+/* He-Ar radial functions (PES-IDS/HeAr.h) */
+double orc_hear_V(double R);
+double orc_hear_dV(double R);
+double orc_hear_dip(double R);
+
+/* CO2-Ar molecular-frame functions (R, theta_m) */
+double orc_co2ar_V_cm(double R, double theta);            /* cm^-1 */
+void   orc_co2ar_dV_cm(double R, double theta, double *dR, double *dTheta);
+void   orc_co2ar_dip_mol(double R, double theta, double d[3]);
+void   orc_co2ar_lab_to_mol(const double *qlab, double *qmol);
+void   orc_co2ar_jacobian(const double *qlab, const double *qmol, double jac[5][5]);
+
+/* ---------------------------------------------------------------- sampler */
+enum { ORC_IC_REJECTION = 0, ORC_IC_R0_INJECT = 1 };
+enum { ORC_PAIR_FREE = 1, ORC_PAIR_BOUND = 2, ORC_PAIR_ALL = 3 };
+
+typedef struct {
+    int mode;          /* ORC_IC_* */
+    int pair_state;    /* ORC_PAIR_*; rejection mode only */
+    double T;          /* K */
+    double Rmin, Rmax; /* bohr */
+    double pesmin;     /* Hartree */
+    double R0;         /* bohr; r0_inject mode only */
+} orc_sampler;
+
+/* Draw ntraj accepted initial conditions from one MT stream seeded with `seed`.
+ * ic is row-major [ntraj][QP].  Returns the number of candidates drawn. */
+long orc_sample_ics(const orc_system *s, const orc_sampler *sp, uint32_t seed,
+                    long ntraj, double *ic, long *n_draws);
+
+/* ---------------------------------------------------------------- integrator / CF */
+enum { ORC_HALVES_BOTH = 0, ORC_HALVES_FORWARD = 1 };
+enum { ORC_OK = 0, ORC_FAIL_DIPOLE_NAN = 1, ORC_FAIL_CF_HUGE = 2, ORC_FAIL_CF_JUMP = 3,
+       ORC_FAIL_STEPS = 4, ORC_FAIL_STEPSIZE = 5 };
+
+typedef struct {
+    double rtol, atol;  /* CVODE_TOLERANCE (src/trajectory.c:214-230): rtol = atol */
+    double dt;          /* sampling_time */
+    int L;              /* MaxTrajectoryLength */
+    double Rcut;
+    int halves;         /* ORC_HALVES_* */
+    long max_steps;     /* per output interval (CVodeSetMaxNumSteps, src/trajectory.c:62) */
+} orc_cf_params;
+
+typedef struct {
+    long nrhs;          /* right-hand-side evaluations */
+    long ndip;          /* dipole evaluations */
+    long naccept, nreject;
+    double e_drift;     /* max |H(end) - H(0)| over the halves */
+} orc_traj_stats;
+
+/* One half: propagate y0 and fill c[0..L-1] with mu(0).mu(t_k).  Returns ORC_* and
+ * writes the number of filled samples to *len (c[len..L-1] is zero). */
+int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
+                double *c, int *len, orc_traj_stats *st);
+
+/* One trajectory as correlation_eval: crln[k] = 1/2 (c_f + c_b) ALU^3 (both halves) or
+ * c_f ALU^3 (forward only).  A dipole-check failure zeroes crln (src/hawaii.c:2183-2237). */
+int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
+                double *crln, int *len_f, int *len_b, orc_traj_stats *st);
+
+/* Per-trajectory rows, threads over trajectories.  crln: [ntraj][L]. */
+void orc_cf_rows(const orc_system *s, const orc_cf_params *p, const double *ic, long ntraj,
+                 int nthreads, double *crln, int *status, int *len_f, int *len_b,
+                 long *nrhs, long *ndip, double *e_drift);
+
+/* Ensemble CF with per-thread partial sums (for timing): cf[k] = sum ppf*crln / ntraj. */
+void orc_cf_ensemble(const orc_system *s, const orc_cf_params *p, const double *ic, long ntraj,
+                     int nthreads, double ppf, double *cf, long *nrhs_total);
+
+/* ---------------------------------------------------------------- CF -> SF */
+void orc_fft_radix2(double *re, double *im, size_t n, int sign); /* in place, unscaled */
+void orc_idct(const double *v, size_t n, double *out);
+void orc_dct(const double *v, size_t n, double *out);
+void orc_connes(double *a, size_t n, double dt);
+/* t[n], cf[n] (apodized in place as the reference does) -> nu[n] (cm^-1), sf[n] */
+void orc_cf_to_sf(const double *t, double *cf, size_t n, double *nu, double *sf);
+
+/* ---------------------------------------------------------------- M0 */
+/* Rejection-sampler M0 (calculate_M0, src/hawaii.c:1423-1491): m = mean(mu.mu)*ZeroCoeff*ppf
+ * q = sqrt(var/n)*ZeroCoeff*ppf, from `seed`. */
+void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+            double *m, double *q);
+
+/* analytic_full_partition_function_by_V (src/hawaii.c:5071-5090) */
+double orc_pf_analytic(const orc_system *s, double T);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

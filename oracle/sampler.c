@@ -1,0 +1,154 @@
+/* Initial-condition sampler (test infrastructure).
+ *
+ * Draw order and arithmetic follow the reference exactly:
+ *   generate_normal  src/hawaii.c:1206-1230  (Box-Muller, cosine branch, retry on non-finite log U)
+ *   q_generator      src/hawaii.c:1232-1290  (R^3 uniform, Phi, Theta, then monomer angles)
+ *   p_generator      src/hawaii.c:1330-1380  (x0, x1, x2 for the pair, then the rotor's x0, x1)
+ *   p_generator_linear_molecule src/hawaii.c:1292-1303
+ *   reject           src/hawaii.c:1382-1421  (u drawn first; weight = desired/proposal/M < u)
+ *   candidate loop   src/hawaii.c:2969-2993  (Hamiltonian before reject; FREE keeps E >= 0)
+ *   R0 injection     src/hawaii.c:3486-3512  (q_generator, R := R0, p_generator, pR := -|pR|)
+ */
+#include <math.h>
+#include "oracle.h"
+
+#define HkT (4.3597447222071e-18 / 1.380649e-23)   /* src/constants.h:24 */
+
+typedef struct {
+    orc_mt mt;
+    long draws;
+} rng;
+
+static double drand(rng *r)
+{
+    r->draws++;
+    return orc_mt_drand(&r->mt);
+}
+
+static double generate_normal(rng *r, double sigma)
+{
+    for (;;) {
+        double U = drand(r);
+        double V = drand(r);
+        double logU = log(U);
+        if (isnan(logU) || isinf(logU)) continue;
+        return sigma * sqrt(-2 * logU) * cos(2.0 * M_PI * V);
+    }
+}
+
+static void q_generator(const orc_system *s, const orc_sampler *sp, rng *r, double *y)
+{
+    double RMIN = sp->Rmin, RMAX = sp->Rmax;
+    double R3 = drand(r) * (RMAX * RMAX * RMAX - RMIN * RMIN * RMIN) + RMIN * RMIN * RMIN;
+    y[4] = pow(R3, 1.0 / 3.0);
+    y[0] = drand(r) * 2.0 * M_PI;
+    y[2] = acos(2.0 * drand(r) - 1.0);
+    if (s->system == ORC_SYS_CO2AR) {
+        y[6] = drand(r) * 2.0 * M_PI;
+        y[8] = acos(2.0 * drand(r) - 1.0);
+    }
+}
+
+static void p_generator(const orc_system *s, double T, rng *r, double *y)
+{
+    double sqrt_MUKT = sqrt(s->mu / HkT * T);
+    double sinTheta = sin(y[2]);
+    double x[3];
+    int i;
+    for (i = 0; i < 3; ++i) x[i] = generate_normal(r, 1.0);
+    y[1] = sqrt_MUKT * y[4] * sinTheta * x[0];
+    y[3] = sqrt_MUKT * y[4] * x[1];
+    y[5] = sqrt_MUKT * x[2];
+    if (s->system == ORC_SYS_CO2AR) {
+        double sqrt_IIKT = sqrt(s->II[0] / HkT * T);
+        double sin_theta = sin(y[8]);
+        double x0 = generate_normal(r, 1.0);
+        double x1 = generate_normal(r, 1.0);
+        y[7] = sqrt_IIKT * sin_theta * x1;
+        y[9] = sqrt_IIKT * x0;
+    }
+}
+
+/* true = rejected */
+static int reject(const orc_system *s, const double *y, double T, double pesmin, rng *r)
+{
+    double u = drand(r);
+    double Mval = exp(-pesmin * HkT / T);
+    double M = 1.5 * Mval;
+    double kin_part = orc_kinetic(s, y);
+    double q[5];
+    int i, nq = orc_qp_size(s) / 2;
+    double pot_part, proposal, desired, weight;
+    for (i = 0; i < nq; ++i) q[i] = y[2 * i];
+    pot_part = orc_pes(s, q);
+    proposal = exp(-kin_part * HkT / T);
+    desired = exp(-(kin_part + pot_part) * HkT / T);
+    weight = desired / proposal / M;
+    return weight < u;
+}
+
+long orc_sample_ics(const orc_system *s, const orc_sampler *sp, uint32_t seed,
+                    long ntraj, double *ic, long *n_draws)
+{
+    rng r;
+    long accepted = 0, candidates = 0;
+    int qp = orc_qp_size(s), i;
+    double y[10];
+    r.draws = 0;
+    orc_mt_seed(&r.mt, seed);
+
+    while (accepted < ntraj) {
+        for (i = 0; i < 10; ++i) y[i] = 0.0;
+        q_generator(s, sp, &r, y);
+        if (sp->mode == ORC_IC_R0_INJECT) {
+            y[4] = sp->R0;
+            p_generator(s, sp->T, &r, y);
+            if (y[5] > 0.0) y[5] = -y[5];
+            ++candidates;
+        } else {
+            double energy;
+            p_generator(s, sp->T, &r, y);
+            energy = orc_hamiltonian(s, y);
+            ++candidates;
+            if (reject(s, y, sp->T, sp->pesmin, &r)) continue;
+            if (sp->pair_state == ORC_PAIR_FREE && energy < 0.0) continue;
+            if (sp->pair_state == ORC_PAIR_BOUND && energy > 0.0) continue;
+        }
+        for (i = 0; i < qp; ++i) ic[accepted * qp + i] = y[i];
+        ++accepted;
+    }
+    if (n_draws) *n_draws = r.draws;
+    return candidates;
+}
+
+void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+            double *m, double *q)
+{
+    const double ZeroCoeff = 0.00361479637 / (4.0 * M_PI);   /* src/constants.h:29 */
+    rng r;
+    long n = 0;
+    int i, nq = orc_qp_size(s) / 2;
+    double y[10], qv[5], d[3];
+    double mm = 0.0, qq = 0.0;
+    r.draws = 0;
+    orc_mt_seed(&r.mt, seed);
+    while (n < npoints) {
+        double energy, fval, diff;
+        for (i = 0; i < 10; ++i) y[i] = 0.0;
+        q_generator(s, sp, &r, y);
+        p_generator(s, sp->T, &r, y);
+        energy = orc_hamiltonian(s, y);
+        if (reject(s, y, sp->T, sp->pesmin, &r)) continue;
+        if (sp->pair_state == ORC_PAIR_FREE && energy < 0.0) continue;
+        if (sp->pair_state == ORC_PAIR_BOUND && energy > 0.0) continue;
+        for (i = 0; i < nq; ++i) qv[i] = y[2 * i];
+        orc_dipole(s, qv, d);
+        fval = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        diff = fval - mm;
+        mm += diff / (n + 1.0);
+        qq += diff * diff * (n / (n + 1.0));
+        n++;
+    }
+    *m = mm * ZeroCoeff * ppf;
+    *q = sqrt(qq / n / (n - 1)) * ZeroCoeff * ppf;
+}

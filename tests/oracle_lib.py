@@ -1,0 +1,178 @@
+"""ctypes access to the CPU oracle (oracle/_build/liboracle.so) and to the reference-built
+checkers (oracle/_ref/*.so).  Test infrastructure: the oracle is the checker, never the
+thing under test."""
+import ctypes as C
+import os
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+REF_DIR = os.path.join(REPO, "oracle", "_ref")
+
+SYS_HEAR, SYS_CO2AR = 0, 1
+IC_REJECTION, IC_R0_INJECT = 0, 1
+PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
+HALVES_BOTH, HALVES_FORWARD = 0, 1
+
+# physical inputs of the example configs (example_configs/CF-F-He-Ar.conf:5,
+# example_configs/CF-F-CO2-Ar.conf:5,42)
+MU_HEAR = 6.632039160324215e3
+MU_CO2AR = 3.817083069753919e4
+I_CO2 = 2.819827114123505e5
+PESMIN_HEAR = -9.424412560638486e-05
+PESMIN_CO2AR = -8.913727688689150e-04
+
+
+class System(C.Structure):
+    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("II", C.c_double * 2)]
+
+
+class Sampler(C.Structure):
+    _fields_ = [("mode", C.c_int), ("pair_state", C.c_int), ("T", C.c_double), ("Rmin", C.c_double),
+                ("Rmax", C.c_double), ("pesmin", C.c_double), ("R0", C.c_double)]
+
+
+class CFParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("L", C.c_int),
+                ("Rcut", C.c_double), ("halves", C.c_int), ("max_steps", C.c_long)]
+
+
+class TrajStats(C.Structure):
+    _fields_ = [("nrhs", C.c_long), ("ndip", C.c_long), ("naccept", C.c_long), ("nreject", C.c_long),
+                ("e_drift", C.c_double)]
+
+
+def hear():
+    return System(SYS_HEAR, MU_HEAR, (C.c_double * 2)(0.0, 0.0))
+
+
+def co2ar():
+    return System(SYS_CO2AR, MU_CO2AR, (C.c_double * 2)(I_CO2, I_CO2))
+
+
+def qp_size(sysid):
+    return 10 if sysid == SYS_CO2AR else 6
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(ORACLE_SO)
+        d, i, v, u32, sz = C.c_double, C.c_int, C.c_void_p, C.c_uint32, C.c_size_t
+        P = C.POINTER
+        sig = {
+            "orc_mt_fill": (None, [u32, v, sz]),
+            "orc_pes": (d, [P(System), v]),
+            "orc_dpes": (None, [P(System), v, v]),
+            "orc_dipole": (None, [P(System), v, v]),
+            "orc_rhs": (None, [P(System), v, v]),
+            "orc_kinetic": (d, [P(System), v]),
+            "orc_hamiltonian": (d, [P(System), v]),
+            "orc_hear_V": (d, [d]), "orc_hear_dV": (d, [d]), "orc_hear_dip": (d, [d]),
+            "orc_co2ar_V_cm": (d, [d, d]),
+            "orc_co2ar_dV_cm": (None, [d, d, v, v]),
+            "orc_co2ar_dip_mol": (None, [d, d, v]),
+            "orc_co2ar_lab_to_mol": (None, [v, v]),
+            "orc_co2ar_jacobian": (None, [v, v, v]),
+            "orc_sample_ics": (C.c_long, [P(System), P(Sampler), u32, C.c_long, v, P(C.c_long)]),
+            "orc_half_cf": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(TrajStats)]),
+            "orc_traj_cf": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(C.c_int), P(TrajStats)]),
+            "orc_cf_rows": (None, [P(System), P(CFParams), v, C.c_long, i, v, v, v, v, v, v, v]),
+            "orc_cf_ensemble": (None, [P(System), P(CFParams), v, C.c_long, i, d, v, P(C.c_long)]),
+            "orc_fft_radix2": (None, [v, v, sz, i]),
+            "orc_idct": (None, [v, sz, v]),
+            "orc_dct": (None, [v, sz, v]),
+            "orc_connes": (None, [v, sz, d]),
+            "orc_cf_to_sf": (None, [v, v, sz, v, v]),
+            "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
+            "orc_pf_analytic": (d, [P(System), d]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def mt_stream(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().orc_mt_fill(seed, ptr(out), n)
+    return out
+
+
+def sample_ics(system, sampler, seed, ntraj):
+    qp = qp_size(system.system)
+    ic = np.zeros((ntraj, qp), dtype=np.float64)
+    nd = C.c_long(0)
+    ncand = lib().orc_sample_ics(C.byref(system), C.byref(sampler), seed, ntraj, ptr(ic), C.byref(nd))
+    return ic, ncand, nd.value
+
+
+def cf_rows(system, params, ic, nthreads=8):
+    ic = np.ascontiguousarray(ic, dtype=np.float64)
+    n = ic.shape[0]
+    crln = np.zeros((n, params.L))
+    status = np.zeros(n, dtype=np.int32)
+    lf = np.zeros(n, dtype=np.int32)
+    lb = np.zeros(n, dtype=np.int32)
+    nrhs = np.zeros(n, dtype=np.int64)
+    ndip = np.zeros(n, dtype=np.int64)
+    ed = np.zeros(n)
+    lib().orc_cf_rows(C.byref(system), C.byref(params), ptr(ic), n, nthreads, ptr(crln), ptr(status),
+                      ptr(lf), ptr(lb), ptr(nrhs), ptr(ndip), ptr(ed))
+    return dict(crln=crln, status=status, len_f=lf, len_b=lb, nrhs=nrhs, ndip=ndip, e_drift=ed)
+
+
+def ensemble_from_rows(crln, ppf=1.0):
+    """Sequential sum in trajectory order, then / ntraj (src/hawaii.c:3007-3009, 3111-3114)."""
+    acc = np.zeros(crln.shape[1])
+    for row in crln:
+        acc += ppf * row
+    return acc / crln.shape[0]
+
+
+def idct(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    lib().orc_idct(ptr(v), v.size, ptr(out))
+    return out
+
+
+def dct(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros_like(v)
+    lib().orc_dct(ptr(v), v.size, ptr(out))
+    return out
+
+
+def cf_to_sf(t, cf):
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    cf = np.array(cf, dtype=np.float64, copy=True)
+    nu = np.zeros_like(cf)
+    sf = np.zeros_like(cf)
+    lib().orc_cf_to_sf(ptr(t), ptr(cf), cf.size, ptr(nu), ptr(sf))
+    return nu, sf
+
+
+def m0(system, sampler, seed, npoints, ppf=1.0):
+    m = C.c_double(0)
+    q = C.c_double(0)
+    lib().orc_m0(C.byref(system), C.byref(sampler), seed, npoints, ppf, C.byref(m), C.byref(q))
+    return m.value, q.value
+
+
+def ref_lib(name):
+    """A reference-built checker from oracle/_ref (None where it has not been built)."""
+    path = os.path.join(REF_DIR, name)
+    return C.CDLL(path) if os.path.exists(path) else None

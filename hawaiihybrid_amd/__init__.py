@@ -1,0 +1,202 @@
+"""hawaiihybrid_amd: MI355X trajectory-ensemble engine (Python side of libhawaii_gpu.so).
+
+The compute path is the C ABI of include/hawaii_gpu.h; this module only binds it with
+ctypes and uses torch for device memory.  There is no CPU fallback: if the library is not
+built, or no gfx950 device is visible, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhawaii_gpu.so")
+
+SYS_HEAR, SYS_CO2AR = 0, 1
+IC_REJECTION, IC_R0_INJECT = 0, 1
+PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
+ENGINE_PLAIN, ENGINE_BATCHED = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+HALVES_BOTH, HALVES_FORWARD = 0, 1
+
+# example_configs/CF-F-He-Ar.conf:5, example_configs/CF-F-CO2-Ar.conf:5,27,42
+MU_HEAR = 6.632039160324215e3
+MU_CO2AR = 3.817083069753919e4
+I_CO2 = 2.819827114123505e5
+PESMIN_HEAR = -9.424412560638486e-05
+PESMIN_CO2AR = -8.913727688689150e-04
+
+
+class System(C.Structure):
+    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("I", C.c_double)]
+
+
+class Sampler(C.Structure):
+    _fields_ = [("mode", C.c_int), ("pair_state", C.c_int), ("T", C.c_double), ("Rmin", C.c_double),
+                ("Rmax", C.c_double), ("pesmin", C.c_double), ("R0", C.c_double)]
+
+
+class CFParams(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("Rcut", C.c_double),
+                ("ppf", C.c_double), ("L", C.c_int), ("halves", C.c_int), ("max_steps", C.c_int),
+                ("engine", C.c_int), ("precision", C.c_int), ("block", C.c_int), ("grid", C.c_int)]
+
+
+class CFStats(C.Structure):
+    _fields_ = [("integrate_ms", C.c_double), ("reduce_ms", C.c_double), ("n_items", C.c_longlong),
+                ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
+                ("n_stepfail", C.c_longlong), ("max_e_drift", C.c_double), ("block", C.c_int), ("grid", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def he_ar():
+    return System(SYS_HEAR, MU_HEAR, 0.0)
+
+
+def co2_ar():
+    return System(SYS_CO2AR, MU_CO2AR, I_CO2)
+
+
+def qp_size(system):
+    return 10 if system == SYS_CO2AR else 6
+
+
+class HHError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded engine library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HHError("libhawaii_gpu.so is not built: run `python -m hawaiihybrid_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        v, i, d, P = C.c_void_p, C.c_int, C.c_double, C.POINTER
+        sigs = {
+            "hh_last_error": (C.c_char_p, []),
+            "hh_qp_size": (i, [i]),
+            "hh_mt_fill": (None, [C.c_uint32, v, C.c_size_t]),
+            "hh_sample_ics": (C.c_long, [P(System), P(Sampler), C.c_uint32, C.c_long, v, v, P(C.c_long)]),
+            "hh_engine_create": (i, [i, P(v)]),
+            "hh_engine_destroy": (None, [v]),
+            "hh_engine_info": (i, [v, C.c_char_p, C.c_size_t]),
+            "hh_cf_run": (i, [v, P(System), P(CFParams), i, v, v, v, P(CFStats)]),
+            "hh_cf_items": (i, [v, i, v, v, v, v, v]),
+            "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
+            "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise HHError(lib().hh_last_error().decode())
+    return rc
+
+
+def mt_stream(seed, n):
+    out = np.zeros(n, dtype=np.uint32)
+    lib().hh_mt_fill(seed, out.ctypes.data_as(C.c_void_p), n)
+    return out
+
+
+def sample_ics(system, sampler, seed, ntraj, pes=None):
+    """Initial conditions [ntraj][QP] (host, bit-exact reference stream).  `pes` is an optional
+    plugin pes_lab address (int) used by the rejection step."""
+    ic = np.zeros((ntraj, qp_size(system.system)), dtype=np.float64)
+    nd = C.c_long(0)
+    n = _check(lib().hh_sample_ics(C.byref(system), C.byref(sampler), seed, ntraj, pes,
+                                   ic.ctypes.data_as(C.c_void_p), C.byref(nd)))
+    return ic, n, nd.value
+
+
+def cf_params(L, dt=200.0, rtol=1e-12, atol=None, Rcut=40.0, ppf=1.0, halves=HALVES_FORWARD, engine=ENGINE_BATCHED,
+              precision=PREC_FP64, max_steps=50000, block=0, grid=0):
+    return CFParams(rtol, rtol if atol is None else atol, dt, Rcut, ppf, L, halves, max_steps, engine, precision,
+                    block, grid)
+
+
+class Engine:
+    """One device of the engine (HIP stream, surface tables, workspaces)."""
+
+    def __init__(self, device=0):
+        import torch  # device memory for callers; the engine itself only needs HIP
+        self.torch = torch
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().hh_engine_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hh_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        import json
+        buf = C.create_string_buffer(512)
+        _check(lib().hh_engine_info(self.h, buf, 512))
+        return json.loads(buf.value.decode())
+
+    def to_device_soa(self, ic):
+        """[ntraj][QP] host array -> device tensor SoA [QP][ntraj] (float64)."""
+        t = self.torch.from_numpy(np.ascontiguousarray(ic.T)).to(f"cuda:{self.device}")
+        return t.contiguous()
+
+    def cf_run(self, system, params, ics_dev, out=None):
+        """ics_dev: list of device tensors SoA [QP][ntraj].  Returns (list of CF tensors, stats)."""
+        torch = self.torch
+        nj = len(ics_dev)
+        if out is None:
+            out = [torch.zeros(params.L, dtype=torch.float64, device=f"cuda:{self.device}") for _ in range(nj)]
+        icp = (C.c_void_p * nj)(*[t.data_ptr() for t in ics_dev])
+        ntr = (C.c_long * nj)(*[t.shape[1] for t in ics_dev])
+        cfp = (C.c_void_p * nj)(*[t.data_ptr() for t in out])
+        st = CFStats()
+        torch.cuda.synchronize(self.device)
+        _check(lib().hh_cf_run(self.h, C.byref(system), C.byref(params), nj, icp, ntr, cfp, C.byref(st)))
+        return out, st
+
+    def items(self, job, n_items, L, rows=False):
+        ln = np.zeros(n_items, np.int32)
+        stt = np.zeros(n_items, np.int32)
+        nr = np.zeros(n_items, np.int32)
+        dr = np.zeros(n_items, np.float32)
+        rw = np.zeros((n_items, L)) if rows else None
+        p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        _check(lib().hh_cf_items(self.h, job, p(ln), p(stt), p(nr), p(dr), p(rw)))
+        return dict(len=ln, status=stt, nrhs=nr, e_drift=dr, rows=rw)
+
+    def cf_to_sf(self, cf_dev, dt):
+        """Apodizes cf_dev in place (as the reference) and returns (nu, sf, ms) device tensors."""
+        torch = self.torch
+        n = cf_dev.numel()
+        nu = torch.empty_like(cf_dev)
+        sf = torch.empty_like(cf_dev)
+        ms = C.c_double(0)
+        _check(lib().hh_cf_to_sf(self.h, C.c_void_p(cf_dev.data_ptr()), n, dt, C.c_void_p(nu.data_ptr()),
+                                 C.c_void_p(sf.data_ptr()), C.byref(ms)))
+        return nu, sf, ms.value
+
+    def m0(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
+        m, q, ms = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().hh_m0(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf, C.byref(m),
+                           C.byref(q), C.byref(ms)))
+        return m.value, q.value, ms.value

@@ -1,0 +1,75 @@
+"""Build the engine library (libhawaii_gpu.so, gfx950) and the CPU oracle in-tree.
+
+    python -m hawaiihybrid_amd.build          # or __graft_entry__.build()
+
+Objects go to hawaiihybrid_amd/_build/; the shared library lands next to this file.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libhawaii_gpu.so")
+ARCH = "gfx950"
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + CSRC]
+# packed FP32 instructions stay off for the device code
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+SOURCES = [
+    ("hh_kernels.hip", DEVICE),
+    ("hh_aux.hip", DEVICE),
+    ("hh_host.cpp", ["-x", "hip"] + DEVICE),
+    # bit-exact initial conditions: no floating-point contraction in the sampler
+    ("hh_sampler.cpp", ["-ffp-contract=off"]),
+]
+
+
+def _compile(src, extra):
+    obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    return obj
+
+
+def build_engine(verbose=True):
+    os.makedirs(OUT, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(lambda a: _compile(*a), SOURCES))
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(LIB + ".tmp", LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def build_oracle(verbose=True):
+    r = subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    if os.path.isdir("/root/reference"):
+        r = subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "-s", "ref"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference checker build failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print("built oracle")
+
+
+def build(verbose=True):
+    build_oracle(verbose)
+    return build_engine(verbose)
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,574 @@
+// Trajectory-ensemble kernels for gfx950.
+//
+//  * plain_cf_kernel      one work item (one half of one trajectory) per thread, FP64,
+//                         runs its item to the end: the straightforward GPU port of
+//                         correlation_eval (src/hawaii.c:2116-2347).
+//  * persistent_cf_kernel resident lanes pull items from a device work queue; a lane whose
+//                         item ends (R > Rcut, L samples, failure) takes the next item in the
+//                         same loop iteration, so uneven trajectory lengths do not idle the
+//                         wavefront.  Surface evaluation in P (float in the mixed engine).
+//  * cf_reduce_*          ensemble sum over trajectories of 1/2 (c_f + c_b) ALU^3 ppf in a
+//                         fixed order with Neumaier-compensated FP64 partial sums, then / ntraj
+//                         (src/hawaii.c:2334-2337, 3007-3009, 3111-3114).
+//
+// Integrator: DOP853 (Dormand-Prince 8(5,3) + 7th-order dense output; Hairer, Norsett &
+// Wanner II.5-II.6), error norm sk = atol + rtol max(|y|,|y_new|), controller safe 0.9,
+// fac1 0.333, fac2 6, beta 0, HINIT start.  Samples at t_k = k dt from the dense output
+// (CVode CV_NORMAL interpolation, src/trajectory.c:155).  State and error control in FP64.
+//
+// Stage storage: the 16 stage vectors K0..K15 and the 7 dense-output coefficient vectors
+// F0..F6 of a lane live in a per-wavefront arena in HBM, interleaved across the 64 lanes,
+// so every stage access is a coalesced 64-lane load or store that stays in L2 / MALL.  Only y, the
+// stage input and the right-hand-side temporaries occupy registers; holding all stage
+// vectors in registers needs ~440 registers per lane for CO2-Ar (one wave per SIMD).
+#include <hip/hip_runtime.h>
+
+#include "dop853_coeffs.h"
+#include "hh_engine.h"
+#include "hh_systems.h"
+
+namespace hh {
+
+constexpr double kEPS = 2.220446049250313e-16;
+constexpr int kMaxTotalSteps = 400000;   // per item; the longest CO2-Ar items take ~5e3 steps
+constexpr int kSlots = 23;               // K0..K15, F0..F6
+constexpr int kSlotF = 16;
+#ifndef HH_MIN_WAVES
+#define HH_MIN_WAVES 2   // occupancy target (waves per SIMD) of the trajectory kernels
+#endif
+
+// Wave-blocked arena: wavefront w owns kSlots*N*64 consecutive doubles; element (slot, i) of
+// lane l is at double offset (slot*N + i)*64 + l of the block.  Accesses go through a buffer
+// resource of the block (wave-uniform, scalar registers), a 32-bit lane offset and a scalar
+// slot offset: a coalesced 64-lane access with no per-lane 64-bit address kept live.  An
+// access outside the block is dropped (stores) or reads 0 (loads) by the hardware.
+typedef unsigned int hh_v2u __attribute__((ext_vector_type(2)));
+constexpr int kBufferConfig = 0x00020000;   // raw 32-bit-element buffer, gfx9 word 3
+
+template <int N>
+struct Arena {
+    __amdgpu_buffer_rsrc_t rsrc;
+    int lane8;
+    __device__ __forceinline__ double ld(int slot, int i) const
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane8, (slot * N + i) * 512, 0));
+    }
+    __device__ __forceinline__ void st(int slot, int i, double v) const
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(hh_v2u, v), rsrc, lane8, (slot * N + i) * 512, 0);
+    }
+};
+
+template <int N>
+__device__ __forceinline__ Arena<N> wave_arena(double* ws, long long wave)
+{
+    const long long w = __builtin_amdgcn_readfirstlane((int)wave);
+    double* block = ws + w * (long long)(kSlots * N * 64);
+    return Arena<N>{__builtin_amdgcn_make_buffer_rsrc(block, 0, kSlots * N * 64 * 8, kBufferConfig),
+                    (int)(threadIdx.x & 63) * 8};
+}
+
+template <int N>
+struct Lane {
+    double y[N];
+    double t, h, E0;
+    double mu0[3];
+    double prev, curr;
+    long long local;   // item index inside its job
+    int job;
+    int k;             // next sample index
+    int steps;         // steps since the last recorded sample
+    int total_steps;
+    int nrhs;
+    int reject;
+    int active;
+};
+
+__device__ __forceinline__ int find_job(const JobDesc* __restrict__ jobs, int njobs, long long g)
+{
+    int j = 0;
+    while (j + 1 < njobs && g >= jobs[j + 1].item_begin) ++j;
+    return j;
+}
+
+// Hairer's HINIT for an order-8 method (one extra right-hand side); f0 is in arena slot 0
+template <int SYS, typename P, int N>
+__device__ __forceinline__ double initial_step(const SysConst& sc, const Arena<N>& ar, const double* y0, double rtol,
+                                               double atol, double hmax, int& nrhs)
+{
+    double dnf = 0.0, dny = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double sk = atol + rtol * fabs(y0[i]);
+        const double f = ar.ld(0, i);
+        dnf += (f / sk) * (f / sk);
+        dny += (y0[i] / sk) * (y0[i] / sk);
+    }
+    double h = (dnf <= 1e-10 || dny <= 1e-10) ? 1e-6 : sqrt(dny / dnf) * 0.01;
+    h = fmin(h, hmax);
+    double y1[N], f1[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) y1[i] = y0[i] + h * ar.ld(0, i);
+    rhs<SYS, P>(sc, y1, f1);
+    ++nrhs;
+    double der2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double sk = atol + rtol * fabs(y0[i]);
+        const double r = (f1[i] - ar.ld(0, i)) / sk;
+        der2 += r * r;
+    }
+    der2 = sqrt(der2) / h;
+    const double der12 = fmax(fabs(der2), sqrt(dnf));
+    const double h1 = (der12 <= 1e-15) ? fmax(1e-6, fabs(h) * 1e-3) : sqrt(sqrt(sqrt(0.01 / der12)));
+    return fmin(fmin(100.0 * fabs(h), h1), hmax);
+}
+
+// `last` is the index of the last stored sample (samples [0, last] are in the row).
+template <int N>
+__device__ __forceinline__ void finish_item(const JobDesc& jb, Lane<N>& ln, int status, int last, float drift)
+{
+    jb.len[ln.local] = last + 1;
+    jb.status[ln.local] = status;
+    jb.nrhs[ln.local] = ln.nrhs;
+    jb.e_drift[ln.local] = drift;
+    ln.active = 0;
+}
+
+// Start item `g`: load the initial condition (momenta inverted for the backward half,
+// invert_momenta src/hawaii.c:1171-1187), mu(0), c[0], H(0), f(y0), first step size.
+// Returns false if the item ended during set-up.
+template <int SYS, typename P, int N>
+__device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc* __restrict__ jobs, int njobs, long long g,
+                                          const Arena<N>& ar, Lane<N>& ln)
+{
+    const int j = find_job(jobs, njobs, g);
+    const JobDesc& jb = jobs[j];
+    ln.job = j;
+    ln.local = g - jb.item_begin;
+    const long long traj = ln.local / jb.halves;
+    const bool back = (ln.local % jb.halves) == 1;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double v = jb.ic[(long long)i * jb.ntraj + traj];
+        ln.y[i] = (back && (i & 1)) ? -v : v;
+    }
+    P m[3];
+    dipole<SYS, P>(sc, ln.y, m);
+    ln.mu0[0] = m[0];
+    ln.mu0[1] = m[1];
+    ln.mu0[2] = m[2];
+    double* row = jb.rows + ln.local * (long long)jb.L;
+    row[0] = ln.mu0[0] * ln.mu0[0] + ln.mu0[1] * ln.mu0[1] + ln.mu0[2] * ln.mu0[2];
+    ln.E0 = hamiltonian<SYS, P>(sc, ln.y);
+    ln.t = 0.0;
+    ln.k = 1;
+    ln.steps = 0;
+    ln.total_steps = 0;
+    ln.reject = 0;
+    ln.nrhs = 0;
+    ln.prev = 0.0;
+    ln.curr = 0.0;
+    ln.active = 1;
+    if (jb.L <= 1) {
+        finish_item<N>(jb, ln, ITEM_OK, 0, 0.0f);
+        return false;
+    }
+    {
+        double f[N];
+        rhs<SYS, P>(sc, ln.y, f);
+#pragma unroll
+        for (int i = 0; i < N; ++i) ar.st(0, i, f[i]);
+    }
+    ln.nrhs = 1;
+    ln.h = initial_step<SYS, P, N>(sc, ar, ln.y, jb.rtol, jb.atol, (jb.L - 1) * jb.dt, ln.nrhs);
+    return true;
+}
+
+// out = sum over the non-zero entries (m, a) of sparse tableau row r of a * K_m (arena slots).
+// The row index is uniform across the wavefront, so the coefficients are scalar loads.
+template <int N>
+__device__ __forceinline__ void row_dot(const Arena<N>& ar, int r, double* out)
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = 0.0;
+    const int nnz = DOP853_SP_NNZ[r];
+#pragma unroll 1
+    for (int q = 0; q < nnz; ++q) {
+        const int m = DOP853_SP_COL[r][q];
+        const double a = DOP853_SP_VAL[r][q];
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[i] += a * ar.ld(m, i);
+    }
+}
+
+// Compiler-only barrier: stage vectors are re-read from the arena after it instead of being
+// kept in registers across stages (which would rebuild the register-resident layout).
+__device__ __forceinline__ void arena_fence() { __asm__ __volatile__("" ::: "memory"); }
+
+// stages s0..s1 (inclusive): K_s = f(y + h sum_m A[s][m] K_m)
+template <int SYS, typename P, int N>
+__device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, const double* y, double h, int s0,
+                                       int s1)
+{
+#pragma unroll 1
+    for (int s = s0; s <= s1; ++s) {
+        arena_fence();
+        double yt[N], k[N];
+        row_dot<N>(ar, s, yt);
+#pragma unroll
+        for (int i = 0; i < N; ++i) yt[i] = y[i] + h * yt[i];
+        rhs<SYS, P>(sc, yt, k);
+#pragma unroll
+        for (int i = 0; i < N; ++i) ar.st(s, i, k[i]);
+    }
+}
+
+// One attempted DOP853 step for a lane.  Returns true when the lane's item has ended.
+template <int SYS, typename P, int N>
+__device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* __restrict__ jobs, const Arena<N>& ar,
+                                            Lane<N>& ln)
+{
+    const JobDesc& jb = jobs[ln.job];
+    const double tend = (jb.L - 1) * jb.dt;
+    if (++ln.steps > jb.max_steps || ++ln.total_steps > kMaxTotalSteps) {
+        finish_item<N>(jb, ln, ITEM_FAIL_STEPS, ln.k - 1, 0.0f);
+        return true;
+    }
+    if (0.1 * fabs(ln.h) <= fabs(ln.t) * kEPS) {
+        finish_item<N>(jb, ln, ITEM_FAIL_STEPSIZE, ln.k - 1, 0.0f);
+        return true;
+    }
+    double h = ln.h;
+    if (ln.t + 1.01 * h - tend > 0.0) h = tend - ln.t;
+
+    // stages 1..11 and stage 12 = f(t + h, y_new) (tableau row 12 is the B row)
+    stages<SYS, P, N>(sc, ar, ln.y, h, 1, 12);
+    ln.nrhs += 12;
+    arena_fence();
+    double yn[N];
+    row_dot<N>(ar, DOP853_ROW_B, yn);
+#pragma unroll
+    for (int i = 0; i < N; ++i) yn[i] = ln.y[i] + h * yn[i];
+
+    double err = 0.0, err2 = 0.0;
+    {
+        double e5[N], e3[N];
+        row_dot<N>(ar, DOP853_ROW_E5, e5);
+        row_dot<N>(ar, DOP853_ROW_E3, e3);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double sk = jb.atol + jb.rtol * fmax(fabs(ln.y[i]), fabs(yn[i]));
+            err += (e5[i] / sk) * (e5[i] / sk);
+            err2 += (e3[i] / sk) * (e3[i] / sk);
+        }
+    }
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (N * deno));
+    const double fac11 = sqrt(sqrt(sqrt(err)));
+    if (!(err == err) || !(h == h)) {   // non-finite state: stop the item
+        finish_item<N>(jb, ln, ITEM_FAIL_STEPSIZE, ln.k - 1, 0.0f);
+        return true;
+    }
+
+    if (err > 1.0) {
+        ln.h = h / fmin(1.0 / 0.333, fac11 / 0.9);
+        ln.reject = 1;
+        return false;
+    }
+
+    // accepted step [t, t + h]: record every sample t_k in (t, t + h]
+    const double tnew = ln.t + h;
+    bool ended = false;
+    int status = ITEM_OK;
+    float drift = 0.0f;
+    if (ln.k < jb.L && ln.k * jb.dt <= tnew) {
+        stages<SYS, P, N>(sc, ar, ln.y, h, 13, 15);
+        ln.nrhs += 3;
+        arena_fence();
+        // F0 = dy, F1 = h K0 - dy, F2 = 2 dy - h (K12 + K0), F3..F6 = h (D K)
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const double dy = yn[i] - ln.y[i];
+            const double k0 = ar.ld(0, i);
+            ar.st(kSlotF + 0, i, dy);
+            ar.st(kSlotF + 1, i, h * k0 - dy);
+            ar.st(kSlotF + 2, i, 2.0 * dy - h * (ar.ld(12, i) + k0));
+        }
+#pragma unroll 1
+        for (int d = 0; d < 4; ++d) {
+            double f[N];
+            row_dot<N>(ar, DOP853_ROW_D + d, f);
+#pragma unroll
+            for (int i = 0; i < N; ++i) ar.st(kSlotF + 3 + d, i, h * f[i]);
+        }
+        double* row = jb.rows + ln.local * (long long)jb.L;
+        while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
+            arena_fence();
+            const double tk = ln.k * jb.dt;
+            double ys[N];
+            if (tk == tnew) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) ys[i] = yn[i];
+            } else {
+                const double x = (tk - ln.t) / h, omx = 1.0 - x;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double v = ar.ld(kSlotF + 5, i) + x * ar.ld(kSlotF + 6, i);
+                    v = ar.ld(kSlotF + 4, i) + omx * v;
+                    v = ar.ld(kSlotF + 3, i) + x * v;
+                    v = ar.ld(kSlotF + 2, i) + omx * v;
+                    v = ar.ld(kSlotF + 1, i) + x * v;
+                    v = ar.ld(kSlotF + 0, i) + omx * v;
+                    ys[i] = ln.y[i] + x * v;
+                }
+            }
+            P m[3];
+            dipole<SYS, P>(sc, ys, m);
+            if (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2])) {
+                status = ITEM_FAIL_DIPOLE_NAN;
+                ended = true;
+                break;
+            }
+            ln.prev = ln.curr;
+            ln.curr = ln.mu0[0] * (double)m[0] + ln.mu0[1] * (double)m[1] + ln.mu0[2] * (double)m[2];
+            if (fabs(ln.curr) > 1e100) {
+                status = ITEM_FAIL_CF_HUGE;
+                ended = true;
+                break;
+            }
+            if (ln.k > 1 && fabs(ln.curr / ln.prev) > 1e10) {
+                status = ITEM_FAIL_CF_JUMP;
+                ended = true;
+                break;
+            }
+            row[ln.k] = ln.curr;
+            ln.steps = 0;
+            if (ys[4] > jb.Rcut || ln.k + 1 >= jb.L) {
+                drift = (float)fabs(hamiltonian<SYS, P>(sc, ys) - ln.E0);
+                ended = true;
+                break;
+            }
+            ++ln.k;
+        }
+    }
+    if (ended) {
+        // on success sample ln.k was stored; a failed check stops before storing it
+        finish_item<N>(jb, ln, status, status == ITEM_OK ? ln.k : ln.k - 1, drift);
+        return true;
+    }
+    // advance: y <- y_new, f(y) <- K12 (first-same-as-last)
+    arena_fence();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        ln.y[i] = yn[i];
+        ar.st(0, i, ar.ld(12, i));
+    }
+    ln.t = tnew;
+    const double fac = fmax(1.0 / 6.0, fmin(1.0 / 0.333, fac11 / 0.9));
+    double hnew = h / fac;
+    if (fabs(hnew) > tend) hnew = tend;
+    if (ln.reject) {
+        hnew = fmin(fabs(hnew), fabs(h));
+        ln.reject = 0;
+    }
+    ln.h = hnew;
+    return false;
+}
+
+// ------------------------------------------------------------------ kernels
+// CO2-Ar surface tables are staged into LDS once per workgroup: every right-hand side reads
+// 6 Legendre slots x 8 coefficients at a lane-dependent radial branch.
+template <int SYS, typename P>
+__device__ __forceinline__ SysConst stage_tables(const SysConst& sc, Co2ArTables<P>* s_tab)
+{
+    SysConst out = sc;
+    if constexpr (SYS == SYS_CO2AR) {
+        const double* src = (const double*)sc.tables;
+        double* dst = (double*)s_tab;
+        for (unsigned i = threadIdx.x; i < sizeof(Co2ArTables<P>) / sizeof(double); i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+        out.tables = s_tab;
+    }
+    return out;
+}
+
+// plain: lane g owns item g; ws holds one arena block per wavefront of the grid
+template <int SYS>
+__global__ void __launch_bounds__(256, HH_MIN_WAVES) plain_cf_kernel(SysConst sc, const JobDesc* __restrict__ jobs, int njobs,
+                                                       long long total_items, double* __restrict__ ws)
+{
+    constexpr int N = SysTraits<SYS>::QP;
+    __shared__ Co2ArTables<double> s_tab;
+    const SysConst scl = stage_tables<SYS, double>(sc, &s_tab);
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const Arena<N> ar = wave_arena<N>(ws, (long long)blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (g >= total_items) return;
+    Lane<N> ln;
+    if (!init_item<SYS, double, N>(scl, jobs, njobs, g, ar, ln)) return;
+    while (!dop853_step<SYS, double, N>(scl, jobs, ar, ln)) {
+    }
+}
+
+// persistent: each resident wavefront owns one arena block for its whole life
+template <int SYS, typename P, int BS>
+__global__ void __launch_bounds__(BS, HH_MIN_WAVES) persistent_cf_kernel(SysConst sc, const JobDesc* __restrict__ jobs, int njobs,
+                                                           long long total_items, QueueState* __restrict__ q,
+                                                           double* __restrict__ ws)
+{
+    constexpr int N = SysTraits<SYS>::QP;
+    __shared__ Co2ArTables<P> s_tab;
+    const SysConst scl = stage_tables<SYS, P>(sc, &s_tab);
+    const Arena<N> ar = wave_arena<N>(ws, (long long)blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+    Lane<N> ln;
+    ln.active = 0;
+    for (;;) {
+        if (!ln.active) {
+            // refill: pull items until one needs integration or the queue is empty
+            bool got = false;
+            while (!got) {
+                const unsigned long long g = atomicAdd(&q->next, 1ull);
+                if (g >= (unsigned long long)total_items) break;
+                got = init_item<SYS, P, N>(scl, jobs, njobs, (long long)g, ar, ln);
+            }
+            if (!got) break;
+        }
+        dop853_step<SYS, P, N>(scl, jobs, ar, ln);
+    }
+}
+
+// ------------------------------------------------------------------ ensemble reduction
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double v)
+{
+    const double t = s + v;
+    if (fabs(s) >= fabs(v))
+        c += (s - t) + v;
+    else
+        c += (v - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ bool dipole_failure(int st)
+{
+    return st == ITEM_FAIL_DIPOLE_NAN || st == ITEM_FAIL_CF_HUGE || st == ITEM_FAIL_CF_JUMP;
+}
+
+// partial[c][k] = sum over trajectories of chunk c of ppf * crln[k]; column-parallel, rows
+// read coalesced along k; entries past an item's length are zero and never read.
+__global__ void __launch_bounds__(256) cf_reduce_partial_kernel(JobDesc jb, long long chunk, double ppf,
+                                                                double* __restrict__ psum,
+                                                                double* __restrict__ pcomp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (k >= jb.L) return;
+    const long long t0 = (long long)c * chunk;
+    long long t1 = t0 + chunk;
+    if (t1 > jb.ntraj) t1 = jb.ntraj;
+    double s = 0.0, cc = 0.0;
+    for (long long t = t0; t < t1; ++t) {
+        double v;
+        if (jb.halves == 2) {
+            const long long a = 2 * t, b = 2 * t + 1;
+            if (dipole_failure(jb.status[a]) || dipole_failure(jb.status[b])) continue;
+            const double cf = k < jb.len[a] ? jb.rows[a * jb.L + k] : 0.0;
+            const double cb = k < jb.len[b] ? jb.rows[b * jb.L + k] : 0.0;
+            v = 0.5 * (cf + cb) * kALU * kALU * kALU;
+        } else {
+            if (dipole_failure(jb.status[t])) continue;
+            const double cf = k < jb.len[t] ? jb.rows[t * jb.L + k] : 0.0;
+            v = cf * kALU * kALU * kALU;
+        }
+        neumaier_add(s, cc, ppf * v);
+    }
+    psum[(long long)c * jb.L + k] = s;
+    pcomp[(long long)c * jb.L + k] = cc;
+}
+
+__global__ void __launch_bounds__(256) cf_reduce_final_kernel(int L, int nchunks, long long ntraj,
+                                                              const double* __restrict__ psum,
+                                                              const double* __restrict__ pcomp,
+                                                              double* __restrict__ cf_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L) return;
+    double s = 0.0, cc = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        neumaier_add(s, cc, psum[(long long)c * L + k]);
+        cc += pcomp[(long long)c * L + k];
+    }
+    cf_out[k] = (s + cc) / (double)ntraj;
+}
+
+// ------------------------------------------------------------------ host-side launchers
+template <int SYS>
+hipError_t launch_plain(const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
+                        hipStream_t st)
+{
+    const long long blocks = (total + 255) / 256;
+    hipLaunchKernelGGL((plain_cf_kernel<SYS>), dim3((unsigned)blocks), dim3(256), 0, st, sc, jobs_dev, njobs, total,
+                       ws);
+    return hipGetLastError();
+}
+
+template <int SYS, typename P, int BS>
+hipError_t launch_persistent(const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, QueueState* q,
+                             int grid, double* ws, hipStream_t st)
+{
+    hipLaunchKernelGGL((persistent_cf_kernel<SYS, P, BS>), dim3(grid), dim3(BS), 0, st, sc, jobs_dev, njobs, total,
+                       q, ws);
+    return hipGetLastError();
+}
+
+}  // namespace hh
+
+// explicit C++ entry points used by hh_host.cpp
+namespace hh {
+int arena_doubles_per_lane(int sys) { return kSlots * (sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP); }
+
+hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
+                     hipStream_t st)
+{
+    return sys == SYS_HEAR ? launch_plain<SYS_HEAR>(sc, jobs_dev, njobs, total, ws, st)
+                           : launch_plain<SYS_CO2AR>(sc, jobs_dev, njobs, total, ws, st);
+}
+
+hipError_t run_persistent(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* jobs_dev, int njobs,
+                          long long total, QueueState* q, int grid, double* ws, hipStream_t st)
+{
+#define HH_LAUNCH(S, PT, B) return launch_persistent<S, PT, B>(sc, jobs_dev, njobs, total, q, grid, ws, st)
+    if (sys == SYS_HEAR) {
+        // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
+        if (mixed) {
+            if (block == 64) HH_LAUNCH(SYS_HEAR, float, 64);
+            if (block == 128) HH_LAUNCH(SYS_HEAR, float, 128);
+            HH_LAUNCH(SYS_HEAR, float, 256);
+        }
+        if (block == 64) HH_LAUNCH(SYS_HEAR, double, 64);
+        if (block == 128) HH_LAUNCH(SYS_HEAR, double, 128);
+        HH_LAUNCH(SYS_HEAR, double, 256);
+    }
+    if (mixed) {
+        if (block == 64) HH_LAUNCH(SYS_CO2AR, float, 64);
+        if (block == 128) HH_LAUNCH(SYS_CO2AR, float, 128);
+        HH_LAUNCH(SYS_CO2AR, float, 256);
+    }
+    if (block == 64) HH_LAUNCH(SYS_CO2AR, double, 64);
+    if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128);
+    HH_LAUNCH(SYS_CO2AR, double, 256);
+#undef HH_LAUNCH
+}
+
+hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
+                      double* cf_out, hipStream_t st)
+{
+    dim3 g1((jb.L + 255) / 256, nchunks);
+    hipLaunchKernelGGL(cf_reduce_partial_kernel, g1, dim3(256), 0, st, jb, chunk, ppf, psum, pcomp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cf_reduce_final_kernel, dim3((jb.L + 255) / 256), dim3(256), 0, st, jb.L, nchunks, jb.ntraj,
+                       psum, pcomp, cf_out);
+    return hipGetLastError();
+}
+}  // namespace hh

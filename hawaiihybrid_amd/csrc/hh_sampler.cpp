@@ -1,0 +1,249 @@
+// Host-side initial-condition sampler and Mersenne Twister of the engine.
+//
+// The initial conditions are a few transcendental functions of a sequential Mersenne
+// Twister stream.  They run on the host on purpose: the reference evaluates them with the
+// host libm (log, cos, sin, acos, pow, exp), and reproducing those last-ulp results is
+// what makes the sampled phase points bit-identical to the reference's from the same seed.
+// The cost is ~15 draws and ~10 libm calls per trajectory (milliseconds per 1e5
+// trajectories), and the result is uploaded once per job.
+//
+// Compiled without floating-point contraction (the reference's default build, FLAGS_DEBUG
+// -O0 in its Makefile, performs none).
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/hawaii_gpu.h"
+#include "co2ar_tables.h"
+
+namespace {
+
+// ---------------------------------------------------------------- MT19937 with mtwist seeding
+// mt_seed32 (thirdparty/mtwist.c:372-406): state from the 69069 LCG, seed 0 -> 4357, refresh
+// at once; mt_lrand / mt_drand (thirdparty/mtwist.h:560-634): tempered words in generation
+// order, drand = word * 2^-32.
+class Mt {
+  public:
+    explicit Mt(uint32_t seed)
+    {
+        if (seed == 0) seed = 4357u;
+        s_[0] = seed;
+        for (int i = 1; i < kN; ++i) s_[i] = 69069u * s_[i - 1];
+        twist();
+    }
+    uint32_t next()
+    {
+        if (pos_ == kN) twist();
+        uint32_t y = s_[pos_++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        return y ^ (y >> 18);
+    }
+    double drand()
+    {
+        ++draws;
+        return static_cast<double>(next()) * (1.0 / 4294967296.0);
+    }
+    long draws = 0;
+
+  private:
+    static constexpr int kN = 624, kM = 397;
+    void twist()
+    {
+        int i = 0;
+        for (; i < kN - kM; ++i) s_[i] = s_[i + kM] ^ mix(s_[i], s_[i + 1]);
+        for (; i < kN - 1; ++i) s_[i] = s_[i + kM - kN] ^ mix(s_[i], s_[i + 1]);
+        s_[kN - 1] = s_[kM - 1] ^ mix(s_[kN - 1], s_[0]);
+        pos_ = 0;
+    }
+    static uint32_t mix(uint32_t a, uint32_t b)
+    {
+        const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+        return (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    uint32_t s_[kN];
+    int pos_ = kN;
+};
+
+constexpr double kHkT = 4.3597447222071e-18 / 1.380649e-23;   // Hartree / Boltzmann (src/constants.h:24)
+constexpr double kHTOCM = 2.1947463136320e5;
+
+// ---------------------------------------------------------------- built-in host surfaces
+// Used by the rejection step when the caller supplies no plugin pes_lab.
+double hear_V(double R)
+{
+    const double A = 0.2303058634e2, a1 = 0.163051044e1, a2 = 0.4680016e-1, b = 0.16274792500000e1;
+    const double C[6] = {0.940835513e1, 0.165523018e3, 0.379715796e4, 0.1165179999e6, 0.466258000e7, 0.236861000e9};
+    const double x = b * R, e = std::exp(-x);
+    double term = 1.0, sum = 1.0, V = A * std::exp(-a1 * R - a2 * R * R), R2n = R * R * R * R;
+    int k = 0;
+    for (int n = 3; n <= 8; ++n) {
+        R2n *= R * R;
+        while (k < 2 * n) {
+            ++k;
+            term *= x / k;
+            sum += term;
+        }
+        V -= C[n - 3] * (1.0 - e * sum) / R2n;
+    }
+    return V;
+}
+
+double co2ar_V(const double* q)
+{
+    // q = [Phi, Theta, R, phi1, theta1]; cos(theta_m) = Rhat . u
+    const double c0 = std::sin(q[1]) * std::cos(q[0]) * std::sin(q[4]) * std::cos(q[3]) +
+                      std::sin(q[1]) * std::sin(q[0]) * std::sin(q[4]) * std::sin(q[3]) +
+                      std::cos(q[1]) * std::cos(q[4]);
+    const double c = c0 > 1.0 ? 1.0 : (c0 < -1.0 ? -1.0 : c0);
+    double P[11];
+    P[0] = 1.0;
+    P[1] = c;
+    for (int l = 1; l < 10; ++l) P[l + 1] = ((2 * l + 1) * c * P[l] - l * P[l - 1]) / (l + 1);
+    const double R = q[2];
+    int b = 0;
+    while (b < HH_CO2AR_NBRANCH - 1 && !(R < HH_CO2AR_BOUNDS[b])) ++b;
+    double V = 0.0;
+    if (b == 0) {
+        for (int s = 0; s < 6; ++s) V += HH_CO2AR_PES_EXP[s][0] * std::exp(HH_CO2AR_PES_EXP[s][1] * R) * P[2 * s];
+    } else {
+        const double x = R - HH_CO2AR_CENTER[b], si = 1.0 / R, s4 = si * si * si * si;
+        for (int s = 0; s < 6; ++s) {
+            const double* cf = HH_CO2AR_PES[b][s];
+            V += (((cf[3] * x + cf[2]) * x + cf[1]) * x + cf[0] + s4 * (((cf[7] * si + cf[6]) * si + cf[5]) * si + cf[4])) *
+                 P[2 * s];
+        }
+    }
+    return V;
+}
+
+struct Sys {
+    int id;
+    double mu, I;
+    hh_pes_fn pes;
+    double V(const double* y) const
+    {
+        double q[5];
+        const int nq = id == HH_SYS_CO2AR ? 5 : 3;
+        for (int i = 0; i < nq; ++i) q[i] = y[2 * i];
+        if (pes) return pes(q);
+        return id == HH_SYS_CO2AR ? co2ar_V(q) : hear_V(q[2]);
+    }
+    // kinetic_energy, src/hawaii.c:1041-1137 (same operation order)
+    double K(const double* y) const
+    {
+        const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5];
+        const double sinTheta = std::sin(Theta);
+        double k1 = pR * pR / (2.0 * mu) + pTheta * pTheta / (2.0 * mu * R * R) +
+                    pPhi * pPhi / (2.0 * mu * R * R * sinTheta * sinTheta);
+        if (id == HH_SYS_CO2AR) {
+            const double st = std::sin(y[8]);
+            k1 = k1 + (y[9] * y[9] / (2.0 * I) + y[7] * y[7] / (2.0 * I * st * st));
+        }
+        return k1;
+    }
+};
+
+// generate_normal, src/hawaii.c:1206-1230: cosine branch of Box-Muller, retry on non-finite log U
+double normal(Mt& mt)
+{
+    for (;;) {
+        const double U = mt.drand();
+        const double V = mt.drand();
+        const double logU = std::log(U);
+        if (std::isnan(logU) || std::isinf(logU)) continue;
+        return 1.0 * std::sqrt(-2 * logU) * std::cos(2.0 * M_PI * V);
+    }
+}
+
+// q_generator, src/hawaii.c:1232-1290
+void gen_q(const Sys& s, const hh_sampler& p, Mt& mt, double* y)
+{
+    const double RMIN = p.Rmin, RMAX = p.Rmax;
+    const double R3 = mt.drand() * (RMAX * RMAX * RMAX - RMIN * RMIN * RMIN) + RMIN * RMIN * RMIN;
+    y[4] = std::pow(R3, 1.0 / 3.0);
+    y[0] = mt.drand() * 2.0 * M_PI;
+    y[2] = std::acos(2.0 * mt.drand() - 1.0);
+    if (s.id == HH_SYS_CO2AR) {
+        y[6] = mt.drand() * 2.0 * M_PI;
+        y[8] = std::acos(2.0 * mt.drand() - 1.0);
+    }
+}
+
+// p_generator + p_generator_linear_molecule, src/hawaii.c:1292-1380
+void gen_p(const Sys& s, double T, Mt& mt, double* y)
+{
+    const double sqrt_MUKT = std::sqrt(s.mu / kHkT * T);
+    const double sinTheta = std::sin(y[2]);
+    const double x0 = normal(mt), x1 = normal(mt), x2 = normal(mt);
+    y[1] = sqrt_MUKT * y[4] * sinTheta * x0;
+    y[3] = sqrt_MUKT * y[4] * x1;
+    y[5] = sqrt_MUKT * x2;
+    if (s.id == HH_SYS_CO2AR) {
+        const double sqrt_IIKT = std::sqrt(s.I / kHkT * T);
+        const double sin_theta = std::sin(y[8]);
+        const double r0 = normal(mt), r1 = normal(mt);
+        y[7] = sqrt_IIKT * sin_theta * r1;
+        y[9] = sqrt_IIKT * r0;
+    }
+}
+
+// reject, src/hawaii.c:1382-1421 (true = rejected)
+bool rejected(const Sys& s, const double* y, double T, double pesmin, Mt& mt)
+{
+    const double u = mt.drand();
+    const double Mval = std::exp(-pesmin * kHkT / T);
+    const double M = 1.5 * Mval;
+    const double kin_part = s.K(y);
+    const double pot_part = s.V(y);
+    const double proposal = std::exp(-kin_part * kHkT / T);
+    const double desired = std::exp(-(kin_part + pot_part) * kHkT / T);
+    const double weight = desired / proposal / M;
+    return weight < u;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n)
+{
+    Mt mt(seed);
+    for (size_t i = 0; i < n; ++i) out[i] = mt.next();
+}
+
+__attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
+                        double* ic, long* n_draws)
+{
+    const Sys s{sys->system, sys->mu, sys->I, pes};
+    const int qp = s.id == HH_SYS_CO2AR ? 10 : 6;
+    Mt mt(seed);
+    long accepted = 0, candidates = 0;
+    double y[10];
+    while (accepted < ntraj) {
+        std::memset(y, 0, sizeof(y));
+        gen_q(s, *smp, mt, y);
+        if (smp->mode == HH_IC_R0_INJECT) {
+            y[4] = smp->R0;
+            gen_p(s, smp->T, mt, y);
+            if (y[5] > 0.0) y[5] = -y[5];
+            ++candidates;
+        } else {
+            gen_p(s, smp->T, mt, y);
+            const double energy = s.K(y) + s.V(y);   // Hamiltonian = K + V (src/hawaii.c:1190-1204)
+            ++candidates;
+            if (rejected(s, y, smp->T, smp->pesmin, mt)) continue;
+            if (smp->pair_state == HH_PAIR_FREE && energy < 0.0) continue;
+            if (smp->pair_state == HH_PAIR_BOUND && energy > 0.0) continue;
+        }
+        std::memcpy(ic + accepted * qp, y, sizeof(double) * qp);
+        ++accepted;
+    }
+    if (n_draws) *n_draws = mt.draws;
+    return candidates;
+}
+
+}  // extern "C"

@@ -1,0 +1,412 @@
+// Device-side molecular systems: Hamilton's equations, PES gradients and induced dipoles
+// for He-Ar (atom-atom) and CO2-Ar (linear rotor + atom).
+//
+// Equations of motion follow src/hawaii.c:553-615 (rhs) and src/hawaii.c:474-538
+// (rhsMonomer, linear molecule); kinetic energy src/hawaii.c:1041-1137.  They stay in FP64.
+// The surface evaluation (PES gradient, dipole) is templated on its arithmetic type P
+// (double, or float in the mixed-precision engine).
+//
+// CO2-Ar surfaces (PES-IDS/ai_pes_co2ar_lib.cpp:5-30, PES-IDS/ai_ids_co2ar_lib.cpp:12-38):
+// the reference builds the molecular-frame angles through rotation matrices, acos and
+// atan2 (src/angles_handler.cpp:106-193) and maps the gradient back with a 5x5 Jacobian.
+// Here the same quantities come from two unit vectors:
+//     Rhat = (sin T cos P, sin T sin P, cos T),  u = (sin t cos p, sin t sin p, cos t),
+//     cos(theta_m) = c = Rhat . u,
+//     dV/dq_i = dV/dR delta_{i,R} + (sum_l c_l(R) P_l'(c)) dc/dq_i,
+//     mu_lab = dipz Rhat - (sum_l dx_l(R) P_l'(c)) (u - c Rhat)
+// (P_l^1 = -(1-c^2)^{1/2} P_l' with the Condon-Shortley phase, and the molecular x axis is
+// (u - c Rhat)/sin(theta_m)), which is the reference's chain without the 1/sin(theta_m)
+// singularity and without inverse trigonometric functions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hh {
+
+constexpr double kALU = 5.29177210903e-11;           // src/constants.h:8
+constexpr double kHTOCM = 2.1947463136320e5;         // src/constants.h:11
+constexpr double kTWO_PI = 6.283185307179586476925;
+
+enum SystemId : int { SYS_HEAR = 0, SYS_CO2AR = 1 };
+
+// Surface tables resident in device memory (filled by the host from co2ar_tables.h).
+template <typename P>
+struct Co2ArTables {
+    P pes[32][6][8];     // centred cubic (4) + R^-4..R^-7 (4), Hartree
+    P pes_exp[6][2];     // R < 4.5: A exp(alpha R), Hartree
+    P dipx[32][5][8];    // P_l^1 slots l = 2..10
+    P dipz[32][6][8];    // P_l slots l = 0..10
+    double center[32];
+};
+
+struct SysConst {
+    double mu;           // reduced mass
+    double I;            // linear-rotor moment of inertia (II[0] == II[1])
+    const void* tables;  // Co2ArTables<P>* for CO2-Ar
+};
+
+// ------------------------------------------------------------------ He-Ar (PES-IDS/HeAr.h)
+// Tang-Toennies form V = A exp(-a1 R - a2 R^2) - sum_{n=3}^{8} C_2n f_2n(R)/R^2n with
+// f_2n = 1 - exp(-bR) sum_{k<=2n} (bR)^k/k!; derivative analytic.  Kept in FP64 in every
+// engine mode: f_2n cancels to 1e-5 below R ~ 5 bohr, beyond what FP32 can carry.
+__device__ __forceinline__ double hear_dV(double R)
+{
+    const double A = 0.2303058634e2, a1 = 0.163051044e1, a2 = 0.4680016e-1, b = 0.16274792500000e1;
+    const double C[6] = {0.940835513e1, 0.165523018e3, 0.379715796e4, 0.1165179999e6, 0.466258000e7,
+                         0.236861000e9};
+    const double x = b * R, e = exp(-x), inv = 1.0 / R, inv2 = inv * inv;
+    double term = 1.0, sum = 1.0;
+    double dV = A * exp(-a1 * R - a2 * R * R) * (-a1 - 2.0 * a2 * R);
+    double ip = inv2 * inv2;  // R^-4
+    int k = 0;
+#pragma unroll
+    for (int n = 3; n <= 8; ++n) {
+        ip *= inv2;  // R^-2n
+#pragma unroll
+        for (int j = 0; j < 2; ++j) { ++k; term *= x * (1.0 / k); sum += term; }
+        if (n == 3) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { ++k; term *= x * (1.0 / k); sum += term; }
+        }
+        dV -= C[n - 3] * (b * e * term * ip - 2.0 * n * (1.0 - e * sum) * ip * inv);
+    }
+    return dV;
+}
+
+__device__ __forceinline__ double hear_V(double R)
+{
+    const double A = 0.2303058634e2, a1 = 0.163051044e1, a2 = 0.4680016e-1, b = 0.16274792500000e1;
+    const double C[6] = {0.940835513e1, 0.165523018e3, 0.379715796e4, 0.1165179999e6, 0.466258000e7,
+                         0.236861000e9};
+    const double x = b * R, e = exp(-x), inv = 1.0 / R, inv2 = inv * inv;
+    double term = 1.0, sum = 1.0, V = A * exp(-a1 * R - a2 * R * R), ip = inv2 * inv2;
+    int k = 0;
+#pragma unroll
+    for (int n = 3; n <= 8; ++n) {
+        ip *= inv2;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) { ++k; term *= x * (1.0 / k); sum += term; }
+        if (n == 3) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) { ++k; term *= x * (1.0 / k); sum += term; }
+        }
+        V -= C[n - 3] * (1.0 - e * sum) * ip;
+    }
+    return V;
+}
+
+template <typename P>
+__device__ __forceinline__ P hear_dip(P R)
+{
+    // PES-IDS/HeAr.h:44-47
+    const P r2 = R * R, inv = P(1) / R, inv2 = inv * inv, inv7 = inv2 * inv2 * inv2 * inv;
+    return exp(P(0.47293181) - P(0.40100613) * R - P(0.10292726) * r2) - P(148.55391) * inv7;
+}
+
+// ------------------------------------------------------------------ CO2-Ar
+__device__ __forceinline__ int co2ar_branch(double R)
+{
+    // breakpoints 4.5, 4.75, ..., 10 (step 1/4), then 11, 12, 14, 16, 18, 20, 25, 30
+    if (R < 4.5) return 0;
+    if (R < 10.0) return 1 + (int)((R - 4.5) * 4.0);   // exact: R - 4.5 and *4 are exact here
+    int b = 23;
+    b += (R >= 11.0) + (R >= 12.0) + (R >= 14.0) + (R >= 16.0) + (R >= 18.0) + (R >= 20.0) + (R >= 25.0) +
+         (R >= 30.0);
+    return b;
+}
+
+// sin and cos for the bounded angles of a trajectory (|x| < 1.6e6 rad).  Cody-Waite
+// reduction by pi/2 in FP64 (pi/2 split into three doubles whose leading two carry 33
+// significant bits, so k*C1 and k*C2 are exact for |k| < 2^20), then the Taylor series of
+// sin and cos on |r| <= pi/4, truncated where the next term is below 2^-60 (FP64) or
+// 2^-30 (FP32) relative, and quadrant selection.  No table, no large-argument path.
+struct Reduced {
+    double r;
+    int q;
+};
+
+__device__ __forceinline__ Reduced reduce_pio2(double x)
+{
+    const double k = rint(x * 0.6366197723675814);
+    double r = fma(-k, 1.5707963267341256, x);
+    r = fma(-k, 6.077100506303966e-11, r);
+    r = fma(-k, 2.0222662487959506e-21, r);
+    return {r, (int)k & 3};
+}
+
+template <typename P>
+__device__ __forceinline__ void sincos_p(double x, P* s, P* c)
+{
+    const Reduced red = reduce_pio2(x);
+    P sr, cr;
+    if constexpr (sizeof(P) == 8) {
+        const double r = red.r, r2 = r * r;
+        sr = r + r * r2 * (-1.0 / 6 + r2 * (1.0 / 120 + r2 * (-1.0 / 5040 + r2 * (1.0 / 362880 + r2 * (-1.0 / 39916800 +
+             r2 * (1.0 / 6227020800.0 + r2 * (-1.0 / 1307674368000.0 + r2 * (1.0 / 355687428096000.0))))))));
+        cr = 1.0 + r2 * (-0.5 + r2 * (1.0 / 24 + r2 * (-1.0 / 720 + r2 * (1.0 / 40320 + r2 * (-1.0 / 3628800 +
+             r2 * (1.0 / 479001600.0 + r2 * (-1.0 / 87178291200.0 + r2 * (1.0 / 20922789888000.0))))))));
+    } else {
+        const float r = (float)red.r, r2 = r * r;
+        sr = r + r * r2 * (-1.0f / 6 + r2 * (1.0f / 120 + r2 * (-1.0f / 5040 + r2 * (1.0f / 362880))));
+        cr = 1.0f + r2 * (-0.5f + r2 * (1.0f / 24 + r2 * (-1.0f / 720 + r2 * (1.0f / 40320 + r2 * (-1.0f / 3628800)))));
+    }
+    const bool swap = red.q & 1;
+    const P s0 = swap ? cr : sr, c0 = swap ? sr : cr;
+    *s = (red.q == 2 || red.q == 3) ? -s0 : s0;
+    *c = (red.q == 1 || red.q == 2) ? -c0 : c0;
+}
+
+// Legendre P_l(c), P_l'(c), l = 0..10 (three-term recurrences, P'_{l+1} = P'_{l-1} + (2l+1) P_l)
+template <typename P>
+__device__ __forceinline__ void legendre10(P c, P* Pl, P* dPl)
+{
+    Pl[0] = P(1);
+    Pl[1] = c;
+    dPl[0] = P(0);
+    dPl[1] = P(1);
+#pragma unroll
+    for (int l = 1; l < 10; ++l) {
+        Pl[l + 1] = (P(2 * l + 1) * c * Pl[l] - P(l) * Pl[l - 1]) * P(1.0 / (l + 1));
+        dPl[l + 1] = dPl[l - 1] + P(2 * l + 1) * Pl[l];
+    }
+}
+
+// radial slot value and derivative: centred cubic in x plus R^-4..R^-7 in s = 1/R
+template <typename P>
+__device__ __forceinline__ void slot_eval(const P* c, P x, P s, P& v, P& dv)
+{
+    const P s2 = s * s, s4 = s2 * s2;
+    v = ((c[3] * x + c[2]) * x + c[1]) * x + c[0] + s4 * (((c[7] * s + c[6]) * s + c[5]) * s + c[4]);
+    dv = (P(3) * c[3] * x + P(2) * c[2]) * x + c[1] -
+         s4 * s * (((P(7) * c[7] * s + P(6) * c[6]) * s + P(5) * c[5]) * s + P(4) * c[4]);
+}
+
+template <typename P>
+__device__ __forceinline__ P slot_value(const P* c, P x, P s)
+{
+    const P s2 = s * s, s4 = s2 * s2;
+    return ((c[3] * x + c[2]) * x + c[1]) * x + c[0] + s4 * (((c[7] * s + c[6]) * s + c[5]) * s + c[4]);
+}
+
+struct Trig {
+    double sT, cT, st, ct;   // FP64 sin/cos of Theta and theta1 (EOM)
+};
+
+// CO2-Ar lab-frame gradient dV/dq, q = [Phi, Theta, R, phi1, theta1] (Hartree / bohr, rad)
+template <typename P>
+__device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb, const double* y,
+                                           const Trig& tr, double* dV)
+{
+    P sP, cP, sp, cp;
+    sincos_p<P>(y[0], &sP, &cP);
+    sincos_p<P>(y[6], &sp, &cp);
+    const P sT = (P)tr.sT, cT = (P)tr.cT, st = (P)tr.st, ct = (P)tr.ct;
+    const P Rx = sT * cP, Ry = sT * sP, Rz = cT;
+    const P ux = st * cp, uy = st * sp, uz = ct;
+    P c = Rx * ux + Ry * uy + Rz * uz;
+    c = c > P(1) ? P(1) : (c < P(-1) ? P(-1) : c);
+    const P dc_dPhi = -Ry * ux + Rx * uy;                       // dRhat/dPhi . u
+    const P dc_dTheta = cT * cP * ux + cT * sP * uy - sT * uz;  // dRhat/dTheta . u
+    const P dc_dphi = -Rx * uy + Ry * ux;                       // Rhat . du/dphi
+    const P dc_dtheta = ct * cp * Rx + ct * sp * Ry - st * Rz;  // Rhat . du/dtheta
+
+    P Pl[11], dPl[11];
+    legendre10<P>(c, Pl, dPl);
+
+    const double R = y[4];
+    const int b = co2ar_branch(R);
+    P dVdR = P(0), dVdc = P(0);
+    if (b == 0) {
+        const P r = (P)R;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const P e = tb->pes_exp[s][0] * exp(tb->pes_exp[s][1] * r);
+            dVdR += tb->pes_exp[s][1] * e * Pl[2 * s];
+            dVdc += e * dPl[2 * s];
+        }
+    } else {
+        const P x = (P)(R - tb->center[b]);
+        const P sinv = (P)(1.0 / R);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            P v, dv;
+            slot_eval<P>(tb->pes[b][s], x, sinv, v, dv);
+            dVdR += dv * Pl[2 * s];
+            dVdc += v * dPl[2 * s];
+        }
+    }
+    dV[0] = (double)(dVdc * dc_dPhi);
+    dV[1] = (double)(dVdc * dc_dTheta);
+    dV[2] = (double)dVdR;
+    dV[3] = (double)(dVdc * dc_dphi);
+    dV[4] = (double)(dVdc * dc_dtheta);
+}
+
+template <typename P>
+__device__ __forceinline__ P co2ar_pes(const Co2ArTables<P>* __restrict__ tb, const double* y)
+{
+    double sTd, cTd, std_, ctd;
+    sincos_p<double>(y[2], &sTd, &cTd);
+    sincos_p<double>(y[8], &std_, &ctd);
+    P sP, cP, sp, cp;
+    sincos_p<P>(y[0], &sP, &cP);
+    sincos_p<P>(y[6], &sp, &cp);
+    const P sT = (P)sTd, cT = (P)cTd, st = (P)std_, ct = (P)ctd;
+    P c = sT * cP * st * cp + sT * sP * st * sp + cT * ct;
+    c = c > P(1) ? P(1) : (c < P(-1) ? P(-1) : c);
+    P Pl[11], dPl[11];
+    legendre10<P>(c, Pl, dPl);
+    const double R = y[4];
+    const int b = co2ar_branch(R);
+    P V = P(0);
+    if (b == 0) {
+#pragma unroll
+        for (int s = 0; s < 6; ++s) V += tb->pes_exp[s][0] * exp(tb->pes_exp[s][1] * (P)R) * Pl[2 * s];
+    } else {
+        const P x = (P)(R - tb->center[b]);
+        const P sinv = (P)(1.0 / R);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) V += slot_value<P>(tb->pes[b][s], x, sinv) * Pl[2 * s];
+    }
+    return V;
+}
+
+// CO2-Ar lab-frame induced dipole at state y
+template <typename P>
+__device__ __forceinline__ void co2ar_dipole(const Co2ArTables<P>* __restrict__ tb, const double* y, P* mu)
+{
+    P sT, cT, sP, cP, st, ct, sp, cp;
+    sincos_p<P>(y[2], &sT, &cT);
+    sincos_p<P>(y[0], &sP, &cP);
+    sincos_p<P>(y[8], &st, &ct);
+    sincos_p<P>(y[6], &sp, &cp);
+    const P Rx = sT * cP, Ry = sT * sP, Rz = cT;
+    const P ux = st * cp, uy = st * sp, uz = ct;
+    P c = Rx * ux + Ry * uy + Rz * uz;
+    c = c > P(1) ? P(1) : (c < P(-1) ? P(-1) : c);
+    P Pl[11], dPl[11];
+    legendre10<P>(c, Pl, dPl);
+    const double R = y[4];
+    const int b = co2ar_branch(R);
+    const P x = (P)(R - tb->center[b]);
+    const P sinv = (P)(1.0 / R);
+    P gx = P(0), dz = P(0);
+#pragma unroll
+    for (int s = 0; s < 5; ++s) gx += slot_value<P>(tb->dipx[b][s], x, sinv) * dPl[2 * s + 2];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) dz += slot_value<P>(tb->dipz[b][s], x, sinv) * Pl[2 * s];
+    // mu = dz Rhat - gx (u - c Rhat)
+    mu[0] = dz * Rx - gx * (ux - c * Rx);
+    mu[1] = dz * Ry - gx * (uy - c * Ry);
+    mu[2] = dz * Rz - gx * (uz - c * Rz);
+}
+
+// ------------------------------------------------------------------ systems as types
+template <int SYS>
+struct SysTraits;
+
+template <>
+struct SysTraits<SYS_HEAR> {
+    static constexpr int QP = 6;
+};
+template <>
+struct SysTraits<SYS_CO2AR> {
+    static constexpr int QP = 10;
+};
+
+// Right-hand side of Hamilton's equations.  Intermolecular part src/hawaii.c:586-591, linear
+// rotor src/hawaii.c:494-497.
+template <int SYS, typename P>
+__device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict__ y, double* __restrict__ yd)
+{
+    const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
+    Trig tr;
+    sincos_p<double>(Theta, &tr.sT, &tr.cT);
+    double dV[5];
+    if constexpr (SYS == SYS_HEAR) {
+        dV[0] = 0.0;
+        dV[1] = 0.0;
+        dV[2] = hear_dV(R);
+    } else {
+        sincos_p<double>(y[8], &tr.st, &tr.ct);
+        co2ar_dpes<P>((const Co2ArTables<P>*)sc.tables, y, tr, dV);
+    }
+    const double iR = 1.0 / R, iR2 = iR * iR, imu = 1.0 / mu;
+    const double is = 1.0 / tr.sT, is2 = is * is;
+    const double a = imu * iR2;               // 1/(mu R^2)
+    yd[4] = pR * imu;
+    yd[5] = (pTheta * pTheta + pPhi * pPhi * is2) * a * iR - dV[2];
+    yd[0] = pPhi * a * is2;
+    yd[1] = -dV[0];
+    yd[2] = pTheta * a;
+    yd[3] = pPhi * pPhi * tr.cT * a * is2 * is - dV[1];
+    if constexpr (SYS == SYS_CO2AR) {
+        const double iI = 1.0 / sc.I, ist = 1.0 / tr.st, ist2 = ist * ist;
+        const double pphi = y[7], ptheta = y[9];
+        yd[6] = pphi * iI * ist2;
+        yd[7] = -dV[3];
+        yd[8] = ptheta * iI;
+        yd[9] = pphi * pphi * tr.ct * iI * ist2 * ist - dV[4];
+    }
+}
+
+template <int SYS, typename P>
+__device__ __forceinline__ void dipole(const SysConst& sc, const double* y, P* mu)
+{
+    if constexpr (SYS == SYS_HEAR) {
+        P sT, cT, sP, cP;
+        sincos_p<P>(y[2], &sT, &cT);
+        sincos_p<P>(y[0], &sP, &cP);
+        const P d = hear_dip<P>((P)y[4]);
+        mu[0] = d * sT * cP;
+        mu[1] = d * sT * sP;
+        mu[2] = d * cT;
+    } else {
+        co2ar_dipole<P>((const Co2ArTables<P>*)sc.tables, y, mu);
+    }
+}
+
+// kinetic_energy (src/hawaii.c:1041-1137), operation order of the reference
+template <int SYS>
+__device__ __forceinline__ double kinetic(const SysConst& sc, const double* y)
+{
+    const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
+    double s, c_unused;
+    sincos_p<double>(Theta, &s, &c_unused);
+    double K = pR * pR / (2.0 * mu) + pTheta * pTheta / (2.0 * mu * R * R) + pPhi * pPhi / (2.0 * mu * R * R * s * s);
+    if constexpr (SYS == SYS_CO2AR) {
+        double st, ct_unused;
+        sincos_p<double>(y[8], &st, &ct_unused);
+        K = K + (y[9] * y[9] / (2.0 * sc.I) + y[7] * y[7] / (2.0 * sc.I * st * st));
+    }
+    return K;
+}
+
+template <int SYS, typename P>
+__device__ __forceinline__ double pes_value(const SysConst& sc, const double* y)
+{
+    if constexpr (SYS == SYS_HEAR)
+        return hear_V(y[4]);
+    else
+        return (double)co2ar_pes<P>((const Co2ArTables<P>*)sc.tables, y);
+}
+
+template <int SYS, typename P>
+__device__ __forceinline__ double hamiltonian(const SysConst& sc, const double* y)
+{
+    const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
+    double s, c_unused;
+    sincos_p<double>(Theta, &s, &c_unused);
+    double K = pR * pR / (2.0 * mu) + pTheta * pTheta / (2.0 * mu * R * R) + pPhi * pPhi / (2.0 * mu * R * R * s * s);
+    double V;
+    if constexpr (SYS == SYS_HEAR) {
+        V = hear_V(R);
+    } else {
+        double st, ct_unused;
+        sincos_p<double>(y[8], &st, &ct_unused);
+        K += y[9] * y[9] / (2.0 * sc.I) + y[7] * y[7] / (2.0 * sc.I * st * st);
+        V = (double)co2ar_pes<P>((const Co2ArTables<P>*)sc.tables, y);
+    }
+    return K + V;
+}
+
+}  // namespace hh

@@ -1,0 +1,151 @@
+/*
+ * hawaii_gpu.h -- C ABI of the MI355X trajectory-ensemble engine (libhawaii_gpu.so).
+ *
+ * Plain C: pointers, sizes and POD structs; no HIP or torch types.  Device buffers are
+ * passed as plain `double*` device addresses (hipMalloc'd or torch-allocated on the same
+ * device).  Every entry point returns 0 on success or a negative HH_E* code; the message
+ * of the last error on the calling thread is available from hh_last_error().  Nothing in
+ * this library calls exit() or abort().
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   hh_sample_ics        q_generator / p_generator / reject / candidate loop
+ *                        (src/hawaii.c:1206-1421, src/hawaii.c:2969-2993) and the R0
+ *                        injection of the PR/mu path (src/hawaii.c:3486-3512), seeded as
+ *                        init_ms(..., seed) -> mt_seed32 (src/hawaii.c:157-163)
+ *   hh_mt_fill           mt_seed32 / mt_lrand (thirdparty/mtwist.c:372-406, 640-796;
+ *                        thirdparty/mtwist.h:560-571)
+ *   hh_cf_run            the trajectory loop of calculate_correlation_and_save
+ *                        (src/hawaii.c:2969-3114) with correlation_eval
+ *                        (src/hawaii.c:2116-2347) and the CVODE propagation it drives
+ *                        (src/trajectory.c:146-200)
+ *   hh_cf_to_sf          idct_cf_to_sf (src/hawaii.c:5569-5601)
+ *   hh_m0                calculate_M0 / mpi_calculate_M0 (src/hawaii.c:1423-1491,
+ *                        src/hawaii.c:1620-1738)
+ * The reference-compatible structs and entry points (MoleculeSystem, CalcParams,
+ * calculate_correlation_and_save, ...) are declared in hawaii_compat.h.
+ */
+#ifndef HAWAII_GPU_H
+#define HAWAII_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+enum {
+    HH_OK = 0,
+    HH_E_INVALID = -1,     /* bad argument */
+    HH_E_DEVICE = -2,      /* HIP runtime error or no usable gfx950 device */
+    HH_E_NOMEM = -3,       /* device or host allocation failed */
+    HH_E_UNSUPPORTED = -4, /* system / mode not provided on the device */
+    HH_E_BUSY = -5,        /* serving: no capacity */
+    HH_E_NOTFOUND = -6,
+};
+const char* hh_last_error(void);
+
+/* ------------------------------------------------------------------ systems */
+enum { HH_SYS_HEAR = 0, HH_SYS_CO2AR = 1 };
+
+typedef struct {
+    int system;     /* HH_SYS_* */
+    double mu;      /* reduced mass, a.u. (pair_reduced_mass) */
+    double I;       /* linear-rotor moment of inertia II[0] (== II[1]); ignored for He-Ar */
+} hh_system;
+
+/* Plugin surface ABI of the reference (src/hawaii.h:480-499): q = [Phi, Theta, R, monomer angles] */
+typedef double (*hh_pes_fn)(double* q);
+typedef void (*hh_dpes_fn)(double* q, double* dVdq);
+typedef void (*hh_dipole_fn)(double* q, double mu[3]);
+
+int hh_qp_size(int system);
+
+/* ------------------------------------------------------------------ Mersenne Twister */
+void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n);
+
+/* ------------------------------------------------------------------ initial conditions */
+enum { HH_IC_REJECTION = 0, HH_IC_R0_INJECT = 1 };
+enum { HH_PAIR_FREE = 1, HH_PAIR_BOUND = 2, HH_PAIR_ALL = 3 }; /* PairState values of src/hawaii.h:291-297 */
+
+typedef struct {
+    int mode;          /* HH_IC_* */
+    int pair_state;    /* rejection mode: FREE keeps E >= 0, BOUND keeps E <= 0 */
+    double T;          /* K */
+    double Rmin, Rmax; /* sampler_Rmin / sampler_Rmax, bohr */
+    double pesmin;     /* Hartree (rejection mode) */
+    double R0;         /* bohr (R0-injection mode) */
+} hh_sampler;
+
+/* Sample `ntraj` initial conditions from the seeded stream, row-major [ntraj][QP], on the
+ * host, in the reference's draw order and arithmetic (bit-exact given the same pes).  The
+ * rejection step calls `pes` (a plugin's pes_lab); NULL selects the engine's built-in
+ * host surface.  Returns the number of candidates drawn (>= ntraj) or a negative code. */
+long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
+                   double* ic_rowmajor, long* n_draws);
+
+/* ------------------------------------------------------------------ engine */
+typedef struct hh_engine hh_engine;
+
+int hh_engine_create(int device, hh_engine** out);
+void hh_engine_destroy(hh_engine* e);
+int hh_engine_info(hh_engine* e, char* buf, size_t len);
+
+enum { HH_ENGINE_PLAIN = 0, HH_ENGINE_BATCHED = 1 };
+enum { HH_PREC_FP64 = 0, HH_PREC_MIXED = 1 };
+enum { HH_HALVES_BOTH = 0, HH_HALVES_FORWARD = 1 };
+
+typedef struct {
+    double rtol, atol;   /* CVODE_TOLERANCE: rtol = atol in the reference (src/trajectory.c:214-230) */
+    double dt;           /* sampling_time, a.t.u. */
+    double Rcut;         /* bohr */
+    double ppf;          /* partial_partition_function_ratio weight */
+    int L;               /* MaxTrajectoryLength */
+    int halves;          /* HH_HALVES_* */
+    int max_steps;       /* max integrator steps between two samples (<= 0: 50000) */
+    int engine;          /* HH_ENGINE_* */
+    int precision;       /* HH_PREC_*; the plain engine is FP64 only */
+    int block;           /* batched: workgroup size 64/128/256 (0: autotuned) */
+    int grid;            /* batched: resident workgroups (0: autotuned) */
+} hh_cf_params;
+
+typedef struct {
+    double integrate_ms;    /* device time of the integration launch (HIP events) */
+    double reduce_ms;       /* device time of the CF reductions */
+    long long n_items;      /* work items (trajectory halves) */
+    long long nrhs;         /* right-hand-side evaluations */
+    long long nsamples;     /* recorded dipole samples */
+    long long n_failed;     /* items that ended on a dipole check (their trajectory counts as zero) */
+    long long n_stepfail;   /* items that ended on a step-count / step-size limit (partial CF kept) */
+    double max_e_drift;     /* max |H(end) - H(0)|, Hartree */
+    int block, grid;        /* launch shape used */
+} hh_cf_stats;
+
+/* Run `njobs` correlation-function jobs of one system on the device.  ic_dev[j] holds
+ * job j's initial conditions, SoA [QP][ntraj[j]] doubles; cf_dev[j] receives its ensemble
+ * CF[L] = sum_traj ppf crln / ntraj (crln as correlation_eval).  The batched engine streams
+ * all jobs through one persistent launch.  Synchronous. */
+int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njobs, const double* const* ic_dev,
+              const long* ntraj, double* const* cf_dev, hh_cf_stats* stats);
+
+/* Per-item results of job `job` of the last hh_cf_run (host buffers of n_items entries;
+ * any pointer may be NULL).  rows: [n_items][L], entries past len are zero. */
+int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* e_drift, double* rows);
+
+/* ------------------------------------------------------------------ CF -> SF on the device */
+/* Connes apodization in place on cf_dev, Makhoul IDCT through a 2n-point complex FFT,
+ * unit scaling: nu_dev[n] in cm^-1, sf_dev[n] (idct_cf_to_sf).  n must be a power of two. */
+int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev, double* sf_dev, double* ms);
+
+/* ------------------------------------------------------------------ M0 phase-space moment */
+/* Rejection-sampler M0 over `npoints` accepted points (calculate_M0 semantics) with the
+ * candidate stream split into `nstreams` independent Mersenne Twister streams seeded
+ * from `seed` (stream s: seed + s); m, q as in the reference (times ppf). */
+int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
+          double ppf, double* m, double* q, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -62,7 +62,7 @@ static double initial_step(fctx *c, int n, const double *y0, const double *f0, d
     der2 = sqrt(der2) / h;
     der12 = max2(fabs(der2), sqrt(dnf));
     if (der12 <= 1e-15) h1 = max2(1e-6, fabs(h) * 1e-3);
-    else h1 = pow(0.01 / der12, 1.0 / 8.0);
+    else h1 = sqrt(sqrt(sqrt(0.01 / der12)));
     return min2(min2(100 * fabs(h), h1), hmax);
 }
 
@@ -137,7 +137,7 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
             if (deno <= 0.0) deno = 1.0;
             err = fabs(h) * err * sqrt(1.0 / (n * deno));
         }
-        fac11 = pow(err, 1.0 / 8.0);
+        fac11 = sqrt(sqrt(sqrt(err)));   /* err^(1/8) */
         fac = max2(1.0 / 6.0, min2(1.0 / 0.333, fac11 / 0.9));
         hnew = h / fac;
 

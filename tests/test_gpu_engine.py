@@ -1,0 +1,161 @@
+"""GPU parity of the engine against the CPU oracle (run with -m gpu on an MI355X).
+
+Tolerances:
+  * FP64 engines, non-chaotic sizes (L <= 1024): per-trajectory CF rows within 1e-6 relative
+    L2 of the oracle for >= 99% of trajectories and ensemble CF within 1e-6 relative L2.
+    Both integrate the same equations with the same DOP853 controller; they differ in the
+    last ulp of sin/cos/exp (own device routines vs glibc) and FMA contraction, which an
+    orbiting (chaotic) trajectory amplifies, so a bound per trajectory is stated for the bulk.
+  * mixed engine (FP32 surfaces, FP64 state and control): ensemble CF within 1e-4 relative L2.
+  * integer and index work (lengths, statuses, the sampled initial conditions) exactly.
+"""
+import numpy as np
+import pytest
+
+import hawaiihybrid_amd as H
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+ALU3 = 5.29177210903e-11 ** 3
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = H.Engine(0)
+    yield e
+    e.close()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def run_and_compare(eng, sys_h, sys_o, smp_h, smp_o, ntraj, L, halves, engine, prec, rtol=1e-12, seed=42):
+    ic, _, _ = H.sample_ics(sys_h, smp_h, seed, ntraj)
+    ref = O.cf_rows(sys_o, O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, halves, 50000), ic, 16)
+    cf_ref = O.ensemble_from_rows(ref["crln"])
+    p = H.cf_params(L, rtol=rtol, halves=halves, engine=engine, precision=prec)
+    out, st = eng.cf_run(sys_h, p, [eng.to_device_soa(ic)])
+    n_items = ntraj * (2 if halves == H.HALVES_BOTH else 1)
+    it = eng.items(0, n_items, L, rows=True)
+    if halves == H.HALVES_BOTH:
+        rows = 0.5 * (it["rows"][0::2] + it["rows"][1::2]) * ALU3
+    else:
+        rows = it["rows"] * ALU3
+    per = np.array([rel_l2(rows[i], ref["crln"][i]) for i in range(ntraj)])
+    return dict(cf=out[0].cpu().numpy(), cf_ref=cf_ref, per=per, items=it, ref=ref, stats=st)
+
+
+def test_engine_is_gfx950(eng):
+    assert eng.info()["arch"].startswith("gfx950")
+
+
+@pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
+def test_hear_r0_fp64_matches_oracle(eng, engine):
+    r = run_and_compare(eng, H.he_ar(), O.hear(), H.Sampler(H.IC_R0_INJECT, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 40.0),
+                        None, 512, 1024, H.HALVES_FORWARD, engine, H.PREC_FP64)
+    assert rel_l2(r["cf"], r["cf_ref"]) < 1e-10
+    assert np.quantile(r["per"], 0.99) < 1e-10
+    assert np.array_equal(r["items"]["len"], r["ref"]["len_f"])
+    assert np.all(r["items"]["status"] == 0)
+
+
+@pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
+def test_hear_rejection_both_halves_matches_oracle(eng, engine):
+    """CORRELATION_SINGLE semantics: rejection-sampled free pairs, forward + backward halves,
+    crln = 1/2 (c_f + c_b) ALU^3."""
+    r = run_and_compare(eng, H.he_ar(), O.hear(), H.Sampler(H.IC_REJECTION, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 0.0),
+                        None, 256, 512, H.HALVES_BOTH, engine, H.PREC_FP64)
+    assert rel_l2(r["cf"], r["cf_ref"]) < 1e-10
+    it, ref = r["items"], r["ref"]
+    assert np.array_equal(it["len"][0::2], ref["len_f"]) and np.array_equal(it["len"][1::2], ref["len_b"])
+
+
+@pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
+def test_co2ar_fp64_matches_oracle(eng, engine):
+    r = run_and_compare(eng, H.co2_ar(), O.co2ar(),
+                        H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0), None, 1024, 1024,
+                        H.HALVES_FORWARD, engine, H.PREC_FP64)
+    assert rel_l2(r["cf"], r["cf_ref"]) < 1e-6
+    assert np.quantile(r["per"], 0.99) < 1e-6
+    assert np.mean(r["items"]["len"] == r["ref"]["len_f"]) > 0.99
+    # same work as the oracle's integrator, to within the chaotic few
+    assert abs(r["stats"].nrhs / r["ref"]["nrhs"].sum() - 1.0) < 0.02
+
+
+@pytest.mark.parametrize("rtol", [1e-10])
+def test_co2ar_mixed_within_1e4_of_fp64_oracle(eng, rtol):
+    r = run_and_compare(eng, H.co2_ar(), O.co2ar(),
+                        H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0), None, 1024, 1024,
+                        H.HALVES_FORWARD, H.ENGINE_BATCHED, H.PREC_MIXED, rtol=rtol)
+    assert rel_l2(r["cf"], r["cf_ref"]) < 1e-4
+    assert r["stats"].n_failed == 0
+
+
+def test_job_result_identical_alone_and_in_a_mix(eng):
+    """Each job's CF depends only on its own items and a fixed reduction order: running it
+    alone or streamed with other jobs through one persistent launch gives identical bits."""
+    smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
+    ics = [eng.to_device_soa(H.sample_ics(H.co2_ar(), smp, s, 300)[0]) for s in (1, 2, 3)]
+    p = H.cf_params(512, engine=H.ENGINE_BATCHED, precision=H.PREC_MIXED, rtol=1e-10)
+    alone, _ = eng.cf_run(H.co2_ar(), p, [ics[1]])
+    mixed, _ = eng.cf_run(H.co2_ar(), p, ics)
+    assert np.array_equal(alone[0].cpu().numpy(), mixed[1].cpu().numpy())
+
+
+def test_failed_trajectory_counts_as_zero(eng):
+    """A NaN dipole ends the item with a failure and the trajectory contributes zero
+    (src/hawaii.c:2183-2237, 3001-3009) while still counting in ntraj."""
+    smp = H.Sampler(H.IC_R0_INJECT, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 40.0)
+    ic, _, _ = H.sample_ics(H.he_ar(), smp, 5, 64)
+    good = ic.copy()
+    ic[3, 4] = float("nan")
+    p = H.cf_params(128, engine=H.ENGINE_BATCHED)
+    out, st = eng.cf_run(H.he_ar(), p, [eng.to_device_soa(ic)])
+    it = eng.items(0, 64, 128)
+    assert it["status"][3] != 0 and st.n_failed >= 1
+    ref = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 128, 40.0, 1, 50000), good, 8)
+    crln = ref["crln"].copy()
+    crln[3] = 0.0
+    assert rel_l2(out[0].cpu().numpy(), crln.sum(0) / 64) < 1e-10
+
+
+def test_cf_to_sf_reference_kat(eng, golden):
+    g = golden("cf_sf_hear_1000.npz")
+    cf = eng.torch.from_numpy(g["cf"].copy()).cuda()
+    nu, sf, _ = eng.cf_to_sf(cf, 200.0)
+    nu, sf = nu.cpu().numpy(), sf.cpu().numpy()
+    peak = np.abs(g["sf"]).max()
+    assert np.max(np.abs(nu - g["sf_nu"])) < 1e-6
+    assert np.max(np.abs(sf - g["sf"])) < 1e-9 * peak
+    big = np.abs(g["sf"]) > 1e-8 * peak
+    assert np.max(np.abs(sf[big] / g["sf"][big] - 1.0)) < 1e-4
+
+
+@pytest.mark.parametrize("n", [2048, 4096])
+def test_cf_to_sf_matches_oracle(eng, n):
+    rng = np.random.default_rng(n)
+    cf = np.exp(-np.arange(n) / 300.0) * (1 + 0.1 * rng.standard_normal(n))
+    t = 200.0 * np.arange(n)
+    onu, osf = O.cf_to_sf(t, cf)
+    d = eng.torch.from_numpy(cf.copy()).cuda()
+    nu, sf, _ = eng.cf_to_sf(d, 200.0)
+    assert np.max(np.abs(sf.cpu().numpy() - osf)) < 1e-12 * np.abs(osf).max()
+    assert np.max(np.abs(nu.cpu().numpy() - onu)) < 1e-9
+
+
+def test_m0_matches_oracle_streams(eng):
+    """M0 over 64 independent streams seeded seed+s equals the oracle's calculate_M0 per
+    stream combined as mpi_calculate_M0 combines ranks."""
+    smp = H.Sampler(H.IC_REJECTION, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 0.0)
+    m, q, _ = eng.m0(H.he_ar(), smp, 7, 64 * 200, 64)
+    mo = [O.m0(O.hear(), O.Sampler(0, 1, 300.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0), 7 + s, 200)[0] for s in range(64)]
+    assert abs(m - np.mean(mo)) < 1e-12 * abs(np.mean(mo))
+    assert q > 0
+
+
+def test_m0_rejects_unreachable_sampler(eng):
+    bad = H.Sampler(H.IC_REJECTION, H.PAIR_BOUND, 300.0, 39.0, 40.0, H.PESMIN_HEAR, 0.0)
+    with pytest.raises(H.HHError):
+        eng.m0(H.he_ar(), bad, 1, 128, 64)

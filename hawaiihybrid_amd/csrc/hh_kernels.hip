@@ -30,7 +30,7 @@
 namespace hh {
 
 constexpr double kEPS = 2.220446049250313e-16;
-constexpr int kMaxTotalSteps = 400000;   // per item; the longest CO2-Ar items take ~5e3 steps
+constexpr int kMaxTotalSteps = 400000;   // attempts per item; the longest CO2-Ar items take ~6e3
 constexpr int kSlots = 23;               // K0..K15, F0..F6
 constexpr int kSlotF = 16;
 #ifndef HH_MIN_WAVES
@@ -77,8 +77,8 @@ struct Lane {
     long long local;   // item index inside its job
     int job;
     int k;             // next sample index
-    int steps;         // steps since the last recorded sample
-    int total_steps;
+    int steps;         // accepted steps since the last recorded sample
+    int total_steps;   // step attempts of the item
     int nrhs;
     int reject;
     int active;
@@ -231,7 +231,10 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
 {
     const JobDesc& jb = jobs[ln.job];
     const double tend = (jb.L - 1) * jb.dt;
-    if (++ln.steps > jb.max_steps || ++ln.total_steps > kMaxTotalSteps) {
+    // CVode's mxstep test (src/trajectory.c:15, 62): max_steps accepted steps without reaching
+    // the next sample time; kMaxTotalSteps bounds the step attempts of the item's whole life,
+    // so every lane's loop ends.
+    if (ln.steps >= jb.max_steps || ++ln.total_steps > kMaxTotalSteps) {
         finish_item<N>(jb, ln, ITEM_FAIL_STEPS, ln.k - 1, 0.0f);
         return true;
     }
@@ -279,6 +282,7 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     }
 
     // accepted step [t, t + h]: record every sample t_k in (t, t + h]
+    ++ln.steps;
     const double tnew = ln.t + h;
     bool ended = false;
     int status = ITEM_OK;
@@ -454,6 +458,15 @@ __device__ __forceinline__ bool dipole_failure(int st)
     return st == ITEM_FAIL_DIPOLE_NAN || st == ITEM_FAIL_CF_HUGE || st == ITEM_FAIL_CF_JUMP;
 }
 
+// A trajectory contributes zero when either half failed a dipole check (crln stays zeroed,
+// src/hawaii.c:2183-2237) or when its last half ended on the step limit: correlation_eval
+// then returns CVode's CV_TOO_MUCH_WORK (-1) and the caller skips it (src/hawaii.c:3001).
+// It still counts in ntraj.
+__device__ __forceinline__ bool dropped(int first, int last)
+{
+    return dipole_failure(first) || dipole_failure(last) || last == ITEM_FAIL_STEPS;
+}
+
 // partial[c][k] = sum over trajectories of chunk c of ppf * crln[k]; column-parallel, rows
 // read coalesced along k; entries past an item's length are zero and never read.
 __global__ void __launch_bounds__(256) cf_reduce_partial_kernel(JobDesc jb, long long chunk, double ppf,
@@ -471,12 +484,12 @@ __global__ void __launch_bounds__(256) cf_reduce_partial_kernel(JobDesc jb, long
         double v;
         if (jb.halves == 2) {
             const long long a = 2 * t, b = 2 * t + 1;
-            if (dipole_failure(jb.status[a]) || dipole_failure(jb.status[b])) continue;
+            if (dropped(jb.status[a], jb.status[b])) continue;
             const double cf = k < jb.len[a] ? jb.rows[a * jb.L + k] : 0.0;
             const double cb = k < jb.len[b] ? jb.rows[b * jb.L + k] : 0.0;
             v = 0.5 * (cf + cb) * kALU * kALU * kALU;
         } else {
-            if (dipole_failure(jb.status[t])) continue;
+            if (dropped(jb.status[t], jb.status[t])) continue;
             const double cf = k < jb.len[t] ? jb.rows[t * jb.L + k] : 0.0;
             v = cf * kALU * kALU * kALU;
         }

@@ -117,7 +117,9 @@ typedef struct {
     long long nrhs;         /* right-hand-side evaluations */
     long long nsamples;     /* recorded dipole samples */
     long long n_failed;     /* items that ended on a dipole check (their trajectory counts as zero) */
-    long long n_stepfail;   /* items that ended on a step-count / step-size limit (partial CF kept) */
+    long long n_stepfail;   /* items that ended on a step-count / step-size limit: the partial CF is
+                               kept, except that a trajectory whose last half hit the step-count
+                               limit (CVode's CV_TOO_MUCH_WORK) counts as zero (src/hawaii.c:3001) */
     double max_e_drift;     /* max |H(end) - H(0)|, Hartree */
     int block, grid;        /* launch shape used */
 } hh_cf_stats;

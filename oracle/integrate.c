@@ -101,7 +101,9 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
 
     for (;;) {
         double err = 0.0, err2 = 0.0, fac11, fac, hnew;
-        if (steps_since_sample++ > p->max_steps) { status = ORC_FAIL_STEPS; break; }
+        /* CVode's mxstep test: max_steps accepted steps without reaching the next output
+         * time ends the call with CV_TOO_MUCH_WORK (src/trajectory.c:15, 62) */
+        if (steps_since_sample >= p->max_steps) { status = ORC_FAIL_STEPS; break; }
         if (0.1 * fabs(h) <= fabs(t) * 2.220446049250313e-16) { status = ORC_FAIL_STEPSIZE; break; }
         if ((t + 1.01 * h - tend) > 0.0) h = tend - t;
 
@@ -151,6 +153,7 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
 
         /* accepted step [t, t+h] */
         st->naccept++;
+        steps_since_sample++;
         {
             double tnew = t + h;
             int have_dense = 0;
@@ -244,6 +247,12 @@ int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
         for (i = 0; i < n; ++i) yb[i] = (i % 2 == 1) ? -y0[i] : y0[i];
         status = orc_half_cf(s, p, yb, cb, len_b, st);
         if (status == ORC_FAIL_DIPOLE_NAN || status == ORC_FAIL_CF_HUGE || status == ORC_FAIL_CF_JUMP) goto out;
+    }
+    /* The caller drops a trajectory whose last integrator call ended on the step limit
+     * (CVode's CV_TOO_MUCH_WORK = -1 returned by correlation_eval, `if (status == -1)
+     * continue;` src/hawaii.c:3001); it still counts in ntraj (src/hawaii.c:3040). */
+    if (status == ORC_FAIL_STEPS) goto out;
+    if (p->halves == ORC_HALVES_BOTH) {
         for (i = 0; i < p->L; ++i) crln[i] = 0.5 * (cf[i] + cb[i]) * ALU * ALU * ALU;
     } else {
         for (i = 0; i < p->L; ++i) crln[i] = cf[i] * ALU * ALU * ALU;

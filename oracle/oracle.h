@@ -116,7 +116,8 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
                 double *c, int *len, orc_traj_stats *st);
 
 /* One trajectory as correlation_eval: crln[k] = 1/2 (c_f + c_b) ALU^3 (both halves) or
- * c_f ALU^3 (forward only).  A dipole-check failure zeroes crln (src/hawaii.c:2183-2237). */
+ * c_f ALU^3 (forward only).  A dipole-check failure zeroes crln (src/hawaii.c:2183-2237), and
+ * so does a last half that ended on the step limit (CV_TOO_MUCH_WORK, src/hawaii.c:3001). */
 int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
                 double *crln, int *len_f, int *len_b, orc_traj_stats *st);
 

@@ -104,21 +104,24 @@ def test_job_result_identical_alone_and_in_a_mix(eng):
     assert np.array_equal(alone[0].cpu().numpy(), mixed[1].cpu().numpy())
 
 
-def test_failed_trajectory_counts_as_zero(eng):
-    """A NaN dipole ends the item with a failure and the trajectory contributes zero
-    (src/hawaii.c:2183-2237, 3001-3009) while still counting in ntraj."""
-    smp = H.Sampler(H.IC_R0_INJECT, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 40.0)
-    ic, _, _ = H.sample_ics(H.he_ar(), smp, 5, 64)
-    good = ic.copy()
-    ic[3, 4] = float("nan")
-    p = H.cf_params(128, engine=H.ENGINE_BATCHED)
+@pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
+def test_step_limit_drops_trajectory_like_cvode(eng, engine):
+    """max_steps accepted steps without reaching the next sample ends a half with the step-limit
+    status (CVode's CV_TOO_MUCH_WORK, src/trajectory.c:15, 62).  When that half is the
+    trajectory's last, correlation_eval returns -1 and the caller drops the trajectory
+    (src/hawaii.c:3001), which still counts in ntraj.  At 1000 K with max_steps = 6, one of
+    these 256 He-Ar trajectories ends that way in the oracle."""
+    smp = H.Sampler(H.IC_REJECTION, 1, 1000.0, 4.0, 40.0, H.PESMIN_HEAR, 0.0)
+    ic, _, _ = H.sample_ics(H.he_ar(), smp, 5, 256)
+    ref = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 6), ic, 8)
+    assert np.sum(ref["status"] == 4) == 1
+    p = H.cf_params(256, halves=H.HALVES_BOTH, engine=engine, max_steps=6)
     out, st = eng.cf_run(H.he_ar(), p, [eng.to_device_soa(ic)])
-    it = eng.items(0, 64, 128)
-    assert it["status"][3] != 0 and st.n_failed >= 1
-    ref = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 128, 40.0, 1, 50000), good, 8)
-    crln = ref["crln"].copy()
-    crln[3] = 0.0
-    assert rel_l2(out[0].cpu().numpy(), crln.sum(0) / 64) < 1e-10
+    it = eng.items(0, 512, 256)
+    assert np.array_equal(it["status"][1::2], ref["status"])
+    assert st.n_stepfail >= 1
+    cf_ref = O.ensemble_from_rows(ref["crln"])
+    assert rel_l2(out[0].cpu().numpy(), cf_ref) < 1e-10
 
 
 def test_cf_to_sf_reference_kat(eng, golden):

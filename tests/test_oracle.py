@@ -296,6 +296,31 @@ def test_failure_quirk_zero_contribution():
     assert status != 0 and np.all(crln == 0.0)
 
 
+def test_step_limit_drops_only_the_limited_trajectory():
+    """max_steps accepted steps without reaching the next sample is CVode's CV_TOO_MUCH_WORK
+    (src/trajectory.c:15, 62); a trajectory whose last half ends that way is dropped
+    (src/hawaii.c:3001).  The limit changes nothing else: every other row equals the row of
+    the unlimited run bit for bit, the dropped row is zero, and no completed half needed
+    more than max_steps accepted steps between two samples."""
+    ic, _, _ = O.sample_ics(O.hear(), O.Sampler(O.IC_REJECTION, 1, 1000.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0), 5, 256)
+    lim = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 6), ic, 8)
+    free = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 50000), ic, 8)
+    dropped = lim["status"] == 4
+    assert dropped.sum() >= 1 and np.all(free["status"] == 0)
+    assert np.all(lim["crln"][dropped] == 0.0)
+    whole = ~dropped & (lim["len_f"] == free["len_f"])
+    assert np.array_equal(lim["crln"][whole], free["crln"][whole])
+    # a forward half cut by the limit, followed by a complete backward half, keeps its partial
+    # samples (make_step's error only breaks the forward loop, src/hawaii.c:2172-2176)
+    for t in np.where(~dropped & ~whole)[0]:
+        n = lim["len_f"][t]
+        assert n < free["len_f"][t]
+        assert np.array_equal(lim["crln"][t, :n], free["crln"][t, :n])
+    # the same trajectories with a limit of one step fewer than needed: all dropped at t = 0
+    lim1 = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 1), ic, 8)
+    assert np.all(lim1["status"] == 4) and np.all(lim1["crln"] == 0.0)
+
+
 def test_both_halves_definition():
     """crln = 1/2 (c_f + c_b) ALU^3 with c_b from inverted momenta (src/hawaii.c:2334-2337)."""
     s = O.hear()

@@ -51,6 +51,20 @@ class CFStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SessionOpts(C.Structure):
+    _fields_ = [("slice_ms", C.c_double), ("max_device_gib", C.c_double), ("block", C.c_int)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("kind", C.c_int), ("state", C.c_int), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double), ("latency_ms", C.c_double), ("trajectories", C.c_longlong),
+                ("cf", CFStats), ("m0", C.c_double), ("m0_err", C.c_double), ("error", C.c_char * 160)]
+
+
+REQ_WAITING, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+REQ_CF, REQ_M0 = 0, 1
+
+
 def he_ar():
     return System(SYS_HEAR, MU_HEAR, 0.0)
 
@@ -90,6 +104,11 @@ def lib():
             "hh_cf_items": (i, [v, i, v, v, v, v, v]),
             "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
             "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
+            "hh_session_create": (i, [v, P(SessionOpts), P(v)]),
+            "hh_submit_cf": (i, [v, P(System), P(CFParams), v, C.c_long, v, P(C.c_uint64)]),
+            "hh_submit_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(C.c_uint64)]),
+            "hh_poll": (i, [v, C.c_uint64, d, P(RequestInfo)]),
+            "hh_session_destroy": (i, [v]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -200,3 +219,56 @@ class Engine:
         _check(lib().hh_m0(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf, C.byref(m),
                            C.byref(q), C.byref(ms)))
         return m.value, q.value, ms.value
+
+
+class Session:
+    """Serving session on an Engine: continuous batching of CF and M0 requests (see
+    hh_session_create).  Device buffers passed to submit_cf must stay alive until the request
+    has finished; this wrapper keeps references to them until then."""
+
+    def __init__(self, engine, slice_ms=10.0, max_device_gib=0.0, block=0):
+        self.engine = engine
+        opts = SessionOpts(slice_ms, max_device_gib, block)
+        h = C.c_void_p()
+        _check(lib().hh_session_create(engine.h, C.byref(opts), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit_cf(self, system, params, ic_dev, out=None):
+        """ic_dev: device tensor SoA [QP][ntraj]; returns (request id, CF output tensor)."""
+        torch = self.engine.torch
+        if out is None:
+            out = torch.zeros(params.L, dtype=torch.float64, device=ic_dev.device)
+        rid = C.c_uint64()
+        torch.cuda.synchronize(ic_dev.device)
+        _check(lib().hh_submit_cf(self.h, C.byref(system), C.byref(params), C.c_void_p(ic_dev.data_ptr()),
+                                  ic_dev.shape[1], C.c_void_p(out.data_ptr()), C.byref(rid)))
+        self._keep[rid.value] = (ic_dev, out)
+        return rid.value, out
+
+    def submit_m0(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
+        rid = C.c_uint64()
+        _check(lib().hh_submit_m0(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf,
+                                  C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid, timeout_ms=-1.0):
+        """None while the request runs; its RequestInfo once finished."""
+        info = RequestInfo()
+        r = _check(lib().hh_poll(self.h, rid, timeout_ms, C.byref(info)))
+        if r == 0:
+            return None
+        self._keep.pop(rid, None)
+        return info
+
+    def close(self):
+        if self.h:
+            _check(lib().hh_session_destroy(self.h))
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

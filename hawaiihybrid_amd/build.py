@@ -25,6 +25,8 @@ SOURCES = [
     ("hh_kernels.hip", DEVICE),
     ("hh_aux.hip", DEVICE),
     ("hh_host.cpp", ["-x", "hip"] + DEVICE),
+    ("hh_pool.cpp", ["-x", "hip"] + DEVICE),
+    ("hh_session.cpp", ["-x", "hip"] + DEVICE),
     # bit-exact initial conditions: no floating-point contraction in the sampler
     ("hh_sampler.cpp", ["-ffp-contract=off"]),
 ]

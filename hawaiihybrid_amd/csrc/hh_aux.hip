@@ -13,6 +13,7 @@
 // filter, dipole, Welford running mean / sum of squared deviations) over its quota.
 #include <hip/hip_runtime.h>
 
+#include "hh_engine.h"
 #include "hh_systems.h"
 
 namespace hh {
@@ -148,12 +149,6 @@ hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2*
 }
 
 // ------------------------------------------------------------------ M0
-struct M0Params {
-    double T, Rmin, Rmax, pesmin;
-    int pair_state;
-    int pad;
-};
-
 constexpr int kMTN = 624, kMTM = 397;
 constexpr double kHkT = 4.3597447222071e-18 / 1.380649e-23;
 
@@ -194,30 +189,41 @@ __device__ double m0_normal(LaneMt& mt)
     }
 }
 
+// One launch draws candidates for every stream until it has its quota, hits the candidate
+// limit, or (ticks > 0) the launch has run `ticks` wall-clock ticks; `init` seeds the streams.
 template <int SYS>
 __global__ void __launch_bounds__(256) m0_kernel(SysConst sc, M0Params p, uint32_t seed, long nstreams, long npoints,
-                                                 uint32_t* __restrict__ state, double* __restrict__ out_m,
-                                                 double* __restrict__ out_q, long* __restrict__ out_n)
+                                                 uint32_t* __restrict__ state, M0Acc acc, int init, long long ticks)
 {
     const long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nstreams) return;
+    const long long t_end = wall_clock64() + ticks;
     LaneMt mt{state, nstreams, s, kMTN};
-    uint32_t v = seed + (uint32_t)s;
-    if (v == 0) v = 4357u;
-    mt.w(0) = v;
-    for (int i = 1; i < kMTN; ++i) mt.w(i) = 69069u * mt.w(i - 1);
-    mt.twist();
     const long quota = npoints / nstreams + (s < npoints % nstreams ? 1 : 0);
     const long max_candidates = quota * 100000 + 100000;   // bounded even if nothing is ever accepted
-    long candidates = 0;
+    long candidates = 0, n = 0;
     double m = 0.0, q = 0.0;
-    long n = 0;
+    if (init) {
+        uint32_t v = seed + (uint32_t)s;
+        if (v == 0) v = 4357u;
+        mt.w(0) = v;
+        for (int i = 1; i < kMTN; ++i) mt.w(i) = 69069u * mt.w(i - 1);
+        mt.twist();
+    } else {
+        mt.pos = acc.pos[s];
+        m = acc.m[s];
+        q = acc.q[s];
+        n = acc.n[s];
+        candidates = acc.cand[s];
+    }
+    const bool was_done = !init && (n >= quota || candidates >= max_candidates);
     double y[10];
     const double RMIN = p.Rmin, RMAX = p.Rmax, T = p.T;
     const double sqrt_MUKT = sqrt(sc.mu / kHkT * T);
     const double sqrt_IIKT = sqrt(sc.I / kHkT * T);
     const double M = 1.5 * exp(-p.pesmin * kHkT / T);
     while (n < quota && candidates < max_candidates) {
+        if (ticks > 0 && wall_clock64() >= t_end) break;
         ++candidates;
         const double R3 = mt.drand() * (RMAX * RMAX * RMAX - RMIN * RMIN * RMIN) + RMIN * RMIN * RMIN;
         y[4] = pow(R3, 1.0 / 3.0);
@@ -256,21 +262,24 @@ __global__ void __launch_bounds__(256) m0_kernel(SysConst sc, M0Params p, uint32
         q += diff * diff * (n / (n + 1.0));
         ++n;
     }
-    out_m[s] = m;
-    out_q[s] = q;
-    out_n[s] = n;
+    acc.m[s] = m;
+    acc.q[s] = q;
+    acc.n[s] = n;
+    acc.cand[s] = candidates;
+    acc.pos[s] = mt.pos;
+    if (!was_done && (n >= quota || candidates >= max_candidates)) atomicAdd(acc.finished, 1);
 }
 
 hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
-                  uint32_t* state, double* m, double* q, long* n, hipStream_t st)
+                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st)
 {
     const unsigned grid = (unsigned)((nstreams + 255) / 256);
     if (sys == SYS_HEAR)
-        hipLaunchKernelGGL(m0_kernel<SYS_HEAR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state, m,
-                           q, n);
+        hipLaunchKernelGGL(m0_kernel<SYS_HEAR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state, acc,
+                           init, ticks);
     else
-        hipLaunchKernelGGL(m0_kernel<SYS_CO2AR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state, m,
-                           q, n);
+        hipLaunchKernelGGL(m0_kernel<SYS_CO2AR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state,
+                           acc, init, ticks);
     return hipGetLastError();
 }
 

@@ -26,9 +26,10 @@ struct JobDesc {
     long long ntraj;
     long long item_begin;    // first global item index of this job
     long long n_items;       // ntraj * halves
+    int* done;               // finished-item counter of the job (nullptr: not counted)
     int halves;              // 1: forward only, 2: forward + momentum-inverted backward
     int L;                   // MaxTrajectoryLength
-    int max_steps;           // max steps between two samples
+    int max_steps;           // max accepted steps between two samples
     int pad;
     double dt, Rcut, rtol, atol;
 };
@@ -36,6 +37,50 @@ struct JobDesc {
 struct QueueState {
     unsigned long long next;     // next global item to hand out
     unsigned long long done;     // items finished
+};
+
+// Resident lane pool of the batched engine: the integrator state of every lane slot,
+// structure-of-arrays with stride S so a wavefront's save / restore is coalesced.  A slot
+// whose job field is negative holds no item.
+//   d: y[N], k0[N] (= f(y), first-same-as-last), t, h, E0, mu0[3], prev, curr -> (2N + 8) x S
+//   i: job (ring index), k, steps, total_steps, nrhs, reject                  -> 6 x S
+//   local: item index inside its job                                           -> S
+struct LaneSlots {
+    double* d;
+    int* i;
+    long long* local;
+    long long S;
+};
+
+// One slice of the pool kernel: items [0, total) of the class's queue have been admitted;
+// jobs with sequence numbers [first_seq, last_seq] live in ring slots seq % ring.  Lanes stop
+// at the first step boundary after `ticks` wall-clock ticks (s_memrealtime) of the slice.
+struct PoolCtl {
+    unsigned long long* next;
+    unsigned long long total;
+    long long first_seq, last_seq;
+    long long ticks;
+    int ring;
+    int pad;
+};
+
+// Rejection-sampler M0 (calculate_M0, src/hawaii.c:1423-1491)
+struct M0Params {
+    double T, Rmin, Rmax, pesmin;
+    int pair_state;
+    int pad;
+};
+
+// Per-stream accumulators of a (possibly sliced) M0 run: running mean m and sum of squared
+// deviations q (Welford, src/hawaii.c:1470-1480), accepted points n, candidates drawn, MT
+// read position; `finished` counts the streams that reached their quota or candidate limit.
+struct M0Acc {
+    double* m;
+    double* q;
+    long* n;
+    long* cand;
+    int* pos;
+    int* finished;
 };
 
 }  // namespace hh

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -10,36 +11,17 @@
 #include <string>
 #include <vector>
 
-#include "../../include/hawaii_gpu.h"
 #include "co2ar_tables.h"
-#include "hh_engine.h"
-#include "hh_systems.h"
+#include "hh_internal.h"
+
+extern "C" __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp,
+                                                                        uint32_t seed, long ntraj, hh_pes_fn pes,
+                                                                        double* ic, long* n_draws);
 
 namespace hh {
-int arena_doubles_per_lane(int sys);
-hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
-                     hipStream_t st);
-hipError_t run_persistent(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* jobs_dev, int njobs,
-                          long long total, QueueState* q, int grid, double* ws, hipStream_t st);
-hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
-                      double* cf_out, hipStream_t st);
-hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2* work_b, double* nu, double* sf,
-                        hipStream_t st, bool lds_ok);
-struct M0Params {
-    double T, Rmin, Rmax, pesmin;
-    int pair_state;
-    int pad;
-};
-hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
-                  uint32_t* state, double* m, double* q, long* n, hipStream_t st);
-}  // namespace hh
-
-extern "C" __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj,
-                                   hh_pes_fn pes, double* ic, long* n_draws);
-
 namespace {
-
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...)
 {
@@ -52,39 +34,13 @@ int fail(int code, const char* fmt, ...)
     return code;
 }
 
-#define HH_CHECK(expr)                                                                                 \
-    do {                                                                                               \
-        hipError_t e_ = (expr);                                                                        \
-        if (e_ != hipSuccess) return fail(HH_E_DEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                                          __FILE__, __LINE__);                                         \
-    } while (0)
+const char* last_error() { return g_err.c_str(); }
+}  // namespace hh
 
-// grow-only device buffer
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t n)
-    {
-        if (n <= bytes) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            if (e != hipSuccess) return e;
-            p = nullptr;
-            bytes = 0;
-        }
-        hipError_t e = hipMalloc(&p, n);
-        if (e == hipSuccess) bytes = n;
-        return e;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <typename T>
-    T* as() const { return static_cast<T*>(p); }
-};
+using hh::DevBuf;
+using hh::fail;
+
+namespace {
 
 template <typename P>
 void fill_tables(hh::Co2ArTables<P>& t)
@@ -106,20 +62,123 @@ void fill_tables(hh::Co2ArTables<P>& t)
 
 }  // namespace
 
-struct hh_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    hipDeviceProp_t prop{};
-    DevBuf tab64, tab32;          // Co2ArTables<double>, Co2ArTables<float>
-    DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, m0_out, arena;
-    std::vector<hh::JobDesc> last_jobs;
-    std::mutex mu;
-};
+namespace hh {
+
+// per-item summary of jobs into stats
+int collect_stats(const std::vector<hh::JobDesc>& jobs, hh_cf_stats* stats)
+{
+    for (const hh::JobDesc& jb : jobs) {
+        const size_t n = (size_t)jb.n_items;
+        std::vector<int> ln(n), st(n), nr(n);
+        std::vector<float> dr(n);
+        HH_CHECK(hipMemcpy(ln.data(), jb.len, n * sizeof(int), hipMemcpyDeviceToHost));
+        HH_CHECK(hipMemcpy(st.data(), jb.status, n * sizeof(int), hipMemcpyDeviceToHost));
+        HH_CHECK(hipMemcpy(nr.data(), jb.nrhs, n * sizeof(int), hipMemcpyDeviceToHost));
+        HH_CHECK(hipMemcpy(dr.data(), jb.e_drift, n * sizeof(float), hipMemcpyDeviceToHost));
+        stats->n_items += (long long)n;
+        for (size_t i = 0; i < n; ++i) {
+            stats->nrhs += nr[i];
+            stats->nsamples += ln[i];
+            if (st[i] >= 1 && st[i] <= 3) ++stats->n_failed;
+            if (st[i] >= 4) ++stats->n_stepfail;
+            stats->max_e_drift = std::max(stats->max_e_drift, (double)dr[i]);
+        }
+    }
+    return HH_OK;
+}
+
+int check_cf_args(const hh_system* sys, const hh_cf_params* p)
+{
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
+        return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
+    if (sys->mu <= 0) return fail(HH_E_INVALID, "invalid reduced mass (%.5e) -- must be positive", sys->mu);
+    if (p->L <= 0 || p->dt <= 0 || p->Rcut <= 0 || p->rtol <= 0 || p->atol <= 0)
+        return fail(HH_E_INVALID, "MaxTrajectoryLength, sampling_time, Rcut and tolerances must be positive");
+    if (p->engine != HH_ENGINE_PLAIN && p->engine != HH_ENGINE_BATCHED)
+        return fail(HH_E_INVALID, "unknown engine %d", p->engine);
+    if (p->precision != HH_PREC_FP64 && p->precision != HH_PREC_MIXED)
+        return fail(HH_E_INVALID, "unknown precision %d", p->precision);
+    if (p->engine == HH_ENGINE_PLAIN && p->precision != HH_PREC_FP64)
+        return fail(HH_E_INVALID, "the plain engine is FP64 only");
+    if (p->halves != HH_HALVES_BOTH && p->halves != HH_HALVES_FORWARD)
+        return fail(HH_E_INVALID, "unknown halves %d", p->halves);
+    return HH_OK;
+}
+
+}  // namespace hh
+
+namespace hh {
+
+const int kZeroIntStorage = 0;
+
+int m0_check(const hh_system* sys, const hh_sampler* smp, long npoints, int nstreams)
+{
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
+        return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
+    if (npoints < 2 || nstreams <= 0 || npoints < nstreams * 2L)
+        return fail(HH_E_INVALID, "M0: need npoints >= 2 * nstreams > 0");
+    if (smp->mode != HH_IC_REJECTION) return fail(HH_E_INVALID, "M0 uses the rejection sampler");
+    if (!(smp->pesmin < 0 || smp->pesmin > 0)) return fail(HH_E_INVALID, "M0: pesmin must be non-zero");
+    if (!(smp->Rmin > 0 && smp->Rmax > smp->Rmin)) return fail(HH_E_INVALID, "M0: need 0 < Rmin < Rmax");
+    if (smp->T <= 0) return fail(HH_E_INVALID, "M0: temperature must be positive");
+    return HH_OK;
+}
+
+// [624 x ns] u32 MT states, then m, q (double), n, cand (long), pos (int) per stream, finished
+size_t m0_bytes(long ns)
+{
+    return (size_t)ns * (624 * sizeof(uint32_t) + 2 * sizeof(double) + 2 * sizeof(long) + sizeof(int)) + sizeof(int);
+}
+
+M0Acc m0_view(void* base, long ns)
+{
+    M0Acc a;
+    char* p = static_cast<char*>(base) + (size_t)ns * 624 * sizeof(uint32_t);
+    a.m = reinterpret_cast<double*>(p);
+    a.q = a.m + ns;
+    a.n = reinterpret_cast<long*>(a.q + ns);
+    a.cand = a.n + ns;
+    a.pos = reinterpret_cast<int*>(a.cand + ns);
+    a.finished = a.pos + ns;
+    return a;
+}
+
+int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q)
+{
+    std::vector<double> hm(nstreams), hq(nstreams);
+    std::vector<long> hn(nstreams);
+    HH_CHECK(hipMemcpy(hm.data(), acc.m, nstreams * sizeof(double), hipMemcpyDeviceToHost));
+    HH_CHECK(hipMemcpy(hq.data(), acc.q, nstreams * sizeof(double), hipMemcpyDeviceToHost));
+    HH_CHECK(hipMemcpy(hn.data(), acc.n, nstreams * sizeof(long), hipMemcpyDeviceToHost));
+    for (long s = 0; s < nstreams; ++s) {
+        const long quota = npoints / nstreams + (s < npoints % nstreams ? 1 : 0);
+        if (hn[s] != quota)
+            return fail(HH_E_INVALID, "M0: stream %ld accepted %ld of %ld points within the candidate limit "
+                                      "(check pesmin, T and the pair state)", s, hn[s], quota);
+    }
+    // combine as mpi_calculate_M0 does across ranks (src/hawaii.c:1722-1733): mean of the
+    // per-stream means and of the per-stream q, q normalised by the per-stream count
+    double sm = 0.0, sq = 0.0;
+    for (long s = 0; s < nstreams; ++s) {
+        sm += hm[s];
+        sq += hq[s];
+    }
+    const double ZeroCoeff = 0.00361479637 / (4.0 * M_PI);   // src/constants.h:29
+    const double n0 = (double)hn[0];
+    *m = (sm / nstreams) * ZeroCoeff * ppf;
+    *q = std::sqrt((sq / nstreams) / n0 / (n0 - 1)) * ZeroCoeff * ppf;
+    return HH_OK;
+}
+
+}  // namespace hh
+
+namespace {
+const int& kZeroInt = hh::kZeroIntStorage;
+}
 
 extern "C" {
 
-const char* hh_last_error(void) { return g_err.c_str(); }
+const char* hh_last_error(void) { return hh::last_error(); }
 
 int hh_qp_size(int system) { return system == HH_SYS_CO2AR ? 10 : (system == HH_SYS_HEAR ? 6 : -1); }
 
@@ -163,6 +222,11 @@ int hh_engine_create(int device, hh_engine** out)
     }
     for (auto& ev : e->ev) (void)hipEventCreate(&ev);
     {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess && khz > 0)
+            e->wall_khz = khz;
+    }
+    {
         static hh::Co2ArTables<double> t64;
         static hh::Co2ArTables<float> t32;
         fill_tables(t64);
@@ -186,8 +250,10 @@ void hh_engine_destroy(hh_engine* e)
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    e->last_pool_jobs.clear();
+    e->pools.clear();
     for (DevBuf* b : {&e->tab64, &e->tab32, &e->rows, &e->len, &e->status, &e->nrhs, &e->drift, &e->jobs, &e->queue,
-                      &e->psum, &e->pcomp, &e->fft_a, &e->fft_b, &e->m0_state, &e->m0_out, &e->arena})
+                      &e->psum, &e->pcomp, &e->fft_a, &e->fft_b, &e->m0_state, &e->arena})
         b->release();
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -210,26 +276,75 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
 {
     if (!e || !sys || !p || njobs <= 0 || !ic_dev || !ntraj || !cf_dev)
         return fail(HH_E_INVALID, "hh_cf_run: null argument or njobs <= 0");
-    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
-        return fail(HH_E_UNSUPPORTED, "hh_cf_run: unknown system %d", sys->system);
-    if (p->L <= 0 || p->dt <= 0 || p->Rcut <= 0 || p->rtol <= 0 || p->atol <= 0)
-        return fail(HH_E_INVALID, "MaxTrajectoryLength, sampling_time, Rcut and tolerances must be positive");
-    if (p->engine == HH_ENGINE_PLAIN && p->precision != HH_PREC_FP64)
-        return fail(HH_E_INVALID, "the plain engine is FP64 only");
+    int rc = hh::check_cf_args(sys, p);
+    if (rc < 0) return rc;
+    for (int j = 0; j < njobs; ++j)
+        if (ntraj[j] <= 0 || !ic_dev[j] || !cf_dev[j]) return fail(HH_E_INVALID, "job %d: empty or null buffers", j);
     const int halves = p->halves == HH_HALVES_BOTH ? 2 : 1;
     std::lock_guard<std::mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
+    e->last_jobs.clear();
+    e->last_pool_jobs.clear();
+    if (stats) std::memset(stats, 0, sizeof *stats);
 
-    // job table and per-item buffers
+    if (p->engine == HH_ENGINE_BATCHED) {
+        // resident lane pool: all jobs are admitted at once and streamed through time slices
+        int block = p->block;
+        if (block != 64 && block != 128 && block != 256) block = 256;
+        hh::LanePool* pl = nullptr;
+        rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, block, &pl);
+        if (rc < 0) return rc;
+        if (!pl->inflight.empty()) return fail(HH_E_BUSY, "lane pool of this class is in use");
+        if (njobs > pl->ring_cap) return fail(HH_E_INVALID, "at most %d jobs per call", pl->ring_cap);
+        std::vector<hh::PoolJob*> mine;
+        for (int j = 0; j < njobs; ++j) {
+            hh::PoolJob* pj = nullptr;
+            rc = hh::pool_admit(e, *pl, *p, ic_dev[j], ntraj[j], cf_dev[j], 0, &pj);
+            if (rc < 0) {
+                while (!pl->inflight.empty()) pl->inflight.pop_back();
+                return rc;
+            }
+            mine.push_back(pj);
+        }
+        double kms = 0.0, rms = 0.0;
+        for (;;) {
+            bool all = true;
+            for (hh::PoolJob* pj : mine) all &= pj->state == 2;
+            if (all) break;
+            rc = hh::pool_step(e, e->pools, 20.0, &kms, &rms);
+            if (rc < 0) {
+                // drop the pool with its state; the next call builds a fresh one
+                (void)hipStreamSynchronize(e->stream);
+                for (auto it = e->pools.begin(); it != e->pools.end(); ++it)
+                    if (it->get() == pl) {
+                        e->pools.erase(it);
+                        break;
+                    }
+                return rc;
+            }
+        }
+        for (hh::PoolJob* pj : mine) {
+            e->last_jobs.push_back(pj->jd);
+            e->last_pool_jobs.push_back(hh::pool_take(*pl, pj));
+        }
+        if (stats) {
+            stats->integrate_ms = kms;
+            stats->reduce_ms = rms;
+            stats->block = block;
+            stats->grid = (int)(pl->S / block);
+            return hh::collect_stats(e->last_jobs, stats);
+        }
+        return HH_OK;
+    }
+
+    // plain: one thread per item, every job in one launch, each item runs to its end
     std::vector<hh::JobDesc> jobs(njobs);
-    long long total = 0, total_traj = 0;
+    long long total = 0;
     for (int j = 0; j < njobs; ++j) {
-        if (ntraj[j] <= 0 || !ic_dev[j] || !cf_dev[j]) return fail(HH_E_INVALID, "job %d: empty or null buffers", j);
         jobs[j].item_begin = total;
         jobs[j].n_items = (long long)ntraj[j] * halves;
         jobs[j].ntraj = ntraj[j];
         total += jobs[j].n_items;
-        total_traj += ntraj[j];
     }
     const size_t rows_bytes = (size_t)total * (size_t)p->L * sizeof(double);
     if (e->rows.ensure(rows_bytes) != hipSuccess)
@@ -239,7 +354,6 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
     HH_CHECK(e->nrhs.ensure(total * sizeof(int)));
     HH_CHECK(e->drift.ensure(total * sizeof(float)));
     HH_CHECK(e->jobs.ensure(njobs * sizeof(hh::JobDesc)));
-    HH_CHECK(e->queue.ensure(sizeof(hh::QueueState)));
     for (int j = 0; j < njobs; ++j) {
         hh::JobDesc& jb = jobs[j];
         const long long off = jb.item_begin;
@@ -249,6 +363,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         jb.status = e->status.as<int>() + off;
         jb.nrhs = e->nrhs.as<int>() + off;
         jb.e_drift = e->drift.as<float>() + off;
+        jb.done = nullptr;
         jb.halves = halves;
         jb.L = p->L;
         jb.max_steps = p->max_steps > 0 ? p->max_steps : 50000;
@@ -258,35 +373,15 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         jb.atol = p->atol;
     }
     HH_CHECK(hipMemcpyAsync(e->jobs.p, jobs.data(), njobs * sizeof(hh::JobDesc), hipMemcpyHostToDevice, e->stream));
-    hh::QueueState q0{0, 0};
-    HH_CHECK(hipMemcpyAsync(e->queue.p, &q0, sizeof q0, hipMemcpyHostToDevice, e->stream));
-
-    const bool mixed = p->precision == HH_PREC_MIXED;
-    hh::SysConst sc{sys->mu, sys->I, mixed ? e->tab32.p : e->tab64.p};
-    int block = p->block, grid = p->grid;
+    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    const int grid = (int)((total + 255) / 256);
     // per-wavefront stage arena (kSlots * QP * 64 doubles per wavefront)
     const size_t per_lane = (size_t)hh::arena_doubles_per_lane(sys->system) * sizeof(double);
-    if (p->engine == HH_ENGINE_PLAIN) {
-        sc.tables = e->tab64.p;
-        block = 256;
-        grid = (int)((total + 255) / 256);
-        if (e->arena.ensure(per_lane * (size_t)grid * 256) != hipSuccess)
-            return fail(HH_E_NOMEM, "cannot allocate the integrator arena");
-        HH_CHECK(hipEventRecord(e->ev[0], e->stream));
-        HH_CHECK(hh::run_plain(sys->system, sc, e->jobs.as<hh::JobDesc>(), njobs, total, e->arena.as<double>(),
-                               e->stream));
-    } else {
-        if (block != 64 && block != 128 && block != 256) block = 256;
-        if (grid <= 0) grid = e->prop.multiProcessorCount * 2 * (256 / block);   // two waves per SIMD
-        if (e->arena.ensure(per_lane * (size_t)grid * block) != hipSuccess)
-            return fail(HH_E_NOMEM, "cannot allocate the integrator arena");
-        HH_CHECK(hipEventRecord(e->ev[0], e->stream));
-        HH_CHECK(hh::run_persistent(sys->system, mixed, block, sc, e->jobs.as<hh::JobDesc>(), njobs, total,
-                                    e->queue.as<hh::QueueState>(), grid, e->arena.as<double>(), e->stream));
-    }
+    if (e->arena.ensure(per_lane * (size_t)grid * 256) != hipSuccess)
+        return fail(HH_E_NOMEM, "cannot allocate the integrator arena");
+    HH_CHECK(hipEventRecord(e->ev[0], e->stream));
+    HH_CHECK(hh::run_plain(sys->system, sc, e->jobs.as<hh::JobDesc>(), njobs, total, e->arena.as<double>(), e->stream));
     HH_CHECK(hipEventRecord(e->ev[1], e->stream));
-
-    // reductions: chunks of trajectories per column tile
     const long long chunk = 256;
     for (int j = 0; j < njobs; ++j) {
         const int nchunks = (int)((jobs[j].ntraj + chunk - 1) / chunk);
@@ -298,30 +393,15 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
     HH_CHECK(hipEventRecord(e->ev[2], e->stream));
     HH_CHECK(hipStreamSynchronize(e->stream));
     e->last_jobs = jobs;
-
     if (stats) {
         float a = 0, b = 0;
         (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
         (void)hipEventElapsedTime(&b, e->ev[1], e->ev[2]);
-        std::memset(stats, 0, sizeof *stats);
         stats->integrate_ms = a;
         stats->reduce_ms = b;
-        stats->n_items = total;
-        stats->block = block;
+        stats->block = 256;
         stats->grid = grid;
-        std::vector<int> ln(total), st(total), nr(total);
-        std::vector<float> dr(total);
-        HH_CHECK(hipMemcpy(ln.data(), e->len.p, total * sizeof(int), hipMemcpyDeviceToHost));
-        HH_CHECK(hipMemcpy(st.data(), e->status.p, total * sizeof(int), hipMemcpyDeviceToHost));
-        HH_CHECK(hipMemcpy(nr.data(), e->nrhs.p, total * sizeof(int), hipMemcpyDeviceToHost));
-        HH_CHECK(hipMemcpy(dr.data(), e->drift.p, total * sizeof(float), hipMemcpyDeviceToHost));
-        for (long long i = 0; i < total; ++i) {
-            stats->nrhs += nr[i];
-            stats->nsamples += ln[i];
-            if (st[i] >= 1 && st[i] <= 3) ++stats->n_failed;
-            if (st[i] >= 4) ++stats->n_stepfail;
-            stats->max_e_drift = std::max(stats->max_e_drift, (double)dr[i]);
-        }
+        return hh::collect_stats(jobs, stats);
     }
     return HH_OK;
 }
@@ -377,45 +457,22 @@ int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t se
           double ppf, double* m, double* q, double* ms)
 {
     if (!e || !sys || !smp || !m || !q) return fail(HH_E_INVALID, "hh_m0: null argument");
-    if (npoints < 2 || nstreams <= 0 || npoints < nstreams * 2L)
-        return fail(HH_E_INVALID, "hh_m0: need npoints >= 2 * nstreams > 0");
-    if (smp->mode != HH_IC_REJECTION) return fail(HH_E_INVALID, "hh_m0: M0 uses the rejection sampler");
-    if (!(smp->pesmin < 0 || smp->pesmin > 0)) return fail(HH_E_INVALID, "hh_m0: pesmin must be non-zero");
+    int rc = hh::m0_check(sys, smp, npoints, nstreams);
+    if (rc < 0) return rc;
     std::lock_guard<std::mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
-    HH_CHECK(e->m0_state.ensure((size_t)nstreams * 624 * sizeof(uint32_t)));
-    HH_CHECK(e->m0_out.ensure((size_t)nstreams * 3 * sizeof(double)));
-    double* om = e->m0_out.as<double>();
-    double* oq = om + nstreams;
-    long* on = reinterpret_cast<long*>(oq + nstreams);
+    HH_CHECK(e->m0_state.ensure(hh::m0_bytes(nstreams)));
+    const hh::M0Acc acc = hh::m0_view(e->m0_state.p, nstreams);
+    HH_CHECK(hipMemcpyAsync(acc.finished, &kZeroInt, sizeof(int), hipMemcpyHostToDevice, e->stream));
     hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
     hh::M0Params mp{smp->T, smp->Rmin, smp->Rmax, smp->pesmin, smp->pair_state, 0};
     HH_CHECK(hipEventRecord(e->ev[0], e->stream));
-    HH_CHECK(hh::run_m0(sys->system, sc, mp, seed, nstreams, npoints, e->m0_state.as<uint32_t>(), om, oq, on, e->stream));
+    HH_CHECK(hh::run_m0(sys->system, sc, mp, seed, nstreams, npoints, reinterpret_cast<uint32_t*>(e->m0_state.p), acc, 1,
+                        0, e->stream));
     HH_CHECK(hipEventRecord(e->ev[1], e->stream));
     HH_CHECK(hipStreamSynchronize(e->stream));
-    std::vector<double> hm(nstreams), hq(nstreams);
-    std::vector<long> hn(nstreams);
-    HH_CHECK(hipMemcpy(hm.data(), om, nstreams * sizeof(double), hipMemcpyDeviceToHost));
-    HH_CHECK(hipMemcpy(hq.data(), oq, nstreams * sizeof(double), hipMemcpyDeviceToHost));
-    HH_CHECK(hipMemcpy(hn.data(), on, nstreams * sizeof(long), hipMemcpyDeviceToHost));
-    // combine as mpi_calculate_M0 does across ranks (src/hawaii.c:1722-1733): mean of the
-    // per-stream means and of the per-stream q, q normalised by the per-stream count
-    for (int s = 0; s < nstreams; ++s) {
-        const long quota = npoints / nstreams + (s < npoints % nstreams ? 1 : 0);
-        if (hn[s] != quota)
-            return fail(HH_E_INVALID, "hh_m0: stream %d accepted %ld of %ld points within the candidate limit "
-                                      "(check pesmin, T and the pair state)", s, hn[s], quota);
-    }
-    double sm = 0.0, sq = 0.0;
-    for (int s = 0; s < nstreams; ++s) {
-        sm += hm[s];
-        sq += hq[s];
-    }
-    const double ZeroCoeff = 0.00361479637 / (4.0 * M_PI);
-    const double n0 = (double)hn[0];
-    *m = (sm / nstreams) * ZeroCoeff * ppf;
-    *q = std::sqrt((sq / nstreams) / n0 / (n0 - 1)) * ZeroCoeff * ppf;
+    rc = hh::m0_combine(acc, npoints, nstreams, ppf, m, q);
+    if (rc < 0) return rc;
     if (ms) {
         float a = 0;
         (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);

@@ -1,0 +1,153 @@
+// Host-side internals shared by hh_host.cpp (C ABI, synchronous entry points) and
+// hh_pool.cpp (resident lane pool and serving session).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/hawaii_gpu.h"
+#include "hh_engine.h"
+#include "hh_systems.h"
+
+namespace hh {
+// kernel launchers (hh_kernels.hip, hh_aux.hip)
+int arena_doubles_per_lane(int sys);
+hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
+                     hipStream_t st);
+hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl,
+                    const LaneSlots& sl, double* ws, hipStream_t st);
+hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st);
+hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st);
+size_t slots_bytes(int sys, long long S);
+LaneSlots slots_view(int sys, void* base, long long S);
+hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
+                      double* cf_out, hipStream_t st);
+hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2* work_b, double* nu, double* sf,
+                        hipStream_t st, bool lds_ok);
+hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
+                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st);
+int m0_check(const hh_system* sys, const hh_sampler* smp, long npoints, int nstreams);
+size_t m0_bytes(long nstreams);
+M0Acc m0_view(void* base, long nstreams);
+// M0 of one request: combine the per-stream results as mpi_calculate_M0 combines ranks
+int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q);
+
+// error reporting: thread-local message, negative code
+int fail(int code, const char* fmt, ...);
+
+#define HH_CHECK(expr)                                                                                     \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return ::hh::fail(HH_E_DEVICE, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// grow-only device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n);
+        if (e == hipSuccess)
+            bytes = n;
+        else
+            p = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(p); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+};
+
+// One correlation-function job inside a lane pool.
+struct PoolJob {
+    long long seq = 0;          // sequence number in its pool (ring slot seq % ring)
+    JobDesc jd{};               // device descriptor (points into `buf` and the caller's ic)
+    DevBuf buf;                 // rows [n_items][L] + len, status, nrhs, e_drift
+    double* cf_out = nullptr;   // caller's CF[L]
+    double ppf = 1.0;
+    int state = 0;              // 0 integrating, 1 reduced (launched), 2 complete
+    uint64_t request = 0;       // serving request id (0: synchronous run)
+    size_t bytes = 0;           // device bytes held by `buf`
+};
+
+// A resident lane pool for one (system, precision, block) class: S = grid * block lane slots
+// with their saved integrator state, the per-wavefront stage arena, a ring of in-flight job
+// descriptors and the class's work queue.
+struct LanePool {
+    int sys = 0;
+    hh_system system{};         // mu and I of every job in the pool
+    bool mixed = false;
+    int block = 256;
+    long long S = 0;
+    int ring_cap = 1024;
+    DevBuf slots[2];            // double-buffered for repacking
+    int cur = 0;
+    DevBuf arena, ring, ring_done, next, scratch;
+    std::deque<std::unique_ptr<PoolJob>> inflight;   // sequence order
+    long long next_seq = 0;
+    unsigned long long total = 0;   // items admitted
+    bool drained_repacked = false;
+    LaneSlots view(int which) const { return slots_view(sys, slots[which].p, S); }
+};
+
+}  // namespace hh
+
+struct hh_session;
+
+struct hh_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipDeviceProp_t prop{};
+    int wall_khz = 100000;        // s_memrealtime frequency
+    hh::DevBuf tab64, tab32;      // Co2ArTables<double>, Co2ArTables<float>
+    hh::DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, arena;
+    std::vector<hh::JobDesc> last_jobs;
+    std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
+    std::vector<std::unique_ptr<hh::LanePool>> pools;
+    std::mutex mu;
+};
+
+namespace hh {
+using PoolSet = std::vector<std::unique_ptr<LanePool>>;
+// the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out);
+// device bytes a job of this shape holds while it runs
+size_t pool_job_bytes(const hh_cf_params& p, long ntraj);
+// admit a correlation-function job: allocates its buffers, uploads its descriptor into the
+// pool's ring and appends its items to the pool's queue; *out (optional) sees the job
+int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* ic_dev, long ntraj, double* cf_dev,
+               uint64_t request, PoolJob** out);
+// true when the ring has room for one more job
+bool pool_has_room(const LanePool& pl);
+// one slice of every pool with work, then reductions of the jobs whose items are all done
+// and repacking of drained pools; synchronises the engine stream.  Jobs whose CF is final
+// have state 2 afterwards.  kernel_ms / reduce_ms accumulate device time.
+int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms);
+// argument checks shared by hh_cf_run and hh_submit_cf
+int check_cf_args(const hh_system* sys, const hh_cf_params* p);
+// per-item summaries (items, RHS, samples, failures, drift) of jobs, added into stats
+int collect_stats(const std::vector<JobDesc>& jobs, hh_cf_stats* stats);
+// remove the complete job from its pool and hand it over
+std::unique_ptr<PoolJob> pool_take(LanePool& pl, PoolJob* job);
+}  // namespace hh

@@ -132,20 +132,19 @@ __device__ __forceinline__ void finish_item(const JobDesc& jb, Lane<N>& ln, int 
     jb.status[ln.local] = status;
     jb.nrhs[ln.local] = ln.nrhs;
     jb.e_drift[ln.local] = drift;
+    if (jb.done) atomicAdd(jb.done, 1);
     ln.active = 0;
 }
 
-// Start item `g`: load the initial condition (momenta inverted for the backward half,
-// invert_momenta src/hawaii.c:1171-1187), mu(0), c[0], H(0), f(y0), first step size.
-// Returns false if the item ended during set-up.
+// Start item `local` of job `j`: load the initial condition (momenta inverted for the
+// backward half, invert_momenta src/hawaii.c:1171-1187), mu(0), c[0], H(0), f(y0), first
+// step size.  Returns false if the item ended during set-up.
 template <int SYS, typename P, int N>
-__device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc* __restrict__ jobs, int njobs, long long g,
+__device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb, int j, long long local,
                                           const Arena<N>& ar, Lane<N>& ln)
 {
-    const int j = find_job(jobs, njobs, g);
-    const JobDesc& jb = jobs[j];
     ln.job = j;
-    ln.local = g - jb.item_begin;
+    ln.local = local;
     const long long traj = ln.local / jb.halves;
     const bool back = (ln.local % jb.halves) == 1;
 #pragma unroll
@@ -410,36 +409,190 @@ __global__ void __launch_bounds__(256, HH_MIN_WAVES) plain_cf_kernel(SysConst sc
     const Arena<N> ar = wave_arena<N>(ws, (long long)blockIdx.x * 4 + (threadIdx.x >> 6));
     if (g >= total_items) return;
     Lane<N> ln;
-    if (!init_item<SYS, double, N>(scl, jobs, njobs, g, ar, ln)) return;
+    const int j = find_job(jobs, njobs, g);
+    if (!init_item<SYS, double, N>(scl, jobs[j], j, g - jobs[j].item_begin, ar, ln)) return;
     while (!dop853_step<SYS, double, N>(scl, jobs, ar, ln)) {
     }
 }
 
-// persistent: each resident wavefront owns one arena block for its whole life
+// ------------------------------------------------------------------ resident lane pool
+template <int N>
+__device__ __forceinline__ void restore_lane(const LaneSlots& sl, long long s, const Arena<N>& ar, Lane<N>& ln)
+{
+    const long long S = sl.S;
+    ln.job = sl.i[s];
+    ln.active = ln.job >= 0;
+    if (!ln.active) return;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        ln.y[i] = sl.d[i * S + s];
+        ar.st(0, i, sl.d[(N + i) * S + s]);
+    }
+    const double* d = sl.d + 2 * N * S + s;
+    ln.t = d[0];
+    ln.h = d[S];
+    ln.E0 = d[2 * S];
+    ln.mu0[0] = d[3 * S];
+    ln.mu0[1] = d[4 * S];
+    ln.mu0[2] = d[5 * S];
+    ln.prev = d[6 * S];
+    ln.curr = d[7 * S];
+    ln.k = sl.i[S + s];
+    ln.steps = sl.i[2 * S + s];
+    ln.total_steps = sl.i[3 * S + s];
+    ln.nrhs = sl.i[4 * S + s];
+    ln.reject = sl.i[5 * S + s];
+    ln.local = sl.local[s];
+}
+
+template <int N>
+__device__ __forceinline__ void save_lane(const LaneSlots& sl, long long s, const Arena<N>& ar, const Lane<N>& ln)
+{
+    const long long S = sl.S;
+    if (!ln.active) {
+        sl.i[s] = -1;
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        sl.d[i * S + s] = ln.y[i];
+        sl.d[(N + i) * S + s] = ar.ld(0, i);
+    }
+    double* d = sl.d + 2 * N * S + s;
+    d[0] = ln.t;
+    d[S] = ln.h;
+    d[2 * S] = ln.E0;
+    d[3 * S] = ln.mu0[0];
+    d[4 * S] = ln.mu0[1];
+    d[5 * S] = ln.mu0[2];
+    d[6 * S] = ln.prev;
+    d[7 * S] = ln.curr;
+    sl.i[s] = ln.job;
+    sl.i[S + s] = ln.k;
+    sl.i[2 * S + s] = ln.steps;
+    sl.i[3 * S + s] = ln.total_steps;
+    sl.i[4 * S + s] = ln.nrhs;
+    sl.i[5 * S + s] = ln.reject;
+    sl.local[s] = ln.local;
+}
+
+// ring slot of the job holding global item g: jobs [first_seq, last_seq] are in flight, their
+// item ranges are consecutive and increasing with the sequence number
+__device__ __forceinline__ int ring_job(const JobDesc* __restrict__ ring, const PoolCtl& c, unsigned long long g)
+{
+    long long lo = c.first_seq, hi = c.last_seq;
+    while (lo < hi) {
+        const long long mid = (lo + hi + 1) >> 1;
+        if ((long long)g >= ring[mid % c.ring].item_begin)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return (int)(lo % c.ring);
+}
+
+// One slice of the batched engine.  Every lane slot restores its item (if any), integrates
+// and refills from the queue until the slice's time is up or no work is left, and saves its
+// state at a step boundary, so slices can be separated by admission of new jobs, reductions
+// of finished ones and repacking of the slots.  Each lane's loop ends: the deadline and the
+// queue bound it, and every item ends within kMaxTotalSteps attempts.
 template <int SYS, typename P, int BS>
-__global__ void __launch_bounds__(BS, HH_MIN_WAVES) persistent_cf_kernel(SysConst sc, const JobDesc* __restrict__ jobs, int njobs,
-                                                           long long total_items, QueueState* __restrict__ q,
-                                                           double* __restrict__ ws)
+__global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, const JobDesc* __restrict__ ring,
+                                                                   PoolCtl ctl, LaneSlots sl, double* __restrict__ ws)
 {
     constexpr int N = SysTraits<SYS>::QP;
     __shared__ Co2ArTables<P> s_tab;
     const SysConst scl = stage_tables<SYS, P>(sc, &s_tab);
-    const Arena<N> ar = wave_arena<N>(ws, (long long)blockIdx.x * (BS / 64) + (threadIdx.x >> 6));
+    const long long wave = (long long)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
+    const Arena<N> ar = wave_arena<N>(ws, wave);
+    const long long slot = (long long)blockIdx.x * BS + threadIdx.x;
+    const long long t_end = wall_clock64() + ctl.ticks;
     Lane<N> ln;
-    ln.active = 0;
+    restore_lane<N>(sl, slot, ar, ln);
     for (;;) {
+        if (wall_clock64() >= t_end) break;
         if (!ln.active) {
-            // refill: pull items until one needs integration or the queue is empty
             bool got = false;
             while (!got) {
-                const unsigned long long g = atomicAdd(&q->next, 1ull);
-                if (g >= (unsigned long long)total_items) break;
-                got = init_item<SYS, P, N>(scl, jobs, njobs, (long long)g, ar, ln);
+                if (__hip_atomic_load(ctl.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= ctl.total) break;
+                const unsigned long long g = atomicAdd(ctl.next, 1ull);
+                if (g >= ctl.total) break;
+                const int j = ring_job(ring, ctl, g);
+                got = init_item<SYS, P, N>(scl, ring[j], j, (long long)g - ring[j].item_begin, ar, ln);
             }
             if (!got) break;
         }
-        dop853_step<SYS, P, N>(scl, jobs, ar, ln);
+        dop853_step<SYS, P, N>(scl, ring, ar, ln);
     }
+    save_lane<N>(sl, slot, ar, ln);
+}
+
+// Repack the active lane slots of `src` into `dst`: active slot number r (in slot order) goes
+// to wavefront r % W, lane r / W, so the remaining items spread over all W wavefronts of the
+// pool (one per wavefront once fewer than W are left) instead of sharing a few diverging ones.
+// Pass 1 counts active slots per 256-slot block; pass 2 scans the counts (one workgroup);
+// pass 3 moves every active slot and marks the others empty.
+__global__ void __launch_bounds__(256) repack_count_kernel(const int* __restrict__ job, long long S, int* __restrict__ counts)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int a = (s < S && job[s] >= 0) ? 1 : 0;
+    const unsigned long long m = __ballot(a);
+    __shared__ int wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ void __launch_bounds__(1024) repack_scan_kernel(int* __restrict__ counts, int nblocks, int* __restrict__ total)
+{
+    __shared__ int part[1024];
+    const int per = (nblocks + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    int acc = 0;
+    for (int b = b0; b < b0 + per && b < nblocks; ++b) acc += counts[b];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = part[threadIdx.x] - acc;   // exclusive prefix of this thread's range
+    for (int b = b0; b < b0 + per && b < nblocks; ++b) {
+        const int c = counts[b];
+        counts[b] = run;
+        run += c;
+    }
+    if (threadIdx.x == 1023) *total = part[1023];
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) repack_move_kernel(LaneSlots src, LaneSlots dst, const int* __restrict__ offsets,
+                                                          long long W)
+{
+    const long long S = src.S;
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int a = (s < S && src.i[s] >= 0) ? 1 : 0;
+    // rank inside the block: wave ballot prefix + preceding waves of the block
+    const unsigned long long m = __ballot(a);
+    const int lane = threadIdx.x & 63;
+    __shared__ int wsum[4];
+    if (lane == 0) wsum[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (!a) return;   // dst was cleared beforehand (slots_clear_kernel)
+    int rank = offsets[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) rank += wsum[w];
+    const long long d = (long long)(rank % W) * 64 + rank / W;
+    for (int f = 0; f < 2 * N + 8; ++f) dst.d[f * S + d] = src.d[f * S + s];
+    for (int f = 0; f < 6; ++f) dst.i[f * S + d] = src.i[f * S + s];
+    dst.local[d] = src.local[s];
+}
+
+__global__ void __launch_bounds__(256) slots_clear_kernel(int* __restrict__ job, long long S)
+{
+    const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (s < S) job[s] = -1;
 }
 
 // ------------------------------------------------------------------ ensemble reduction
@@ -526,11 +679,11 @@ hipError_t launch_plain(const SysConst& sc, const JobDesc* jobs_dev, int njobs, 
 }
 
 template <int SYS, typename P, int BS>
-hipError_t launch_persistent(const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, QueueState* q,
-                             int grid, double* ws, hipStream_t st)
+hipError_t launch_pool(const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl, const LaneSlots& sl, double* ws,
+                       hipStream_t st)
 {
-    hipLaunchKernelGGL((persistent_cf_kernel<SYS, P, BS>), dim3(grid), dim3(BS), 0, st, sc, jobs_dev, njobs, total,
-                       q, ws);
+    const unsigned grid = (unsigned)(sl.S / BS);
+    hipLaunchKernelGGL((pool_cf_kernel<SYS, P, BS>), dim3(grid), dim3(BS), 0, st, sc, ring, ctl, sl, ws);
     return hipGetLastError();
 }
 
@@ -547,10 +700,11 @@ hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int n
                            : launch_plain<SYS_CO2AR>(sc, jobs_dev, njobs, total, ws, st);
 }
 
-hipError_t run_persistent(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* jobs_dev, int njobs,
-                          long long total, QueueState* q, int grid, double* ws, hipStream_t st)
+// the pool holds sl.S = grid * block lane slots; block is 64, 128 or 256
+hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl,
+                    const LaneSlots& sl, double* ws, hipStream_t st)
 {
-#define HH_LAUNCH(S, PT, B) return launch_persistent<S, PT, B>(sc, jobs_dev, njobs, total, q, grid, ws, st)
+#define HH_LAUNCH(S, PT, B) return launch_pool<S, PT, B>(sc, ring, ctl, sl, ws, st)
     if (sys == SYS_HEAR) {
         // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
         if (mixed) {
@@ -571,6 +725,48 @@ hipError_t run_persistent(int sys, bool mixed, int block, const SysConst& sc, co
     if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128);
     HH_LAUNCH(SYS_CO2AR, double, 256);
 #undef HH_LAUNCH
+}
+
+hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st)
+{
+    hipLaunchKernelGGL(slots_clear_kernel, dim3((unsigned)((sl.S + 255) / 256)), dim3(256), 0, st, sl.i, sl.S);
+    return hipGetLastError();
+}
+
+// src -> dst (dst is cleared first); `scratch` holds S/256 + 1 ints; the number of active
+// slots lands in scratch[S/256]
+hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st)
+{
+    const unsigned nb = (unsigned)((src.S + 255) / 256);
+    hipLaunchKernelGGL(repack_count_kernel, dim3(nb), dim3(256), 0, st, src.i, src.S, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(repack_scan_kernel, dim3(1), dim3(1024), 0, st, scratch, (int)nb, scratch + nb);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = run_slots_clear(dst, st)) != hipSuccess) return e;
+    const long long W = src.S / 64;
+    if (sys == SYS_HEAR)
+        hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_HEAR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
+    else
+        hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_CO2AR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
+    return hipGetLastError();
+}
+
+size_t slots_bytes(int sys, long long S)
+{
+    const int N = sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP;
+    return (size_t)S * ((2 * N + 8) * sizeof(double) + 6 * sizeof(int) + sizeof(long long));
+}
+
+LaneSlots slots_view(int sys, void* base, long long S)
+{
+    const int N = sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP;
+    LaneSlots sl;
+    sl.S = S;
+    sl.d = static_cast<double*>(base);
+    sl.local = reinterpret_cast<long long*>(sl.d + (size_t)(2 * N + 8) * S);
+    sl.i = reinterpret_cast<int*>(sl.local + S);
+    return sl;
 }
 
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,

@@ -1,0 +1,207 @@
+// Resident lane pools of the batched engine.
+//
+// A pool owns S = 2 waves/SIMD x 64 lanes of integrator slots for one class of work (system,
+// mu, I, precision, workgroup size).  Work runs in slices: one launch of pool_cf_kernel
+// restores every slot, integrates and refills from the pool's queue until its time is up, and
+// saves the slots again.  Between slices the host
+//   * admits new jobs (their items are appended to the queue: continuous batching),
+//   * launches the ensemble reduction of every job whose items have all finished, so a job's
+//     CF does not wait for the slowest trajectory of other jobs,
+//   * repacks the slots once the queue is drained, spreading the remaining items over all
+//     wavefronts so the long tail of the ensemble runs with little divergence.
+// Every item's arithmetic is independent of the slice boundaries and of its neighbours, and
+// each job's reduction has a fixed order, so a job's CF is the same bits alone or in a mix.
+#include <algorithm>
+#include <cstring>
+
+#include "hh_internal.h"
+
+namespace hh {
+
+namespace {
+const int kZero = 0;
+}  // namespace
+
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out)
+{
+    for (auto& p : set)
+        if (p->sys == sys.system && p->mixed == mixed && p->block == block && p->system.mu == sys.mu &&
+            p->system.I == sys.I) {
+            *out = p.get();
+            return HH_OK;
+        }
+    auto pl = std::make_unique<LanePool>();
+    pl->sys = sys.system;
+    pl->system = sys;
+    pl->mixed = mixed;
+    pl->block = block;
+    const long long grid = (long long)e->prop.multiProcessorCount * 2 * (256 / block);   // two waves per SIMD
+    pl->S = grid * block;
+    const size_t sb = slots_bytes(pl->sys, pl->S);
+    const size_t arena = (size_t)arena_doubles_per_lane(pl->sys) * sizeof(double) * (size_t)pl->S;
+    if (pl->slots[0].ensure(sb) != hipSuccess || pl->slots[1].ensure(sb) != hipSuccess ||
+        pl->arena.ensure(arena) != hipSuccess || pl->ring.ensure(pl->ring_cap * sizeof(JobDesc)) != hipSuccess ||
+        pl->ring_done.ensure(pl->ring_cap * sizeof(int)) != hipSuccess ||
+        pl->next.ensure(sizeof(unsigned long long)) != hipSuccess ||
+        pl->scratch.ensure((size_t)(pl->S / 256 + 1) * sizeof(int)) != hipSuccess)
+        return fail(HH_E_NOMEM, "cannot allocate a lane pool (%.1f MiB of slots and arena)",
+                    (2.0 * sb + arena) / 1048576.0);
+    HH_CHECK(run_slots_clear(pl->view(0), e->stream));
+    const unsigned long long z = 0;
+    HH_CHECK(hipMemcpyAsync(pl->next.p, &z, sizeof z, hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipStreamSynchronize(e->stream));
+    *out = pl.get();
+    set.push_back(std::move(pl));
+    return HH_OK;
+}
+
+size_t pool_job_bytes(const hh_cf_params& p, long ntraj)
+{
+    const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
+    return (size_t)items * ((size_t)p.L * sizeof(double) + 3 * sizeof(int) + sizeof(float));
+}
+
+bool pool_has_room(const LanePool& pl)
+{
+    return pl.inflight.empty() || pl.next_seq - pl.inflight.front()->seq < pl.ring_cap;
+}
+
+int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* ic_dev, long ntraj, double* cf_dev,
+               uint64_t request, PoolJob** out)
+{
+    if (!pool_has_room(pl)) return fail(HH_E_BUSY, "lane pool ring full (%d jobs in flight)", pl.ring_cap);
+    auto job = std::make_unique<PoolJob>();
+    const int halves = p.halves == HH_HALVES_BOTH ? 2 : 1;
+    const long long n_items = (long long)ntraj * halves;
+    const size_t bytes = pool_job_bytes(p, ntraj);
+    if (job->buf.ensure(bytes) != hipSuccess)
+        return fail(HH_E_NOMEM, "cannot allocate %.2f GiB for a job of %ld trajectories", bytes / 1073741824.0, ntraj);
+    job->bytes = bytes;
+    job->seq = pl.next_seq;
+    job->cf_out = cf_dev;
+    job->ppf = p.ppf;
+    job->request = request;
+    const int slot = (int)(job->seq % pl.ring_cap);
+    JobDesc& jd = job->jd;
+    jd.ic = ic_dev;
+    jd.rows = job->buf.as<double>();
+    jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L);
+    jd.status = jd.len + n_items;
+    jd.nrhs = jd.status + n_items;
+    jd.e_drift = reinterpret_cast<float*>(jd.nrhs + n_items);
+    jd.ntraj = ntraj;
+    jd.item_begin = (long long)pl.total;
+    jd.n_items = n_items;
+    jd.done = pl.ring_done.as<int>() + slot;
+    jd.halves = halves;
+    jd.L = p.L;
+    jd.max_steps = p.max_steps > 0 ? p.max_steps : 50000;
+    jd.dt = p.dt;
+    jd.Rcut = p.Rcut;
+    jd.rtol = p.rtol;
+    jd.atol = p.atol;
+    HH_CHECK(hipMemcpyAsync(pl.ring.as<JobDesc>() + slot, &job->jd, sizeof(JobDesc), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipMemcpyAsync(jd.done, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    pl.total += (unsigned long long)n_items;
+    pl.next_seq++;
+    pl.drained_repacked = false;
+    if (out) *out = job.get();
+    pl.inflight.push_back(std::move(job));
+    return HH_OK;
+}
+
+int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms)
+{
+    // 1. one slice of every pool that still integrates
+    std::vector<LanePool*> run;
+    for (auto& pp : set) {
+        LanePool& pl = *pp;
+        bool work = false;
+        for (auto& j : pl.inflight) work |= j->state == 0;
+        if (work) run.push_back(&pl);
+    }
+    if (!run.empty()) {
+        HH_CHECK(hipEventRecord(e->ev[0], e->stream));
+        for (LanePool* pl : run) {
+            PoolCtl ctl{};
+            ctl.next = pl->next.as<unsigned long long>();
+            ctl.total = pl->total;
+            ctl.first_seq = pl->inflight.front()->seq;
+            ctl.last_seq = pl->inflight.back()->seq;
+            ctl.ticks = (long long)(slice_ms * e->wall_khz);
+            ctl.ring = pl->ring_cap;
+            const bool mixed = pl->mixed && pl->sys == HH_SYS_CO2AR;
+            SysConst sc{pl->system.mu, pl->system.I, mixed ? e->tab32.p : e->tab64.p};
+            HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
+                              pl->arena.as<double>(), e->stream));
+        }
+        HH_CHECK(hipEventRecord(e->ev[1], e->stream));
+        HH_CHECK(hipStreamSynchronize(e->stream));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+        if (kernel_ms) *kernel_ms += ms;
+    }
+
+    // 2. finished jobs -> reductions; drained pools -> repack
+    bool launched = false;
+    HH_CHECK(hipEventRecord(e->ev[2], e->stream));
+    for (LanePool* pl : run) {
+        std::vector<int> done(pl->ring_cap);
+        HH_CHECK(hipMemcpy(done.data(), pl->ring_done.p, pl->ring_cap * sizeof(int), hipMemcpyDeviceToHost));
+        unsigned long long next = 0;
+        HH_CHECK(hipMemcpy(&next, pl->next.p, sizeof next, hipMemcpyDeviceToHost));
+        if (next > pl->total) {
+            // lanes that found the queue empty advanced the head past the end; none of those
+            // indices was handed out, so the head goes back to the first unadmitted item
+            next = pl->total;
+            HH_CHECK(hipMemcpy(pl->next.p, &next, sizeof next, hipMemcpyHostToDevice));
+        }
+        bool integrating = false;
+        for (auto& jp : pl->inflight) {
+            PoolJob& j = *jp;
+            if (j.state != 0) continue;
+            if (done[j.seq % pl->ring_cap] < j.jd.n_items) {
+                integrating = true;
+                continue;
+            }
+            const long long chunk = 256;
+            const int nchunks = (int)((j.jd.ntraj + chunk - 1) / chunk);
+            HH_CHECK(e->psum.ensure((size_t)nchunks * j.jd.L * sizeof(double)));
+            HH_CHECK(e->pcomp.ensure((size_t)nchunks * j.jd.L * sizeof(double)));
+            HH_CHECK(run_reduce(j.jd, j.ppf, e->psum.as<double>(), e->pcomp.as<double>(), chunk, nchunks, j.cf_out,
+                                e->stream));
+            j.state = 1;
+            launched = true;
+        }
+        if (integrating && next >= pl->total) {
+            // queue drained, items left: spread them over the pool's wavefronts
+            HH_CHECK(run_repack(pl->sys, pl->view(pl->cur), pl->view(1 - pl->cur), pl->scratch.as<int>(), e->stream));
+            pl->cur = 1 - pl->cur;
+            launched = true;
+        }
+    }
+    HH_CHECK(hipEventRecord(e->ev[3], e->stream));
+    if (launched) {
+        HH_CHECK(hipStreamSynchronize(e->stream));
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->ev[2], e->ev[3]);
+        if (reduce_ms) *reduce_ms += ms;
+    }
+    for (LanePool* pl : run)
+        for (auto& j : pl->inflight)
+            if (j->state == 1) j->state = 2;
+    return HH_OK;
+}
+
+std::unique_ptr<PoolJob> pool_take(LanePool& pl, PoolJob* job)
+{
+    for (auto it = pl.inflight.begin(); it != pl.inflight.end(); ++it)
+        if (it->get() == job) {
+            std::unique_ptr<PoolJob> j = std::move(*it);
+            pl.inflight.erase(it);
+            return j;
+        }
+    return nullptr;
+}
+
+}  // namespace hh

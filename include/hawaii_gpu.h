@@ -147,6 +147,51 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
 int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
           double ppf, double* m, double* q, double* ms);
 
+/* ------------------------------------------------------------------ serving session */
+/* A session keeps lane pools resident on the device and runs submitted requests with
+ * continuous batching: a background thread runs the pools in time slices; between slices it
+ * admits waiting requests while their device buffers fit the memory budget (FIFO), reduces
+ * every CF job whose trajectories have all finished and advances M0 requests by one slice.
+ * Requests of different systems, temperatures and sizes overlap on the device, and each
+ * result is the same bits as the same request run alone.  Caller buffers (ic_dev, cf_dev)
+ * must stay valid until the request has finished.  Thread-safe.  New interface (the
+ * reference has no serving path); it drives the same work as hh_cf_run and hh_m0. */
+typedef struct hh_session hh_session;
+
+typedef struct {
+    double slice_ms;          /* pool kernel time slice (<= 0: 10 ms) */
+    double max_device_gib;    /* admission budget for request buffers (<= 0: 80% of the free HBM) */
+    int block;                /* lane-pool workgroup size 64 / 128 / 256 (0: 256) */
+} hh_session_opts;
+
+enum { HH_REQ_WAITING = 0, HH_REQ_RUNNING = 1, HH_REQ_DONE = 2, HH_REQ_FAILED = 3 };
+enum { HH_REQ_CF = 0, HH_REQ_M0 = 1 };
+
+typedef struct {
+    int kind;                 /* HH_REQ_CF / HH_REQ_M0 */
+    int state;                /* HH_REQ_* */
+    double submit_ms;         /* session clock (ms since hh_session_create) */
+    double admit_ms;
+    double done_ms;
+    double latency_ms;        /* done - submit */
+    long long trajectories;   /* CF: ntraj; M0: accepted points */
+    hh_cf_stats cf;           /* CF requests (integrate_ms / reduce_ms are the session's
+                                 device time while the request was running) */
+    double m0, m0_err;        /* M0 requests: m, q as hh_m0 */
+    char error[160];          /* HH_REQ_FAILED */
+} hh_request_info;
+
+int hh_session_create(hh_engine* e, const hh_session_opts* opts, hh_session** out);
+int hh_submit_cf(hh_session* s, const hh_system* sys, const hh_cf_params* p, const double* ic_dev, long ntraj,
+                 double* cf_dev, uint64_t* id);
+int hh_submit_m0(hh_session* s, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints,
+                 int nstreams, double ppf, uint64_t* id);
+/* Wait up to timeout_ms (< 0: forever) for request `id`; returns 1 when it has finished
+ * (done or failed; `info` filled, the request forgotten), 0 on time-out. */
+int hh_poll(hh_session* s, uint64_t id, double timeout_ms, hh_request_info* info);
+/* Finish every submitted request, stop the worker and free the session. */
+int hh_session_destroy(hh_session* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@ PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
 ENGINE_PLAIN, ENGINE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 HALVES_BOTH, HALVES_FORWARD = 0, 1
+EVAL_PES, EVAL_DIPOLE, EVAL_HAMILTONIAN, EVAL_RHS = 0, 1, 2, 3
 
 # example_configs/CF-F-He-Ar.conf:5, example_configs/CF-F-CO2-Ar.conf:5,27,42
 MU_HEAR = 6.632039160324215e3
@@ -104,6 +105,8 @@ def lib():
             "hh_cf_items": (i, [v, i, v, v, v, v, v]),
             "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
             "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
+            "hh_eval": (i, [v, P(System), i, v, C.c_long, v]),
+            "hh_conf_parse": (i, [C.c_char_p, v]),
             "hh_session_create": (i, [v, P(SessionOpts), P(v)]),
             "hh_submit_cf": (i, [v, P(System), P(CFParams), v, C.c_long, v, P(C.c_uint64)]),
             "hh_submit_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(C.c_uint64)]),
@@ -213,6 +216,17 @@ class Engine:
         _check(lib().hh_cf_to_sf(self.h, C.c_void_p(cf_dev.data_ptr()), n, dt, C.c_void_p(nu.data_ptr()),
                                  C.c_void_p(sf.data_ptr()), C.byref(ms)))
         return nu, sf, ms.value
+
+    def eval(self, system, what, y_dev):
+        """Device surfaces at phase points y_dev (SoA [QP][n]): EVAL_PES -> [n], EVAL_DIPOLE ->
+        [3][n], EVAL_HAMILTONIAN -> [n], EVAL_RHS -> [QP][n]."""
+        n = y_dev.shape[1]
+        rows = {EVAL_PES: 1, EVAL_DIPOLE: 3, EVAL_HAMILTONIAN: 1, EVAL_RHS: y_dev.shape[0]}[what]
+        out = self.torch.empty((rows, n), dtype=self.torch.float64, device=y_dev.device)
+        self.torch.cuda.synchronize(self.device)
+        _check(lib().hh_eval(self.h, C.byref(system), what, C.c_void_p(y_dev.data_ptr()), n,
+                             C.c_void_p(out.data_ptr())))
+        return out
 
     def m0(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
         m, q, ms = C.c_double(), C.c_double(), C.c_double()

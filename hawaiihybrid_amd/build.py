@@ -15,6 +15,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhawaii_gpu.so")
+DRIVER = os.path.join(HERE, "hawaii_gpu")
 ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -27,6 +28,7 @@ SOURCES = [
     ("hh_host.cpp", ["-x", "hip"] + DEVICE),
     ("hh_pool.cpp", ["-x", "hip"] + DEVICE),
     ("hh_session.cpp", ["-x", "hip"] + DEVICE),
+    ("hh_conf.cpp", ["-x", "hip"] + DEVICE),
     # bit-exact initial conditions: no floating-point contraction in the sampler
     ("hh_sampler.cpp", ["-ffp-contract=off"]),
 ]
@@ -50,8 +52,16 @@ def build_engine(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(LIB + ".tmp", LIB)
+    # the .conf driver executable, linked against the library next to it
+    main_obj = _compile("hh_driver_main.cpp", [])
+    cmd = [HIPCC, "-o", DRIVER + ".tmp", main_obj, "-L" + HERE, "-lhawaii_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
     if verbose:
         print("built", LIB)
+        print("built", DRIVER)
     return LIB
 
 

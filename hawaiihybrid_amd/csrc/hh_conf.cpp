@@ -1,0 +1,732 @@
+// The reference driver's .conf front end on the engine (see include/hawaii_conf.h).
+//
+// Parsing follows the reference's grammar (src/driver.c:54-1522): blocks opened by &INPUT,
+// &MONOMER or &PROCESSING and closed by &END; `KEYWORD = value` with case-insensitive
+// keywords, values being numbers, bare words, double-quoted strings or {a, b, c} lists;
+// `!` starts a comment; PROCESSING holds calls NAME(args).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/hawaii_conf.h"
+#include "hh_internal.h"
+
+namespace {
+
+using hh::fail;
+
+constexpr double kALU = 5.29177210903e-11;                    // src/constants.h:8
+constexpr double kZeroCoeff = 0.00361479637 / (4.0 * M_PI);   // src/constants.h:29
+
+struct Tok {
+    enum Kind { END, IDENT, NUMBER, STRING, SYM, BLOCK } kind = END;
+    std::string text;
+    double num = 0.0;
+    int line = 0;
+};
+
+class Lexer {
+  public:
+    explicit Lexer(const std::string& s) : s_(s) {}
+    Tok next()
+    {
+        skip();
+        Tok t;
+        t.line = line_;
+        if (i_ >= s_.size()) return t;
+        const char c = s_[i_];
+        if (c == '&') {
+            size_t j = ++i_;
+            while (j < s_.size() && (std::isalnum((unsigned char)s_[j]) || s_[j] == '_')) ++j;
+            t.kind = Tok::BLOCK;
+            t.text = upper(s_.substr(i_, j - i_));
+            i_ = j;
+            return t;
+        }
+        if (c == '"') {
+            size_t j = i_ + 1;
+            while (j < s_.size() && s_[j] != '"' && s_[j] != '\n') ++j;
+            t.kind = Tok::STRING;
+            t.text = s_.substr(i_ + 1, j - i_ - 1);
+            i_ = j < s_.size() && s_[j] == '"' ? j + 1 : j;
+            return t;
+        }
+        if (std::isdigit((unsigned char)c) || c == '-' || c == '+' || c == '.') {
+            // numbers may group digits with underscores (100_000_000)
+            size_t j = i_;
+            std::string clean;
+            while (j < s_.size()) {
+                const char d = s_[j];
+                const bool sign_ok = (j == i_) || s_[j - 1] == 'e' || s_[j - 1] == 'E';
+                if (std::isdigit((unsigned char)d) || d == '.' || d == 'e' || d == 'E' || ((d == '+' || d == '-') && sign_ok)) {
+                    clean += d;
+                } else if (d != '_') {
+                    break;
+                }
+                ++j;
+            }
+            char* e = nullptr;
+            const double v = std::strtod(clean.c_str(), &e);
+            if (!clean.empty() && e == clean.c_str() + clean.size()) {
+                t.kind = Tok::NUMBER;
+                t.num = v;
+                t.text = s_.substr(i_, j - i_);
+                i_ = j;
+                return t;
+            }
+        }
+        if (std::isalpha((unsigned char)c) || c == '_') {
+            size_t j = i_;
+            while (j < s_.size() && (std::isalnum((unsigned char)s_[j]) || s_[j] == '_' || s_[j] == '.')) ++j;
+            t.kind = Tok::IDENT;
+            t.text = s_.substr(i_, j - i_);
+            i_ = j;
+            return t;
+        }
+        t.kind = Tok::SYM;
+        t.text = std::string(1, c);
+        ++i_;
+        return t;
+    }
+    static std::string upper(std::string x)
+    {
+        for (char& ch : x) ch = (char)std::toupper((unsigned char)ch);
+        return x;
+    }
+
+  private:
+    void skip()
+    {
+        while (i_ < s_.size()) {
+            const char c = s_[i_];
+            if (c == '\n') {
+                ++line_;
+                ++i_;
+            } else if (std::isspace((unsigned char)c)) {
+                ++i_;
+            } else if (c == '!') {
+                while (i_ < s_.size() && s_[i_] != '\n') ++i_;
+            } else {
+                break;
+            }
+        }
+    }
+    const std::string& s_;
+    size_t i_ = 0;
+    int line_ = 1;
+};
+
+// keywords of the reference that steer parts outside this engine: accepted, ignored
+const char* kIgnored[] = {"TEMPERATURES", "SATELLITE_TEMPERATURES", "INITIALM2_NPOINTS", "HEP_M0_NITERATIONS",
+                          "HEP_M0_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS", "HEP_PPF_NITERATIONS",
+                          "HEP_PPF_NPOINTS", "CF_FILENAMES", "PARTIAL_PARTITION_FUNCTION_RATIOS",
+                          "APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
+                          "AVERAGE_TIME_BETWEEN_COLLISIONS", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
+                          "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
+                          "JINI_HISTOGRAM_BINS", "JINI_HISTOGRAM_MAX", "JINI_HISTOGRAM_FILENAME",
+                          "JFIN_HISTOGRAM_BINS", "JFIN_HISTOGRAM_MAX", "JFIN_HISTOGRAM_FILENAME",
+                          "SPECTRUM_FREQUENCY_MAX"};
+
+bool ignored(const std::string& k)
+{
+    for (const char* x : kIgnored)
+        if (k == x) return true;
+    return false;
+}
+
+void copy_str(char* dst, const std::string& s) { snprintf(dst, HH_CONF_PATH, "%s", s.c_str()); }
+
+int parse_text(const std::string& text, const char* path, hh_conf* c)
+{
+    std::memset(c, 0, sizeof *c);
+    c->niterations = 1;
+    Lexer lx(text);
+    std::string block;
+    Tok t = lx.next();
+    auto err = [&](int line, const char* fmt, const std::string& what) -> int {
+        char msg[256];
+        snprintf(msg, sizeof msg, fmt, what.c_str());
+        return fail(HH_E_INVALID, "%s:%d: %s", path, line, msg);
+    };
+    while (t.kind != Tok::END) {
+        if (t.kind == Tok::BLOCK) {
+            if (t.text == "END") {
+                if (block.empty()) return err(t.line, "%s without an open block", "&END");
+                block.clear();
+            } else if (t.text == "INPUT" || t.text == "PROCESSING") {
+                block = t.text;
+            } else if (t.text == "MONOMER") {
+                if (c->n_monomers >= 2) return err(t.line, "%s: at most two monomers", "&MONOMER");
+                block = t.text;
+                ++c->n_monomers;
+            } else {
+                return err(t.line, "unknown block &%s", t.text);
+            }
+            t = lx.next();
+            continue;
+        }
+        if (block.empty()) return err(t.line, "'%s' outside of a block", t.text);
+        if (t.kind != Tok::IDENT) return err(t.line, "unexpected '%s'", t.text);
+        const std::string key = Lexer::upper(t.text);
+        const int line = t.line;
+
+        if (block == "PROCESSING") {
+            // NAME(arg, ...), or a setting KEY = value of the processing stack
+            Tok open = lx.next();
+            if (open.text == "=") {
+                (void)lx.next();
+                if (!ignored(key)) return err(line, "unknown PROCESSING setting %s", key);
+                t = lx.next();
+                continue;
+            }
+            if (open.text != "(") return err(line, "expected '(' after %s", key);
+            std::vector<Tok> args;
+            Tok a = lx.next();
+            while (a.kind != Tok::END && a.text != ")") {
+                if (a.text != ",") args.push_back(a);
+                a = lx.next();
+            }
+            if (c->nproc >= HH_CONF_MAX_PROC) return err(line, "too many PROCESSING calls (%s)", key);
+            hh_conf_proc& pr = c->proc[c->nproc];
+            static const struct { const char* name; int op; int nargs; } ops[] = {
+                {"READ_CF", HH_PROC_READ_CF, 1}, {"READ_SF", HH_PROC_READ_SF, 1}, {"CF_TO_SF", HH_PROC_CF_TO_SF, 0},
+                {"CMP", HH_PROC_CMP, 0},          {"DUP", HH_PROC_DUP, 0},         {"DUP2", HH_PROC_DUP2, 0},
+                {"DROP", HH_PROC_DROP, 0},        {"DROP2", HH_PROC_DROP2, 0},     {"WRITE_CF", HH_PROC_WRITE_CF, 1},
+                {"WRITE_SF", HH_PROC_WRITE_SF, 1}};
+            int found = -1;
+            for (const auto& o : ops)
+                if (key == o.name) {
+                    found = o.op;
+                    if ((int)args.size() != o.nargs || (o.nargs == 1 && args[0].kind != Tok::STRING))
+                        return err(line, "%s: wrong arguments", key);
+                    if (o.nargs == 1) copy_str(pr.arg, args[0].text);
+                }
+            if (found < 0) return err(line, "PROCESSING function %s is not part of this engine", key);
+            pr.op = found;
+            ++c->nproc;
+            t = lx.next();
+            continue;
+        }
+
+        Tok eq = lx.next();
+        if (eq.text != "=") return err(line, "expected '=' after %s", key);
+        Tok v = lx.next();
+        std::vector<double> list;
+        if (v.text == "{") {
+            Tok a = lx.next();
+            while (a.kind != Tok::END && a.text != "}") {
+                if (a.kind == Tok::NUMBER) list.push_back(a.num);
+                a = lx.next();
+            }
+        }
+        auto num = [&](double& dst) -> int {
+            if (v.kind != Tok::NUMBER) return err(line, "%s expects a number", key);
+            dst = v.num;
+            return 0;
+        };
+        auto inum = [&](long& dst) -> int {
+            if (v.kind != Tok::NUMBER || v.num != std::floor(v.num)) return err(line, "%s expects an integer", key);
+            dst = (long)v.num;
+            return 0;
+        };
+        auto str = [&](char* dst) -> int {
+            if (v.kind != Tok::STRING) return err(line, "%s expects a double-quoted string", key);
+            copy_str(dst, v.text);
+            return 0;
+        };
+        int rc = 0;
+        if (block == "MONOMER") {
+            const int m = c->n_monomers - 1;
+            if (key == "MONOMER_TYPE") {
+                const std::string w = Lexer::upper(v.text);
+                c->monomer_type[m] = w == "ATOM" ? HH_MONOMER_ATOM
+                                     : w == "LINEAR_MOLECULE" ? HH_MONOMER_LINEAR_MOLECULE
+                                                              : HH_MONOMER_OTHER;
+            } else if (key == "II") {
+                if (list.size() != 3) return err(line, "%s expects {Ixx, Iyy, Izz}", key);
+                for (int k = 0; k < 3; ++k) c->II[m][k] = list[k];
+            } else if (!ignored(key)) {
+                return err(line, "unknown MONOMER keyword %s", key);
+            }
+        } else if (key == "CALCULATION_TYPE") {
+            const std::string w = Lexer::upper(v.text);
+            if (w == "CORRELATION_SINGLE")
+                c->calculation_type = HH_CALC_CORRELATION_SINGLE;
+            else if (w == "PROCESSING")
+                c->calculation_type = HH_CALC_PROCESSING;
+            else
+                return err(line, "CALCULATION_TYPE %s is not part of this engine", w);
+        } else if (key == "PAIR_STATE") {
+            const std::string w = Lexer::upper(v.text);
+            if (w == "FREE_AND_METASTABLE" || w == "FREE")
+                c->pair_state = HH_PAIR_FREE;
+            else if (w == "BOUND")
+                c->pair_state = HH_PAIR_BOUND;
+            else if (w == "ALL")
+                c->pair_state = HH_PAIR_ALL;
+            else
+                return err(line, "unknown PAIR_STATE %s", w);
+        } else if (key == "PAIR_REDUCED_MASS") {
+            rc = num(c->reduced_mass);
+        } else if (key == "SO_POTENTIAL") {
+            rc = str(c->so_potential);
+        } else if (key == "SO_DIPOLE" || key == "SO_DIPOLE.1") {
+            rc = str(c->so_dipole);
+        } else if (key == "SO_DIPOLE.2") {
+            if (v.kind != Tok::STRING || (c->so_dipole[0] && v.text != c->so_dipole))
+                return err(line, "%s: cross-correlation of two dipoles is not part of this engine", key);
+        } else if (key == "TEMPERATURE") {
+            rc = num(c->temperature);
+        } else if (key == "NITERATIONS") {
+            rc = inum(c->niterations);
+        } else if (key == "TOTAL_TRAJECTORIES") {
+            rc = inum(c->total_trajectories);
+        } else if (key == "CVODE_TOLERANCE") {
+            rc = num(c->cvode_tolerance);
+        } else if (key == "SAMPLING_TIME") {
+            rc = num(c->sampling_time);
+        } else if (key == "MAXTRAJECTORYLENGTH") {
+            rc = inum(c->max_trajectory_length);
+        } else if (key == "SAMPLER_RMIN") {
+            rc = num(c->sampler_Rmin);
+        } else if (key == "SAMPLER_RMAX") {
+            rc = num(c->sampler_Rmax);
+        } else if (key == "PESMIN") {
+            rc = num(c->pesmin);
+        } else if (key == "INITIALM0_NPOINTS") {
+            rc = inum(c->initialM0_npoints);
+        } else if (key == "CF_FILENAME") {
+            rc = str(c->cf_filename);
+        } else if (key == "SF_FILENAME") {
+            rc = str(c->sf_filename);
+        } else if (key == "R0") {
+            rc = num(c->R0);
+        } else if (key == "RCUT") {
+            rc = num(c->Rcut);
+        } else if (key == "PARTIAL_PARTITION_FUNCTION_RATIO") {
+            rc = num(c->ppf);
+        } else if (key == "USE_ZIMMERMANN_TRICK") {
+            const std::string w = Lexer::upper(v.text);
+            c->use_zimmermann_trick = (w == "TRUE" || (v.kind == Tok::NUMBER && v.num != 0));
+        } else if (!ignored(key)) {
+            return err(line, "unknown keyword %s", key);
+        }
+        if (rc < 0) return rc;
+        t = lx.next();
+    }
+    if (!block.empty()) return fail(HH_E_INVALID, "%s: block &%s is not closed", path, block.c_str());
+    return HH_OK;
+}
+
+// ------------------------------------------------------------------ CF / SF files
+struct Series {
+    bool sf = false;
+    std::vector<double> x, y;   // t or nu; normalised values
+    double T = 0.0, ntraj = 0.0;
+};
+
+std::string now_stamp()
+{
+    time_t raw;
+    time(&raw);
+    struct tm tmv;
+    localtime_r(&raw, &tmv);
+    char b[64];
+    snprintf(b, sizeof b, "%04d-%02d-%02d %02d:%02d:%02d", tmv.tm_year + 1900, tmv.tm_mon + 1, tmv.tm_mday, tmv.tm_hour,
+             tmv.tm_min, tmv.tm_sec);
+    return b;
+}
+
+// write_correlation_function_ext / write_spectral_function formats (src/hawaii.c:4150-4264)
+int write_series(const char* path, const Series& s)
+{
+    FILE* fp = std::fopen(path, "w");
+    if (!fp) return fail(HH_E_INVALID, "could not open '%s' for writing", path);
+    std::fprintf(fp, "# HAWAII HYBRID v0.1\n# Saved on %s\n# TEMPERATURE: %.2f\n# AVERAGE OVER %.2f TRAJECTORIES\n",
+                 now_stamp().c_str(), s.T, s.ntraj);
+    if (s.sf) {
+        std::fprintf(fp, "# SPECTRAL FUNCTION LENGTH: %zu\n", s.x.size());
+        for (size_t i = 0; i < s.x.size(); ++i) std::fprintf(fp, "%.10f   %.10e\n", s.x[i], s.y[i]);
+    } else {
+        std::fprintf(fp, "# MAXIMUM TRAJECTORY LENGTH: %zu\n", s.x.size());
+        for (size_t i = 0; i < s.x.size(); ++i) std::fprintf(fp, "%.2f %.10e\n", s.x[i], s.y[i]);
+    }
+    std::fclose(fp);
+    return HH_OK;
+}
+
+// read_correlation_function / read_spectral_function: TEMPERATURE and AVERAGE OVER from the
+// header, then two columns (src/hawaii.c:4856-4939)
+int read_series(const char* path, bool sf, Series& s)
+{
+    std::ifstream in(path);
+    if (!in) return fail(HH_E_INVALID, "could not open '%s'", path);
+    s = Series{};
+    s.sf = sf;
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        if (line[0] == '#') {
+            double v;
+            if (std::sscanf(line.c_str(), "# TEMPERATURE: %lf", &v) == 1) s.T = v;
+            if (std::sscanf(line.c_str(), "# AVERAGE OVER %lf", &v) == 1) s.ntraj = v;
+            continue;
+        }
+        double a, b;
+        if (std::sscanf(line.c_str(), "%lf %lf", &a, &b) == 2) {
+            s.x.push_back(a);
+            s.y.push_back(b);
+        }
+    }
+    if (s.x.empty()) return fail(HH_E_INVALID, "'%s' holds no data", path);
+    return HH_OK;
+}
+
+// CMP (src/driver.c execute_cmp): `a` is the element that was on top.  Same length, T within
+// 1e-2 K, ntraj within 1e-3, abscissae within 1e-3 a.t.u. (t) / 1e-6 cm^-1 (nu); a value
+// differs when |a/b| - 1 > 1e-4 (the reference's one-sided test), or when b is 0 and a is not.
+bool cmp_series(const Series& a, const Series& b, std::string& why)
+{
+    char buf[256];
+    if (a.sf != b.sf) return why = "incompatible element types", false;
+    if (a.x.size() != b.x.size()) {
+        snprintf(buf, sizeof buf, "'len' differs (%zu and %zu)", a.x.size(), b.x.size());
+        return why = buf, false;
+    }
+    if (std::fabs(a.T - b.T) > 1e-2) return why = "'Temperature' differs", false;
+    if (std::fabs(a.ntraj - b.ntraj) > 1e-3) return why = "'ntraj' differs", false;
+    const double xtol = a.sf ? 1e-6 : 1e-3;
+    for (size_t i = 0; i < a.x.size(); ++i) {
+        if (std::fabs(a.x[i] - b.x[i]) > xtol) {
+            snprintf(buf, sizeof buf, "abscissae at index %zu differ (%.6e and %.6e)", i, a.x[i], b.x[i]);
+            return why = buf, false;
+        }
+        const bool differ = b.y[i] != 0.0 ? (std::fabs(a.y[i] / b.y[i]) - 1.0) > 1e-4 : a.y[i] != 0.0;
+        if (differ) {
+            snprintf(buf, sizeof buf, "values at index %zu differ (%.3e and %.3e)", i, a.y[i], b.y[i]);
+            return why = buf, false;
+        }
+    }
+    return true;
+}
+
+int cf_to_sf(hh_engine* e, const Series& cf, Series& sf)
+{
+    const long n = (long)cf.y.size();
+    if (n < 2 || (n & (n - 1))) return fail(HH_E_INVALID, "CF_TO_SF: length %ld is not a power of 2", n);
+    const double dt = cf.x[1] - cf.x[0];
+    double *d = nullptr, *nu = nullptr, *s = nullptr;
+    HH_CHECK(hipMalloc(&d, n * sizeof(double)));
+    HH_CHECK(hipMalloc(&nu, n * sizeof(double)));
+    HH_CHECK(hipMalloc(&s, n * sizeof(double)));
+    int rc = HH_OK;
+    if (hipMemcpy(d, cf.y.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(HH_E_DEVICE, "CF_TO_SF: upload failed");
+    if (rc == HH_OK) rc = hh_cf_to_sf(e, d, n, dt, nu, s, nullptr);
+    sf = Series{};
+    sf.sf = true;
+    sf.T = cf.T;
+    sf.ntraj = cf.ntraj;
+    sf.x.resize(n);
+    sf.y.resize(n);
+    if (rc == HH_OK && (hipMemcpy(sf.x.data(), nu, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(sf.y.data(), s, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+        rc = fail(HH_E_DEVICE, "CF_TO_SF: download failed");
+    (void)hipFree(d);
+    (void)hipFree(nu);
+    (void)hipFree(s);
+    return rc;
+}
+
+int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
+{
+    std::vector<Series> st;
+    int code = HH_OK;
+    for (int i = 0; i < c->nproc; ++i) {
+        const hh_conf_proc& p = c->proc[i];
+        auto need = [&](size_t k) -> int {
+            if (st.size() < k) return fail(HH_E_INVALID, "PROCESSING call %d needs %zu stack elements", i + 1, k);
+            return (int)HH_OK;
+        };
+        int rc = HH_OK;
+        switch (p.op) {
+            case HH_PROC_READ_CF:
+            case HH_PROC_READ_SF: {
+                Series s;
+                rc = read_series(p.arg, p.op == HH_PROC_READ_SF, s);
+                if (rc == HH_OK) st.push_back(std::move(s));
+                break;
+            }
+            case HH_PROC_CF_TO_SF: {
+                if ((rc = need(1)) < 0) break;
+                if (st.back().sf) {
+                    rc = fail(HH_E_INVALID, "CF_TO_SF expects a correlation function on the stack");
+                    break;
+                }
+                Series sf;
+                rc = cf_to_sf(e, st.back(), sf);
+                if (rc == HH_OK) st.back() = std::move(sf);
+                break;
+            }
+            case HH_PROC_CMP: {
+                if ((rc = need(2)) < 0) break;
+                std::string why;
+                const bool same = cmp_series(st[st.size() - 1], st[st.size() - 2], why);
+                st.pop_back();
+                st.pop_back();
+                if (!quiet) std::printf("CMP: %s%s\n", same ? "equal" : "differ: ", same ? "" : why.c_str());
+                if (!same) code = fail(HH_E_INVALID, "CMP failed: %s", why.c_str());
+                break;
+            }
+            case HH_PROC_DUP:
+                if ((rc = need(1)) == HH_OK) st.push_back(st.back());
+                break;
+            case HH_PROC_DUP2:
+                if ((rc = need(2)) == HH_OK) {
+                    Series a = st[st.size() - 2], b = st[st.size() - 1];
+                    st.push_back(a);
+                    st.push_back(b);
+                }
+                break;
+            case HH_PROC_DROP:
+                if ((rc = need(1)) == HH_OK) st.pop_back();
+                break;
+            case HH_PROC_DROP2:
+                if ((rc = need(2)) == HH_OK) st.resize(st.size() - 2);
+                break;
+            case HH_PROC_WRITE_CF:
+            case HH_PROC_WRITE_SF:
+                if ((rc = need(1)) < 0) break;
+                if (st.back().sf != (p.op == HH_PROC_WRITE_SF)) {
+                    rc = fail(HH_E_INVALID, "WRITE_%s: wrong element type on the stack", p.op == HH_PROC_WRITE_SF ? "SF" : "CF");
+                    break;
+                }
+                rc = write_series(p.arg, st.back());
+                st.pop_back();
+                break;
+            default:
+                rc = fail(HH_E_INVALID, "unknown PROCESSING op %d", p.op);
+        }
+        if (rc < 0) return rc;
+    }
+    return code;
+}
+
+uint32_t good_seed()
+{
+    uint32_t s = 0;
+    FILE* f = std::fopen("/dev/urandom", "rb");
+    if (f) {
+        if (std::fread(&s, sizeof s, 1, f) != 1) s = 0;
+        std::fclose(f);
+    }
+    if (s == 0) s = (uint32_t)time(nullptr);
+    return s;
+}
+
+int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+{
+    // the reference's checks (src/hawaii.c:2802-2811)
+    if (c->max_trajectory_length <= 0 || c->Rcut <= 0 || c->sampling_time <= 0 || c->total_trajectories <= 0 ||
+        c->cvode_tolerance <= 0 || c->niterations < 1 || !c->cf_filename[0])
+        return fail(HH_E_INVALID, "MaxTrajectoryLength, Rcut, sampling_time, total_trajectories, cvode_tolerance, "
+                                  "niterations and cf_filename must be set");
+    if (c->use_zimmermann_trick) return fail(HH_E_UNSUPPORTED, "Zimmermann's trick is not part of this engine");
+    if (c->ppf <= 0)
+        return fail(HH_E_UNSUPPORTED, "PARTIAL_PARTITION_FUNCTION_RATIO must be given: the VEGAS estimate of the "
+                                      "reference is not part of this engine");
+    if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
+        return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
+    hh_system sys{};
+    sys.mu = c->reduced_mass;
+    if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_ATOM && c->monomer_type[1] == HH_MONOMER_ATOM) {
+        sys.system = HH_SYS_HEAR;
+    } else if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_LINEAR_MOLECULE &&
+               c->monomer_type[1] == HH_MONOMER_ATOM) {
+        sys.system = HH_SYS_CO2AR;
+        sys.I = c->II[0][0];
+        if (c->II[0][0] != c->II[0][1] || sys.I <= 0)
+            return fail(HH_E_INVALID, "linear molecule needs II = {I, I, 0} with I > 0");
+    } else {
+        return fail(HH_E_UNSUPPORTED, "monomers other than ATOM+ATOM or LINEAR_MOLECULE+ATOM are not part of this engine");
+    }
+    const int qp = hh_qp_size(sys.system);
+    const long n_iter = c->total_trajectories / c->niterations;
+    if (n_iter <= 0) return fail(HH_E_INVALID, "total_trajectories < niterations");
+    const uint32_t seed = o.seed ? o.seed : good_seed();
+
+    // the plugin's pes_lab drives the host rejection sampler (the reference's accept/reject)
+    hh_pes_fn pes = nullptr;
+    void* so = nullptr;
+    if (c->so_potential[0]) {
+        so = dlopen(c->so_potential, RTLD_LAZY);
+        if (so) {
+            pes = reinterpret_cast<hh_pes_fn>(dlsym(so, "pes_lab"));
+            if (!pes) {
+                dlclose(so);
+                return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
+            }
+            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
+        } else if (!o.quiet) {
+            std::printf("note: %s not loadable; the engine's built-in surface drives the sampler\n", c->so_potential);
+        }
+    }
+
+    hh_sampler smp{HH_IC_REJECTION, c->pair_state ? c->pair_state : HH_PAIR_FREE, c->temperature, c->sampler_Rmin,
+                   c->sampler_Rmax, c->pesmin, 0.0};
+    const long m0n = c->initialM0_npoints > 1 ? c->initialM0_npoints : 0;
+    const long total = m0n + n_iter * c->niterations;
+    std::vector<double> ic((size_t)total * qp);
+    long nd = 0;
+    if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
+    long rc = hh_sample_ics(&sys, &smp, seed, total, pes, ic.data(), &nd);
+    if (rc < 0) {
+        if (so) dlclose(so);
+        return (int)rc;
+    }
+
+    // device buffers: one iteration of initial conditions (SoA), the CF, the M0 dipoles
+    const long nmax = std::max(n_iter, m0n);
+    double *ic_dev = nullptr, *cf_dev = nullptr, *out_dev = nullptr;
+    int code = HH_OK;
+    std::vector<double> soa((size_t)nmax * qp);
+    auto upload = [&](long first, long n) -> int {
+        for (long t = 0; t < n; ++t)
+            for (int i = 0; i < qp; ++i) soa[(size_t)i * n + t] = ic[(size_t)(first + t) * qp + i];
+        if (hipMemcpy(ic_dev, soa.data(), (size_t)n * qp * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(HH_E_DEVICE, "upload of initial conditions failed");
+        return (int)HH_OK;
+    };
+    do {
+        if (hipMalloc(&ic_dev, (size_t)nmax * qp * sizeof(double)) != hipSuccess ||
+            hipMalloc(&cf_dev, c->max_trajectory_length * sizeof(double)) != hipSuccess ||
+            hipMalloc(&out_dev, (size_t)3 * std::max(m0n, 1L) * sizeof(double)) != hipSuccess) {
+            code = fail(HH_E_NOMEM, "device allocation failed");
+            break;
+        }
+        // preliminary M0 over the first initialM0_npoints accepted points (calculate_M0,
+        // src/hawaii.c:1423-1491: the same candidates the CF stream then continues after)
+        double m0 = 0.0;
+        if (m0n > 0) {
+            if ((code = upload(0, m0n)) < 0) break;
+            if ((code = hh_eval(e, &sys, HH_EVAL_DIPOLE, ic_dev, m0n, out_dev)) < 0) break;
+            std::vector<double> d((size_t)3 * m0n);
+            if (hipMemcpy(d.data(), out_dev, d.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            double m = 0.0, q = 0.0;
+            for (long k = 0; k < m0n; ++k) {
+                const double f = d[k] * d[k] + d[m0n + k] * d[m0n + k] + d[2 * m0n + k] * d[2 * m0n + k];
+                const double diff = f - m;
+                m += diff / (k + 1.0);
+                q += diff * diff * (k / (k + 1.0));
+            }
+            m0 = m * kZeroCoeff * c->ppf;
+            const double m0s = std::sqrt(q / m0n / (m0n - 1.0)) * kZeroCoeff * c->ppf;
+            if (!o.quiet) {
+                std::printf("M0 = %.10e +/- %.10e [%.10e ... %.10e]\n", m0, m0s, m0 - m0s, m0 + m0s);
+                std::printf("Error: %.3f%%\n", m0s / m0 * 100.0);
+            }
+        }
+        hh_cf_params p{};
+        const bool mixed = o.precision == HH_PREC_MIXED;
+        p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+        p.dt = c->sampling_time;
+        p.Rcut = c->Rcut;
+        p.ppf = c->ppf;
+        p.L = (int)c->max_trajectory_length;
+        p.halves = HH_HALVES_BOTH;
+        p.max_steps = 50000;
+        p.engine = HH_ENGINE_BATCHED;
+        p.precision = mixed ? HH_PREC_MIXED : HH_PREC_FP64;
+        Series cf;
+        cf.T = c->temperature;
+        cf.x.resize(p.L);
+        cf.y.assign(p.L, 0.0);
+        for (int k = 0; k < p.L; ++k) cf.x[k] = c->sampling_time * k;
+        std::vector<double> sum(p.L, 0.0), it(p.L);
+        double ntraj = 0.0;
+        for (long iter = 0; iter < c->niterations && code == HH_OK; ++iter) {
+            if ((code = upload(m0n + iter * n_iter, n_iter)) < 0) break;
+            const double* icp[1] = {ic_dev};
+            double* cfp[1] = {cf_dev};
+            const long nt[1] = {n_iter};
+            hh_cf_stats st;
+            if ((code = hh_cf_run(e, &sys, &p, 1, icp, nt, cfp, &st)) < 0) break;
+            if (hipMemcpy(it.data(), cf_dev, p.L * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            for (int k = 0; k < p.L; ++k) sum[k] += it[k] * (double)n_iter;
+            ntraj += (double)n_iter;
+            cf.ntraj = ntraj;
+            for (int k = 0; k < p.L; ++k) cf.y[k] = sum[k] / ntraj;
+            if (!o.quiet) {
+                std::printf("ITERATION %ld/%ld: accumulated %.0f trajectories. Saving the temporary result to '%s'\n",
+                            iter + 1, c->niterations, ntraj, c->cf_filename);
+                const double m0cf = cf.y[0] * kZeroCoeff / kALU / kALU / kALU;
+                std::printf("M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e\n", m0cf, m0);
+                std::printf("device: %.1f ms integration, %lld RHS, %lld failed, %lld step-limited items\n",
+                            st.integrate_ms, st.nrhs, st.n_failed, st.n_stepfail);
+                std::fflush(stdout);
+            }
+            if ((code = write_series(c->cf_filename, cf)) < 0) break;
+        }
+        if (code == HH_OK && c->sf_filename[0]) {
+            Series sf;
+            if ((code = cf_to_sf(e, cf, sf)) == HH_OK) code = write_series(c->sf_filename, sf);
+        }
+    } while (false);
+    if (ic_dev) (void)hipFree(ic_dev);
+    if (cf_dev) (void)hipFree(cf_dev);
+    if (out_dev) (void)hipFree(out_dev);
+    if (so) dlclose(so);
+    return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hh_conf_parse(const char* path, hh_conf* out)
+{
+    if (!path || !out) return fail(HH_E_INVALID, "hh_conf_parse: null argument");
+    std::ifstream in(path);
+    if (!in) return fail(HH_E_INVALID, "could not open '%s'", path);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return parse_text(ss.str(), path, out);
+}
+
+int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
+{
+    if (!conf) return fail(HH_E_INVALID, "hh_conf_run: null configuration");
+    hh_conf_run_opts o{};
+    if (opts) o = *opts;
+    hh_engine* e = nullptr;
+    int rc = hh_engine_create(o.device, &e);
+    if (rc < 0) return rc;
+    if (conf->calculation_type == HH_CALC_CORRELATION_SINGLE) {
+        rc = run_correlation_single(e, conf, o);
+        if (rc == HH_OK) rc = run_processing(e, conf, o.quiet != 0);
+    } else if (conf->calculation_type == HH_CALC_PROCESSING) {
+        rc = run_processing(e, conf, o.quiet != 0);
+    } else {
+        rc = fail(HH_E_INVALID, "CALCULATION_TYPE is not set");
+    }
+    hh_engine_destroy(e);
+    return rc;
+}
+
+}  // extern "C"

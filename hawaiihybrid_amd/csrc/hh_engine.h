@@ -27,6 +27,7 @@ struct JobDesc {
     long long item_begin;    // first global item index of this job
     long long n_items;       // ntraj * halves
     int* done;               // finished-item counter of the job (nullptr: not counted)
+    const int* order;        // queue position -> item index (nullptr: identity)
     int halves;              // 1: forward only, 2: forward + momentum-inverted backward
     int L;                   // MaxTrajectoryLength
     int max_steps;           // max accepted steps between two samples

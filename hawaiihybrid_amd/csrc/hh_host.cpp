@@ -430,6 +430,20 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
     return HH_OK;
 }
 
+int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev)
+{
+    if (!e || !sys || !y_dev || !out_dev || n <= 0) return fail(HH_E_INVALID, "hh_eval: bad argument");
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
+        return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
+    if (what < HH_EVAL_PES || what > HH_EVAL_RHS) return fail(HH_E_INVALID, "hh_eval: unknown quantity %d", what);
+    std::lock_guard<std::mutex> lock(e->mu);
+    HH_CHECK(hipSetDevice(e->device));
+    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    HH_CHECK(hh::run_eval(sys->system, sc, what, y_dev, n, out_dev, e->stream));
+    HH_CHECK(hipStreamSynchronize(e->stream));
+    return HH_OK;
+}
+
 int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev, double* sf_dev, double* ms)
 {
     if (!e || !cf_dev || !nu_dev || !sf_dev) return fail(HH_E_INVALID, "hh_cf_to_sf: null argument");

@@ -25,6 +25,8 @@ hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const Jo
                     const LaneSlots& sl, double* ws, hipStream_t st);
 hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st);
 hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st);
+hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long long n, double* out, hipStream_t st);
+hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, hipStream_t st);
 size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
@@ -102,7 +104,7 @@ struct LanePool {
     int ring_cap = 1024;
     DevBuf slots[2];            // double-buffered for repacking
     int cur = 0;
-    DevBuf arena, ring, ring_done, next, scratch;
+    DevBuf arena, ring, ring_done, next, scratch, sort_tmp;
     std::deque<std::unique_ptr<PoolJob>> inflight;   // sequence order
     long long next_seq = 0;
     unsigned long long total = 0;   // items admitted

@@ -518,13 +518,74 @@ __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, 
                 const unsigned long long g = atomicAdd(ctl.next, 1ull);
                 if (g >= ctl.total) break;
                 const int j = ring_job(ring, ctl, g);
-                got = init_item<SYS, P, N>(scl, ring[j], j, (long long)g - ring[j].item_begin, ar, ln);
+                const long long pos = (long long)g - ring[j].item_begin;
+                const long long local = ring[j].order ? (long long)ring[j].order[pos] : pos;
+                got = init_item<SYS, P, N>(scl, ring[j], j, local, ar, ln);
             }
             if (!got) break;
         }
         dop853_step<SYS, P, N>(scl, ring, ar, ln);
     }
     save_lane<N>(sl, slot, ar, ln);
+}
+
+// Scheduling key of every item of a job: the straight-line time (in samples, at most L - 1)
+// from the initial point to the Rcut sphere along the item's initial velocity, forward for
+// the first half and momentum-inverted for the second.  Trajectory cost grows with its
+// duration, so handing out items in decreasing key order starts the long ones first.  Only
+// the order of work changes; every item's result is the same.
+__global__ void __launch_bounds__(256) order_keys_kernel(const double* __restrict__ ic, long long ntraj, int halves,
+                                                         double mu, double Rcut, double dt, int L,
+                                                         float* __restrict__ keys, int* __restrict__ idx)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= ntraj * halves) return;
+    const long long t = i / halves;
+    const double pPhi = ic[1 * ntraj + t], Theta = ic[2 * ntraj + t], pTheta = ic[3 * ntraj + t];
+    const double R = ic[4 * ntraj + t], pR = ic[5 * ntraj + t];
+    const double sT = sin(Theta);
+    const double L2 = pTheta * pTheta + (sT != 0.0 ? pPhi * pPhi / (sT * sT) : 0.0);
+    const double p = sqrt(pR * pR + L2 / (R * R));
+    double key = (double)(L - 1);
+    if (p > 0.0) {
+        const double rr = ((i % halves) == 1 ? -1.0 : 1.0) * R * pR / p;   // r . u
+        const double disc = rr * rr - R * R + Rcut * Rcut;
+        const double tx = disc > 0.0 ? (-rr + sqrt(disc)) * mu / p : 0.0;
+        key = fmin(tx / dt, (double)(L - 1));
+    }
+    keys[i] = (float)key;
+    idx[i] = (int)i;
+}
+
+// Batched surface evaluation at phase points y (SoA [QP][n], FP64): what=0 the potential
+// (pes_lab, Hartree), 1 the dipole (dipole_lab, 3 x n), 2 the Hamiltonian, 3 the right-hand side
+// of Hamilton's equations (QP x n).
+template <int SYS>
+__global__ void __launch_bounds__(256) eval_kernel(SysConst sc, int what, const double* __restrict__ y, long long n,
+                                                   double* __restrict__ out)
+{
+    constexpr int N = SysTraits<SYS>::QP;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double v[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = y[k * n + i];
+    if (what == 0) {
+        out[i] = pes_value<SYS, double>(sc, v);
+    } else if (what == 1) {
+        double m[3];
+        dipole<SYS, double>(sc, v, m);
+        out[i] = m[0];
+        out[n + i] = m[1];
+        out[2 * n + i] = m[2];
+    } else if (what == 2) {
+        out[i] = hamiltonian<SYS, double>(sc, v);
+    } else {
+        double f[N];
+        rhs<SYS, double>(sc, v, f);
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k * n + i] = f[k];
+    }
 }
 
 // Repack the active lane slots of `src` into `dst`: active slot number r (in slot order) goes
@@ -749,6 +810,24 @@ hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* 
         hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_HEAR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
     else
         hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_CO2AR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
+    return hipGetLastError();
+}
+
+hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long long n, double* out, hipStream_t st)
+{
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (sys == SYS_HEAR)
+        hipLaunchKernelGGL(eval_kernel<SYS_HEAR>, dim3(g), dim3(256), 0, st, sc, what, y, n, out);
+    else
+        hipLaunchKernelGGL(eval_kernel<SYS_CO2AR>, dim3(g), dim3(256), 0, st, sc, what, y, n, out);
+    return hipGetLastError();
+}
+
+hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, hipStream_t st)
+{
+    const long long n = jb.n_items;
+    hipLaunchKernelGGL(order_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, jb.ic, jb.ntraj,
+                       jb.halves, mu, jb.Rcut, jb.dt, jb.L, keys, idx);
     return hipGetLastError();
 }
 

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "hh_internal.h"
 
 namespace hh {
@@ -58,7 +60,8 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj)
 {
     const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
-    return (size_t)items * ((size_t)p.L * sizeof(double) + 3 * sizeof(int) + sizeof(float));
+    // rows, len, status, nrhs, e_drift, order
+    return (size_t)items * ((size_t)p.L * sizeof(double) + 4 * sizeof(int) + sizeof(float));
 }
 
 bool pool_has_room(const LanePool& pl)
@@ -100,6 +103,25 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     jd.Rcut = p.Rcut;
     jd.rtol = p.rtol;
     jd.atol = p.atol;
+    int* order = reinterpret_cast<int*>(jd.e_drift + n_items);
+    {
+        // longest-expected-first order of the job's items (order_keys_kernel)
+        size_t tmp_bytes = 0;
+        HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, (const float*)nullptr,
+                                                              (float*)nullptr, (const int*)nullptr, (int*)nullptr,
+                                                              (int)n_items, 0, 32, e->stream));
+        const size_t kv = (size_t)n_items * (2 * sizeof(float) + sizeof(int));
+        if (pl.sort_tmp.ensure(kv + tmp_bytes + 256) != hipSuccess)
+            return fail(HH_E_NOMEM, "cannot allocate the ordering workspace");
+        float* keys_in = pl.sort_tmp.as<float>();
+        float* keys_out = keys_in + n_items;
+        int* idx_in = reinterpret_cast<int*>(keys_out + n_items);
+        void* tmp = reinterpret_cast<char*>(pl.sort_tmp.p) + ((kv + 255) / 256) * 256;
+        HH_CHECK(run_order_keys(jd, pl.system.mu, keys_in, idx_in, e->stream));
+        HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, keys_in, keys_out, idx_in, order,
+                                                              (int)n_items, 0, 32, e->stream));
+        jd.order = order;
+    }
     HH_CHECK(hipMemcpyAsync(pl.ring.as<JobDesc>() + slot, &job->jd, sizeof(JobDesc), hipMemcpyHostToDevice, e->stream));
     HH_CHECK(hipMemcpyAsync(jd.done, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     pl.total += (unsigned long long)n_items;

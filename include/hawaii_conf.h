@@ -1,0 +1,92 @@
+/*
+ * hawaii_conf.h -- the reference driver's .conf front end on the MI355X engine.
+ *
+ * Reads the configuration format of the reference driver (src/driver.c:54-1522: &INPUT,
+ * &MONOMER and &PROCESSING blocks, `key = value` lines, `!` comments, case-insensitive
+ * keywords) and runs the part of it that is this engine's path:
+ *   CALCULATION_TYPE = CORRELATION_SINGLE   calculate_correlation_and_save
+ *                                           (src/hawaii.c:2795-3117): preliminary M0 from the
+ *                                           rejection sampler, NITERATIONS iterations of
+ *                                           TOTAL_TRAJECTORIES/NITERATIONS trajectories from the
+ *                                           seeded stream, the CF file rewritten after each
+ *                                           iteration (write_correlation_function_ext,
+ *                                           src/hawaii.c:4150-4212)
+ *   CALCULATION_TYPE = PROCESSING           the stack machine of run_processing
+ *                                           (src/driver.c:2956-3081) restricted to READ_CF,
+ *                                           READ_SF, CF_TO_SF, CMP, DUP, DUP2, DROP, DROP2,
+ *                                           WRITE_CF, WRITE_SF
+ * Systems: two ATOM monomers (He-Ar surfaces) or LINEAR_MOLECULE + ATOM (CO2-Ar surfaces).
+ * The VEGAS integrals of the reference (partition-function ratio, M0, M2 by hep-mc) are not
+ * part of this engine: PARTIAL_PARTITION_FUNCTION_RATIO must be given.  A plugin named by
+ * so_potential is dlopen'ed and its pes_lab drives the host rejection sampler, so the sampled
+ * phase points follow the reference's draws; the device integrates with its built-in surfaces.
+ */
+#ifndef HAWAII_CONF_H
+#define HAWAII_CONF_H
+
+#include "hawaii_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2 };
+enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
+enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
+       HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF };
+
+#define HH_CONF_PATH 512
+#define HH_CONF_MAX_PROC 128
+
+typedef struct {
+    int op;                       /* HH_PROC_* */
+    char arg[HH_CONF_PATH];       /* file name for READ_* / WRITE_* */
+} hh_conf_proc;
+
+typedef struct {
+    int calculation_type;         /* HH_CALC_* */
+    int pair_state;               /* HH_PAIR_* (FREE_AND_METASTABLE -> HH_PAIR_FREE) */
+    double temperature;
+    double reduced_mass;
+    char so_potential[HH_CONF_PATH];
+    char so_dipole[HH_CONF_PATH];
+    long niterations;
+    long total_trajectories;
+    double cvode_tolerance;
+    double sampling_time;
+    long max_trajectory_length;
+    double sampler_Rmin, sampler_Rmax;
+    double pesmin;
+    long initialM0_npoints;
+    char cf_filename[HH_CONF_PATH];
+    char sf_filename[HH_CONF_PATH];
+    double R0, Rcut;
+    double ppf;                   /* partial_partition_function_ratio (0: not given) */
+    int use_zimmermann_trick;
+    int n_monomers;
+    int monomer_type[2];          /* HH_MONOMER_* */
+    double II[2][3];
+    int nproc;
+    hh_conf_proc proc[HH_CONF_MAX_PROC];
+} hh_conf;
+
+/* Parse a configuration file (no device use).  Unknown keywords and unsupported PROCESSING
+ * functions are errors naming the line; keywords of the reference that only steer parts
+ * outside this engine (VEGAS point counts, M2, histograms) are accepted and ignored. */
+int hh_conf_parse(const char* path, hh_conf* out);
+
+typedef struct {
+    uint32_t seed;                /* 0: from /dev/urandom, as the reference driver */
+    int precision;                /* HH_PREC_FP64 (default) or HH_PREC_MIXED */
+    double rtol_mixed;            /* rtol = atol of the mixed engine (<= 0: 1e-10) */
+    int device;
+    int quiet;
+} hh_conf_run_opts;
+
+/* Run a parsed configuration on the engine (creates one on opts->device). */
+int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

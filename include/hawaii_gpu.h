@@ -135,6 +135,17 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
  * any pointer may be NULL).  rows: [n_items][L], entries past len are zero. */
 int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* e_drift, double* rows);
 
+/* ------------------------------------------------------------------ batched surfaces */
+/* Evaluate the engine's device surfaces at n phase points y_dev (SoA [QP][n] doubles, state
+ * layout [Phi, pPhi, Theta, pTheta, R, pR, (phi1, pphi1, theta1, ptheta1)]), FP64:
+ *   HH_EVAL_PES        out[n]       V (Hartree)                    pes_lab (src/hawaii.h:480-499)
+ *   HH_EVAL_DIPOLE     out[3][n]    lab-frame dipole (a.u.)        dipole_lab
+ *   HH_EVAL_HAMILTONIAN out[n]      H = K + V                      Hamiltonian (src/hawaii.c:1190-1204)
+ *   HH_EVAL_RHS        out[QP][n]   Hamilton's equations           rhs (src/hawaii.c:553-615)
+ * The batched device form of a plugin's entry points. */
+enum { HH_EVAL_PES = 0, HH_EVAL_DIPOLE = 1, HH_EVAL_HAMILTONIAN = 2, HH_EVAL_RHS = 3 };
+int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev);
+
 /* ------------------------------------------------------------------ CF -> SF on the device */
 /* Connes apodization in place on cf_dev, Makhoul IDCT through a 2n-point complex FFT,
  * unit scaling: nu_dev[n] in cm^-1, sf_dev[n] (idct_cf_to_sf).  n must be a power of two. */

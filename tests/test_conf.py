@@ -1,0 +1,109 @@
+"""The .conf front end (include/hawaii_conf.h): parsing of the reference driver's format on
+the CPU.  Runs on the GPU are in test_gpu_driver.py."""
+import ctypes as C
+import os
+
+import pytest
+
+import hawaiihybrid_amd as H
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_CONFIGS = "/root/reference/example_configs"
+
+
+class Proc(C.Structure):
+    _fields_ = [("op", C.c_int), ("arg", C.c_char * 512)]
+
+
+class Conf(C.Structure):
+    _fields_ = [("calculation_type", C.c_int), ("pair_state", C.c_int), ("temperature", C.c_double),
+                ("reduced_mass", C.c_double), ("so_potential", C.c_char * 512), ("so_dipole", C.c_char * 512),
+                ("niterations", C.c_long), ("total_trajectories", C.c_long), ("cvode_tolerance", C.c_double),
+                ("sampling_time", C.c_double), ("max_trajectory_length", C.c_long), ("sampler_Rmin", C.c_double),
+                ("sampler_Rmax", C.c_double), ("pesmin", C.c_double), ("initialM0_npoints", C.c_long),
+                ("cf_filename", C.c_char * 512), ("sf_filename", C.c_char * 512), ("R0", C.c_double),
+                ("Rcut", C.c_double), ("ppf", C.c_double), ("use_zimmermann_trick", C.c_int),
+                ("n_monomers", C.c_int), ("monomer_type", C.c_int * 2), ("II", (C.c_double * 3) * 2),
+                ("nproc", C.c_int), ("proc", Proc * 128)]
+
+
+def parse_text(tmp_path, text):
+    p = tmp_path / "x.conf"
+    p.write_text(text)
+    c = Conf()
+    rc = H.lib().hh_conf_parse(str(p).encode(), C.byref(c))
+    return rc, c
+
+
+def parse(path):
+    c = Conf()
+    rc = H.lib().hh_conf_parse(path.encode(), C.byref(c))
+    return rc, c
+
+
+def test_parse_small_correlation_config():
+    rc, c = parse(os.path.join(HERE, "conf", "cf-hear-small.conf"))
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 1 and c.pair_state == H.PAIR_FREE
+    assert c.temperature == 300.0 and c.reduced_mass == 6.632039160324215e3
+    assert c.niterations == 2 and c.total_trajectories == 512 and c.max_trajectory_length == 256
+    assert c.cvode_tolerance == 1e-12 and c.sampling_time == 200.0 and c.Rcut == 40.0
+    assert c.initialM0_npoints == 100 and c.ppf == 1.0
+    assert c.cf_filename == b"CF-hear-small.txt" and c.n_monomers == 2 and list(c.monomer_type) == [1, 1]
+
+
+@pytest.mark.parametrize("name", ["CF-F-CO2-Ar.conf", "CF-F-He-Ar.conf", "CF-B-CO2-Ar.conf"])
+def test_parse_reference_example_configs(name):
+    """The reference's own example configurations parse (read in place; skipped where the
+    reference tree is absent)."""
+    path = os.path.join(REF_CONFIGS, name)
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    rc, c = parse(path)
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 1 and c.n_monomers == 2
+    assert c.total_trajectories > 0 and c.max_trajectory_length > 0 and c.Rcut > 0
+
+
+def test_parse_linear_molecule_and_processing(tmp_path):
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = correlation_single   ! keywords and values are case-insensitive
+  PAIR_STATE = BOUND
+  pair_reduced_mass = 3.817083069753919e+04
+  TEMPERATURE = 100.0
+  hep_m0_npoints = 1000                    ! VEGAS settings are accepted and ignored
+&END
+&MONOMER
+  MONOMER_TYPE = LINEAR_MOLECULE
+  II = {2.819827114123505e+05, 2.819827114123505e+05, 0.0}
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+&PROCESSING
+  READ_CF("a.txt")
+  DUP()
+  CF_TO_SF()
+  WRITE_SF("b.txt")
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.pair_state == H.PAIR_BOUND and list(c.monomer_type) == [2, 1]
+    assert c.II[0][0] == 2.819827114123505e5 and c.II[0][2] == 0.0
+    assert c.nproc == 4 and [c.proc[i].op for i in range(4)] == [0, 4, 2, 9]
+    assert c.proc[0].arg == b"a.txt" and c.proc[3].arg == b"b.txt"
+
+
+@pytest.mark.parametrize("text,needle", [
+    ("&INPUT\n  FOO = 1\n&END\n", b"x.conf:2: unknown keyword FOO"),
+    ("&INPUT\n  CALCULATION_TYPE = PR_MU\n&END\n", b"PR_MU is not part of this engine"),
+    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  FIT_BASELINE()\n&END\n",
+     b"FIT_BASELINE is not part of this engine"),
+    ("&INPUT\n  TEMPERATURE = \"hot\"\n&END\n", b"TEMPERATURE expects a number"),
+    ("&INPUT\n  TEMPERATURE = 300.0\n", b"not closed"),
+])
+def test_parse_errors_name_the_problem(tmp_path, text, needle):
+    rc, _ = parse_text(tmp_path, text)
+    assert rc < 0
+    assert needle in H.lib().hh_last_error()

@@ -1,0 +1,95 @@
+"""The .conf driver executable (hawaiihybrid_amd/hawaii_gpu) on the GPU: a CORRELATION_SINGLE
+run reproduces the oracle's CF for the same seeded stream (the preliminary-M0 points consumed
+first, as calculate_correlation_and_save does), written in the reference's file format, and a
+PROCESSING run of the reference's CF -> SF known-answer test passes its CMP."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hawaiihybrid_amd as H
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(os.path.dirname(HERE), "hawaiihybrid_amd", "hawaii_gpu")
+
+
+def run_driver(args, cwd):
+    print("driver:", args, flush=True)
+    return subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def read_columns(path):
+    head, rows = [], []
+    for line in open(path):
+        if line.startswith("#"):
+            head.append(line.strip())
+        elif line.strip():
+            rows.append([float(v) for v in line.split()])
+    return head, np.array(rows)
+
+
+def test_driver_correlation_single_matches_oracle(tmp_path):
+    conf = os.path.join(HERE, "conf", "cf-hear-small.conf")
+    r = run_driver([conf, "--seed", "42"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "ITERATION 2/2: accumulated 512 trajectories" in r.stdout
+    head, cf = read_columns(tmp_path / "CF-hear-small.txt")
+    assert "# TEMPERATURE: 300.00" in head and "# AVERAGE OVER 512.00 TRAJECTORIES" in head
+    assert "# MAXIMUM TRAJECTORY LENGTH: 256" in head
+    assert cf.shape == (256, 2) and np.allclose(cf[:, 0], 200.0 * np.arange(256))
+    # oracle: the first 100 accepted points went to the preliminary M0, then 2 x 256
+    ic, _, _ = O.sample_ics(O.hear(), O.Sampler(O.IC_REJECTION, 1, 300.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0), 42,
+                            100 + 512)
+    p = O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 50000)
+    ref = O.cf_rows(O.hear(), p, ic[100:], 8)
+    cf_ref = ref["crln"].sum(0) / 512
+    err = np.linalg.norm(cf[:, 1] - cf_ref) / np.linalg.norm(cf_ref)
+    assert err < 1e-8, err
+    _, sf = read_columns(tmp_path / "SF-hear-small.txt")
+    assert sf.shape == (256, 2)
+
+
+def write_ref_format(path, x, y, T, ntraj, sf):
+    with open(path, "w") as f:
+        f.write(f"# HAWAII HYBRID v0.1\n# TEMPERATURE: {T:.2f}\n# AVERAGE OVER {ntraj:.2f} TRAJECTORIES\n")
+        if sf:
+            f.write(f"# SPECTRAL FUNCTION LENGTH: {len(x)}\n")
+            f.writelines(f"{a:.10f}   {b:.10e}\n" for a, b in zip(x, y))
+        else:
+            f.write(f"# MAXIMUM TRAJECTORY LENGTH: {len(x)}\n")
+            f.writelines(f"{a:.2f} {b:.10e}\n" for a, b in zip(x, y))
+
+
+@pytest.mark.parametrize("perturb", [False, True])
+def test_driver_cf_to_sf_known_answer(tmp_path, golden, perturb):
+    """tests/convert-cf-to-sf.conf of the reference (its CF_TO_SF part) on the committed
+    fixture: CMP passes, and fails once one SF value is moved by 1e-3."""
+    g = golden("cf_sf_hear_1000.npz")
+    sf = g["sf"].copy()
+    if perturb:
+        k = int(np.argmax(np.abs(sf)))
+        sf[k] *= 1.001
+    write_ref_format(tmp_path / "cf.txt", g["cf_t"], g["cf"], float(g["T"]), float(g["ntraj"]), False)
+    write_ref_format(tmp_path / "sf.txt", g["sf_nu"], sf, float(g["T"]), float(g["ntraj"]), True)
+    (tmp_path / "kat.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  DUP()
+    CF_TO_SF()
+    READ_SF("sf.txt")
+    CMP()
+  DROP()
+&END
+""")
+    r = run_driver(["kat.conf"], str(tmp_path))
+    if perturb:
+        assert r.returncode != 0 and "CMP failed" in r.stderr
+    else:
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert "CMP: equal" in r.stdout

@@ -162,3 +162,34 @@ def test_m0_rejects_unreachable_sampler(eng):
     bad = H.Sampler(H.IC_REJECTION, H.PAIR_BOUND, 300.0, 39.0, 40.0, H.PESMIN_HEAR, 0.0)
     with pytest.raises(H.HHError):
         eng.m0(H.he_ar(), bad, 1, 128, 64)
+
+
+@pytest.mark.parametrize("which", ["hear", "co2ar"])
+def test_batched_surfaces_match_oracle(eng, which):
+    """hh_eval (device pes_lab / dipole_lab / Hamiltonian / rhs in FP64) at sampled phase points
+    against the oracle's restated surfaces.  The device evaluates its own sin/cos and the
+    CO2-Ar tables re-expanded about each interval's centre: agreement to rounding."""
+    if which == "hear":
+        sh, so = H.he_ar(), O.hear()
+        smp = H.Sampler(H.IC_REJECTION, 3, 300.0, 3.5, 40.0, H.PESMIN_HEAR, 0.0)
+    else:
+        sh, so = H.co2_ar(), O.co2ar()
+        smp = H.Sampler(H.IC_REJECTION, 3, 300.0, 4.0, 40.0, H.PESMIN_CO2AR, 0.0)
+    y, _, _ = H.sample_ics(sh, smp, 17, 400)
+    yd = eng.to_device_soa(y)
+    V = eng.eval(sh, H.EVAL_PES, yd).cpu().numpy()[0]
+    D = eng.eval(sh, H.EVAL_DIPOLE, yd).cpu().numpy()
+    E = eng.eval(sh, H.EVAL_HAMILTONIAN, yd).cpu().numpy()[0]
+    F = eng.eval(sh, H.EVAL_RHS, yd).cpu().numpy()
+    L = O.lib()
+    for i in range(y.shape[0]):
+        q = np.ascontiguousarray(y[i, 0::2])
+        v = L.orc_pes(so, O.ptr(q))
+        assert abs(V[i] - v) <= 1e-11 * abs(v) + 1e-16
+        mu = np.zeros(3)
+        L.orc_dipole(so, O.ptr(q), O.ptr(mu))
+        assert np.allclose(D[:, i], mu, rtol=1e-11, atol=1e-16)
+        assert abs(E[i] - L.orc_hamiltonian(so, O.ptr(np.ascontiguousarray(y[i])))) <= 1e-11 * abs(E[i]) + 1e-15
+        f = np.zeros(y.shape[1])
+        L.orc_rhs(so, O.ptr(np.ascontiguousarray(y[i])), O.ptr(f))
+        assert np.allclose(F[:, i], f, rtol=1e-9, atol=1e-15)

@@ -213,6 +213,7 @@ class Engine:
         nu = torch.empty_like(cf_dev)
         sf = torch.empty_like(cf_dev)
         ms = C.c_double(0)
+        torch.cuda.synchronize(self.device)
         _check(lib().hh_cf_to_sf(self.h, C.c_void_p(cf_dev.data_ptr()), n, dt, C.c_void_p(nu.data_ptr()),
                                  C.c_void_p(sf.data_ptr()), C.byref(ms)))
         return nu, sf, ms.value
@@ -230,6 +231,7 @@ class Engine:
 
     def m0(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
         m, q, ms = C.c_double(), C.c_double(), C.c_double()
+        self.torch.cuda.synchronize(self.device)
         _check(lib().hh_m0(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf, C.byref(m),
                            C.byref(q), C.byref(ms)))
         return m.value, q.value, ms.value

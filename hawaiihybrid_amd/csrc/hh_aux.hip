@@ -21,7 +21,10 @@ namespace hh {
 constexpr double kPI = 3.14159265358979323846;
 
 // ------------------------------------------------------------------ CF -> SF
-__global__ void __launch_bounds__(256) sf_pack_kernel(double* __restrict__ cf, long n, double dt,
+// The pack kernel only reads cf (thread k > n needs the original cf[2n - k] that thread 2n - k
+// would otherwise be overwriting); the in-place apodization of the input that the reference
+// performs is written by sf_unpack_kernel, after the transform.
+__global__ void __launch_bounds__(256) sf_pack_kernel(const double* __restrict__ cf, long n, double dt,
                                                       double2* __restrict__ z)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,14 +38,13 @@ __global__ void __launch_bounds__(256) sf_pack_kernel(double* __restrict__ cf, l
         const double tc = dt * k, r = tc / tmax;
         const double f = (1.0 - r * r) * (1.0 - r * r);
         v = cf[k] * f;
-        cf[k] = v;   // apodized in place, as the reference does
     } else if (k == n) {
         z[k] = make_double2(0.0, 0.0);
         return;
     } else {
         const long i = 2 * n - k;
         const double tc = dt * i, r = tc / tmax;
-        v = -(cf[i] * ((1.0 - r * r) * (1.0 - r * r)));   // value of the not-yet-written apodized cf[i]
+        v = -(cf[i] * ((1.0 - r * r) * (1.0 - r * r)));
     }
     z[k] = make_double2(v * c * inv2n, v * s * inv2n);
 }
@@ -105,10 +107,16 @@ __global__ void __launch_bounds__(1024) fft_lds_kernel(const double2* __restrict
 }
 
 __global__ void __launch_bounds__(256) sf_unpack_kernel(const double2* __restrict__ Z, long n, double dt,
-                                                        double* __restrict__ nu, double* __restrict__ sf)
+                                                        double* __restrict__ nu, double* __restrict__ sf,
+                                                        double* __restrict__ cf)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    {
+        // Connes apodization of the input in place, as the reference does (src/hawaii.c:5581)
+        const double r = (dt * k) / (dt * n);
+        cf[k] *= (1.0 - r * r) * (1.0 - r * r);
+    }
     const double LightSpeed_cm = 2.99792458e10, ATU = 2.4188843265857e-17;
     const double ADIPMOMU = 8.4783536255e-30, EPSILON0 = 8.8541878128e-12;
     const double Xscale = 1.0 / LightSpeed_cm / ATU / 2.0 / kPI;
@@ -144,7 +152,7 @@ hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2*
         }
         res = src;
     }
-    hipLaunchKernelGGL(sf_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, n, dt, nu, sf);
+    hipLaunchKernelGGL(sf_unpack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, res, n, dt, nu, sf, cf);
     return hipGetLastError();
 }
 

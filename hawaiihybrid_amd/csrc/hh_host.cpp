@@ -216,7 +216,9 @@ int hh_engine_create(int device, hh_engine** out)
         delete e;
         return fail(HH_E_DEVICE, "device %d is %s; this build targets gfx950 (MI355X)", device, arch.c_str());
     }
-    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    // a blocking stream: the engine's work is ordered after the caller's copies and kernels on
+    // the legacy default stream (hipMemcpy of pageable memory can return before its DMA ends)
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamDefault) != hipSuccess) {
         delete e;
         return fail(HH_E_DEVICE, "hipStreamCreate failed");
     }

@@ -184,32 +184,67 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
     return true;
 }
 
+// out[j][i] = sum over the non-zeros q of tableau row r0 + j of A[r0 + j][q] * K_{col q}[i] for
+// NR consecutive rows that share one column list (rows 16-18: B, E5, E3; rows 19-22: the dense
+// output D rows).  Columns are loaded G at a time and each loaded column serves all NR rows, so
+// the row set costs ceil(nnz / G) memory round trips instead of NR * nnz.  Components i with
+// i % STRIDE == 0 only (STRIDE 2: the coordinates).  Per row the sum runs in the same order as
+// row_dot, so the results are identical.
+template <int N, int NR, int G, int STRIDE>
+__device__ __forceinline__ void rows_dot(const Arena<N>& ar, int r0, double (&out)[NR][N])
+{
+#pragma unroll
+    for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i) out[j][i] = 0.0;
+    const int nnz = DOP853_SP_NNZ[r0];
+#pragma unroll 1
+    for (int q0 = 0; q0 < nnz; q0 += G) {
+        double v[G][N];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (q0 + g < nnz) {
+                const int m = DOP853_SP_COL[r0][q0 + g];
+#pragma unroll
+                for (int i = 0; i < N; i += STRIDE) v[g][i] = ar.ld(m, i);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            if (q0 + g < nnz) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j) {
+                    const double a = DOP853_SP_VAL[r0 + j][q0 + g];
+#pragma unroll
+                    for (int i = 0; i < N; i += STRIDE) out[j][i] += a * v[g][i];
+                }
+            }
+        }
+    }
+}
+
 // out = sum over the non-zero entries (m, a) of sparse tableau row r of a * K_m (arena slots).
 // The row index is uniform across the wavefront, so the coefficients are scalar loads.
 template <int N>
 __device__ __forceinline__ void row_dot(const Arena<N>& ar, int r, double* out)
 {
+    double o[1][N];
+    rows_dot<N, 1, 3, 1>(ar, r, o);
 #pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = 0.0;
-    const int nnz = DOP853_SP_NNZ[r];
-#pragma unroll 1
-    for (int q = 0; q < nnz; ++q) {
-        const int m = DOP853_SP_COL[r][q];
-        const double a = DOP853_SP_VAL[r][q];
-#pragma unroll
-        for (int i = 0; i < N; ++i) out[i] += a * ar.ld(m, i);
-    }
+    for (int i = 0; i < N; ++i) out[i] = o[0][i];
 }
 
 // Compiler-only barrier: stage vectors are re-read from the arena after it instead of being
 // kept in registers across stages (which would rebuild the register-resident layout).
 __device__ __forceinline__ void arena_fence() { __asm__ __volatile__("" ::: "memory"); }
 
-// stages s0..s1 (inclusive): K_s = f(y + h sum_m A[s][m] K_m)
+// stages s0..s1 (inclusive): K_s = f(y + h sum_m A[s][m] K_m); noise: the largest FP32 noise
+// bound of the stages' momentum derivatives
 template <int SYS, typename P, int N>
 __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, const double* y, double h, int s0,
-                                       int s1)
+                                       int s1, double& noise)
 {
+    noise = 0.0;
 #pragma unroll 1
     for (int s = s0; s <= s1; ++s) {
         arena_fence();
@@ -217,7 +252,9 @@ __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, c
         row_dot<N>(ar, s, yt);
 #pragma unroll
         for (int i = 0; i < N; ++i) yt[i] = y[i] + h * yt[i];
-        rhs<SYS, P>(sc, yt, k);
+        double nz;
+        rhs<SYS, P>(sc, yt, k, nz);
+        noise = fmax(noise, nz);
 #pragma unroll
         for (int i = 0; i < N; ++i) ar.st(s, i, k[i]);
     }
@@ -245,24 +282,27 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     if (ln.t + 1.01 * h - tend > 0.0) h = tend - ln.t;
 
     // stages 1..11 and stage 12 = f(t + h, y_new) (tableau row 12 is the B row)
-    stages<SYS, P, N>(sc, ar, ln.y, h, 1, 12);
+    double noise;
+    stages<SYS, P, N>(sc, ar, ln.y, h, 1, 12, noise);
     ln.nrhs += 12;
     arena_fence();
+    // y_new (B row) and the two error estimates (E5, E3 rows) share their columns
+    static_assert(DOP853_ROW_E5 == DOP853_ROW_B + 1 && DOP853_ROW_E3 == DOP853_ROW_B + 2, "tableau row order");
     double yn[N];
-    row_dot<N>(ar, DOP853_ROW_B, yn);
-#pragma unroll
-    for (int i = 0; i < N; ++i) yn[i] = ln.y[i] + h * yn[i];
-
     double err = 0.0, err2 = 0.0;
     {
-        double e5[N], e3[N];
-        row_dot<N>(ar, DOP853_ROW_E5, e5);
-        row_dot<N>(ar, DOP853_ROW_E3, e3);
+        double be[3][N];
+        rows_dot<N, 3, 2, 1>(ar, DOP853_ROW_B, be);
+        // FP32 surfaces: the momentum tolerance is raised by the step's force rounding noise
+        // times h, so the controller does not shrink the step to resolve FP32 noise (zero and
+        // without effect in FP64)
+        const double floor_p = fabs(h) * noise;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            const double sk = jb.atol + jb.rtol * fmax(fabs(ln.y[i]), fabs(yn[i]));
-            err += (e5[i] / sk) * (e5[i] / sk);
-            err2 += (e3[i] / sk) * (e3[i] / sk);
+            yn[i] = ln.y[i] + h * be[0][i];
+            const double sk = jb.atol + jb.rtol * fmax(fabs(ln.y[i]), fabs(yn[i])) + ((i & 1) ? floor_p : 0.0);
+            err += (be[1][i] / sk) * (be[1][i] / sk);
+            err2 += (be[2][i] / sk) * (be[2][i] / sk);
         }
     }
     double deno = err + 0.01 * err2;
@@ -287,24 +327,27 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     int status = ITEM_OK;
     float drift = 0.0f;
     if (ln.k < jb.L && ln.k * jb.dt <= tnew) {
-        stages<SYS, P, N>(sc, ar, ln.y, h, 13, 15);
+        double nz_dense;
+        stages<SYS, P, N>(sc, ar, ln.y, h, 13, 15, nz_dense);
         ln.nrhs += 3;
         arena_fence();
-        // F0 = dy, F1 = h K0 - dy, F2 = 2 dy - h (K12 + K0), F3..F6 = h (D K)
+        // F0 = dy, F1 = h K0 - dy, F2 = 2 dy - h (K12 + K0), F3..F6 = h (D K), for the
+        // coordinates only (even state indices): the dipole and the R > Rcut test need no momenta
 #pragma unroll
-        for (int i = 0; i < N; ++i) {
+        for (int i = 0; i < N; i += 2) {
             const double dy = yn[i] - ln.y[i];
             const double k0 = ar.ld(0, i);
             ar.st(kSlotF + 0, i, dy);
             ar.st(kSlotF + 1, i, h * k0 - dy);
             ar.st(kSlotF + 2, i, 2.0 * dy - h * (ar.ld(12, i) + k0));
         }
-#pragma unroll 1
-        for (int d = 0; d < 4; ++d) {
-            double f[N];
-            row_dot<N>(ar, DOP853_ROW_D + d, f);
+        {
+            double f[4][N];
+            rows_dot<N, 4, 3, 2>(ar, DOP853_ROW_D, f);
 #pragma unroll
-            for (int i = 0; i < N; ++i) ar.st(kSlotF + 3 + d, i, h * f[i]);
+            for (int d = 0; d < 4; ++d)
+#pragma unroll
+                for (int i = 0; i < N; i += 2) ar.st(kSlotF + 3 + d, i, h * f[d][i]);
         }
         double* row = jb.rows + ln.local * (long long)jb.L;
         while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
@@ -317,7 +360,9 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
             } else {
                 const double x = (tk - ln.t) / h, omx = 1.0 - x;
 #pragma unroll
-                for (int i = 0; i < N; ++i) {
+                for (int i = 1; i < N; i += 2) ys[i] = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; i += 2) {
                     double v = ar.ld(kSlotF + 5, i) + x * ar.ld(kSlotF + 6, i);
                     v = ar.ld(kSlotF + 4, i) + omx * v;
                     v = ar.ld(kSlotF + 3, i) + x * v;
@@ -349,7 +394,9 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
             row[ln.k] = ln.curr;
             ln.steps = 0;
             if (ys[4] > jb.Rcut || ln.k + 1 >= jb.L) {
-                drift = (float)fabs(hamiltonian<SYS, P>(sc, ys) - ln.E0);
+                // energy check of make_step (src/trajectory.c:167-180), at the end of the step
+                // that holds the last sample
+                drift = (float)fabs(hamiltonian<SYS, P>(sc, yn) - ln.E0);
                 ended = true;
                 break;
             }

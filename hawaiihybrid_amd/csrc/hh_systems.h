@@ -191,10 +191,16 @@ struct Trig {
     double sT, cT, st, ct;   // FP64 sin/cos of Theta and theta1 (EOM)
 };
 
-// CO2-Ar lab-frame gradient dV/dq, q = [Phi, Theta, R, phi1, theta1] (Hartree / bohr, rad)
+// Rounding-noise scale of an FP32 gradient: 8 units of FP32 roundoff (2^-24) times the sum of
+// the magnitudes of the Legendre terms that are added up (their cancellation is what makes the
+// FP32 gradient noisy, 1e-6 relative and more where the terms nearly cancel).
+constexpr double kF32NoiseUnits = 8.0 / 16777216.0;
+
+// CO2-Ar lab-frame gradient dV/dq, q = [Phi, Theta, R, phi1, theta1] (Hartree / bohr, rad).
+// noise: for P = float an absolute bound on the rounding noise of every component (0 for FP64).
 template <typename P>
 __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb, const double* y,
-                                           const Trig& tr, double* dV)
+                                           const Trig& tr, double* dV, double& noise)
 {
     P sP, cP, sp, cp;
     sincos_p<P>(y[0], &sP, &cP);
@@ -215,13 +221,16 @@ __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb
     const double R = y[4];
     const int b = co2ar_branch(R);
     P dVdR = P(0), dVdc = P(0);
+    P mag = P(0);   // sum of |terms| (FP32 only)
     if (b == 0) {
         const P r = (P)R;
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             const P e = tb->pes_exp[s][0] * exp(tb->pes_exp[s][1] * r);
-            dVdR += tb->pes_exp[s][1] * e * Pl[2 * s];
-            dVdc += e * dPl[2 * s];
+            const P tR = tb->pes_exp[s][1] * e * Pl[2 * s], tc = e * dPl[2 * s];
+            dVdR += tR;
+            dVdc += tc;
+            if constexpr (sizeof(P) == 4) mag += fabs(tR) + fabs(tc);
         }
     } else {
         const P x = (P)(R - tb->center[b]);
@@ -230,10 +239,13 @@ __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb
         for (int s = 0; s < 6; ++s) {
             P v, dv;
             slot_eval<P>(tb->pes[b][s], x, sinv, v, dv);
-            dVdR += dv * Pl[2 * s];
-            dVdc += v * dPl[2 * s];
+            const P tR = dv * Pl[2 * s], tc = v * dPl[2 * s];
+            dVdR += tR;
+            dVdc += tc;
+            if constexpr (sizeof(P) == 4) mag += fabs(tR) + fabs(tc);
         }
     }
+    noise = sizeof(P) == 4 ? kF32NoiseUnits * (double)mag : 0.0;
     dV[0] = (double)(dVdc * dc_dPhi);
     dV[1] = (double)(dVdc * dc_dTheta);
     dV[2] = (double)dVdR;
@@ -314,10 +326,13 @@ struct SysTraits<SYS_CO2AR> {
 };
 
 // Right-hand side of Hamilton's equations.  Intermolecular part src/hawaii.c:586-591, linear
-// rotor src/hawaii.c:494-497.
+// rotor src/hawaii.c:494-497.  noise: bound on the FP32 rounding noise of the momentum
+// derivatives (0 when the surfaces are evaluated in FP64).
 template <int SYS, typename P>
-__device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict__ y, double* __restrict__ yd)
+__device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict__ y, double* __restrict__ yd,
+                                    double& noise)
 {
+    noise = 0.0;
     const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
     Trig tr;
     sincos_p<double>(Theta, &tr.sT, &tr.cT);
@@ -328,7 +343,7 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
         dV[2] = hear_dV(R);
     } else {
         sincos_p<double>(y[8], &tr.st, &tr.ct);
-        co2ar_dpes<P>((const Co2ArTables<P>*)sc.tables, y, tr, dV);
+        co2ar_dpes<P>((const Co2ArTables<P>*)sc.tables, y, tr, dV, noise);
     }
     const double iR = 1.0 / R, iR2 = iR * iR, imu = 1.0 / mu;
     const double is = 1.0 / tr.sT, is2 = is * is;
@@ -347,6 +362,13 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
         yd[8] = ptheta * iI;
         yd[9] = pphi * pphi * tr.ct * iI * ist2 * ist - dV[4];
     }
+}
+
+template <int SYS, typename P>
+__device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict__ y, double* __restrict__ yd)
+{
+    double noise;
+    rhs<SYS, P>(sc, y, yd, noise);
 }
 
 template <int SYS, typename P>

@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
 """CO2-Ar trajectory-ensemble benchmark on one MI355X.
 
-A step is one batch of `--ntraj` CO2-Ar trajectories (T = 200 K, R0 = Rcut = 40 bohr,
-incoming momentum, L = 4096 samples at dt = 200 a.u.) taken from consecutive chunks of the
-seed-42 Mersenne Twister stream: sample, integrate, record the dipole correlation function,
-reduce to the ensemble CF.  Initial conditions are resident in HBM before timing.
+Workload: a step is one CO2-Ar dipole correlation function over `--ntraj` trajectories
+(T = 200 K, R0 = Rcut = 40 bohr, incoming radial momentum, L = 4096 samples at dt = 200 a.u.;
+IC mode r0_inject, forward half), from consecutive chunks of the seed-42 Mersenne Twister
+stream: integrate every trajectory, record mu(0).mu(t_k), reduce to the ensemble CF.  The
+initial conditions are sampled on the host (bit-exact reference stream) and resident in HBM
+before any timed region.  W warm-up steps, then K timed steps.
 
-Timed engines (same batches):
-  plain    FP64, one trajectory per thread, one launch per batch
-  batched  persistent kernel + device work queue, FP32 surfaces with FP64 state and error
-           control (rtol = atol = --rtol-mixed), all timed batches streamed through one launch
-The headline `value` is the batched engine's trajectories/s; `speedup_vs_plain` compares it
-with the plain FP64 engine.  Accuracy of both against the CPU oracle (FP64) is reported for
-the first timed batch.  One JSON line on stdout.
+Engines, timed on the same K batches:
+  plain    FP64 at the reference tolerance (rtol = atol = 1e-12), one trajectory per thread,
+           one launch per step (each launch waits for its slowest trajectory)
+  batched  resident lane pool (time-sliced persistent kernel, longest-first queue, continuous
+           batching: all K steps are admitted together and stream through the pool), FP32
+           surfaces with FP64 state and error control at rtol = atol = --rtol-mixed
+`value` is the batched engine's trajectories/s over the K steps; `speedup_vs_plain` divides it
+by the plain engine's.  Also reported: the plain engine given all K steps in one launch, the
+batched engine in FP64, accuracy against the CPU oracle for the first timed step (FP64 DOP853
+at 1e-12), the roofline of the pool kernel, a serving run with Poisson arrivals (p50/p99
+request latency), and the CPU oracle timed on the host cores.  One JSON line on stdout.
 """
 import argparse
-import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -27,63 +33,161 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 sys.path.insert(0, os.path.join(HERE, "tests"))
 
+ALU3 = 5.29177210903e-11 ** 3
+MI355X_FP64_VECTOR_TFLOPS = 78.6
+MI355X_HBM_GBS = 8000.0
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
+    """Poisson arrivals of CF requests (He-Ar or CO2-Ar, T ~ U[50, 500] K, ntraj ~ 2^U[10, 16])
+    and M0 requests, 4:1; inputs staged in HBM before the clock starts.  M0 requests draw
+    10^U[6, 7] points (the serving configuration allows up to 10^8; at 50 K the rejection
+    sampler needs hundreds of candidates per point, so the bench caps them to keep the run
+    within its time budget)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    n_req = max(1, int(rng.poisson(rate_hz * seconds)))
+    t_arr = np.sort(rng.uniform(0.0, seconds, n_req))
+    reqs = []
+    for i in range(n_req):
+        if rng.uniform() < 0.8:
+            co2 = rng.uniform() < 0.5
+            T = float(rng.uniform(50.0, 500.0))
+            ntraj = int(2 ** rng.uniform(10, 16))
+            if co2:
+                sysd, pes, L = H.co2_ar(), H.PESMIN_CO2AR, 4096
+                p = H.cf_params(L, rtol=1e-10, precision=H.PREC_MIXED)
+            else:
+                sysd, pes, L = H.he_ar(), H.PESMIN_HEAR, 2048
+                p = H.cf_params(L, rtol=1e-12, precision=H.PREC_FP64)
+            smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, T, 4.0, 40.0, pes, 40.0)
+            ic = eng.to_device_soa(H.sample_ics(sysd, smp, 1000 + i, ntraj)[0])
+            reqs.append(("cf", sysd, p, ic, ntraj))
+        else:
+            co2 = rng.uniform() < 0.5
+            sysd = H.co2_ar() if co2 else H.he_ar()
+            pes = H.PESMIN_CO2AR if co2 else H.PESMIN_HEAR
+            T = float(rng.uniform(50.0, 500.0))
+            npts = int(10 ** rng.uniform(6, 7))
+            smp = H.Sampler(H.IC_REJECTION, H.PAIR_FREE, T, 4.0, 40.0, pes, 0.0)
+            reqs.append(("m0", sysd, smp, npts, 16384))
+    torch.cuda.synchronize()
+    s = H.Session(eng, slice_ms=10.0)
+    ids = []
+    t0 = time.perf_counter()
+    for i, (ta, r) in enumerate(zip(t_arr, reqs)):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        if r[0] == "cf":
+            rid, _ = s.submit_cf(r[1], r[2], r[3])
+        else:
+            rid = s.submit_m0(r[1], r[2], 7 + i, r[3], r[4])
+        ids.append((rid, r[0], r[4] if r[0] == "cf" else r[3]))
+    infos = [(kind, n, s.poll(rid, 600000.0)) for rid, kind, n in ids]
+    wall = time.perf_counter() - t0
+    s.close()
+    lat = np.array([inf.latency_ms for _, _, inf in infos if inf.state == H.REQ_DONE])
+    lat_cf = np.array([inf.latency_ms for k, _, inf in infos if k == "cf" and inf.state == H.REQ_DONE])
+    failed = sum(1 for _, _, inf in infos if inf.state != H.REQ_DONE)
+    traj = sum(n for k, n, inf in infos if k == "cf" and inf.state == H.REQ_DONE)
+    return {"requests": len(infos), "cf_requests": int(sum(1 for k, _, _ in infos if k == "cf")),
+            "m0_requests": int(sum(1 for k, _, _ in infos if k == "m0")), "failed": failed,
+            "arrival_rate_hz": rate_hz, "arrival_window_s": seconds, "wall_s": wall,
+            "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
+            "p50_ms_cf": float(np.percentile(lat_cf, 50)) if lat_cf.size else None,
+            "p99_ms_cf": float(np.percentile(lat_cf, 99)) if lat_cf.size else None,
+            "aggregate_traj_per_s": traj / wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--ntraj", type=int, default=65536)
     ap.add_argument("--L", type=int, default=4096)
     ap.add_argument("--rtol-mixed", type=float, default=1e-10)
     ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--no-oracle", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: the trajectory ensemble of this path is measured on "
-              "one MI355X (multi-GPU sharding is a design note, not a measured configuration)", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: this engine is measured on one MI355X; running the "
+              "ensemble over several GPUs is a design note (trajectories shard by index), not a "
+              "measured configuration", file=sys.stderr)
         sys.exit(2)
 
     import torch
     import hawaiihybrid_amd as H
 
     eng = H.Engine(0)
-    info = eng.info()
-    log("device", info)
+    log("device", eng.info())
+    if args.calibrate_copy:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     sysd = H.co2_ar()
     smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
-    nb = args.warmup + args.steps
-    n = args.ntraj
+    W, K, n, L = args.warmup, args.steps, args.ntraj, args.L
     t0 = time.time()
-    ic_all, _, _ = H.sample_ics(sysd, smp, 42, nb * n)
-    log(f"sampled {nb * n} initial conditions in {time.time() - t0:.2f} s (host, bit-exact stream)")
-    ics = [eng.to_device_soa(ic_all[b * n:(b + 1) * n]) for b in range(nb)]
-    warm, timed = ics[:args.warmup], ics[args.warmup:]
+    ic_all, _, _ = H.sample_ics(sysd, smp, 42, (W + K) * n)
+    log(f"sampled {(W + K) * n} initial conditions in {time.time() - t0:.2f} s (host, reference stream)")
+    ics = [eng.to_device_soa(ic_all[b * n:(b + 1) * n]) for b in range(W + K)]
+    warm, timed = ics[:W], ics[W:]
     torch.cuda.synchronize()
 
-    p_plain = H.cf_params(args.L, rtol=1e-12, engine=H.ENGINE_PLAIN, precision=H.PREC_FP64)
-    p_mixed = H.cf_params(args.L, rtol=args.rtol_mixed, engine=H.ENGINE_BATCHED, precision=H.PREC_MIXED)
+    p_plain = H.cf_params(L, rtol=1e-12, engine=H.ENGINE_PLAIN, precision=H.PREC_FP64)
+    p_mixed = H.cf_params(L, rtol=args.rtol_mixed, engine=H.ENGINE_BATCHED, precision=H.PREC_MIXED)
+    p_b64 = H.cf_params(L, rtol=1e-12, engine=H.ENGINE_BATCHED, precision=H.PREC_FP64)
 
-    # plain FP64: one launch per batch
+    # plain FP64: one launch per step
     for x in warm:
         eng.cf_run(sysd, p_plain, [x])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    plain_cf0 = None
-    for i, x in enumerate(timed):
+    plain_k_ms = []
+    for x in timed:
         out, st = eng.cf_run(sysd, p_plain, [x])
-        if i == 0:
-            plain_cf0 = out[0].cpu().numpy()
-            plain_items0 = eng.items(0, n, args.L, rows=True)
+        plain_k_ms.append(st.integrate_ms)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    log(f"plain fp64: {t_plain * 1e3:.1f} ms for {args.steps} steps -> {args.steps * n / t_plain:.0f} traj/s")
+    log(f"plain fp64, one launch per step: {t_plain * 1e3:.1f} ms for {K} steps -> {K * n / t_plain:.0f} traj/s")
 
-    # batched mixed: all timed batches through one persistent launch
+    # plain FP64 with every timed step in one launch (for comparison)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.cf_run(sysd, p_plain, timed)
+    torch.cuda.synchronize()
+    t_plain_one = time.perf_counter() - t0
+    log(f"plain fp64, all steps in one launch: {t_plain_one * 1e3:.1f} ms")
+
+    # batched FP64 (same tolerance as plain)
+    if warm:
+        eng.cf_run(sysd, p_b64, warm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, st64 = eng.cf_run(sysd, p_b64, timed)
+    torch.cuda.synchronize()
+    t_b64 = time.perf_counter() - t0
+    log(f"batched fp64: {t_b64 * 1e3:.1f} ms")
+
+    # batched mixed: the headline
     if warm:
         eng.cf_run(sysd, p_mixed, warm)
     torch.cuda.synchronize()
@@ -92,77 +196,98 @@ def main():
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     mixed_cf0 = outs[0].cpu().numpy()
-    mixed_items0 = eng.items(0, n, args.L, rows=True)
-    log(f"batched mixed: {t_mixed * 1e3:.1f} ms for {args.steps} steps -> {args.steps * n / t_mixed:.0f} traj/s; "
-        f"kernel {stm.integrate_ms:.1f} ms, reduce {stm.reduce_ms:.2f} ms, nrhs {stm.nrhs}, grid {stm.grid}x{stm.block}")
+    mixed_items0 = eng.items(0, n, L, rows=True)
+    # per-trajectory rows of the plain engine for the first timed step (outside the timings)
+    out, _ = eng.cf_run(sysd, p_plain, [timed[0]])
+    plain_cf0 = out[0].cpu().numpy()
+    plain_items0 = eng.items(0, n, L, rows=True)
+    log(f"batched mixed: {t_mixed * 1e3:.1f} ms for {K} steps -> {K * n / t_mixed:.0f} traj/s; kernel "
+        f"{stm.integrate_ms:.1f} ms in {stm.launches} slices, reduce {stm.reduce_ms:.2f} ms, nrhs {stm.nrhs}")
 
-    value = args.steps * n / t_mixed
-    plain_rate = args.steps * n / t_plain
+    value = K * n / t_mixed
+    plain_rate = K * n / t_plain
 
-    # roofline of the persistent kernel (FP64 vector pipe: state, error control, EOM)
+    # roofline of the pool kernel (FP64 vector ALU; the surfaces in FP32 are counted at the
+    # same weight, so the fraction is an upper bound of the FP64-equivalent rate)
     fl = json.load(open(os.path.join(HERE, "hawaiihybrid_amd", "data", "flops.json")))["co2ar"]
     nsteps = max(stm.nrhs - 2 * stm.n_items, 0) / 13.0
     flops = stm.nrhs * fl["rhs"] + stm.nsamples * (fl["dip"] + fl["sample"]) + nsteps * fl["step"]
-    achieved = flops / (stm.integrate_ms * 1e-3) / 1e12
-    peak = 78.6
+    launch_ms = stm.integrate_ms / max(stm.launches, 1)
+    achieved = flops / max(stm.launches, 1) / (launch_ms * 1e-3) / 1e12
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "traffic_persistent_cf_co2ar.json")
+    tpath = os.path.join(HERE, "profiles", "pool_cf_kernel_co2ar_mixed_traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
-    roofline = {"bound": "mfma", "pipe": "fp64 vector ALU (no matrix work in this kernel)", "achieved": achieved,
-                "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
-                "kernel": "persistent_cf_kernel<CO2-Ar, float>", "launch_ms": stm.integrate_ms,
-                "algorithmic_flop": flops}
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MI355X_FP64_VECTOR_TFLOPS, "traffic": traffic,
+                "kernel": "pool_cf_kernel<CO2-Ar, float, 256>", "pipe": "FP64 vector ALU (no matrix work)",
+                "launches": stm.launches, "avg_launch_ms": launch_ms,
+                "algorithmic_flop_per_launch": flops / max(stm.launches, 1),
+                "flop_model": "hawaiihybrid_amd/data/flops.json: nrhs*rhs + nsamples*(dip+sample) + steps*step"}
 
     result = {
         "metric": "CO2-Ar traj/s batched vs plain GPU at 1e-4 CF rel err; p50/p99 request latency",
-        "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": t_mixed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64 state + f32 surfaces", "data": "synthetic (seeded initial conditions)",
+        "value": value, "unit": "trajectories/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": t_mixed / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 state and error control, f32 surfaces",
+        "data": "synthetic: seeded initial conditions from the reference Mersenne Twister stream",
         "config": {"workload": f"CO2-Ar dipole correlation function, {n} trajectories per step, T=200 K, "
-                               f"R0=Rcut=40 bohr (incoming), L={args.L}, dt=200 a.u.",
-                   "ntraj_per_step": n, "L": args.L, "rtol_mixed": args.rtol_mixed, "rtol_plain": 1e-12},
+                               f"R0=Rcut=40 bohr incoming, L={L}, dt=200 a.u.",
+                   "ntraj_per_step": n, "L": L, "rtol_batched_mixed": args.rtol_mixed, "rtol_plain": 1e-12},
         "plain_fp64_traj_per_s": plain_rate, "speedup_vs_plain": value / plain_rate,
+        "plain_definition": "FP64 rtol=atol=1e-12, one thread per trajectory, one launch per step",
+        "plain_fp64_one_launch_traj_per_s": K * n / t_plain_one,
+        "speedup_vs_plain_one_launch": value / (K * n / t_plain_one),
+        "batched_fp64_traj_per_s": K * n / t_b64,
+        "plain_fp64_kernel_ms_per_step": float(np.mean(plain_k_ms)),
         "roofline": roofline,
     }
 
+    if not args.no_serve:
+        # offered load ~0.7 of the measured capacity (mean CF request 2^U[10,16] ~ 15.5k trajectories)
+        rate = max(1.0, 0.7 * value / 15512.0 * 0.8)
+        log(f"serving mix: Poisson arrivals at {rate:.1f} requests/s for {args.serve_seconds:.0f} s")
+        sv = serving_mix(H, eng, rate, args.serve_seconds, 5)
+        log("serving", sv)
+        result["serving"] = sv
+
     if not args.no_oracle:
+        import ctypes as C
         import oracle_lib as O
         cores = min(16, os.cpu_count() or 1)
-        ic0 = ic_all[args.warmup * n:(args.warmup + 1) * n]
+        ic0 = ic_all[W * n:(W + 1) * n]
         t0 = time.time()
-        ref = O.cf_rows(O.co2ar(), O.CFParams(1e-12, 1e-12, 200.0, args.L, 40.0, 1, 50000), ic0, cores)
+        ref = O.cf_rows(O.co2ar(), O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000), ic0, cores)
         t_or = time.time() - t0
         cf_ref = O.ensemble_from_rows(ref["crln"])
-        alu3 = 5.29177210903e-11 ** 3
-
-        def rel(a, b):
-            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        # trajectories where two FP64 integrations (GPU plain vs CPU oracle) disagree are chaotic
-        d = np.linalg.norm(plain_items0["rows"] * alu3 - ref["crln"], axis=1) / np.maximum(
-            np.linalg.norm(ref["crln"], axis=1), 1e-300)
-        regular = d < 1e-6
-        mrows = mixed_items0["rows"] * alu3
+        prow = plain_items0["rows"] * ALU3
+        mrow = mixed_items0["rows"] * ALU3
+        dn = np.maximum(np.linalg.norm(ref["crln"], axis=1), 1e-300)
+        dp = np.linalg.norm(prow - ref["crln"], axis=1) / dn
+        dm = np.linalg.norm(mrow - ref["crln"], axis=1) / dn
+        # trajectories on which two FP64 integrations (GPU plain, CPU oracle) already disagree
+        # beyond 1e-6 are chaotic: their rows depend on the last bit of the arithmetic
+        regular = dp < 1e-6
         result["accuracy_vs_fp64_oracle"] = {
-            "batch": "first timed batch", "cf_rel_l2_plain_fp64": rel(plain_cf0, cf_ref),
-            "cf_rel_l2_batched_mixed": rel(mixed_cf0, cf_ref),
+            "step": "first timed step", "oracle": "CPU DOP853 FP64 rtol=atol=1e-12 (oracle/)",
+            "cf_rel_l2_batched_mixed": rel(mixed_cf0, cf_ref), "cf_rel_l2_plain_fp64": rel(plain_cf0, cf_ref),
             "chaotic_trajectories": int((~regular).sum()),
-            "cf_rel_l2_batched_mixed_regular": rel(mrows[regular].sum(0), ref["crln"][regular].sum(0)),
-            "cf_rel_l2_plain_fp64_regular": rel((plain_items0["rows"] * alu3)[regular].sum(0),
-                                                ref["crln"][regular].sum(0)),
-        }
+            "cf_rel_l2_batched_mixed_regular": rel(mrow[regular].sum(0), ref["crln"][regular].sum(0)),
+            "cf_rel_l2_plain_fp64_regular": rel(prow[regular].sum(0), ref["crln"][regular].sum(0)),
+            "per_traj_rel_l2_batched_mixed_median": float(np.median(dm)),
+            "per_traj_rel_l2_batched_mixed_p99": float(np.quantile(dm, 0.99)),
+            "max_energy_drift_hartree": float(stm.max_e_drift)}
         log("accuracy", result["accuracy_vs_fp64_oracle"], f"(oracle {t_or:.1f} s on {cores} threads)")
-        # CPU baseline: the oracle on a bounded prefix of the batch
         m = min(args.cpu_sample, n)
-        cf = np.zeros(args.L)
+        cf = np.zeros(L)
         t0 = time.time()
-        O.lib().orc_cf_ensemble(C.byref(O.co2ar()), C.byref(O.CFParams(1e-12, 1e-12, 200.0, args.L, 40.0, 1, 50000)),
+        O.lib().orc_cf_ensemble(C.byref(O.co2ar()), C.byref(O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000)),
                                 O.ptr(np.ascontiguousarray(ic0[:m])), m, cores, 1.0, O.ptr(cf), None)
         t_cpu = time.time() - t0
         result["cpu_baseline"] = {"value": m / t_cpu, "unit": "trajectories/s", "cores": cores, "kind": "port",
-                                  "sample": f"first {m} trajectories of the first timed batch, FP64 oracle "
-                                            f"(same DOP853 at rtol=atol=1e-12), {cores} threads, {t_cpu:.1f} s"}
-    print(json.dumps(result))
+                                  "sample": f"first {m} trajectories of the first timed step, CPU oracle "
+                                            f"(DOP853 FP64, rtol=atol=1e-12), {cores} threads, {t_cpu:.1f} s"}
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

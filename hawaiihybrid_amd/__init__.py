@@ -46,7 +46,8 @@ class CFParams(C.Structure):
 class CFStats(C.Structure):
     _fields_ = [("integrate_ms", C.c_double), ("reduce_ms", C.c_double), ("n_items", C.c_longlong),
                 ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
-                ("n_stepfail", C.c_longlong), ("max_e_drift", C.c_double), ("block", C.c_int), ("grid", C.c_int)]
+                ("n_stepfail", C.c_longlong), ("max_e_drift", C.c_double), ("block", C.c_int), ("grid", C.c_int),
+                ("launches", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -309,11 +309,12 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
             mine.push_back(pj);
         }
         double kms = 0.0, rms = 0.0;
+        int launches = 0;
         for (;;) {
             bool all = true;
             for (hh::PoolJob* pj : mine) all &= pj->state == 2;
             if (all) break;
-            rc = hh::pool_step(e, e->pools, 20.0, &kms, &rms);
+            rc = hh::pool_step(e, e->pools, 20.0, &kms, &rms, &launches);
             if (rc < 0) {
                 // drop the pool with its state; the next call builds a fresh one
                 (void)hipStreamSynchronize(e->stream);
@@ -334,6 +335,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
             stats->reduce_ms = rms;
             stats->block = block;
             stats->grid = (int)(pl->S / block);
+            stats->launches = launches;
             return hh::collect_stats(e->last_jobs, stats);
         }
         return HH_OK;
@@ -403,6 +405,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         stats->reduce_ms = b;
         stats->block = 256;
         stats->grid = grid;
+        stats->launches = 1;
         return hh::collect_stats(jobs, stats);
     }
     return HH_OK;
@@ -421,13 +424,15 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
     if (nrhs) HH_CHECK(hipMemcpy(nrhs, jb.nrhs, n * sizeof(int), hipMemcpyDeviceToHost));
     if (e_drift) HH_CHECK(hipMemcpy(e_drift, jb.e_drift, n * sizeof(float), hipMemcpyDeviceToHost));
     if (rows) {
+        // one copy of the whole block, then zero what lies past each item's length (never written
+        // on the device)
         std::vector<int> ln(n);
         HH_CHECK(hipMemcpy(ln.data(), jb.len, n * sizeof(int), hipMemcpyDeviceToHost));
-        std::memset(rows, 0, n * (size_t)jb.L * sizeof(double));
-        for (size_t i = 0; i < n; ++i)
-            if (ln[i] > 0)
-                HH_CHECK(hipMemcpy(rows + i * jb.L, jb.rows + i * (size_t)jb.L, ln[i] * sizeof(double),
-                                   hipMemcpyDeviceToHost));
+        HH_CHECK(hipMemcpy(rows, jb.rows, n * (size_t)jb.L * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) {
+            const size_t from = (size_t)std::max(0, std::min(ln[i], jb.L));
+            std::memset(rows + i * jb.L + from, 0, (jb.L - from) * sizeof(double));
+        }
     }
     return HH_OK;
 }

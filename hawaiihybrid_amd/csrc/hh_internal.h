@@ -145,7 +145,8 @@ bool pool_has_room(const LanePool& pl);
 // one slice of every pool with work, then reductions of the jobs whose items are all done
 // and repacking of drained pools; synchronises the engine stream.  Jobs whose CF is final
 // have state 2 afterwards.  kernel_ms / reduce_ms accumulate device time.
-int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms);
+int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms,
+              int* launches = nullptr);
 // argument checks shared by hh_cf_run and hh_submit_cf
 int check_cf_args(const hh_system* sys, const hh_cf_params* p);
 // per-item summaries (items, RHS, samples, failures, drift) of jobs, added into stats

@@ -132,7 +132,7 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     return HH_OK;
 }
 
-int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms)
+int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms, int* launches)
 {
     // 1. one slice of every pool that still integrates
     std::vector<LanePool*> run;
@@ -156,6 +156,7 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             SysConst sc{pl->system.mu, pl->system.I, mixed ? e->tab32.p : e->tab64.p};
             HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
                               pl->arena.as<double>(), e->stream));
+            if (launches) ++*launches;
         }
         HH_CHECK(hipEventRecord(e->ev[1], e->stream));
         HH_CHECK(hipStreamSynchronize(e->stream));

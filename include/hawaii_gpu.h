@@ -122,6 +122,7 @@ typedef struct {
                                limit (CVode's CV_TOO_MUCH_WORK) counts as zero (src/hawaii.c:3001) */
     double max_e_drift;     /* max |H(end) - H(0)|, Hartree */
     int block, grid;        /* launch shape used */
+    int launches;           /* integration kernel launches (time slices of the batched engine) */
 } hh_cf_stats;
 
 /* Run `njobs` correlation-function jobs of one system on the device.  ic_dev[j] holds

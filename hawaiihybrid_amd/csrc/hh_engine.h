@@ -27,7 +27,8 @@ struct JobDesc {
     long long item_begin;    // first global item index of this job
     long long n_items;       // ntraj * halves
     int* done;               // finished-item counter of the job (nullptr: not counted)
-    const int* order;        // queue position -> item index (nullptr: identity)
+    const int* order;        // admission group's order: the item run at queue position g is
+    long long order_base;    //   order_base + order[g - order_base] (nullptr: the item g itself)
     int halves;              // 1: forward only, 2: forward + momentum-inverted backward
     int L;                   // MaxTrajectoryLength
     int max_steps;           // max accepted steps between two samples

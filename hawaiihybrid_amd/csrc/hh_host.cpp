@@ -308,6 +308,11 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
             }
             mine.push_back(pj);
         }
+        rc = hh::pool_order_group(e, *pl);
+        if (rc < 0) {
+            while (!pl->inflight.empty()) pl->inflight.pop_back();
+            return rc;
+        }
         double kms = 0.0, rms = 0.0;
         int launches = 0;
         for (;;) {

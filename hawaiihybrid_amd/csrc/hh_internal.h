@@ -26,7 +26,7 @@ hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const Jo
 hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st);
 hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st);
 hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long long n, double* out, hipStream_t st);
-hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, hipStream_t st);
+hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st);
 size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
@@ -90,6 +90,7 @@ struct PoolJob {
     int state = 0;              // 0 integrating, 1 reduced (launched), 2 complete
     uint64_t request = 0;       // serving request id (0: synchronous run)
     size_t bytes = 0;           // device bytes held by `buf`
+    std::shared_ptr<DevBuf> group_order;   // order of the admission group this job belongs to
 };
 
 // A resident lane pool for one (system, precision, block) class: S = grid * block lane slots
@@ -109,6 +110,7 @@ struct LanePool {
     long long next_seq = 0;
     unsigned long long total = 0;   // items admitted
     bool drained_repacked = false;
+    long long unordered_seq = 0;    // first job not yet ordered (pool_order_group)
     LaneSlots view(int which) const { return slots_view(sys, slots[which].p, S); }
 };
 
@@ -140,6 +142,9 @@ size_t pool_job_bytes(const hh_cf_params& p, long ntraj);
 // pool's ring and appends its items to the pool's queue; *out (optional) sees the job
 int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* ic_dev, long ntraj, double* cf_dev,
                uint64_t request, PoolJob** out);
+// hand out the items of the jobs admitted since the last call (a group) longest-expected-first
+// across the whole group; call after admitting them and before the next pool_step
+int pool_order_group(hh_engine* e, LanePool& pl);
 // true when the ring has room for one more job
 bool pool_has_room(const LanePool& pl);
 // one slice of every pool with work, then reductions of the jobs whose items are all done

@@ -31,8 +31,7 @@ namespace hh {
 
 constexpr double kEPS = 2.220446049250313e-16;
 constexpr int kMaxTotalSteps = 400000;   // attempts per item; the longest CO2-Ar items take ~6e3
-constexpr int kSlots = 23;               // K0..K15, F0..F6
-constexpr int kSlotF = 16;
+constexpr int kSlots = 16;               // K0..K15
 #ifndef HH_MIN_WAVES
 #define HH_MIN_WAVES 2   // occupancy target (waves per SIMD) of the trajectory kernels
 #endif
@@ -332,14 +331,18 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
         ln.nrhs += 3;
         arena_fence();
         // F0 = dy, F1 = h K0 - dy, F2 = 2 dy - h (K12 + K0), F3..F6 = h (D K), for the
-        // coordinates only (even state indices): the dipole and the R > Rcut test need no momenta
+        // coordinates only (even state indices: the dipole and the R > Rcut test need no
+        // momenta), kept in registers for all samples of the step
+        constexpr int NQ = N / 2;
+        double Fc[7][NQ];
 #pragma unroll
-        for (int i = 0; i < N; i += 2) {
+        for (int c = 0; c < NQ; ++c) {
+            const int i = 2 * c;
             const double dy = yn[i] - ln.y[i];
             const double k0 = ar.ld(0, i);
-            ar.st(kSlotF + 0, i, dy);
-            ar.st(kSlotF + 1, i, h * k0 - dy);
-            ar.st(kSlotF + 2, i, 2.0 * dy - h * (ar.ld(12, i) + k0));
+            Fc[0][c] = dy;
+            Fc[1][c] = h * k0 - dy;
+            Fc[2][c] = 2.0 * dy - h * (ar.ld(12, i) + k0);
         }
         {
             double f[4][N];
@@ -347,11 +350,10 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
 #pragma unroll
             for (int d = 0; d < 4; ++d)
 #pragma unroll
-                for (int i = 0; i < N; i += 2) ar.st(kSlotF + 3 + d, i, h * f[d][i]);
+                for (int c = 0; c < NQ; ++c) Fc[3 + d][c] = h * f[d][2 * c];
         }
         double* row = jb.rows + ln.local * (long long)jb.L;
         while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
-            arena_fence();
             const double tk = ln.k * jb.dt;
             double ys[N];
             if (tk == tnew) {
@@ -362,14 +364,14 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
 #pragma unroll
                 for (int i = 1; i < N; i += 2) ys[i] = 0.0;
 #pragma unroll
-                for (int i = 0; i < N; i += 2) {
-                    double v = ar.ld(kSlotF + 5, i) + x * ar.ld(kSlotF + 6, i);
-                    v = ar.ld(kSlotF + 4, i) + omx * v;
-                    v = ar.ld(kSlotF + 3, i) + x * v;
-                    v = ar.ld(kSlotF + 2, i) + omx * v;
-                    v = ar.ld(kSlotF + 1, i) + x * v;
-                    v = ar.ld(kSlotF + 0, i) + omx * v;
-                    ys[i] = ln.y[i] + x * v;
+                for (int c = 0; c < NQ; ++c) {
+                    double v = Fc[5][c] + x * Fc[6][c];
+                    v = Fc[4][c] + omx * v;
+                    v = Fc[3][c] + x * v;
+                    v = Fc[2][c] + omx * v;
+                    v = Fc[1][c] + x * v;
+                    v = Fc[0][c] + omx * v;
+                    ys[2 * c] = ln.y[2 * c] + x * v;
                 }
             }
             P m[3];
@@ -564,10 +566,14 @@ __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, 
                 if (__hip_atomic_load(ctl.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= ctl.total) break;
                 const unsigned long long g = atomicAdd(ctl.next, 1ull);
                 if (g >= ctl.total) break;
-                const int j = ring_job(ring, ctl, g);
-                const long long pos = (long long)g - ring[j].item_begin;
-                const long long local = ring[j].order ? (long long)ring[j].order[pos] : pos;
-                got = init_item<SYS, P, N>(scl, ring[j], j, local, ar, ln);
+                int j = ring_job(ring, ctl, g);
+                unsigned long long item = g;
+                if (ring[j].order) {
+                    item = (unsigned long long)(ring[j].order_base +
+                                                ring[j].order[(long long)g - ring[j].order_base]);
+                    j = ring_job(ring, ctl, item);
+                }
+                got = init_item<SYS, P, N>(scl, ring[j], j, (long long)item - ring[j].item_begin, ar, ln);
             }
             if (!got) break;
         }
@@ -583,7 +589,8 @@ __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, 
 // the order of work changes; every item's result is the same.
 __global__ void __launch_bounds__(256) order_keys_kernel(const double* __restrict__ ic, long long ntraj, int halves,
                                                          double mu, double Rcut, double dt, int L,
-                                                         float* __restrict__ keys, int* __restrict__ idx)
+                                                         float* __restrict__ keys, int* __restrict__ idx,
+                                                         int idx_offset)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     if (i >= ntraj * halves) return;
@@ -601,7 +608,7 @@ __global__ void __launch_bounds__(256) order_keys_kernel(const double* __restric
         key = fmin(tx / dt, (double)(L - 1));
     }
     keys[i] = (float)key;
-    idx[i] = (int)i;
+    idx[i] = idx_offset + (int)i;
 }
 
 // Batched surface evaluation at phase points y (SoA [QP][n], FP64): what=0 the potential
@@ -870,11 +877,13 @@ hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long
     return hipGetLastError();
 }
 
-hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, hipStream_t st)
+// keys and item numbers of job jb's items, written at keys[0..n_items) and idx[0..n_items) with
+// item numbers counted from idx_offset
+hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st)
 {
     const long long n = jb.n_items;
     hipLaunchKernelGGL(order_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, jb.ic, jb.ntraj,
-                       jb.halves, mu, jb.Rcut, jb.dt, jb.L, keys, idx);
+                       jb.halves, mu, jb.Rcut, jb.dt, jb.L, keys, idx, idx_offset);
     return hipGetLastError();
 }
 

@@ -12,6 +12,8 @@
 // Every item's arithmetic is independent of the slice boundaries and of its neighbours, and
 // each job's reduction has a fixed order, so a job's CF is the same bits alone or in a mix.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <hipcub/hipcub.hpp>
@@ -60,7 +62,7 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj)
 {
     const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
-    // rows, len, status, nrhs, e_drift, order
+    // rows, len, status, nrhs, e_drift, and the job's share of its group's order
     return (size_t)items * ((size_t)p.L * sizeof(double) + 4 * sizeof(int) + sizeof(float));
 }
 
@@ -103,25 +105,8 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     jd.Rcut = p.Rcut;
     jd.rtol = p.rtol;
     jd.atol = p.atol;
-    int* order = reinterpret_cast<int*>(jd.e_drift + n_items);
-    {
-        // longest-expected-first order of the job's items (order_keys_kernel)
-        size_t tmp_bytes = 0;
-        HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, (const float*)nullptr,
-                                                              (float*)nullptr, (const int*)nullptr, (int*)nullptr,
-                                                              (int)n_items, 0, 32, e->stream));
-        const size_t kv = (size_t)n_items * (2 * sizeof(float) + sizeof(int));
-        if (pl.sort_tmp.ensure(kv + tmp_bytes + 256) != hipSuccess)
-            return fail(HH_E_NOMEM, "cannot allocate the ordering workspace");
-        float* keys_in = pl.sort_tmp.as<float>();
-        float* keys_out = keys_in + n_items;
-        int* idx_in = reinterpret_cast<int*>(keys_out + n_items);
-        void* tmp = reinterpret_cast<char*>(pl.sort_tmp.p) + ((kv + 255) / 256) * 256;
-        HH_CHECK(run_order_keys(jd, pl.system.mu, keys_in, idx_in, e->stream));
-        HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, keys_in, keys_out, idx_in, order,
-                                                              (int)n_items, 0, 32, e->stream));
-        jd.order = order;
-    }
+    jd.order = nullptr;
+    jd.order_base = 0;
     HH_CHECK(hipMemcpyAsync(pl.ring.as<JobDesc>() + slot, &job->jd, sizeof(JobDesc), hipMemcpyHostToDevice, e->stream));
     HH_CHECK(hipMemcpyAsync(jd.done, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
     pl.total += (unsigned long long)n_items;
@@ -129,6 +114,44 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     pl.drained_repacked = false;
     if (out) *out = job.get();
     pl.inflight.push_back(std::move(job));
+    return HH_OK;
+}
+
+int pool_order_group(hh_engine* e, LanePool& pl)
+{
+    std::vector<PoolJob*> group;
+    for (auto& j : pl.inflight)
+        if (j->seq >= pl.unordered_seq) group.push_back(j.get());
+    pl.unordered_seq = pl.next_seq;
+    if (group.empty()) return HH_OK;
+    const long long base = group.front()->jd.item_begin;
+    const long long n = group.back()->jd.item_begin + group.back()->jd.n_items - base;
+    if (n > 0x7fffffffLL) return fail(HH_E_INVALID, "admission group of %lld items is too large", n);
+    auto order = std::make_shared<DevBuf>();
+    size_t tmp_bytes = 0;
+    HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp_bytes, (const float*)nullptr, (float*)nullptr,
+                                                          (const int*)nullptr, (int*)nullptr, (int)n, 0, 32,
+                                                          e->stream));
+    const size_t kv = (size_t)n * (2 * sizeof(float) + sizeof(int));
+    if (order->ensure((size_t)n * sizeof(int)) != hipSuccess || pl.sort_tmp.ensure(kv + tmp_bytes + 256) != hipSuccess)
+        return fail(HH_E_NOMEM, "cannot allocate the ordering workspace");
+    float* keys_in = pl.sort_tmp.as<float>();
+    float* keys_out = keys_in + n;
+    int* idx_in = reinterpret_cast<int*>(keys_out + n);
+    void* tmp = reinterpret_cast<char*>(pl.sort_tmp.p) + ((kv + 255) / 256) * 256;
+    for (PoolJob* j : group) {
+        const long long off = j->jd.item_begin - base;
+        HH_CHECK(run_order_keys(j->jd, pl.system.mu, keys_in + off, idx_in + off, (int)off, e->stream));
+    }
+    HH_CHECK(hipcub::DeviceRadixSort::SortPairsDescending(tmp, tmp_bytes, keys_in, keys_out, idx_in, order->as<int>(),
+                                                          (int)n, 0, 32, e->stream));
+    for (PoolJob* j : group) {
+        j->group_order = order;
+        j->jd.order = order->as<int>();
+        j->jd.order_base = base;
+        HH_CHECK(hipMemcpyAsync(pl.ring.as<JobDesc>() + j->seq % pl.ring_cap, &j->jd, sizeof(JobDesc),
+                                hipMemcpyHostToDevice, e->stream));
+    }
     return HH_OK;
 }
 
@@ -166,6 +189,7 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
     }
 
     // 2. finished jobs -> reductions; drained pools -> repack
+    static const bool trace = std::getenv("HH_POOL_TRACE") != nullptr;   // progress lines on stderr
     bool launched = false;
     HH_CHECK(hipEventRecord(e->ev[2], e->stream));
     for (LanePool* pl : run) {
@@ -201,6 +225,15 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             HH_CHECK(run_repack(pl->sys, pl->view(pl->cur), pl->view(1 - pl->cur), pl->scratch.as<int>(), e->stream));
             pl->cur = 1 - pl->cur;
             launched = true;
+            if (trace) {
+                int active = 0;
+                HH_CHECK(hipMemcpy(&active, pl->scratch.as<int>() + pl->S / 256, sizeof(int), hipMemcpyDeviceToHost));
+                std::fprintf(stderr, "[hh pool] sys %d %s: queue drained, %d items in flight\n", pl->sys,
+                             pl->mixed ? "mixed" : "fp64", active);
+            }
+        } else if (trace && integrating) {
+            std::fprintf(stderr, "[hh pool] sys %d %s: %llu of %llu items handed out\n", pl->sys,
+                         pl->mixed ? "mixed" : "fp64", next, pl->total);
         }
     }
     HH_CHECK(hipEventRecord(e->ev[3], e->stream));

@@ -167,6 +167,15 @@ void hh_session::run()
             in_use += rr->bytes;
             running.push_back(std::move(rr));
         }
+        {
+            // the CF jobs admitted in this round form one group: longest-expected-first across it
+            std::lock_guard<std::mutex> el(e->mu);
+            for (auto& pl : pools)
+                if (hh::pool_order_group(e, *pl) < 0) {
+                    // ordering is an optimisation: on failure the items run in queue order
+                    pl->unordered_seq = pl->next_seq;
+                }
+        }
         cv.notify_all();
         if (running.empty()) {
             if (stop && waiting.empty()) break;

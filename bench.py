@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--ntraj", type=int, default=65536)
     ap.add_argument("--L", type=int, default=4096)
     ap.add_argument("--rtol-mixed", type=float, default=1e-10)
-    ap.add_argument("--cpu-sample", type=int, default=16384)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -278,14 +278,14 @@ def main():
             "per_traj_rel_l2_batched_mixed_p99": float(np.quantile(dm, 0.99)),
             "max_energy_drift_hartree": float(stm.max_e_drift)}
         log("accuracy", result["accuracy_vs_fp64_oracle"], f"(oracle {t_or:.1f} s on {cores} threads)")
-        m = min(args.cpu_sample, n)
+        m = min(args.cpu_sample, K * n)
         cf = np.zeros(L)
         t0 = time.time()
         O.lib().orc_cf_ensemble(C.byref(O.co2ar()), C.byref(O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000)),
-                                O.ptr(np.ascontiguousarray(ic0[:m])), m, cores, 1.0, O.ptr(cf), None)
+                                O.ptr(np.ascontiguousarray(ic_all[W * n:W * n + m])), m, cores, 1.0, O.ptr(cf), None)
         t_cpu = time.time() - t0
         result["cpu_baseline"] = {"value": m / t_cpu, "unit": "trajectories/s", "cores": cores, "kind": "port",
-                                  "sample": f"first {m} trajectories of the first timed step, CPU oracle "
+                                  "sample": f"first {m} trajectories of the timed steps, CPU oracle "
                                             f"(DOP853 FP64, rtol=atol=1e-12), {cores} threads, {t_cpu:.1f} s"}
     print(json.dumps(result), flush=True)
 

@@ -18,7 +18,9 @@ Engines, timed on the same K batches:
 by the plain engine's.  Also reported: the plain engine given all K steps in one launch, the
 batched engine in FP64, accuracy against the CPU oracle for the first timed step (FP64 DOP853
 at 1e-12), the roofline of the pool kernel, a serving run with Poisson arrivals (p50/p99
-request latency), and the CPU oracle timed on the host cores.  One JSON line on stdout.
+request latency), the N2-Ar PIP-NN surfaces on the matrix cores (2^20 phase points at
+100-500 K; BF16 and split-BF16 errors against the FP64 oracle), and the CPU oracle timed on
+the host cores.  One JSON line on stdout.
 """
 import argparse
 import json
@@ -109,6 +111,65 @@ def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
             "aggregate_traj_per_s": traj / wall}
 
 
+def n2ar_section(H, eng, npoints, sample_check):
+    """N2-Ar PIP-NN surfaces (the reference's trained PES and dipole models, and the synthetic
+    17 -> 3 x 128 tanh shape) at `npoints` phase points from q_generator + p_generator at 64
+    temperatures spread over 100-500 K: device time per evaluation, points/s, matrix-core
+    TFLOP/s of the network products, and the error against the FP64 oracle on a subsample."""
+    import ctypes as C
+    import torch
+    import oracle_lib as O
+    from hawaiihybrid_amd import n2ar
+    HT = 2.1947463136320e5
+    nchunk = 64
+    per = (npoints + nchunk - 1) // nchunk
+    pts = np.concatenate([n2ar.phase_points(per, 100.0 + 400.0 * (c + 0.5) / nchunk, seed=1000 + c)
+                          for c in range(nchunk)])[:npoints]
+    q = n2ar.lab_q(pts)
+    qd = torch.from_numpy(q).to("cuda:0")
+    W = n2ar.load_weights()
+    m, keep = O.n2ar_model(W, n2ar.PES_EN_INF)
+    idx = np.linspace(0, npoints - 1, sample_check).astype(np.int64)
+    refE = np.array([O.lib().orc_n2ar_pes(C.byref(m), O.ptr(np.ascontiguousarray(q[:, i]))) for i in idx]) * HT
+    refmu = np.zeros((3, len(idx)))
+    mu = np.zeros(3)
+    for j, i in enumerate(idx):
+        O.lib().orc_n2ar_dipole(C.byref(m), O.ptr(np.ascontiguousarray(q[:, i])), O.ptr(mu))
+        refmu[:, j] = mu
+    musc = np.max(np.linalg.norm(refmu, axis=0))
+    # algorithmic matrix-core FLOPs per point: PES 17-32-1; dipole 2 models x (119-16-2, 119-16-2, 70-8-1)
+    f_pes = 2 * (17 * 32 + 32)
+    f_dip = 2 * 2 * (2 * (119 * 16 + 16 * 2) + 70 * 8 + 8)
+    f_syn = 2 * (17 * 128 + 2 * 128 * 128 + 128)
+    out = {"points": npoints, "T_K": "64 temperatures evenly over 100-500", "check_points": len(idx)}
+    for prec, name in ((n2ar.MLP_BF16, "bf16"), (n2ar.MLP_BF16X3, "bf16x3")):
+        model = n2ar.PipNN(eng, precision=prec, weights=W)
+        model.eval(qd)                      # warm-up (code objects)
+        times = []
+        for _ in range(5):
+            E, MU, ms = model.eval(qd)
+            times.append(ms)
+        ms = float(np.median(times))
+        E = E.cpu().numpy()[idx] * HT
+        MU = MU.cpu().numpy()[:, idx]
+        dE = np.abs(E - refE)
+        out[name] = {"ms": ms, "points_per_s": npoints / (ms * 1e-3),
+                     "matrix_tflops": npoints * (f_pes + f_dip) / (ms * 1e-3) / 1e12,
+                     "dE_cm1_median": float(np.median(dE)), "dE_cm1_max": float(dE.max()),
+                     "dE_rel_max_above_100cm1": float(np.max((dE / np.abs(refE))[np.abs(refE) > 100.0])),
+                     "dmu_over_max_mu": float(np.max(np.abs(MU - refmu)) / musc)}
+        model.close()
+        syn = n2ar.synthetic_pes(128)
+        sm = n2ar.PipNN(eng, precision=prec, pes_layers=syn, pes_act=n2ar.ACT_TANH, with_dipole=False, weights=W)
+        sm.eval(qd, dipole=False)
+        times = [sm.eval(qd, dipole=False)[2] for _ in range(5)]
+        ms = float(np.median(times))
+        out[name]["synthetic_17_128x3_1_tanh"] = {"ms": ms, "points_per_s": npoints / (ms * 1e-3),
+                                                   "matrix_tflops": npoints * f_syn / (ms * 1e-3) / 1e12}
+        sm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +182,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-n2ar", action="store_true")
+    ap.add_argument("--n2ar-points", type=int, default=1 << 20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -250,6 +313,11 @@ def main():
         sv = serving_mix(H, eng, rate, args.serve_seconds, 5)
         log("serving", sv)
         result["serving"] = sv
+
+    if not args.no_n2ar:
+        log(f"N2-Ar PIP-NN surfaces: {args.n2ar_points} phase points")
+        result["n2ar_pipnn"] = n2ar_section(H, eng, args.n2ar_points, 2000 if not args.no_oracle else 64)
+        log("n2ar", result["n2ar_pipnn"])
 
     if not args.no_oracle:
         import ctypes as C

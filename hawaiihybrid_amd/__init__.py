@@ -12,8 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhawaii_gpu.so")
 
-SYS_HEAR, SYS_CO2AR = 0, 1
-IC_REJECTION, IC_R0_INJECT = 0, 1
+SYS_HEAR, SYS_CO2AR, SYS_N2AR, SYS_CH4CO2 = 0, 1, 2, 3
+IC_REJECTION, IC_R0_INJECT, IC_PHASE_SPACE = 0, 1, 2
 PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
 ENGINE_PLAIN, ENGINE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -26,10 +26,18 @@ MU_CO2AR = 3.817083069753919e4
 I_CO2 = 2.819827114123505e5
 PESMIN_HEAR = -9.424412560638486e-05
 PESMIN_CO2AR = -8.913727688689150e-04
+# src/constants.h:36-98: RAMTOAMU, m_N, m_Ar, L_N2, II_N2 = m_N/2 L_N2^2
+_RAMTOAMU = 1822.888485332
+_M_N = 14.003074004460 * _RAMTOAMU
+_M_AR = 39.9623831237 * _RAMTOAMU
+L_N2 = 2.07856749
+MU_N2AR = 2.0 * _M_N * _M_AR / (2.0 * _M_N + _M_AR)
+I_N2 = _M_N / 2.0 * L_N2 * L_N2
+PESMIN_N2AR = -97.3 / 2.1947463136320e5     # examples/correlation_n2_ar.cpp:214
 
 
 class System(C.Structure):
-    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("I", C.c_double)]
+    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("I", C.c_double), ("I_rotor", C.c_double * 3)]
 
 
 class Sampler(C.Structure):
@@ -75,8 +83,13 @@ def co2_ar():
     return System(SYS_CO2AR, MU_CO2AR, I_CO2)
 
 
+def n2_ar():
+    """N2-Ar (examples/correlation_n2_ar.cpp:184-186): mu = m_N2 m_Ar / (m_N2 + m_Ar), I = II_N2."""
+    return System(SYS_N2AR, MU_N2AR, I_N2)
+
+
 def qp_size(system):
-    return 10 if system == SYS_CO2AR else 6
+    return {SYS_HEAR: 6, SYS_CO2AR: 10, SYS_N2AR: 10, SYS_CH4CO2: 16}[system]
 
 
 class HHError(RuntimeError):
@@ -113,6 +126,11 @@ def lib():
             "hh_submit_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(C.c_uint64)]),
             "hh_poll": (i, [v, C.c_uint64, d, P(RequestInfo)]),
             "hh_session_destroy": (i, [v]),
+            "hh_pf_analytic": (d, [P(System), d]),
+            "hh_cf_moment": (i, [v, C.c_long, d, d, i, P(d)]),
+            "hh_pipnn_create": (i, [v, v, i, P(v)]),
+            "hh_pipnn_eval": (i, [v, v, C.c_long, v, v, P(d)]),
+            "hh_pipnn_destroy": (None, [v]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -142,6 +160,22 @@ def sample_ics(system, sampler, seed, ntraj, pes=None):
     n = _check(lib().hh_sample_ics(C.byref(system), C.byref(sampler), seed, ntraj, pes,
                                    ic.ctypes.data_as(C.c_void_p), C.byref(nd)))
     return ic, n, nd.value
+
+
+def pf_analytic(system, T):
+    """analytic_full_partition_function_by_V (hh_pf_analytic)."""
+    v = lib().hh_pf_analytic(C.byref(system), float(T))
+    if v < 0:
+        raise HHError(lib().hh_last_error().decode())
+    return v
+
+
+def cf_moment(cf, dt, n, ntraj=0.0):
+    """M0 (n = 0) or M2 (n = 2) of a host CF (hh_cf_moment); ntraj > 0 for an unnormalised CF."""
+    cf = np.ascontiguousarray(cf, dtype=np.float64)
+    out = C.c_double()
+    _check(lib().hh_cf_moment(cf.ctypes.data_as(C.c_void_p), len(cf), float(dt), float(ntraj), int(n), C.byref(out)))
+    return out.value
 
 
 def cf_params(L, dt=200.0, rtol=1e-12, atol=None, Rcut=40.0, ppf=1.0, halves=HALVES_FORWARD, engine=ENGINE_BATCHED,

@@ -25,6 +25,8 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [
     ("hh_kernels.hip", DEVICE),
     ("hh_aux.hip", DEVICE),
+    ("hh_pipnn.hip", DEVICE),
+    ("hh_moments.cpp", ["-x", "hip"] + DEVICE),
     ("hh_host.cpp", ["-x", "hip"] + DEVICE),
     ("hh_pool.cpp", ["-x", "hip"] + DEVICE),
     ("hh_session.cpp", ["-x", "hip"] + DEVICE),

@@ -180,17 +180,30 @@ extern "C" {
 
 const char* hh_last_error(void) { return hh::last_error(); }
 
-int hh_qp_size(int system) { return system == HH_SYS_CO2AR ? 10 : (system == HH_SYS_HEAR ? 6 : -1); }
+int hh_qp_size(int system)
+{
+    switch (system) {
+    case HH_SYS_HEAR: return 6;
+    case HH_SYS_CO2AR:
+    case HH_SYS_N2AR: return 10;
+    case HH_SYS_CH4CO2: return 16;
+    default: return -1;
+    }
+}
 
 long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
                    double* ic, long* n_draws)
 {
     if (!sys || !smp || !ic || ntraj < 0) return fail(HH_E_INVALID, "hh_sample_ics: null argument or ntraj < 0");
-    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
-        return fail(HH_E_UNSUPPORTED, "hh_sample_ics: unknown system %d", sys->system);
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_N2AR)
+        return fail(HH_E_UNSUPPORTED, "hh_sample_ics: system %d has no sampler", sys->system);
+    if (smp->mode != HH_IC_REJECTION && smp->mode != HH_IC_R0_INJECT && smp->mode != HH_IC_PHASE_SPACE)
+        return fail(HH_E_INVALID, "hh_sample_ics: unknown mode %d", smp->mode);
+    if (sys->system == HH_SYS_N2AR && smp->mode == HH_IC_REJECTION && !pes)
+        return fail(HH_E_UNSUPPORTED, "hh_sample_ics: N2-Ar rejection sampling needs the plugin's pes_lab");
     if (sys->mu <= 0) return fail(HH_E_INVALID, "invalid reduced mass (%.5e) -- must be positive", sys->mu);
     if (smp->T <= 0) return fail(HH_E_INVALID, "temperature must be positive");
-    if (smp->mode == HH_IC_REJECTION && !(smp->Rmin > 0 && smp->Rmax > smp->Rmin))
+    if (smp->mode != HH_IC_R0_INJECT && !(smp->Rmin > 0 && smp->Rmax > smp->Rmin))
         return fail(HH_E_INVALID, "sampler_Rmin/sampler_Rmax must satisfy 0 < Rmin < Rmax");
     if (smp->mode == HH_IC_REJECTION && !(smp->pesmin < 0 || smp->pesmin > 0))
         return fail(HH_E_INVALID, "pesmin must be non-zero in rejection mode");

@@ -1,0 +1,728 @@
+// N2-Ar PIP-NN surfaces on the matrix cores.
+//
+// Reference: PES-IDS/ai_pes_n2_ar_pip_nn.cpp:9-146 (Morse-like variables, 17-polynomial basis,
+// MLPES forward of PES-IDS/mlp.hpp:178-209, en_inf, HTOCM), PES-IDS/ai_ids_n2_ar_pip_nn.cpp:24-127
+// with PES-IDS/QModel.hpp:206-272 (two charge models on 308 polynomials, three subnets each,
+// q - q(inf), mu = sum q_a r_a, switch over 9.5..12.5 bohr), lab wrappers of
+// examples/correlation_n2_ar.cpp:11-75.
+//
+// One wavefront evaluates 64 phase points per block.  Every lane computes the polynomial
+// features of its own point in FP32 (pip_n2ar.h), 16 at a time (one k-block), standardises
+// them and stages them in LDS as BF16 (high, and low part in the split mode).  The network
+// runs with the points as the columns of v_mfma_f32_32x32x16_bf16 tiles: D[unit][point] =
+// W^T[unit][k] X^T[k][point], weights as the A operand (pre-swizzled 16-byte lane fragments
+// in global memory, L2-resident), features as the B operand.  The accumulator of one layer
+// feeds the next layer's B operand straight from registers: lane l holds units
+// (l/32)*4 + 8g + j (g, j < 4) of each 32-unit tile, so the host permutes the k index of every
+// layer after the first within blocks of 16 (the middle two quads swap) and no activation
+// passes through LDS.  Activations (SiLU, tanh) are applied in FP32 on the accumulators.
+//
+// Every network is evaluated as a deviation from its value at the reference point where all
+// polynomials vanish (infinite separation; QModel::forward_inf): with x = (p - mean)/scale and
+// x_ref = -mean/scale, the first layer takes dx = p/scale and starts its accumulator from
+// z_ref = W^T x_ref + b, each hidden layer passes dh = act(z) - act(z_ref), and the last layer
+// yields d(out) = out - out_ref directly.  The references are computed once on the host in FP64.
+// The BF16 operands then carry only the deviations, so their rounding scales with the signal
+// (the far-field energy, the induced dipole) instead of with the standardised inputs, which sit
+// at O(1) even where the surfaces vanish; the energy adds the FP64 constant
+// yscale out_ref + ymean - en_inf, and the dipole charges q - q(inf) are d(out) itself.
+//
+// MFMA operand layouts (32x32x16, BF16): A[i][k] in lane i + 32*(k/8), element k%8;
+// B[k][j] in lane j + 32*(k/8), element k%8; D[i][j] in lane j + 32*((i/4)%2), register
+// 4*(i/8) + i%4.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hawaii_gpu.h"
+#include "hh_internal.h"
+#include "pip_n2ar.h"
+
+namespace hh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr double kLN2 = 2.07856749;   // N2 bond length, bohr (src/constants.h:80)
+
+// ------------------------------------------------------------------ device helpers
+__device__ __forceinline__ unsigned bf16_bits(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even
+    return u >> 16;
+}
+__device__ __forceinline__ float bf16_value(unsigned h) { return __uint_as_float(h << 16); }
+
+__device__ __forceinline__ f32x16 mfma16(const u32x4& a, const u32x4& b, const f32x16& c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+template <int ACT>
+__device__ __forceinline__ float activate(float x)
+{
+    if constexpr (ACT == HH_ACT_TANH) {
+        return tanhf(x);
+    } else {
+        return x / (1.0f + __expf(-x));
+    }
+}
+
+// 8 floats -> 8 BF16 (high part) and their residuals (low part)
+template <int SPLIT>
+__device__ __forceinline__ void pack8(const float* v, u32x4& hi, u32x4& lo)
+{
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const unsigned h0 = bf16_bits(v[2 * w]), h1 = bf16_bits(v[2 * w + 1]);
+        hi[w] = h0 | (h1 << 16);
+        if constexpr (SPLIT) {
+            const unsigned l0 = bf16_bits(v[2 * w] - bf16_value(h0)), l1 = bf16_bits(v[2 * w + 1] - bf16_value(h1));
+            lo[w] = l0 | (l1 << 16);
+        }
+    }
+}
+
+// A layer on the device: weight fragments [NT][KB][64 lanes] (high), then the low parts;
+// the accumulator start z_ref (hidden layers) or 0 (last layer); act(z_ref) (hidden layers).
+struct LayerDev {
+    const u32x4* A;
+    const float* bias;   // [NT * 32] accumulator start
+    const float* href;   // [NT * 32] activation at the reference point (hidden layers)
+};
+
+template <int NT>
+__device__ __forceinline__ void bias_init(const float* __restrict__ bias, f32x16 (&acc)[NT], int lane)
+{
+    const int base = (lane >> 5) * 4;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[nt][r] = bias[32 * nt + base + 8 * (r >> 2) + (r & 3)];
+}
+
+// one k-block of a layer: acc[nt] += A[nt][kb] x B for the n-tiles in [nt0, nt1)
+template <int NT, int KB, int SPLIT>
+__device__ __forceinline__ void kblock(const u32x4* __restrict__ A, int kb, const u32x4& bh, const u32x4& bl,
+                                       f32x16 (&acc)[NT], int lane, int nt0 = 0, int nt1 = NT)
+{
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        if (nt < nt0 || nt >= nt1) continue;
+        const u32x4 ah = A[(nt * KB + kb) * 64 + lane];
+        acc[nt] = mfma16(ah, bh, acc[nt]);
+        if constexpr (SPLIT) {
+            const u32x4 al = A[NT * KB * 64 + (nt * KB + kb) * 64 + lane];
+            acc[nt] = mfma16(al, bh, acc[nt]);
+            acc[nt] = mfma16(ah, bl, acc[nt]);
+        }
+    }
+}
+
+// act(acc) - act(z_ref) of a layer -> the next layer's B operand (2 k-blocks per n-tile)
+template <int NT, int ACT, int SPLIT>
+__device__ __forceinline__ void act_to_operand(const f32x16 (&acc)[NT], const float* __restrict__ href,
+                                               u32x4 (&bh)[2 * NT], u32x4 (&bl)[2 * NT], int lane)
+{
+    const int base = (lane >> 5) * 4;
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int r = 8 * h + e;
+                v[e] = activate<ACT>(acc[nt][r]) - href[32 * nt + base + 8 * (r >> 2) + (r & 3)];
+            }
+            pack8<SPLIT>(v, bh[2 * nt + h], bl[2 * nt + h]);
+        }
+}
+
+// a full layer whose input is in registers.  The scheduling barrier after each k-block keeps
+// the weight fragments of one k-block in flight at a time (the unrolled loop would otherwise
+// load every fragment of the layer up front and spill); occupancy hides the load latency.
+template <int NT, int KB, int SPLIT>
+__device__ __forceinline__ void layer_regs(const LayerDev& L, const u32x4 (&bh)[KB], const u32x4 (&bl)[KB],
+                                           f32x16 (&acc)[NT], int lane)
+{
+    bias_init<NT>(L.bias, acc, lane);
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        kblock<NT, KB, SPLIT>(L.A, kb, bh[kb], bl[kb], acc, lane);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// ------------------------------------------------------------------ geometry and features
+__device__ __forceinline__ double quintic_switch(double x, double xi, double xf)
+{
+    if (x < xi) return 0.0;
+    if (x >= xf) return 1.0;
+    const double r = (x - xi) / (xf - xi);
+    return r * r * r * (10.0 - 15.0 * r + 6.0 * r * r);
+}
+
+struct Frame {
+    double R, c;                  // R and cos(theta_m) = Rhat . u
+    double Rh[3], ex[3];          // Rhat and the molecular x axis (u - c Rhat)/sin(theta_m)
+};
+
+__device__ __forceinline__ Frame frame_of(const double* __restrict__ q, long n, long i)
+{
+    Frame f;
+    const double Phi = q[i], Theta = q[n + i], R = q[2 * n + i], p1 = q[3 * n + i], t1 = q[4 * n + i];
+    double sT, cT, sP, cP, st, ct, sp, cp;
+    sincos(Theta, &sT, &cT);
+    sincos(Phi, &sP, &cP);
+    sincos(t1, &st, &ct);
+    sincos(p1, &sp, &cp);
+    f.R = R;
+    f.Rh[0] = sT * cP;
+    f.Rh[1] = sT * sP;
+    f.Rh[2] = cT;
+    const double u[3] = {st * cp, st * sp, ct};
+    double c = f.Rh[0] * u[0] + f.Rh[1] * u[1] + f.Rh[2] * u[2];
+    c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+    f.c = c;
+    const double s = sqrt(fmax(1.0 - c * c, 0.0));
+    const double is = s > 1e-12 ? 1.0 / s : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f.ex[k] = (u[k] - c * f.Rh[k]) * is;
+    return f;
+}
+
+// PES variables y (make_yij_n2_ar_purify): N1N2 exp(-r/2); N-Ar (1-s) exp(-r/2) + s 1e4/r^6
+__device__ __forceinline__ void pes_variables(const Frame& f, float* y)
+{
+    const double a = 0.5 * kLN2, base = f.R * f.R + a * a, t = 2.0 * a * f.R * f.c;
+    const double r[2] = {sqrt(fmax(base - t, 0.0)), sqrt(base + t)};   // N1-Ar, N2-Ar
+    y[0] = (float)exp(-0.5 * kLN2);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const double d = r[k], d2 = d * d, d6 = d2 * d2 * d2, s = quintic_switch(d, 6.0, 20.0);
+        y[1 + k] = (float)((1.0 - s) * exp(-0.5 * d) + s * 1e4 / d6);
+    }
+}
+
+// dipole sites (xyz4q_from_bf): N1 (a s, 0, a c), N2 = -N1, N1' (a c, 0, -a s), N2' = -N1', Ar (0, 0, R)
+struct Sites {
+    double x[5], z[5];
+};
+__device__ __forceinline__ Sites sites_of(const Frame& f)
+{
+    const double a = 0.5 * kLN2, c = f.c, s = sqrt(fmax(1.0 - c * c, 0.0));
+    Sites S;
+    S.x[0] = a * s; S.z[0] = a * c;
+    S.x[1] = -a * s; S.z[1] = -a * c;
+    S.x[2] = a * c; S.z[2] = -a * s;
+    S.x[3] = -a * c; S.z[3] = a * s;
+    S.x[4] = 0.0; S.z[4] = f.R;
+    return S;
+}
+
+__device__ __forceinline__ void dipole_variables(const Sites& S, double a0, float* y)
+{
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = i + 1; j < 5; ++j, ++k) {
+            const double dx = S.x[i] - S.x[j], dz = S.z[i] - S.z[j];
+            y[k] = (float)exp(-sqrt(dx * dx + dz * dz) / a0);
+        }
+}
+
+// ------------------------------------------------------------------ model parameters
+template <int NL>
+struct PesDev {
+    LayerDev L[NL];
+    const float* xinv;      // [32] 1 / scale (zero past 17)
+    double yscale;          // E = (e0 + yscale d(out)) / HTOCM
+    double e0;              // yscale out_ref + ymean - en_inf (FP64, host)
+};
+
+struct DipModelDev {
+    LayerDev L1, L2;        // 308 -> 40 (2 n-tiles, 20 k-blocks), 40 -> 5 (1 n-tile, 3 k-blocks)
+    const float* xinv;      // [320]
+};
+struct DipDev {
+    DipModelDev m[2];
+};
+
+// stage the 16 input deviations p/scale of k-block kb of this lane's point into LDS
+template <int SPLIT>
+__device__ __forceinline__ void stage_features(const float* p, const float* __restrict__ xinv, int kb,
+                                               u32x4 (*fh)[2], u32x4 (*fl)[2], int lane)
+{
+    float v[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = p[j] * xinv[16 * kb + j];
+    u32x4 h0, h1, l0, l1;
+    pack8<SPLIT>(v, h0, l0);
+    pack8<SPLIT>(v + 8, h1, l1);
+    fh[lane][0] = h0;
+    fh[lane][1] = h1;
+    if constexpr (SPLIT) {
+        fl[lane][0] = l0;
+        fl[lane][1] = l1;
+    }
+}
+
+// ------------------------------------------------------------------ PES kernel
+// H hidden units per layer, NH hidden layers; NL = NH + 1 linear layers.
+template <int H, int NH, int ACT, int SPLIT>
+__global__ __launch_bounds__(64) void pipnn_pes_kernel(PesDev<NH + 1> m, const double* __restrict__ q, long n,
+                                                       double* __restrict__ energy)
+{
+    constexpr int NT = H / 32, KBH = H / 16;
+    __shared__ u32x4 fh[2][64][2], fl[2][64][2];   // [k-block][point][half]
+    const int lane = threadIdx.x;
+    const long i = (long)blockIdx.x * 64 + lane;
+    const long ii = i < n ? i : n - 1;
+    const Frame f = frame_of(q, n, ii);
+    float y[3];
+    pes_variables(f, y);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        float p[16];
+        pip_n2ar_pes_block<float>(kb, y, p);
+        stage_features<SPLIT>(p, m.xinv, kb, fh[kb], fl[kb], lane);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int t = 0; t < 2; ++t) {
+        const int pt = 32 * t + (lane & 31), half = lane >> 5;
+        f32x16 acc[NT];
+        bias_init<NT>(m.L[0].bias, acc, lane);
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            const u32x4 bh = fh[kb][pt][half];
+            const u32x4 bl = SPLIT ? fl[kb][pt][half] : bh;
+            kblock<NT, 2, SPLIT>(m.L[0].A, kb, bh, bl, acc, lane);
+        }
+        u32x4 bh[KBH], bl[KBH];
+        act_to_operand<NT, ACT, SPLIT>(acc, m.L[0].href, bh, bl, lane);
+#pragma unroll
+        for (int l = 1; l < NH; ++l) {
+            layer_regs<NT, KBH, SPLIT>(m.L[l], bh, bl, acc, lane);
+            act_to_operand<NT, ACT, SPLIT>(acc, m.L[l].href, bh, bl, lane);
+        }
+        f32x16 out[1];
+        layer_regs<1, KBH, SPLIT>(m.L[NH], bh, bl, out, lane);
+        const long j = (long)blockIdx.x * 64 + pt;
+        if (half == 0 && j < n) energy[j] = (m.e0 + m.yscale * (double)out[0][0]) / kHTOCM;
+    }
+}
+
+// ------------------------------------------------------------------ dipole kernel
+template <int SPLIT>
+__global__ __launch_bounds__(64) void pipnn_dip_kernel(DipDev d, const double* __restrict__ q, long n,
+                                                       double* __restrict__ mu)
+{
+    __shared__ u32x4 fh[64][2], fl[64][2];
+    __shared__ float qs[64][5];
+    const int lane = threadIdx.x, half = lane >> 5;
+    const long i = (long)blockIdx.x * 64 + lane;
+    const long ii = i < n ? i : n - 1;
+    double R, c;
+    {
+        const Frame f = frame_of(q, n, ii);
+        R = f.R;
+        c = f.c;
+    }
+    // blended body-frame dipole: (1 - w) mu_short + w mu_long
+    const double w = quintic_switch(R, 9.5, 12.5);
+    double dx = 0.0, dz = 0.0;
+#pragma unroll 1
+    for (int model = 0; model < 2; ++model) {
+        const DipModelDev& M = d.m[model];
+        float y[10];
+        {
+            Frame f;
+            f.R = R;
+            f.c = c;
+            dipole_variables(sites_of(f), model == 0 ? 2.0 : 6.0, y);
+        }
+        f32x16 acc[2][2];
+        bias_init<2>(M.L1.bias, acc[0], lane);
+        bias_init<2>(M.L1.bias, acc[1], lane);
+#pragma unroll 1
+        for (int kb = 0; kb < 20; ++kb) {
+            float p[16];
+            pip_n2ar_dip_block<float>(kb, y, p);
+            __syncthreads();   // the previous block's operands have been read
+            stage_features<SPLIT>(p, M.xinv, kb, fh, fl, lane);
+            __syncthreads();
+            // subnets 0, 1 (units 0..31) read polynomials 0..237 (k-blocks 0..14);
+            // subnet 2 (units 32..39) reads 238..307 (k-blocks 14..19)
+            const int nt0 = kb <= 14 ? 0 : 1, nt1 = kb >= 14 ? 2 : 1;
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int pt = 32 * t + (lane & 31);
+                const u32x4 bh = fh[pt][half];
+                const u32x4 bl = SPLIT ? fl[pt][half] : bh;
+                kblock<2, 20, SPLIT>(M.L1.A, kb, bh, bl, acc[t], lane, nt0, nt1);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            u32x4 bh[4], bl[4];
+            act_to_operand<2, HH_ACT_SILU, SPLIT>(acc[t], M.L1.href, bh, bl, lane);
+            f32x16 o[1];
+            bias_init<1>(M.L2.bias, o, lane);
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) kblock<1, 3, SPLIT>(M.L2.A, kb, bh[kb], bl[kb], o, lane);
+            // q - q(inf): units 0..3 in lanes 0..31 (registers 0..3), unit 4 in lanes 32..63 (register 0)
+            const int pt = 32 * t + (lane & 31);
+            if (half == 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) qs[pt][r] = o[0][r];
+            } else {
+                qs[pt][4] = o[0][0];
+            }
+        }
+        __syncthreads();
+        {
+            Frame f;
+            f.R = R;
+            f.c = c;
+            const Sites S = sites_of(f);
+            double ax = 0.0, az = 0.0;
+#pragma unroll
+            for (int a = 0; a < 5; ++a) {
+                const double dq = (double)qs[lane][a];
+                ax += dq * S.x[a];
+                az += dq * S.z[a];
+            }
+            const double wm = model == 0 ? 1.0 - w : w;
+            dx += wm * ax;
+            dz += wm * az;
+        }
+        __syncthreads();
+    }
+    if (i >= n) return;
+    const Frame f = frame_of(q, n, i);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mu[k * n + i] = dz * f.Rh[k] + dx * f.ex[k];
+}
+
+// ------------------------------------------------------------------ host: weight preparation
+namespace {
+
+unsigned short host_bf16(float x)
+{
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+float host_bf16_value(unsigned short h)
+{
+    unsigned u = (unsigned)h << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+
+// row index, within a 16-block, of the previous layer's unit that k-slot q carries when the
+// operand comes straight from the accumulator registers
+int chained_slot(int q) { return (q / 8) * 4 + 8 * ((q % 8) / 4) + (q % 4); }
+
+// Dense W^T[n_pad][k_pad] (FP64) -> fragments [NT][KB][64] u32x4 (high parts), then the low parts.
+void make_fragments(const std::vector<double>& WT, int n_pad, int k_pad, bool chained, std::vector<uint32_t>& out)
+{
+    const int NT = n_pad / 32, KB = k_pad / 16;
+    out.assign((size_t)2 * NT * KB * 64 * 4, 0u);
+    for (int nt = 0; nt < NT; ++nt)
+        for (int kb = 0; kb < KB; ++kb)
+            for (int lane = 0; lane < 64; ++lane)
+                for (int e = 0; e < 8; ++e) {
+                    const int row = 32 * nt + (lane & 31);
+                    const int qslot = 8 * (lane >> 5) + e;
+                    const int k = 16 * kb + (chained ? chained_slot(qslot) : qslot);
+                    const float wf = (float)WT[(size_t)row * k_pad + k];
+                    const unsigned short hi = host_bf16(wf);
+                    const unsigned short lo = host_bf16(wf - host_bf16_value(hi));
+                    const size_t frag = ((size_t)(nt * KB + kb) * 64 + lane) * 4 + e / 2;
+                    const int shift = (e & 1) ? 16 : 0;
+                    out[frag] |= (uint32_t)hi << shift;
+                    out[(size_t)NT * KB * 64 * 4 + frag] |= (uint32_t)lo << shift;
+                }
+}
+
+double act64(int act, double x) { return act == HH_ACT_TANH ? std::tanh(x) : x / (1.0 + std::exp(-x)); }
+
+bool mlp_ok(const hh_mlp_desc& m)
+{
+    if (m.n_layers < 1 || m.n_layers > 4) return false;
+    for (int l = 0; l < m.n_layers; ++l)
+        if (!m.W[l] || !m.b[l] || m.dims[l] < 1 || m.dims[l + 1] < 1) return false;
+    return true;
+}
+
+// appends a host block to the parameter blob, 256-byte aligned; returns its offset
+size_t stage(std::vector<char>& blob, const void* src, size_t bytes)
+{
+    const size_t off = (blob.size() + 255) / 256 * 256;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, src, bytes);
+    return off;
+}
+
+struct StagedLayer {
+    size_t a, init, href;
+};
+
+// One layer from its padded W^T and reference pre-activation z_ref (FP64): hidden layers start
+// their accumulators at z_ref and subtract act(z_ref); the last layer starts at 0.
+StagedLayer stage_layer(std::vector<char>& blob, const std::vector<double>& WT, int n_pad, int k_pad, bool chained,
+                        const std::vector<double>& zref, bool last, int act)
+{
+    std::vector<uint32_t> frags;
+    make_fragments(WT, n_pad, k_pad, chained, frags);
+    std::vector<float> init(n_pad, 0.0f), href(n_pad, 0.0f);
+    if (!last)
+        for (int j = 0; j < n_pad; ++j) {
+            init[j] = (float)zref[j];
+            href[j] = (float)act64(act, zref[j]);
+        }
+    StagedLayer st;
+    st.a = stage(blob, frags.data(), frags.size() * 4);
+    st.init = stage(blob, init.data(), init.size() * 4);
+    st.href = stage(blob, href.data(), href.size() * 4);
+    return st;
+}
+
+// z = W^T x + b on padded arrays
+std::vector<double> affine(const std::vector<double>& WT, int n_pad, int k_pad, const std::vector<double>& x,
+                           const std::vector<double>& b)
+{
+    std::vector<double> z(n_pad, 0.0);
+    for (int j = 0; j < n_pad; ++j) {
+        double acc = b[j];
+        for (int k = 0; k < k_pad; ++k) acc += WT[(size_t)j * k_pad + k] * x[k];
+        z[j] = acc;
+    }
+    return z;
+}
+
+}  // namespace
+}  // namespace hh
+
+struct hh_pipnn {
+    hh_engine* e = nullptr;
+    int precision = HH_MLP_BF16;
+    int H = 32, NH = 1, act = HH_ACT_SILU;
+    bool has_dip = false;
+    hh::DevBuf params;
+    hh::PesDev<4> pes{};            // layers used: NH + 1
+    hh::DipDev dip{};
+};
+
+extern "C" {
+
+int hh_pipnn_create(hh_engine* e, const hh_pipnn_desc* d, int precision, hh_pipnn** out)
+{
+    using namespace hh;
+    if (!e || !d || !out) return fail(HH_E_INVALID, "hh_pipnn_create: engine, desc and out required");
+    if (precision != HH_MLP_BF16 && precision != HH_MLP_BF16X3)
+        return fail(HH_E_INVALID, "hh_pipnn_create: precision must be HH_MLP_BF16 or HH_MLP_BF16X3");
+    const hh_mlp_desc& P = d->pes;
+    if (!mlp_ok(P) || P.dims[0] != kPipPes || P.dims[P.n_layers] != 1 || !d->pes_xmean || !d->pes_xscale)
+        return fail(HH_E_INVALID, "hh_pipnn_create: the PES network must map %d polynomials to 1 output", kPipPes);
+    const int H = P.dims[1], NH = P.n_layers - 1;
+    for (int l = 1; l < P.n_layers; ++l)
+        if (P.dims[l] != H) return fail(HH_E_UNSUPPORTED, "hh_pipnn_create: hidden layers must have equal widths");
+    if (!((H == 32 && NH == 1) || ((H == 64 || H == 128) && NH == 3)))
+        return fail(HH_E_UNSUPPORTED, "hh_pipnn_create: PES shape 17-%d x%d-1 not provided (17-32-1, 17-64x3-1, "
+                    "17-128x3-1)", H, NH);
+    if (P.activation != HH_ACT_SILU && P.activation != HH_ACT_TANH)
+        return fail(HH_E_INVALID, "hh_pipnn_create: unknown activation %d", P.activation);
+    for (int k = 0; k < kPipPes; ++k)
+        if (!(d->pes_xscale[k] != 0.0)) return fail(HH_E_INVALID, "hh_pipnn_create: zero PES input scale");
+    const bool has_dip = d->dip[0][0].n_layers != 0;
+    static const int kSubIn[3] = {119, 119, 70}, kSubHid[3] = {16, 16, 8}, kSubOut[3] = {2, 2, 1};
+    if (has_dip) {
+        for (int m = 0; m < 2; ++m) {
+            if (!d->dip_xmean[m] || !d->dip_xscale[m]) return fail(HH_E_INVALID, "hh_pipnn_create: dipole scalers");
+            for (int k = 0; k < kPipDip; ++k)
+                if (!(d->dip_xscale[m][k] != 0.0)) return fail(HH_E_INVALID, "hh_pipnn_create: zero dipole input scale");
+            for (int s = 0; s < 3; ++s) {
+                const hh_mlp_desc& D = d->dip[m][s];
+                if (!mlp_ok(D) || D.n_layers != 2 || D.dims[0] != kSubIn[s] || D.dims[1] != kSubHid[s] ||
+                    D.dims[2] != kSubOut[s] || D.activation != HH_ACT_SILU)
+                    return fail(HH_E_UNSUPPORTED, "hh_pipnn_create: dipole subnet %d of model %d must be %d-%d-%d SiLU", s,
+                                m, kSubIn[s], kSubHid[s], kSubOut[s]);
+            }
+        }
+    }
+
+    std::vector<char> blob;
+    // PES: the first layer reads 17 (padded to 32) staged deviations, the rest are chained
+    std::vector<StagedLayer> pl;
+    std::vector<double> a(32, 0.0);
+    for (int k = 0; k < kPipPes; ++k) a[k] = -d->pes_xmean[k] / d->pes_xscale[k];   // x_ref
+    double out_ref = 0.0;
+    for (int l = 0; l < P.n_layers; ++l) {
+        const int k_in = P.dims[l], n_out = P.dims[l + 1];
+        const int k_pad = l == 0 ? 32 : H, n_pad = (n_out + 31) / 32 * 32;
+        std::vector<double> WT((size_t)n_pad * k_pad, 0.0), b(n_pad, 0.0);
+        for (int k = 0; k < k_in; ++k)
+            for (int j = 0; j < n_out; ++j) WT[(size_t)j * k_pad + k] = P.W[l][(size_t)k * n_out + j];
+        for (int j = 0; j < n_out; ++j) b[j] = P.b[l][j];
+        const std::vector<double> z = affine(WT, n_pad, k_pad, a, b);
+        const bool last = l == P.n_layers - 1;
+        pl.push_back(stage_layer(blob, WT, n_pad, k_pad, l > 0, z, last, P.activation));
+        if (last) {
+            out_ref = z[0];
+        } else {
+            a.assign(n_pad, 0.0);
+            for (int j = 0; j < n_out; ++j) a[j] = act64(P.activation, z[j]);
+        }
+    }
+    std::vector<float> xi(32, 0.0f);
+    for (int k = 0; k < kPipPes; ++k) xi[k] = (float)(1.0 / d->pes_xscale[k]);
+    const size_t xi_off = stage(blob, xi.data(), xi.size() * 4);
+
+    // dipole: subnets merged block-diagonally; layer 1 308 (pad 320) -> 40 (pad 64),
+    // layer 2 40 (pad 48) -> 5 (pad 32)
+    StagedLayer dl[2][2] = {};
+    size_t dxi[2] = {0, 0};
+    if (has_dip) {
+        for (int m = 0; m < 2; ++m) {
+            std::vector<double> W1T((size_t)64 * 320, 0.0), W2T((size_t)32 * 48, 0.0), b1(64, 0.0), b2(32, 0.0);
+            int in_off = 0, hid_off = 0, out_off = 0;
+            for (int s = 0; s < 3; ++s) {
+                const hh_mlp_desc& D = d->dip[m][s];
+                for (int k = 0; k < kSubIn[s]; ++k)
+                    for (int j = 0; j < kSubHid[s]; ++j)
+                        W1T[(size_t)(hid_off + j) * 320 + in_off + k] = D.W[0][(size_t)k * kSubHid[s] + j];
+                for (int j = 0; j < kSubHid[s]; ++j) b1[hid_off + j] = D.b[0][j];
+                for (int k = 0; k < kSubHid[s]; ++k)
+                    for (int j = 0; j < kSubOut[s]; ++j)
+                        W2T[(size_t)(out_off + j) * 48 + hid_off + k] = D.W[1][(size_t)k * kSubOut[s] + j];
+                for (int j = 0; j < kSubOut[s]; ++j) b2[out_off + j] = D.b[1][j];
+                in_off += kSubIn[s];
+                hid_off += kSubHid[s];
+                out_off += kSubOut[s];
+            }
+            std::vector<double> xref(320, 0.0);
+            for (int k = 0; k < kPipDip; ++k) xref[k] = -d->dip_xmean[m][k] / d->dip_xscale[m][k];
+            const std::vector<double> z1 = affine(W1T, 64, 320, xref, b1);
+            dl[m][0] = stage_layer(blob, W1T, 64, 320, false, z1, false, HH_ACT_SILU);
+            dl[m][1] = stage_layer(blob, W2T, 32, 48, true, std::vector<double>(32, 0.0), true, HH_ACT_SILU);
+            std::vector<float> di(320, 0.0f);
+            for (int k = 0; k < kPipDip; ++k) di[k] = (float)(1.0 / d->dip_xscale[m][k]);
+            dxi[m] = stage(blob, di.data(), di.size() * 4);
+        }
+    }
+
+    std::lock_guard<std::mutex> lk(e->mu);
+    auto* M = new hh_pipnn;
+    M->e = e;
+    M->precision = precision;
+    M->H = H;
+    M->NH = NH;
+    M->act = P.activation;
+    M->has_dip = has_dip;
+    hipError_t err = hipSetDevice(e->device);
+    if (err == hipSuccess) err = M->params.ensure(blob.size());
+    if (err == hipSuccess) err = hipMemcpy(M->params.p, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        delete M;
+        return fail(HH_E_DEVICE, "hh_pipnn_create: %s (%zu bytes of parameters)", hipGetErrorString(err), blob.size());
+    }
+    char* base = M->params.as<char>();
+    auto view = [&](const StagedLayer& st) {
+        LayerDev L;
+        L.A = reinterpret_cast<const u32x4*>(base + st.a);
+        L.bias = reinterpret_cast<const float*>(base + st.init);
+        L.href = reinterpret_cast<const float*>(base + st.href);
+        return L;
+    };
+    for (int l = 0; l < P.n_layers; ++l) M->pes.L[l] = view(pl[l]);
+    M->pes.xinv = reinterpret_cast<const float*>(base + xi_off);
+    M->pes.yscale = d->pes_yscale;
+    M->pes.e0 = d->pes_yscale * out_ref + d->pes_ymean - d->pes_en_inf;
+    if (has_dip)
+        for (int m = 0; m < 2; ++m) {
+            M->dip.m[m].L1 = view(dl[m][0]);
+            M->dip.m[m].L2 = view(dl[m][1]);
+            M->dip.m[m].xinv = reinterpret_cast<const float*>(base + dxi[m]);
+        }
+    *out = M;
+    return HH_OK;
+}
+
+void hh_pipnn_destroy(hh_pipnn* m) { delete m; }
+
+int hh_pipnn_eval(hh_pipnn* M, const double* q_dev, long n, double* energy_dev, double* dipole_dev, double* ms)
+{
+    using namespace hh;
+    if (!M || (n > 0 && !q_dev)) return fail(HH_E_INVALID, "hh_pipnn_eval: model and q required");
+    if (n < 0) return fail(HH_E_INVALID, "hh_pipnn_eval: n < 0");
+    if (dipole_dev && !M->has_dip) return fail(HH_E_INVALID, "hh_pipnn_eval: the model has no dipole");
+    if (ms) *ms = 0.0;
+    if (n == 0) return HH_OK;
+    hh_engine* e = M->e;
+    std::lock_guard<std::mutex> lk(e->mu);
+    HH_CHECK(hipSetDevice(e->device));
+    const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+    const bool split = M->precision == HH_MLP_BF16X3;
+    HH_CHECK(hipEventRecord(e->ev[0], e->stream));
+    if (energy_dev) {
+        const int key = (M->H == 32 ? 0 : (M->H == 64 ? 1 : 2)) * 4 + (M->act == HH_ACT_TANH ? 2 : 0) + (split ? 1 : 0);
+#define HH_PES_CASE(K, H_, NH_, ACT_, SPLIT_)                                                                   \
+    case K: {                                                                                                   \
+        PesDev<NH_ + 1> pd;                                                                                     \
+        for (int l = 0; l <= NH_; ++l) pd.L[l] = M->pes.L[l];                                                  \
+        pd.xinv = M->pes.xinv;                                                                                  \
+        pd.yscale = M->pes.yscale;                                                                              \
+        pd.e0 = M->pes.e0;                                                                                      \
+        hipLaunchKernelGGL((pipnn_pes_kernel<H_, NH_, ACT_, SPLIT_>), grid, block, 0, e->stream, pd, q_dev, n,   \
+                           energy_dev);                                                                          \
+        break;                                                                                                  \
+    }
+        switch (key) {
+            HH_PES_CASE(0, 32, 1, HH_ACT_SILU, 0)
+            HH_PES_CASE(1, 32, 1, HH_ACT_SILU, 1)
+            HH_PES_CASE(2, 32, 1, HH_ACT_TANH, 0)
+            HH_PES_CASE(3, 32, 1, HH_ACT_TANH, 1)
+            HH_PES_CASE(4, 64, 3, HH_ACT_SILU, 0)
+            HH_PES_CASE(5, 64, 3, HH_ACT_SILU, 1)
+            HH_PES_CASE(6, 64, 3, HH_ACT_TANH, 0)
+            HH_PES_CASE(7, 64, 3, HH_ACT_TANH, 1)
+            HH_PES_CASE(8, 128, 3, HH_ACT_SILU, 0)
+            HH_PES_CASE(9, 128, 3, HH_ACT_SILU, 1)
+            HH_PES_CASE(10, 128, 3, HH_ACT_TANH, 0)
+            HH_PES_CASE(11, 128, 3, HH_ACT_TANH, 1)
+        default:
+            return fail(HH_E_UNSUPPORTED, "hh_pipnn_eval: no kernel for this shape");
+        }
+#undef HH_PES_CASE
+        HH_CHECK(hipGetLastError());
+    }
+    if (dipole_dev) {
+        if (split)
+            hipLaunchKernelGGL(pipnn_dip_kernel<1>, grid, block, 0, e->stream, M->dip, q_dev, n, dipole_dev);
+        else
+            hipLaunchKernelGGL(pipnn_dip_kernel<0>, grid, block, 0, e->stream, M->dip, q_dev, n, dipole_dev);
+        HH_CHECK(hipGetLastError());
+    }
+    HH_CHECK(hipEventRecord(e->ev[1], e->stream));
+    HH_CHECK(hipEventSynchronize(e->ev[1]));
+    if (ms) {
+        float t = 0.0f;
+        HH_CHECK(hipEventElapsedTime(&t, e->ev[0], e->ev[1]));
+        *ms = t;
+    }
+    return HH_OK;
+}
+
+}  // extern "C"

@@ -124,10 +124,11 @@ struct Sys {
     int id;
     double mu, I;
     hh_pes_fn pes;
+    bool linear() const { return id == HH_SYS_CO2AR || id == HH_SYS_N2AR; }   // LINEAR_MOLECULE + ATOM
     double V(const double* y) const
     {
         double q[5];
-        const int nq = id == HH_SYS_CO2AR ? 5 : 3;
+        const int nq = linear() ? 5 : 3;
         for (int i = 0; i < nq; ++i) q[i] = y[2 * i];
         if (pes) return pes(q);
         return id == HH_SYS_CO2AR ? co2ar_V(q) : hear_V(q[2]);
@@ -139,7 +140,7 @@ struct Sys {
         const double sinTheta = std::sin(Theta);
         double k1 = pR * pR / (2.0 * mu) + pTheta * pTheta / (2.0 * mu * R * R) +
                     pPhi * pPhi / (2.0 * mu * R * R * sinTheta * sinTheta);
-        if (id == HH_SYS_CO2AR) {
+        if (linear()) {
             const double st = std::sin(y[8]);
             k1 = k1 + (y[9] * y[9] / (2.0 * I) + y[7] * y[7] / (2.0 * I * st * st));
         }
@@ -167,7 +168,7 @@ void gen_q(const Sys& s, const hh_sampler& p, Mt& mt, double* y)
     y[4] = std::pow(R3, 1.0 / 3.0);
     y[0] = mt.drand() * 2.0 * M_PI;
     y[2] = std::acos(2.0 * mt.drand() - 1.0);
-    if (s.id == HH_SYS_CO2AR) {
+    if (s.linear()) {
         y[6] = mt.drand() * 2.0 * M_PI;
         y[8] = std::acos(2.0 * mt.drand() - 1.0);
     }
@@ -182,7 +183,7 @@ void gen_p(const Sys& s, double T, Mt& mt, double* y)
     y[1] = sqrt_MUKT * y[4] * sinTheta * x0;
     y[3] = sqrt_MUKT * y[4] * x1;
     y[5] = sqrt_MUKT * x2;
-    if (s.id == HH_SYS_CO2AR) {
+    if (s.linear()) {
         const double sqrt_IIKT = std::sqrt(s.I / kHkT * T);
         const double sin_theta = std::sin(y[8]);
         const double r0 = normal(mt), r1 = normal(mt);
@@ -219,7 +220,7 @@ __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* s
                         double* ic, long* n_draws)
 {
     const Sys s{sys->system, sys->mu, sys->I, pes};
-    const int qp = s.id == HH_SYS_CO2AR ? 10 : 6;
+    const int qp = s.linear() ? 10 : 6;
     Mt mt(seed);
     long accepted = 0, candidates = 0;
     double y[10];
@@ -230,6 +231,9 @@ __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* s
             y[4] = smp->R0;
             gen_p(s, smp->T, mt, y);
             if (y[5] > 0.0) y[5] = -y[5];
+            ++candidates;
+        } else if (smp->mode == HH_IC_PHASE_SPACE) {
+            gen_p(s, smp->T, mt, y);
             ++candidates;
         } else {
             gen_p(s, smp->T, mt, y);

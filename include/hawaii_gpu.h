@@ -21,8 +21,12 @@
  *   hh_cf_to_sf          idct_cf_to_sf (src/hawaii.c:5569-5601)
  *   hh_m0                calculate_M0 / mpi_calculate_M0 (src/hawaii.c:1423-1491,
  *                        src/hawaii.c:1620-1738)
- * The reference-compatible structs and entry points (MoleculeSystem, CalcParams,
- * calculate_correlation_and_save, ...) are declared in hawaii_compat.h.
+ *   hh_pf_analytic       analytic_full_partition_function_by_V (src/hawaii.c:5071-5090)
+ *   hh_cf_moment         compute_Mn_from_cf_using_classical_detailed_balance
+ *                        (src/hawaii.c:5945-5979)
+ *   hh_pipnn_*           the N2-Ar PIP-NN plugin surfaces (PES-IDS/ai_pes_n2_ar_pip_nn.cpp,
+ *                        PES-IDS/ai_ids_n2_ar_pip_nn.cpp) as batched device evaluations
+ * The .conf driver front end is declared in hawaii_conf.h.
  */
 #ifndef HAWAII_GPU_H
 #define HAWAII_GPU_H
@@ -47,12 +51,16 @@ enum {
 const char* hh_last_error(void);
 
 /* ------------------------------------------------------------------ systems */
-enum { HH_SYS_HEAR = 0, HH_SYS_CO2AR = 1 };
+/* He-Ar (ATOM + ATOM), CO2-Ar and N2-Ar (LINEAR_MOLECULE + ATOM), CH4-CO2 (ROTOR +
+ * LINEAR_MOLECULE).  The trajectory engine integrates He-Ar and CO2-Ar; N2-Ar surfaces are
+ * evaluated by hh_pipnn_*; the partition function covers all four. */
+enum { HH_SYS_HEAR = 0, HH_SYS_CO2AR = 1, HH_SYS_N2AR = 2, HH_SYS_CH4CO2 = 3 };
 
 typedef struct {
-    int system;     /* HH_SYS_* */
-    double mu;      /* reduced mass, a.u. (pair_reduced_mass) */
-    double I;       /* linear-rotor moment of inertia II[0] (== II[1]); ignored for He-Ar */
+    int system;          /* HH_SYS_* */
+    double mu;           /* reduced mass, a.u. (pair_reduced_mass) */
+    double I;            /* linear-rotor moment of inertia II[0] (== II[1]); ignored for He-Ar */
+    double I_rotor[3];   /* CH4-CO2: principal moments of the rotor (monomer 1); else ignored */
 } hh_system;
 
 /* Plugin surface ABI of the reference (src/hawaii.h:480-499): q = [Phi, Theta, R, monomer angles] */
@@ -66,7 +74,9 @@ int hh_qp_size(int system);
 void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n);
 
 /* ------------------------------------------------------------------ initial conditions */
-enum { HH_IC_REJECTION = 0, HH_IC_R0_INJECT = 1 };
+/* REJECTION: calculate_correlation_and_save's candidate loop; R0_INJECT: the PR/mu path's R0
+ * injection; PHASE_SPACE: q_generator + p_generator only (every candidate kept) */
+enum { HH_IC_REJECTION = 0, HH_IC_R0_INJECT = 1, HH_IC_PHASE_SPACE = 2 };
 enum { HH_PAIR_FREE = 1, HH_PAIR_BOUND = 2, HH_PAIR_ALL = 3 }; /* PairState values of src/hawaii.h:291-297 */
 
 typedef struct {
@@ -158,6 +168,58 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
  * from `seed` (stream s: seed + s); m, q as in the reference (times ppf). */
 int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
           double ppf, double* m, double* q, double* ms);
+
+/* ------------------------------------------------------------------ moments, partition function */
+/* analytic_full_partition_function_by_V (src/hawaii.c:5071-5090): classical kinetic partition
+ * function per unit volume at T (K).  Returns a negative value (and sets the error) on bad input. */
+double hh_pf_analytic(const hh_system* sys, double T);
+
+/* compute_Mn_from_cf_using_classical_detailed_balance (src/hawaii.c:5945-5979) on a host CF:
+ * n = 0: M0 = cf[0] ZeroCoeff / ALU^3;  n = 2: M2 = -SecondCoeff cf''(0) / ALU^3 from the central
+ * difference over 21 points (len >= 11) or 9 points (len >= 5), cf even in t, sample step dt.
+ * ntraj > 0 marks an unnormalised CF (divided by ntraj); ntraj <= 0 a normalised one. */
+int hh_cf_moment(const double* cf, long len, double dt, double ntraj, int n, double* out);
+
+/* ------------------------------------------------------------------ PIP-NN surfaces (N2-Ar) */
+/* The N2-Ar permutation-invariant-polynomial neural-network surfaces
+ * (PES-IDS/ai_pes_n2_ar_pip_nn.cpp, PES-IDS/ai_ids_n2_ar_pip_nn.cpp, PES-IDS/mlp.hpp,
+ * PES-IDS/QModel.hpp; the plugin's pes_lab / dipole_lab of examples/correlation_n2_ar.cpp:11-75)
+ * evaluated in batches on the device: the polynomial features in FP32, every network layer as
+ * BF16 matrix-core products with FP32 accumulation across the batch of phase points. */
+enum { HH_ACT_SILU = 0, HH_ACT_TANH = 1 };
+enum { HH_MLP_BF16 = 0,      /* one BF16 product per layer */
+       HH_MLP_BF16X3 = 1 };  /* operands split into BF16 high + low parts, three products
+                                (hi*hi + hi*lo + lo*hi): FP32-level accuracy on the same units */
+
+typedef struct {
+    int n_layers;            /* linear layers (hidden + output), 2..4; 0 = absent */
+    int dims[5];             /* dims[0] inputs ... dims[n_layers] outputs */
+    int activation;          /* HH_ACT_*, after every layer but the last */
+    const double* W[4];      /* layer k: row-major [dims[k]][dims[k+1]] (y = x W + b) */
+    const double* b[4];      /* [dims[k+1]] */
+} hh_mlp_desc;
+
+typedef struct {
+    /* PES: 17 polynomials -> network -> E = (yscale * out + ymean - en_inf) / HTOCM (Hartree).
+     * Shapes: 17-32-1 (the reference model) or 17-H-H-H-1 with H = 64 or 128. */
+    hh_mlp_desc pes;
+    const double* pes_xmean;     /* [17] StandardScaler of the inputs */
+    const double* pes_xscale;    /* [17] */
+    double pes_ymean, pes_yscale, pes_en_inf;
+    /* dipole: [short (a0 = 2), long (a0 = 6)] x 3 subnets 119-16-2, 119-16-2, 70-8-1 (SiLU) on
+     * 308 standardised polynomials; dip[0][0].n_layers = 0 leaves the dipole out */
+    hh_mlp_desc dip[2][3];
+    const double* dip_xmean[2];  /* [308] */
+    const double* dip_xscale[2];
+} hh_pipnn_desc;
+
+typedef struct hh_pipnn hh_pipnn;
+int hh_pipnn_create(hh_engine* e, const hh_pipnn_desc* desc, int precision, hh_pipnn** out);
+/* q_dev: lab coordinates SoA [5][n] (Phi, Theta, R, phi1, theta1).  energy_dev[n] (Hartree,
+ * pes_lab) and dipole_dev[3][n] (a.u., dipole_lab) may each be NULL.  ms (optional): device
+ * time of the launches.  Synchronous. */
+int hh_pipnn_eval(hh_pipnn* m, const double* q_dev, long n, double* energy_dev, double* dipole_dev, double* ms);
+void hh_pipnn_destroy(hh_pipnn* m);
 
 /* ------------------------------------------------------------------ serving session */
 /* A session keeps lane pools resident on the device and runs submitted requests with

@@ -1,0 +1,49 @@
+/*
+ * Spectral moments from a correlation function (test infrastructure).
+ *
+ * compute_Mn_from_cf_using_classical_detailed_balance (src/hawaii.c:5945-5979):
+ *   M0 = CF[0] ZeroCoeff / ALU^3,  M2 = -SecondCoeff CF''(0) / ALU^3  (/ ntraj if unnormalised),
+ * CF''(0) by the maximal-order central difference on 2N+1 points (N = 10 when at least 11
+ * samples exist, N = 4 when 5..10 do), CF even in t.  The weights here come from the closed
+ * form of the central second-difference weights,
+ *   f''(0) ~ sum_{j=1..N} c_j (f(j h) - 2 f(0) + f(-j h)) / h^2,
+ *   c_j = 2 (-1)^(j+1) (N!)^2 / (j^2 (N-j)! (N+j)!),
+ * an expression independent of the recursion the engine uses.
+ */
+#include <math.h>
+
+#include "oracle.h"
+
+#define ALU 5.29177210903e-11                     /* src/constants.h:8 */
+#define ZERO_COEFF (0.00361479637 / (4.0 * M_PI)) /* src/constants.h:29 */
+#define SECOND_COEFF 13856114.29344114            /* src/constants.h:30 */
+
+static long double fact(int n)
+{
+    long double f = 1.0L;
+    int i;
+    for (i = 2; i <= n; ++i) f *= i;
+    return f;
+}
+
+int orc_cf_moment(const double *cf, size_t len, double dt, double ntraj, int n, double *out)
+{
+    const double scale = (ntraj > 0.0 ? 1.0 / ntraj : 1.0) / (ALU * ALU * ALU);
+    long double d2 = 0.0L;
+    int N, j;
+    if (n == 0) {
+        *out = cf[0] * ZERO_COEFF * scale;
+        return 0;
+    }
+    if (n != 2) return -1;
+    if (len >= 11) N = 10;
+    else if (len >= 5) N = 4;
+    else return -1;
+    for (j = 1; j <= N; ++j) {
+        const long double c = 2.0L * ((j & 1) ? 1.0L : -1.0L) * fact(N) * fact(N) /
+                              ((long double)j * j * fact(N - j) * fact(N + j));
+        d2 += c * (2.0L * cf[j] - 2.0L * cf[0]);
+    }
+    *out = -SECOND_COEFF * (double)(d2 / ((long double)dt * dt)) * scale;
+    return 0;
+}

@@ -70,9 +70,11 @@ void   orc_co2ar_dV_cm(double R, double theta, double *dR, double *dTheta);
 void   orc_co2ar_dip_mol(double R, double theta, double d[3]);
 void   orc_co2ar_lab_to_mol(const double *qlab, double *qmol);
 void   orc_co2ar_jacobian(const double *qlab, const double *qmol, double jac[5][5]);
+/* molecular-frame vector of a linear molecule + atom -> lab frame */
+void   orc_lm_mol_to_lab(const double *qmol, const double dm[3], double mu[3]);
 
 /* ---------------------------------------------------------------- sampler */
-enum { ORC_IC_REJECTION = 0, ORC_IC_R0_INJECT = 1 };
+enum { ORC_IC_REJECTION = 0, ORC_IC_R0_INJECT = 1, ORC_IC_PHASE_SPACE = 2 };
 enum { ORC_PAIR_FREE = 1, ORC_PAIR_BOUND = 2, ORC_PAIR_ALL = 3 };
 
 typedef struct {
@@ -146,6 +148,48 @@ void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoi
 
 /* analytic_full_partition_function_by_V (src/hawaii.c:5071-5090) */
 double orc_pf_analytic(const orc_system *s, double T);
+
+/* compute_Mn_from_cf_using_classical_detailed_balance (src/hawaii.c:5945-5979), n = 0 or 2;
+ * ntraj > 0 divides (an unnormalised CF).  Returns 0, or -1 when the CF is too short. */
+int orc_cf_moment(const double *cf, size_t len, double dt, double ntraj, int n, double *out);
+
+/* ---------------------------------------------------------------- N2-Ar PIP-NN surfaces */
+/* A multilayer perceptron as the reference's MLPES / qm::MLP (PES-IDS/mlp.hpp:178-209,
+ * PES-IDS/QModel.hpp:206-230): layer k maps dims[k] -> dims[k+1] with W[k] row-major
+ * [dims[k]][dims[k+1]] (x W + b); the activation follows every layer but the last. */
+enum { ORC_ACT_SILU = 0, ORC_ACT_TANH = 1 };
+typedef struct {
+    int n_layers;
+    int dims[5];
+    int act;
+    const double *W[4];
+    const double *b[4];
+} orc_mlp;
+
+/* The N2-Ar surfaces (PES-IDS/ai_pes_n2_ar_pip_nn.cpp, PES-IDS/ai_ids_n2_ar_pip_nn.cpp):
+ * PES = network on 17 standardised PIPs, inverse-scaled, minus en_inf, in cm^-1;
+ * dipole = two charge models (short a0 = 2, long a0 = 6) of three subnets each on 308
+ * standardised PIPs, blended over R in [9.5, 12.5]. */
+typedef struct {
+    orc_mlp pes;
+    const double *pes_xmean, *pes_xscale;
+    double pes_ymean, pes_yscale, pes_en_inf;
+    orc_mlp dip[2][3];
+    const double *dip_xmean[2], *dip_xscale[2];
+} orc_n2ar;
+
+double orc_mlp_forward(const orc_mlp *m, const double *x, double *out);   /* returns out[0] */
+void   orc_pip_n2ar_pes(const double y[3], double p[17]);
+void   orc_pip_n2ar_dip(const double y[10], double p[308]);
+double orc_n2ar_pes_xyz(const orc_n2ar *m, const double xyz[9]);             /* cm^-1 */
+void   orc_n2ar_dpes_xyz(const orc_n2ar *m, const double xyz[9], double g[9]); /* cm^-1 / bohr */
+double orc_n2ar_pes(const orc_n2ar *m, const double q[5]);                    /* Hartree, lab q */
+void   orc_n2ar_dpes(const orc_n2ar *m, const double q[5], double dvdq[5]);   /* Hartree / (bohr, rad) */
+void   orc_n2ar_dipole_bf(const orc_n2ar *m, double R, double theta, double d[3]);
+void   orc_n2ar_dipole(const orc_n2ar *m, const double q[5], double mu[3]);   /* lab, a.u. */
+/* plain network forward of the synthetic shape on the PES features (no scalers) */
+double orc_n2ar_mlp_on_pes_features(const orc_mlp *net, const double *xmean, const double *xscale,
+                                    const double q[5]);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,9 @@ long orc_sample_ics(const orc_system *s, const orc_sampler *sp, uint32_t seed,
             p_generator(s, sp->T, &r, y);
             if (y[5] > 0.0) y[5] = -y[5];
             ++candidates;
+        } else if (sp->mode == ORC_IC_PHASE_SPACE) {   /* q_generator + p_generator only */
+            p_generator(s, sp->T, &r, y);
+            ++candidates;
         } else {
             double energy;
             p_generator(s, sp->T, &r, y);

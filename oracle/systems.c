@@ -292,17 +292,24 @@ void orc_dipole(const orc_system *s, const double *q, double mu[3])
         return;
     }
     {                                                      /* ai_ids_co2ar_lib.cpp:12-37 */
-        double qmol[5], dm[3], t1[3], t2[3];
-        m3 Sphiem, Sthetaem, Spsiem;
+        double qmol[5], dm[3];
         orc_co2ar_lab_to_mol(q, qmol);
         orc_co2ar_dip_mol(qmol[0], qmol[4], dm);
-        Sz(Sphiem, sin(qmol[1]), cos(qmol[1]));
-        Sx(Sthetaem, sin(qmol[2]), cos(qmol[2]));
-        Sz(Spsiem, sin(qmol[3]), cos(qmol[3]));
-        mtv(Spsiem, dm, t1);
-        mtv(Sthetaem, t1, t2);
-        mtv(Sphiem, t2, mu);
+        orc_lm_mol_to_lab(qmol, dm, mu);
     }
+}
+
+/* molecular-frame vector -> lab: Sphiem^T Sthetaem^T Spsiem^T d (ai_ids_co2ar_lib.cpp:25-33) */
+void orc_lm_mol_to_lab(const double *qmol, const double dm[3], double mu[3])
+{
+    double t1[3], t2[3];
+    m3 Sphiem, Sthetaem, Spsiem;
+    Sz(Sphiem, sin(qmol[1]), cos(qmol[1]));
+    Sx(Sthetaem, sin(qmol[2]), cos(qmol[2]));
+    Sz(Spsiem, sin(qmol[3]), cos(qmol[3]));
+    mtv(Spsiem, dm, t1);
+    mtv(Sthetaem, t1, t2);
+    mtv(Sphiem, t2, mu);
 }
 
 static void extract_q(const orc_system *s, const double *y, double *q)

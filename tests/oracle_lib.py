@@ -11,7 +11,7 @@ ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
 REF_DIR = os.path.join(REPO, "oracle", "_ref")
 
 SYS_HEAR, SYS_CO2AR = 0, 1
-IC_REJECTION, IC_R0_INJECT = 0, 1
+IC_REJECTION, IC_R0_INJECT, IC_PHASE_SPACE = 0, 1, 2
 PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
 HALVES_BOTH, HALVES_FORWARD = 0, 1
 
@@ -41,6 +41,55 @@ class CFParams(C.Structure):
 class TrajStats(C.Structure):
     _fields_ = [("nrhs", C.c_long), ("ndip", C.c_long), ("naccept", C.c_long), ("nreject", C.c_long),
                 ("e_drift", C.c_double)]
+
+
+class OrcMlp(C.Structure):
+    _fields_ = [("n_layers", C.c_int), ("dims", C.c_int * 5), ("act", C.c_int),
+                ("W", C.c_void_p * 4), ("b", C.c_void_p * 4)]
+
+
+class OrcN2ar(C.Structure):
+    _fields_ = [("pes", OrcMlp), ("pes_xmean", C.c_void_p), ("pes_xscale", C.c_void_p),
+                ("pes_ymean", C.c_double), ("pes_yscale", C.c_double), ("pes_en_inf", C.c_double),
+                ("dip", (OrcMlp * 3) * 2), ("dip_xmean", C.c_void_p * 2), ("dip_xscale", C.c_void_p * 2)]
+
+
+ACT_SILU, ACT_TANH = 0, 1
+
+
+def orc_mlp(layers, act, keep):
+    m = OrcMlp()
+    m.n_layers = len(layers)
+    m.dims[0] = layers[0][0].shape[0]
+    for k, (W, b) in enumerate(layers):
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        keep += [W, b]
+        m.dims[k + 1] = W.shape[1]
+        m.W[k] = W.ctypes.data
+        m.b[k] = b.ctypes.data
+    m.act = act
+    return m
+
+
+def n2ar_model(weights, en_inf):
+    """The oracle's N2-Ar model over the weight arrays of hawaiihybrid_amd.n2ar.load_weights()
+    (the arrays are data; the oracle does its own arithmetic).  Returns (model, keep-alive)."""
+    keep = []
+    m = OrcN2ar()
+    layers, xm, xs, ym, ys = weights["pes"]
+    m.pes = orc_mlp(layers, ACT_SILU, keep)
+    xm, xs = np.ascontiguousarray(xm), np.ascontiguousarray(xs)
+    keep += [xm, xs]
+    m.pes_xmean, m.pes_xscale = xm.ctypes.data, xs.ctypes.data
+    m.pes_ymean, m.pes_yscale, m.pes_en_inf = ym, ys, en_inf
+    for k, (subnets, dxm, dxs) in enumerate(weights["dip"]):
+        for s, sub in enumerate(subnets):
+            m.dip[k][s] = orc_mlp(sub, ACT_SILU, keep)
+        dxm, dxs = np.ascontiguousarray(dxm), np.ascontiguousarray(dxs)
+        keep += [dxm, dxs]
+        m.dip_xmean[k], m.dip_xscale[k] = dxm.ctypes.data, dxs.ctypes.data
+    return m, keep
 
 
 def hear():
@@ -92,6 +141,17 @@ def lib():
             "orc_cf_to_sf": (None, [v, v, sz, v, v]),
             "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
             "orc_pf_analytic": (d, [P(System), d]),
+            "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
+            "orc_mlp_forward": (d, [P(OrcMlp), v, v]),
+            "orc_pip_n2ar_pes": (None, [v, v]),
+            "orc_pip_n2ar_dip": (None, [v, v]),
+            "orc_n2ar_pes_xyz": (d, [P(OrcN2ar), v]),
+            "orc_n2ar_dpes_xyz": (None, [P(OrcN2ar), v, v]),
+            "orc_n2ar_pes": (d, [P(OrcN2ar), v]),
+            "orc_n2ar_dpes": (None, [P(OrcN2ar), v, v]),
+            "orc_n2ar_dipole_bf": (None, [P(OrcN2ar), d, d, v]),
+            "orc_n2ar_dipole": (None, [P(OrcN2ar), v, v]),
+            "orc_n2ar_mlp_on_pes_features": (d, [P(OrcMlp), v, v, v]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

@@ -1,0 +1,227 @@
+"""N2-Ar PIP-NN surfaces in the oracle (CPU) and the host pieces of the engine that need no GPU.
+
+What pins the restatement (the reference's surfaces need Eigen and cannot be built here):
+  * the polynomial bases: the orbit expansion of the representative table equals the reference's
+    basis text term for term (where the reference tree is present), the oracle's evaluation equals
+    that expansion, and every polynomial is invariant under its permutation group;
+  * the networks: the oracle's forward pass equals torch's float64 Linear/SiLU stack on the same
+    weight arrays (an independent implementation);
+  * the PES: the analytic gradient equals finite differences at the reference's own test point
+    (examples/correlation_n2_ar.cpp:77-102) and elsewhere; the T-shaped minimum lies just above the
+    pesmin = -97.3 cm^-1 bound the reference samples with (examples/correlation_n2_ar.cpp:214);
+  * the dipole: mu_y = 0 in the body frame, |mu_lab| = |mu_bf|, rotation covariance of the lab
+    dipole about z, and the long-range R^-4 fall-off of an induced dipole.
+"""
+import itertools
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from hawaiihybrid_amd import n2ar
+from hawaiihybrid_amd.data import gen_pip_n2ar as G
+
+HTOCM = 2.1947463136320e5
+REF = "/root/reference/PES-IDS"
+
+
+@pytest.fixture(scope="module")
+def model():
+    W = n2ar.load_weights()
+    m, keep = O.n2ar_model(W, n2ar.PES_EN_INF)
+    return m, keep, W
+
+
+def expansions():
+    pes = G.expand(G.parse_reps(G.PES_REPS), 3, G.PES_GENS)
+    dip = []
+    for reps, gens in zip(G.DIP_REPS, G.DIP_GENS):
+        dip.append(G.expand(G.parse_reps(reps), 5, gens))
+    return pes, dip
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+def test_basis_table_equals_reference_text():
+    """Each polynomial's orbit expansion equals the reference's expression, monomial for monomial."""
+    pes, dip = expansions()
+    files = ["c_basis_2_1_4_purify.cc", "c_basis_1_1_2_1_3_purify.cc", "c_basis_2_1_1_1_3_purify.cc",
+             "c_basis_2_2_1_3_purify.cc"]
+    for fname, ours in zip(files, [pes] + dip):
+        ref = {}
+        for line in open(os.path.join(REF, fname)):
+            mt = re.match(r"\s*p\((\d+)\)\s*=\s*(.*);", line)
+            if not mt:
+                continue
+            rhs = mt.group(2).strip()
+            terms = [] if rhs == "0.0" else sorted(
+                tuple(sorted(int(x) for x in re.findall(r"y\[(\d+)\]", t))) for t in rhs.split("+"))
+            ref[int(mt.group(1))] = terms
+        assert len(ref) == len(ours), fname
+        for k in range(len(ours)):
+            assert ref[k] == ours[k], (fname, k)
+
+
+def numpy_pips(polys, y):
+    return np.array([sum(np.prod([y[i] for i in mono]) for mono in poly) for poly in polys])
+
+
+def test_oracle_polynomials_equal_table_expansion():
+    pes, dip = expansions()
+    dip_all = dip[0] + dip[1] + dip[2]
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        y3 = rng.uniform(0.01, 1.0, 3)
+        p = np.zeros(17)
+        O.lib().orc_pip_n2ar_pes(O.ptr(y3), O.ptr(p))
+        np.testing.assert_allclose(p, numpy_pips(pes, y3), rtol=1e-14, atol=0)
+        y10 = rng.uniform(0.01, 1.0, 10)
+        q = np.zeros(308)
+        O.lib().orc_pip_n2ar_dip(O.ptr(y10), O.ptr(q))
+        np.testing.assert_allclose(q, numpy_pips(dip_all, y10), rtol=1e-14, atol=0)
+
+
+def pair_y(xyz, n):
+    return np.array([np.exp(-np.linalg.norm(xyz[i] - xyz[j]) / 2.0) for i, j in itertools.combinations(range(n), 2)])
+
+
+def test_polynomials_invariant_under_their_groups():
+    rng = np.random.default_rng(4)
+    xyz3 = rng.normal(size=(3, 3)) * 2
+    p0, p1 = np.zeros(17), np.zeros(17)
+    O.lib().orc_pip_n2ar_pes(O.ptr(pair_y(xyz3, 3)), O.ptr(p0))
+    O.lib().orc_pip_n2ar_pes(O.ptr(pair_y(xyz3[[1, 0, 2]], 3)), O.ptr(p1))
+    np.testing.assert_allclose(p0, p1, rtol=1e-13)
+    # a wrong symmetry must show: swapping N1 with Ar changes the basis
+    O.lib().orc_pip_n2ar_pes(O.ptr(pair_y(xyz3[[2, 1, 0]], 3)), O.ptr(p1))
+    assert np.max(np.abs(p0 - p1)) > 1e-6
+    xyz5 = rng.normal(size=(5, 3)) * 2
+    base = np.zeros(308)
+    O.lib().orc_pip_n2ar_dip(O.ptr(pair_y(xyz5, 5)), O.ptr(base))
+    segs = [(0, 119, [(0, 1, 3, 2, 4)]), (119, 238, [(1, 0, 2, 3, 4)]), (238, 308, [(1, 0, 2, 3, 4), (0, 1, 3, 2, 4)])]
+    for lo, hi, perms in segs:
+        for perm in perms:
+            p = np.zeros(308)
+            O.lib().orc_pip_n2ar_dip(O.ptr(pair_y(xyz5[list(perm)], 5)), O.ptr(p))
+            np.testing.assert_allclose(p[lo:hi], base[lo:hi], rtol=1e-12)
+
+
+def test_mlp_forward_matches_torch(model):
+    import torch
+    m, keep, W = model
+    layers = W["pes"][0]
+    net = torch.nn.Sequential(torch.nn.Linear(17, 32), torch.nn.SiLU(), torch.nn.Linear(32, 1)).double()
+    with torch.no_grad():
+        net[0].weight.copy_(torch.from_numpy(layers[0][0].T.copy()))
+        net[0].bias.copy_(torch.from_numpy(layers[0][1]))
+        net[2].weight.copy_(torch.from_numpy(layers[1][0].T.copy()))
+        net[2].bias.copy_(torch.from_numpy(layers[1][1]))
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        x = rng.normal(size=17)
+        out = np.zeros(1)
+        O.lib().orc_mlp_forward(C_ref(m.pes), O.ptr(x), O.ptr(out))
+        ref = net(torch.from_numpy(x)).item()
+        assert abs(out[0] - ref) < 1e-13 * max(1.0, abs(ref))
+    # the synthetic tanh shape as well
+    syn = n2ar.synthetic_pes(64)
+    keep2 = []
+    om = O.orc_mlp(syn, O.ACT_TANH, keep2)
+    tnet = []
+    for k, (Wk, bk) in enumerate(syn):
+        lin = torch.nn.Linear(Wk.shape[0], Wk.shape[1]).double()
+        with torch.no_grad():
+            lin.weight.copy_(torch.from_numpy(Wk.T.copy()))
+            lin.bias.copy_(torch.from_numpy(bk))
+        tnet.append(lin)
+        if k < len(syn) - 1:
+            tnet.append(torch.nn.Tanh())
+    tnet = torch.nn.Sequential(*tnet)
+    x = rng.normal(size=17)
+    out = np.zeros(1)
+    O.lib().orc_mlp_forward(C_ref(om), O.ptr(x), O.ptr(out))
+    assert abs(out[0] - tnet(torch.from_numpy(x)).item()) < 1e-13
+
+
+def C_ref(x):
+    import ctypes
+    return ctypes.byref(x)
+
+
+def test_pes_gradient_matches_fd(model):
+    m, _, _ = model
+    L = O.lib()
+    pts = [np.array([0.1, 0.2, 8.0, 0.3, 0.4])]      # examples/correlation_n2_ar.cpp:81
+    rng = np.random.default_rng(6)
+    for _ in range(6):
+        pts.append(np.array([rng.uniform(0, 6.28), rng.uniform(0.3, 2.8), rng.uniform(5.5, 25.0),
+                             rng.uniform(0, 6.28), rng.uniform(0.3, 2.8)]))
+    for q in pts:
+        g = np.zeros(5)
+        L.orc_n2ar_dpes(C_ref(m), O.ptr(q), O.ptr(g))
+        scale = np.max(np.abs(g))
+        for i in range(5):
+            h = 1e-5
+            qp, qm = q.copy(), q.copy()
+            qp[i] += h
+            qm[i] -= h
+            fd = (L.orc_n2ar_pes(C_ref(m), O.ptr(qp)) - L.orc_n2ar_pes(C_ref(m), O.ptr(qm))) / (2 * h)
+            # the energy is a difference of ~1e3 cm^-1 network terms: its rounding (~1e-18 Hartree)
+            # over h bounds what finite differences resolve (1e-12 absolute)
+            assert abs(g[i] - fd) < 1e-6 * scale + 1e-12, (q, i, g[i], fd)
+
+
+def test_pes_minimum_is_t_shaped_above_reference_bound(model):
+    m, _, _ = model
+    L = O.lib()
+    best = (np.inf, None)
+    for R in np.linspace(6.0, 8.5, 51):
+        for th in np.linspace(0.0, np.pi / 2, 31):
+            q = np.array([0.0, 0.0, R, 0.0, th])
+            v = L.orc_n2ar_pes(C_ref(m), O.ptr(q)) * HTOCM
+            if v < best[0]:
+                best = (v, (R, th))
+    vmin, (R, th) = best
+    assert -97.3 < vmin < -95.0, vmin
+    assert abs(th - np.pi / 2) < 1e-9 and 6.8 < R < 7.3
+    # far away the surface vanishes (en_inf removed)
+    q = np.array([0.0, 0.0, 200.0, 0.0, 0.7])
+    assert abs(L.orc_n2ar_pes(C_ref(m), O.ptr(q)) * HTOCM) < 0.05
+
+
+def test_dipole_frame_properties(model):
+    m, _, _ = model
+    L = O.lib()
+    d = np.zeros(3)
+    for R, th in [(5.0, 0.3), (9.0, 1.2), (11.0, 2.0), (20.0, 0.9)]:
+        L.orc_n2ar_dipole_bf(C_ref(m), R, th, O.ptr(d))
+        assert d[1] == 0.0
+    # lab dipole: norm equals the body-frame norm; rotating the system about z rotates mu
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        q = np.array([rng.uniform(0, 6.28), rng.uniform(0.3, 2.8), rng.uniform(5.0, 20.0),
+                      rng.uniform(0, 6.28), rng.uniform(0.3, 2.8)])
+        mu = np.zeros(3)
+        L.orc_n2ar_dipole(C_ref(m), O.ptr(q), O.ptr(mu))
+        qm = np.zeros(5)
+        O.lib().orc_co2ar_lab_to_mol(O.ptr(q), O.ptr(qm))
+        L.orc_n2ar_dipole_bf(C_ref(m), qm[0], qm[4], O.ptr(d))
+        assert abs(np.linalg.norm(mu) - np.linalg.norm(d)) < 1e-12 * np.linalg.norm(d)
+        a = 0.4
+        q2 = q.copy()
+        q2[0] += a
+        q2[3] += a
+        mu2 = np.zeros(3)
+        L.orc_n2ar_dipole(C_ref(m), O.ptr(q2), O.ptr(mu2))
+        rot = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+        np.testing.assert_allclose(mu2, rot @ mu, atol=1e-12 * np.linalg.norm(mu))
+
+
+def test_dipole_long_range_falls_as_r4(model):
+    m, _, _ = model
+    d1, d2 = np.zeros(3), np.zeros(3)
+    O.lib().orc_n2ar_dipole_bf(C_ref(m), 20.0, 0.6, O.ptr(d1))
+    O.lib().orc_n2ar_dipole_bf(C_ref(m), 40.0, 0.6, O.ptr(d2))
+    k = np.log(np.linalg.norm(d1) / np.linalg.norm(d2)) / np.log(2.0)
+    assert 3.5 < k < 4.5, k

@@ -48,7 +48,8 @@ class Sampler(C.Structure):
 class CFParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("Rcut", C.c_double),
                 ("ppf", C.c_double), ("L", C.c_int), ("halves", C.c_int), ("max_steps", C.c_int),
-                ("engine", C.c_int), ("precision", C.c_int), ("block", C.c_int), ("grid", C.c_int)]
+                ("engine", C.c_int), ("precision", C.c_int), ("block", C.c_int), ("grid", C.c_int),
+                ("zimmermann", C.c_int)]
 
 
 class CFStats(C.Structure):
@@ -105,6 +106,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise HHError("libhawaii_gpu.so is not built: run `python -m hawaiihybrid_amd.build`")
+        # PyTorch ships its own HIP/HSA runtime under a different soname; loaded after ours it
+        # would start a second runtime that finds no GPU.  Loading torch first makes the process
+        # share torch's runtime (the engine only needs one).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         v, i, d, P = C.c_void_p, C.c_int, C.c_double, C.POINTER
         sigs = {
@@ -179,9 +187,9 @@ def cf_moment(cf, dt, n, ntraj=0.0):
 
 
 def cf_params(L, dt=200.0, rtol=1e-12, atol=None, Rcut=40.0, ppf=1.0, halves=HALVES_FORWARD, engine=ENGINE_BATCHED,
-              precision=PREC_FP64, max_steps=50000, block=0, grid=0):
+              precision=PREC_FP64, max_steps=50000, block=0, grid=0, zimmermann=False):
     return CFParams(rtol, rtol if atol is None else atol, dt, Rcut, ppf, L, halves, max_steps, engine, precision,
-                    block, grid)
+                    block, grid, 1 if zimmermann else 0)
 
 
 class Engine:
@@ -231,12 +239,14 @@ class Engine:
         _check(lib().hh_cf_run(self.h, C.byref(system), C.byref(params), nj, icp, ntr, cfp, C.byref(st)))
         return out, st
 
-    def items(self, job, n_items, L, rows=False):
+    def items(self, job, n_items, L, rows=False, zimmermann=False):
+        """Per-item results of job `job` of the last cf_run; rows [n_items][L], or [n_items][3][L]
+        dipole components for a Zimmermann run."""
         ln = np.zeros(n_items, np.int32)
         stt = np.zeros(n_items, np.int32)
         nr = np.zeros(n_items, np.int32)
         dr = np.zeros(n_items, np.float32)
-        rw = np.zeros((n_items, L)) if rows else None
+        rw = (np.zeros((n_items, 3, L)) if zimmermann else np.zeros((n_items, L))) if rows else None
         p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
         _check(lib().hh_cf_items(self.h, job, p(ln), p(stt), p(nr), p(dr), p(rw)))
         return dict(len=ln, status=stt, nrhs=nr, e_drift=dr, rows=rw)

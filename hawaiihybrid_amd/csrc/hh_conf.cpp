@@ -541,7 +541,9 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         c->cvode_tolerance <= 0 || c->niterations < 1 || !c->cf_filename[0])
         return fail(HH_E_INVALID, "MaxTrajectoryLength, Rcut, sampling_time, total_trajectories, cvode_tolerance, "
                                   "niterations and cf_filename must be set");
-    if (c->use_zimmermann_trick) return fail(HH_E_UNSUPPORTED, "Zimmermann's trick is not part of this engine");
+    // src/hawaii.c:2942-2945
+    if (c->use_zimmermann_trick && c->pair_state != HH_PAIR_BOUND)
+        return fail(HH_E_INVALID, "Zimmermann's trick can be used only for bound states");
     if (c->ppf <= 0)
         return fail(HH_E_UNSUPPORTED, "PARTIAL_PARTITION_FUNCTION_RATIO must be given: the VEGAS estimate of the "
                                       "reference is not part of this engine");
@@ -647,6 +649,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         p.ppf = c->ppf;
         p.L = (int)c->max_trajectory_length;
         p.halves = HH_HALVES_BOTH;
+        p.zimmermann = c->use_zimmermann_trick ? 1 : 0;
         p.max_steps = 50000;
         p.engine = HH_ENGINE_BATCHED;
         p.precision = mixed ? HH_PREC_MIXED : HH_PREC_FP64;

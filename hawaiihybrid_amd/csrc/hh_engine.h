@@ -1,5 +1,7 @@
 // Structures shared by the host library and the device kernels of the trajectory engine.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace hh {
@@ -14,11 +16,25 @@ enum ItemStatus : int {
     ITEM_FAIL_STEPSIZE = 5,     // step size underflow
 };
 
+__host__ __device__ __forceinline__ bool dipole_failure(int st)
+{
+    return st == ITEM_FAIL_DIPOLE_NAN || st == ITEM_FAIL_CF_HUGE || st == ITEM_FAIL_CF_JUMP;
+}
+
+// A trajectory contributes zero when either half failed a dipole check (crln stays zeroed,
+// src/hawaii.c:2183-2237, and src/hawaii.c:1957-2065 with the Zimmermann trick) or when its
+// last half ended on the step limit: correlation_eval then returns CVode's CV_TOO_MUCH_WORK
+// (-1) and the caller skips it (src/hawaii.c:3001).  It still counts in ntraj.
+__host__ __device__ __forceinline__ bool dropped(int first, int last)
+{
+    return dipole_failure(first) || dipole_failure(last) || last == ITEM_FAIL_STEPS;
+}
+
 // One correlation-function job: a batch of trajectories sharing system, temperature-free
 // integration parameters and output buffers.  All pointers are device pointers.
 struct JobDesc {
     const double* ic;        // initial conditions, SoA [QP][ntraj]
-    double* rows;            // per-item CF samples mu(0).mu(t_k), [n_items][L] (only [0, len) written)
+    double* rows;            // per-item samples, [n_items][L] or [n_items][3][L] (record), only [0, len) written
     int* len;                // per-item number of written samples
     int* status;             // per-item ItemStatus
     int* nrhs;               // per-item right-hand-side evaluations
@@ -32,7 +48,7 @@ struct JobDesc {
     int halves;              // 1: forward only, 2: forward + momentum-inverted backward
     int L;                   // MaxTrajectoryLength
     int max_steps;           // max accepted steps between two samples
-    int pad;
+    int record;              // 0: rows hold mu(0).mu(t_k) [L]; 1: the dipole series mu_x, mu_y, mu_z [3][L]
     double dt, Rcut, rtol, atol;
 };
 

@@ -102,6 +102,45 @@ int check_cf_args(const hh_system* sys, const hh_cf_params* p)
         return fail(HH_E_INVALID, "the plain engine is FP64 only");
     if (p->halves != HH_HALVES_BOTH && p->halves != HH_HALVES_FORWARD)
         return fail(HH_E_INVALID, "unknown halves %d", p->halves);
+    if (p->zimmermann && p->halves != HH_HALVES_BOTH)
+        return fail(HH_E_INVALID, "Zimmermann's trick integrates both halves of every trajectory");
+    if (p->zimmermann && p->L > zimm_max_length())
+        return fail(HH_E_UNSUPPORTED, "Zimmermann's trick: MaxTrajectoryLength %d above %d", p->L, zimm_max_length());
+    return HH_OK;
+}
+
+// The ensemble CF of a finished job: the per-trajectory rows summed in a fixed order, or for the
+// Zimmermann series the FFT cross-correlation over every time origin.
+int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out)
+{
+    if (jd.record) {
+        const size_t ws = zimm_workspace_doubles(jd.L);
+        HH_CHECK(e->psum.ensure(ws * sizeof(double)));
+        HH_CHECK(e->pcomp.ensure(ws * sizeof(double)));
+        const int logP = zimm_log_points(jd.L);
+        if (e->zimm_logP != logP) {
+            const long P = 1L << logP;
+            std::vector<double> tw((size_t)P);   // e^{+2 pi i j/P}, j < P/2, as (re, im) pairs
+            for (long j = 0; j < P / 2; ++j) {
+                const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)P;
+                tw[2 * j] = (double)cosl(a);
+                tw[2 * j + 1] = (double)sinl(a);
+            }
+            HH_CHECK(e->zimm_tw.ensure(tw.size() * sizeof(double)));
+            HH_CHECK(hipMemcpyAsync(e->zimm_tw.p, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice,
+                                    e->stream));
+            HH_CHECK(hipStreamSynchronize(e->stream));
+            e->zimm_logP = logP;
+        }
+        HH_CHECK(run_zimm_reduce(jd, ppf, e->zimm_tw.as<double2>(), e->psum.as<double>(), e->pcomp.as<double>(),
+                                 cf_out, e->stream));
+        return HH_OK;
+    }
+    const long long chunk = 256;
+    const int nchunks = (int)((jd.ntraj + chunk - 1) / chunk);
+    HH_CHECK(e->psum.ensure((size_t)nchunks * jd.L * sizeof(double)));
+    HH_CHECK(e->pcomp.ensure((size_t)nchunks * jd.L * sizeof(double)));
+    HH_CHECK(run_reduce(jd, ppf, e->psum.as<double>(), e->pcomp.as<double>(), chunk, nchunks, cf_out, e->stream));
     return HH_OK;
 }
 
@@ -368,7 +407,8 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         jobs[j].ntraj = ntraj[j];
         total += jobs[j].n_items;
     }
-    const size_t rows_bytes = (size_t)total * (size_t)p->L * sizeof(double);
+    const int rec = p->zimmermann ? 3 : 1;
+    const size_t rows_bytes = (size_t)total * (size_t)p->L * rec * sizeof(double);
     if (e->rows.ensure(rows_bytes) != hipSuccess)
         return fail(HH_E_NOMEM, "cannot allocate %.2f GiB of per-trajectory CF rows", rows_bytes / 1073741824.0);
     HH_CHECK(e->len.ensure(total * sizeof(int)));
@@ -380,7 +420,8 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         hh::JobDesc& jb = jobs[j];
         const long long off = jb.item_begin;
         jb.ic = ic_dev[j];
-        jb.rows = e->rows.as<double>() + off * p->L;
+        jb.rows = e->rows.as<double>() + off * p->L * rec;
+        jb.record = p->zimmermann ? 1 : 0;
         jb.len = e->len.as<int>() + off;
         jb.status = e->status.as<int>() + off;
         jb.nrhs = e->nrhs.as<int>() + off;
@@ -404,13 +445,9 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
     HH_CHECK(hipEventRecord(e->ev[0], e->stream));
     HH_CHECK(hh::run_plain(sys->system, sc, e->jobs.as<hh::JobDesc>(), njobs, total, e->arena.as<double>(), e->stream));
     HH_CHECK(hipEventRecord(e->ev[1], e->stream));
-    const long long chunk = 256;
     for (int j = 0; j < njobs; ++j) {
-        const int nchunks = (int)((jobs[j].ntraj + chunk - 1) / chunk);
-        HH_CHECK(e->psum.ensure((size_t)nchunks * p->L * sizeof(double)));
-        HH_CHECK(e->pcomp.ensure((size_t)nchunks * p->L * sizeof(double)));
-        HH_CHECK(hh::run_reduce(jobs[j], p->ppf, e->psum.as<double>(), e->pcomp.as<double>(), chunk, nchunks, cf_dev[j],
-                                e->stream));
+        const int rc = hh::reduce_job(e, jobs[j], p->ppf, cf_dev[j]);
+        if (rc != HH_OK) return rc;
     }
     HH_CHECK(hipEventRecord(e->ev[2], e->stream));
     HH_CHECK(hipStreamSynchronize(e->stream));
@@ -446,9 +483,10 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
         // on the device)
         std::vector<int> ln(n);
         HH_CHECK(hipMemcpy(ln.data(), jb.len, n * sizeof(int), hipMemcpyDeviceToHost));
-        HH_CHECK(hipMemcpy(rows, jb.rows, n * (size_t)jb.L * sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < n; ++i) {
-            const size_t from = (size_t)std::max(0, std::min(ln[i], jb.L));
+        const int rec = jb.record ? 3 : 1;
+        HH_CHECK(hipMemcpy(rows, jb.rows, n * (size_t)jb.L * rec * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n * rec; ++i) {
+            const size_t from = (size_t)std::max(0, std::min(ln[i / rec], jb.L));
             std::memset(rows + i * jb.L + from, 0, (jb.L - from) * sizeof(double));
         }
     }

@@ -31,6 +31,12 @@ size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
                       double* cf_out, hipStream_t st);
+// Zimmermann's trick (hh_zimm.hip)
+int zimm_log_points(int L);
+size_t zimm_workspace_doubles(int L);
+int zimm_max_length();
+hipError_t run_zimm_reduce(const JobDesc& jb, double ppf, const double2* tw, double* psum, double* pcomp,
+                           double* cf_out, hipStream_t st);
 hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2* work_b, double* nu, double* sf,
                         hipStream_t st, bool lds_ok);
 hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
@@ -126,6 +132,8 @@ struct hh_engine {
     int wall_khz = 100000;        // s_memrealtime frequency
     hh::DevBuf tab64, tab32;      // Co2ArTables<double>, Co2ArTables<float>
     hh::DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, arena;
+    hh::DevBuf zimm_tw;           // twiddles e^{2 pi i j/P} of the Zimmermann transforms
+    int zimm_logP = 0;
     std::vector<hh::JobDesc> last_jobs;
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
     std::vector<std::unique_ptr<hh::LanePool>> pools;
@@ -133,6 +141,8 @@ struct hh_engine {
 };
 
 namespace hh {
+// the ensemble CF of a finished job into cf_out (on the engine stream)
+int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;
 // the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
 int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out);

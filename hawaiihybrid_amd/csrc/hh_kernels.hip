@@ -156,8 +156,14 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
     ln.mu0[0] = m[0];
     ln.mu0[1] = m[1];
     ln.mu0[2] = m[2];
-    double* row = jb.rows + ln.local * (long long)jb.L;
-    row[0] = ln.mu0[0] * ln.mu0[0] + ln.mu0[1] * ln.mu0[1] + ln.mu0[2] * ln.mu0[2];
+    double* row = jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1);
+    if (jb.record) {
+        row[0] = ln.mu0[0];
+        row[jb.L] = ln.mu0[1];
+        row[2 * (long long)jb.L] = ln.mu0[2];
+    } else {
+        row[0] = ln.mu0[0] * ln.mu0[0] + ln.mu0[1] * ln.mu0[1] + ln.mu0[2] * ln.mu0[2];
+    }
     ln.E0 = hamiltonian<SYS, P>(sc, ln.y);
     ln.t = 0.0;
     ln.k = 1;
@@ -352,7 +358,7 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
 #pragma unroll
                 for (int c = 0; c < NQ; ++c) Fc[3 + d][c] = h * f[d][2 * c];
         }
-        double* row = jb.rows + ln.local * (long long)jb.L;
+        double* row = jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1);
         while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
             const double tk = ln.k * jb.dt;
             double ys[N];
@@ -393,7 +399,13 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
                 ended = true;
                 break;
             }
-            row[ln.k] = ln.curr;
+            if (jb.record) {
+                row[ln.k] = (double)m[0];
+                row[jb.L + ln.k] = (double)m[1];
+                row[2 * (long long)jb.L + ln.k] = (double)m[2];
+            } else {
+                row[ln.k] = ln.curr;
+            }
             ln.steps = 0;
             if (ys[4] > jb.Rcut || ln.k + 1 >= jb.L) {
                 // energy check of make_step (src/trajectory.c:167-180), at the end of the step
@@ -719,20 +731,6 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double v)
     else
         c += (v - t) + s;
     s = t;
-}
-
-__device__ __forceinline__ bool dipole_failure(int st)
-{
-    return st == ITEM_FAIL_DIPOLE_NAN || st == ITEM_FAIL_CF_HUGE || st == ITEM_FAIL_CF_JUMP;
-}
-
-// A trajectory contributes zero when either half failed a dipole check (crln stays zeroed,
-// src/hawaii.c:2183-2237) or when its last half ended on the step limit: correlation_eval
-// then returns CVode's CV_TOO_MUCH_WORK (-1) and the caller skips it (src/hawaii.c:3001).
-// It still counts in ntraj.
-__device__ __forceinline__ bool dropped(int first, int last)
-{
-    return dipole_failure(first) || dipole_failure(last) || last == ITEM_FAIL_STEPS;
 }
 
 // partial[c][k] = sum over trajectories of chunk c of ppf * crln[k]; column-parallel, rows

@@ -62,8 +62,10 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj)
 {
     const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
-    // rows, len, status, nrhs, e_drift, and the job's share of its group's order
-    return (size_t)items * ((size_t)p.L * sizeof(double) + 4 * sizeof(int) + sizeof(float));
+    // rows ([3][L] per item for the Zimmermann series), len, status, nrhs, e_drift, and the job's
+    // share of its group's order
+    const size_t row = (size_t)p.L * (p.zimmermann ? 3 : 1);
+    return (size_t)items * (row * sizeof(double) + 4 * sizeof(int) + sizeof(float));
 }
 
 bool pool_has_room(const LanePool& pl)
@@ -90,7 +92,8 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     JobDesc& jd = job->jd;
     jd.ic = ic_dev;
     jd.rows = job->buf.as<double>();
-    jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L);
+    jd.record = p.zimmermann ? 1 : 0;
+    jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L * (jd.record ? 3 : 1));
     jd.status = jd.len + n_items;
     jd.nrhs = jd.status + n_items;
     jd.e_drift = reinterpret_cast<float*>(jd.nrhs + n_items);
@@ -211,12 +214,8 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
                 integrating = true;
                 continue;
             }
-            const long long chunk = 256;
-            const int nchunks = (int)((j.jd.ntraj + chunk - 1) / chunk);
-            HH_CHECK(e->psum.ensure((size_t)nchunks * j.jd.L * sizeof(double)));
-            HH_CHECK(e->pcomp.ensure((size_t)nchunks * j.jd.L * sizeof(double)));
-            HH_CHECK(run_reduce(j.jd, j.ppf, e->psum.as<double>(), e->pcomp.as<double>(), chunk, nchunks, j.cf_out,
-                                e->stream));
+            const int rc = reduce_job(e, j.jd, j.ppf, j.cf_out);
+            if (rc != HH_OK) return rc;
             j.state = 1;
             launched = true;
         }

@@ -118,6 +118,9 @@ typedef struct {
     int precision;       /* HH_PREC_*; the plain engine is FP64 only */
     int block;           /* batched: workgroup size 64/128/256 (0: autotuned) */
     int grid;            /* batched: resident workgroups (0: autotuned) */
+    int zimmermann;      /* use_zimmermann_trick: CF averaged over every time origin of the
+                            trajectory's dipole series (correlation_eval_zimmerman_trick,
+                            src/hawaii.c:1888-2114); both halves, L <= 4096 */
 } hh_cf_params;
 
 typedef struct {
@@ -143,7 +146,8 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
               const long* ntraj, double* const* cf_dev, hh_cf_stats* stats);
 
 /* Per-item results of job `job` of the last hh_cf_run (host buffers of n_items entries;
- * any pointer may be NULL).  rows: [n_items][L], entries past len are zero. */
+ * any pointer may be NULL).  rows: [n_items][L] samples mu(0).mu(t_k), or with zimmermann set
+ * [n_items][3][L] dipole components; entries past len are zero. */
 int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* e_drift, double* rows);
 
 /* ------------------------------------------------------------------ batched surfaces */

@@ -74,8 +74,9 @@ static void dipole_of(const orc_system *s, const double *y, double mu[3])
     orc_dipole(s, q, mu);
 }
 
-int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *c, int *len,
-                orc_traj_stats *st)
+/* vec (optional): the dipole at every recorded sample, vec[3k + i] */
+static int half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *c, double *vec, int *len,
+                   orc_traj_stats *st)
 {
     const int n = orc_qp_size(s);
     fctx fc = {s, 0};
@@ -92,6 +93,12 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
     dipole_of(s, y, mu0);
     st->ndip++;
     c[0] = mu0[0] * mu0[0] + mu0[1] * mu0[1] + mu0[2] * mu0[2];
+    if (vec) {
+        memset(vec, 0, sizeof(double) * 3 * p->L);
+        vec[0] = mu0[0];
+        vec[1] = mu0[1];
+        vec[2] = mu0[2];
+    }
     *len = 1;
     E0 = orc_hamiltonian(s, y0);
     if (p->L <= 1) return ORC_OK;
@@ -208,6 +215,11 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, d
                 if (fabs(curr) > 1e100) { status = ORC_FAIL_CF_HUGE; goto done; }
                 if (k > 1 && fabs(curr / prev) > 1e10) { status = ORC_FAIL_CF_JUMP; goto done; }
                 c[k] = curr;
+                if (vec) {
+                    vec[3 * k] = mut[0];
+                    vec[3 * k + 1] = mut[1];
+                    vec[3 * k + 2] = mut[2];
+                }
                 *len = k + 1;
                 memcpy(ylast, ys, sizeof(double) * n);
                 steps_since_sample = 0;
@@ -233,9 +245,60 @@ done:
     return status;
 }
 
+int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *c, int *len,
+                orc_traj_stats *st)
+{
+    return half_cf(s, p, y0, c, NULL, len, st);
+}
+
+void orc_zimmermann_series_cf(const double *d, int L, double *crln)
+{
+    int sh, c, i;
+    for (sh = 0; sh < L; ++sh) {
+        double acc = 0.0;
+        for (c = 0; c < L; ++c)
+            for (i = 0; i < 3; ++i) acc += d[3 * c + i] * d[3 * (c + sh) + i] * ALU * ALU * ALU;
+        crln[sh] = acc / L;
+    }
+}
+
+/* correlation_eval_zimmerman_trick (src/hawaii.c:1888-2114): the series d[0 .. 2L-2] holds the
+ * backward half reversed below mu(0) at d[L-1] and the forward half above it; a dipole failure
+ * in either half returns with crln zeroed (the forward one before the backward half runs); a
+ * step failure ends that half and the status of the backward half is the one returned. */
+static int traj_cf_zimmermann(const orc_system *s, const orc_cf_params *p, const double *y0, double *crln,
+                              int *len_f, int *len_b, orc_traj_stats *st)
+{
+    double *c = calloc(p->L, sizeof(double)), *vf = calloc(3 * (size_t)p->L, sizeof(double));
+    double *vb = calloc(3 * (size_t)p->L, sizeof(double)), *d = calloc(3 * (size_t)(2 * p->L - 1), sizeof(double));
+    double yb[NMAX];
+    int n = orc_qp_size(s), i, k, status;
+    *len_b = 0;
+    memset(crln, 0, sizeof(double) * p->L);
+    status = half_cf(s, p, y0, c, vf, len_f, st);
+    if (status == ORC_FAIL_DIPOLE_NAN || status == ORC_FAIL_CF_HUGE || status == ORC_FAIL_CF_JUMP) goto out;
+    for (i = 0; i < n; ++i) yb[i] = (i % 2 == 1) ? -y0[i] : y0[i];
+    status = half_cf(s, p, yb, c, vb, len_b, st);
+    if (status == ORC_FAIL_DIPOLE_NAN || status == ORC_FAIL_CF_HUGE || status == ORC_FAIL_CF_JUMP) goto out;
+    if (status == ORC_FAIL_STEPS) goto out;
+    for (k = 0; k < *len_f; ++k)
+        for (i = 0; i < 3; ++i) d[3 * (p->L - 1 + k) + i] = vf[3 * k + i];
+    for (k = 1; k < *len_b; ++k)
+        for (i = 0; i < 3; ++i) d[3 * (p->L - 1 - k) + i] = vb[3 * k + i];
+    orc_zimmermann_series_cf(d, p->L, crln);
+out:
+    free(c);
+    free(vf);
+    free(vb);
+    free(d);
+    return status;
+}
+
 int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *crln,
                 int *len_f, int *len_b, orc_traj_stats *st)
 {
+    if (p->zimmermann) return traj_cf_zimmermann(s, p, y0, crln, len_f, len_b, st);
+    {
     double *cf = calloc(p->L, sizeof(double)), *cb = calloc(p->L, sizeof(double));
     double yb[NMAX];
     int n = orc_qp_size(s), i, status;
@@ -261,6 +324,7 @@ out:
     free(cf);
     free(cb);
     return status;
+    }
 }
 
 typedef struct {

@@ -103,6 +103,7 @@ typedef struct {
     double Rcut;
     int halves;         /* ORC_HALVES_* */
     long max_steps;     /* per output interval (CVodeSetMaxNumSteps, src/trajectory.c:62) */
+    int zimmermann;     /* use_zimmermann_trick (both halves; src/hawaii.c:1888-2114) */
 } orc_cf_params;
 
 typedef struct {
@@ -122,6 +123,11 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
  * so does a last half that ended on the step limit (CV_TOO_MUCH_WORK, src/hawaii.c:3001). */
 int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
                 double *crln, int *len_f, int *len_b, orc_traj_stats *st);
+
+/* The Zimmermann sum over time origins of one dipole series d[3 * (2L - 1)] (d[3c + i] is
+ * component i at series index c, mu(0) at c = L - 1):
+ * crln[s] = ALU^3 / L sum_{c=0}^{L-1} d(c) . d(c + s)  (src/hawaii.c:2089-2094). */
+void orc_zimmermann_series_cf(const double *d, int L, double *crln);
 
 /* Per-trajectory rows, threads over trajectories.  crln: [ntraj][L]. */
 void orc_cf_rows(const orc_system *s, const orc_cf_params *p, const double *ic, long ntraj,

@@ -35,7 +35,7 @@ class Sampler(C.Structure):
 
 class CFParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("L", C.c_int),
-                ("Rcut", C.c_double), ("halves", C.c_int), ("max_steps", C.c_long)]
+                ("Rcut", C.c_double), ("halves", C.c_int), ("max_steps", C.c_long), ("zimmermann", C.c_int)]
 
 
 class TrajStats(C.Structure):
@@ -142,6 +142,7 @@ def lib():
             "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
             "orc_pf_analytic": (d, [P(System), d]),
             "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
+            "orc_zimmermann_series_cf": (None, [v, i, v]),
             "orc_mlp_forward": (d, [P(OrcMlp), v, v]),
             "orc_pip_n2ar_pes": (None, [v, v]),
             "orc_pip_n2ar_dip": (None, [v, v]),

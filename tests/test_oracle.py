@@ -382,3 +382,28 @@ def test_m0_rejection_sampler_consistent_with_ensemble_cf0():
     ZeroCoeff = 0.00361479637 / (4.0 * math.pi)
     assert m == pytest.approx((d * d).mean() * ZeroCoeff, rel=1e-12)
     assert 0 < q < m
+
+
+def test_zimmermann_series_sum_matches_numpy_correlate():
+    """The Zimmermann sum over the first L time origins (src/hawaii.c:2089-2094) equals numpy's
+    cross-correlation of the series with its first L samples, lags 0..L-1."""
+    rng = np.random.default_rng(11)
+    L = 37
+    d = rng.normal(size=(2 * L - 1, 3))
+    crln = np.zeros(L)
+    O.lib().orc_zimmermann_series_cf(O.ptr(np.ascontiguousarray(d)), L, O.ptr(crln))
+    ALU3 = 5.29177210903e-11 ** 3
+    ref = np.zeros(L)
+    for i in range(3):
+        full = np.correlate(d[:, i], d[:L, i], mode="full")   # lag m at index m + L - 1
+        ref += full[L - 1:2 * L - 1]
+    np.testing.assert_allclose(crln, ref * ALU3 / L, rtol=1e-12, atol=1e-12 * np.abs(ref).max() * ALU3 / L)
+
+
+def test_zimmermann_constant_dipole_is_flat():
+    """A constant dipole over the whole series gives crln[s] = ALU^3 |d|^2 for every shift."""
+    L = 16
+    d = np.tile([0.3, -0.1, 0.2], (2 * L - 1, 1))
+    crln = np.zeros(L)
+    O.lib().orc_zimmermann_series_cf(O.ptr(np.ascontiguousarray(d)), L, O.ptr(crln))
+    np.testing.assert_allclose(crln, 0.14 * 5.29177210903e-11 ** 3, rtol=1e-13)

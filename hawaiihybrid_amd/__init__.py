@@ -72,6 +72,20 @@ class RequestInfo(C.Structure):
                 ("cf", CFStats), ("m0", C.c_double), ("m0_err", C.c_double), ("error", C.c_char * 160)]
 
 
+class VegasParams(C.Structure):
+    _fields_ = [("integrand", C.c_int), ("pair_state", C.c_int), ("T", C.c_double), ("Rmin", C.c_double),
+                ("Rmax", C.c_double), ("niterations", C.c_int), ("npoints", C.c_long), ("seed", C.c_uint32)]
+
+
+class VegasResult(C.Structure):
+    _fields_ = [("value", C.c_double), ("error", C.c_double), ("chi2_dof", C.c_double), ("iterations", C.c_int),
+                ("points", C.c_longlong), ("nonfinite", C.c_longlong), ("ms", C.c_double),
+                ("iter_value", C.c_double * 32), ("iter_error", C.c_double * 32)]
+
+
+INTEGRAND_M0, INTEGRAND_M2, INTEGRAND_PF = 0, 1, 2
+ZERO_COEFF = 0.00361479637 / (4.0 * 3.141592653589793)   # src/constants.h:29
+
 REQ_WAITING, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 REQ_CF, REQ_M0 = 0, 1
 
@@ -135,6 +149,7 @@ def lib():
             "hh_poll": (i, [v, C.c_uint64, d, P(RequestInfo)]),
             "hh_session_destroy": (i, [v]),
             "hh_pf_analytic": (d, [P(System), d]),
+            "hh_vegas": (i, [v, P(System), P(VegasParams), P(VegasResult)]),
             "hh_cf_moment": (i, [v, C.c_long, d, d, i, P(d)]),
             "hh_pipnn_create": (i, [v, v, i, P(v)]),
             "hh_pipnn_eval": (i, [v, v, C.c_long, v, v, P(d)]),
@@ -250,6 +265,15 @@ class Engine:
         p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
         _check(lib().hh_cf_items(self.h, job, p(ln), p(stt), p(nr), p(dr), p(rw)))
         return dict(len=ln, status=stt, nrhs=nr, e_drift=dr, rows=rw)
+
+    def vegas(self, system, integrand, T, Rmin, Rmax, pair_state=PAIR_FREE, niterations=12, npoints=1000000,
+              seed=1):
+        """VEGAS phase-space integral (hh_vegas): returns the VegasResult."""
+        p = VegasParams(integrand, pair_state, float(T), float(Rmin), float(Rmax), int(niterations), int(npoints),
+                        int(seed))
+        r = VegasResult()
+        _check(lib().hh_vegas(self.h, C.byref(system), C.byref(p), C.byref(r)))
+        return r
 
     def cf_to_sf(self, cf_dev, dt):
         """Apodizes cf_dev in place (as the reference) and returns (nu, sf, ms) device tensors."""

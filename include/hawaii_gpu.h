@@ -173,6 +173,37 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
 int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
           double ppf, double* m, double* q, double* ms);
 
+/* ------------------------------------------------------------------ VEGAS phase-space integrals */
+/* c_mpi_perform_integration / mpi_perform_integration (src/hep_hawaii.cpp:332-380) with the
+ * integrands integrand_M0 and integrand_pf (src/hep_hawaii.cpp:116-201) over the phase space of
+ * transform_variables (src/hep_hawaii.cpp:8-114): adaptive Monte Carlo (Lepage's VEGAS) on the
+ * device, niterations of npoints, the first iteration discarded.  M0 = value * ZeroCoeff /
+ * hh_pf_analytic; the partition-function ratio = value(PF) / hh_pf_analytic.  The estimate
+ * agrees with the reference's hep-mc run statistically (same integrand, own sampler). */
+enum { HH_INTEGRAND_M0 = 0, HH_INTEGRAND_M2 = 1, HH_INTEGRAND_PF = 2 };   /* IntegrandType order */
+
+typedef struct {
+    int integrand;          /* HH_INTEGRAND_M0 or HH_INTEGRAND_PF */
+    int pair_state;         /* HH_PAIR_FREE (E >= 0), HH_PAIR_BOUND (E <= 0), HH_PAIR_ALL */
+    double T;               /* K */
+    double Rmin, Rmax;      /* sampler_Rmin, sampler_Rmax */
+    int niterations;        /* the reference's default: 12 */
+    long npoints;           /* per iteration; the reference's default: 1000000 */
+    uint32_t seed;
+} hh_vegas_params;
+
+typedef struct {
+    double value, error;    /* the integral and its standard error */
+    double chi2_dof;        /* consistency of the accumulated iterations */
+    int iterations;
+    long long points;
+    long long nonfinite;    /* points whose integrand was not finite (counted as 0) */
+    double ms;              /* device time */
+    double iter_value[32], iter_error[32];   /* per iteration (first 32) */
+} hh_vegas_result;
+
+int hh_vegas(hh_engine* e, const hh_system* sys, const hh_vegas_params* p, hh_vegas_result* out);
+
 /* ------------------------------------------------------------------ moments, partition function */
 /* analytic_full_partition_function_by_V (src/hawaii.c:5071-5090): classical kinetic partition
  * function per unit volume at T (K).  Returns a negative value (and sets the error) on bad input. */

@@ -127,9 +127,8 @@ class Lexer {
 };
 
 // keywords of the reference that steer parts outside this engine: accepted, ignored
-const char* kIgnored[] = {"TEMPERATURES", "SATELLITE_TEMPERATURES", "INITIALM2_NPOINTS", "HEP_M0_NITERATIONS",
-                          "HEP_M0_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS", "HEP_PPF_NITERATIONS",
-                          "HEP_PPF_NPOINTS", "CF_FILENAMES", "PARTIAL_PARTITION_FUNCTION_RATIOS",
+const char* kIgnored[] = {"SATELLITE_TEMPERATURES", "INITIALM2_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS",
+                          "CF_FILENAMES", "PARTIAL_PARTITION_FUNCTION_RATIOS",
                           "APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
                           "AVERAGE_TIME_BETWEEN_COLLISIONS", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
                           "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
@@ -264,6 +263,8 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 c->calculation_type = HH_CALC_CORRELATION_SINGLE;
             else if (w == "PROCESSING")
                 c->calculation_type = HH_CALC_PROCESSING;
+            else if (w == "PHASE_SPACE_M0")
+                c->calculation_type = HH_CALC_PHASE_SPACE_M0;
             else
                 return err(line, "CALCULATION_TYPE %s is not part of this engine", w);
         } else if (key == "PAIR_STATE") {
@@ -315,6 +316,18 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             rc = num(c->Rcut);
         } else if (key == "PARTIAL_PARTITION_FUNCTION_RATIO") {
             rc = num(c->ppf);
+        } else if (key == "HEP_PPF_NITERATIONS") {
+            rc = inum(c->hep_ppf_niterations);
+        } else if (key == "HEP_PPF_NPOINTS") {
+            rc = inum(c->hep_ppf_npoints);
+        } else if (key == "HEP_M0_NITERATIONS") {
+            rc = inum(c->hep_m0_niterations);
+        } else if (key == "HEP_M0_NPOINTS") {
+            rc = inum(c->hep_m0_npoints);
+        } else if (key == "TEMPERATURES") {
+            if (list.empty() || list.size() > 64) return err(line, "%s expects {T1, T2, ...} (at most 64)", key);
+            c->ntemperatures = (int)list.size();
+            for (size_t k = 0; k < list.size(); ++k) c->temperatures[k] = list[k];
         } else if (key == "USE_ZIMMERMANN_TRICK") {
             const std::string w = Lexer::upper(v.text);
             c->use_zimmermann_trick = (w == "TRUE" || (v.kind == Tok::NUMBER && v.num != 0));
@@ -534,6 +547,52 @@ uint32_t good_seed()
     return s;
 }
 
+// the engine's system for the configured monomers (ATOM+ATOM: He-Ar surfaces, LINEAR_MOLECULE+ATOM:
+// CO2-Ar surfaces)
+int system_of(const hh_conf* c, hh_system* sys)
+{
+    sys->mu = c->reduced_mass;
+    if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_ATOM && c->monomer_type[1] == HH_MONOMER_ATOM) {
+        sys->system = HH_SYS_HEAR;
+    } else if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_LINEAR_MOLECULE &&
+               c->monomer_type[1] == HH_MONOMER_ATOM) {
+        sys->system = HH_SYS_CO2AR;
+        sys->I = c->II[0][0];
+        if (c->II[0][0] != c->II[0][1] || sys->I <= 0)
+            return fail(HH_E_INVALID, "linear molecule needs II = {I, I, 0} with I > 0");
+    } else {
+        return fail(HH_E_UNSUPPORTED, "monomers other than ATOM+ATOM or LINEAR_MOLECULE+ATOM are not part of this engine");
+    }
+    if (!(sys->mu > 0)) return fail(HH_E_INVALID, "pair_reduced_mass must be positive");
+    return HH_OK;
+}
+
+// CALCULATION_PHASE_SPACE_M0 (src/driver.c:3389-3445): VEGAS M0 at each temperature
+int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+{
+    if (!(c->sampler_Rmin > 0) || !(c->sampler_Rmax > c->sampler_Rmin))
+        return fail(HH_E_INVALID, "sampler_Rmin and sampler_Rmax must be set for the phase-space integral");
+    hh_system sys{};
+    int rc = system_of(c, &sys);
+    if (rc < 0) return rc;
+    std::vector<double> Ts(c->temperatures, c->temperatures + c->ntemperatures);
+    if (Ts.empty() && c->temperature > 0) Ts.push_back(c->temperature);
+    if (Ts.empty()) return fail(HH_E_INVALID, "no temperature is provided to run PHASE_SPACE_M0");
+    for (double T : Ts) {
+        const double pf_an = hh_pf_analytic(&sys, T);
+        if (pf_an < 0) return (int)HH_E_INVALID;
+        if (!o.quiet) std::printf("Analytic partition function divided by V: %.5e\n", pf_an);
+        hh_vegas_params vp{HH_INTEGRAND_M0, c->pair_state ? c->pair_state : HH_PAIR_FREE, T, c->sampler_Rmin,
+                           c->sampler_Rmax, (int)(c->hep_m0_niterations > 0 ? c->hep_m0_niterations : 12),
+                           c->hep_m0_npoints > 0 ? c->hep_m0_npoints : 1000000L, o.seed ? o.seed : 1u};
+        hh_vegas_result vr{};
+        rc = hh_vegas(e, &sys, &vp, &vr);
+        if (rc < 0) return rc;
+        std::printf("T = %.2e => M0: %.5e +- %.5e\n", T, vr.value * kZeroCoeff / pf_an, vr.error * kZeroCoeff / pf_an);
+    }
+    return HH_OK;
+}
+
 int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
 {
     // the reference's checks (src/hawaii.c:2802-2811)
@@ -544,24 +603,43 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     // src/hawaii.c:2942-2945
     if (c->use_zimmermann_trick && c->pair_state != HH_PAIR_BOUND)
         return fail(HH_E_INVALID, "Zimmermann's trick can be used only for bound states");
-    if (c->ppf <= 0)
-        return fail(HH_E_UNSUPPORTED, "PARTIAL_PARTITION_FUNCTION_RATIO must be given: the VEGAS estimate of the "
-                                      "reference is not part of this engine");
     if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
-    sys.mu = c->reduced_mass;
-    if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_ATOM && c->monomer_type[1] == HH_MONOMER_ATOM) {
-        sys.system = HH_SYS_HEAR;
-    } else if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_LINEAR_MOLECULE &&
-               c->monomer_type[1] == HH_MONOMER_ATOM) {
-        sys.system = HH_SYS_CO2AR;
-        sys.I = c->II[0][0];
-        if (c->II[0][0] != c->II[0][1] || sys.I <= 0)
-            return fail(HH_E_INVALID, "linear molecule needs II = {I, I, 0} with I > 0");
-    } else {
-        return fail(HH_E_UNSUPPORTED, "monomers other than ATOM+ATOM or LINEAR_MOLECULE+ATOM are not part of this engine");
+    {
+        const int rc0 = system_of(c, &sys);
+        if (rc0 < 0) return rc0;
     }
+    // partition-function ratio and M0 by VEGAS when the ratio is not given (src/hawaii.c:2829-2880)
+    double ppf = c->ppf;
+    if (ppf <= 0) {
+        const double pf_an = hh_pf_analytic(&sys, c->temperature);
+        if (pf_an < 0) return (int)HH_E_INVALID;
+        if (!o.quiet) {
+            std::printf("Ratio of partial partition functions to full (analytic) partial partition function is not "
+                        "provided.\nConducting calculations using adaptive Monte Carlo method.\n\n\n");
+            std::printf("Analytic partition function divided by V: %.5e\n", pf_an);
+        }
+        hh_vegas_params vp{HH_INTEGRAND_PF, c->pair_state ? c->pair_state : HH_PAIR_FREE, c->temperature,
+                           c->sampler_Rmin, c->sampler_Rmax,
+                           (int)(c->hep_ppf_niterations > 0 ? c->hep_ppf_niterations : 12),
+                           c->hep_ppf_npoints > 0 ? c->hep_ppf_npoints : 1000000L, o.seed ? o.seed : 1u};
+        hh_vegas_result vr{};
+        int vrc = hh_vegas(e, &sys, &vp, &vr);
+        if (vrc < 0) return vrc;
+        ppf = vr.value / pf_an;
+        if (!o.quiet) std::printf("T = %.2e => PPF ratio: %.5e\n", c->temperature, ppf);
+        vp.integrand = HH_INTEGRAND_M0;
+        vp.niterations = (int)(c->hep_m0_niterations > 0 ? c->hep_m0_niterations : 12);
+        vp.npoints = c->hep_m0_npoints > 0 ? c->hep_m0_npoints : 1000000L;
+        vrc = hh_vegas(e, &sys, &vp, &vr);
+        if (vrc < 0) return vrc;
+        if (!o.quiet) {
+            std::printf("T = %.2e => M0: %.5e\n", c->temperature, vr.value * kZeroCoeff / pf_an);
+            std::printf("note: the VEGAS M2 integral is not part of this engine\n");
+        }
+    }
+
     const int qp = hh_qp_size(sys.system);
     const long n_iter = c->total_trajectories / c->niterations;
     if (n_iter <= 0) return fail(HH_E_INVALID, "total_trajectories < niterations");
@@ -634,8 +712,8 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
                 m += diff / (k + 1.0);
                 q += diff * diff * (k / (k + 1.0));
             }
-            m0 = m * kZeroCoeff * c->ppf;
-            const double m0s = std::sqrt(q / m0n / (m0n - 1.0)) * kZeroCoeff * c->ppf;
+            m0 = m * kZeroCoeff * ppf;
+            const double m0s = std::sqrt(q / m0n / (m0n - 1.0)) * kZeroCoeff * ppf;
             if (!o.quiet) {
                 std::printf("M0 = %.10e +/- %.10e [%.10e ... %.10e]\n", m0, m0s, m0 - m0s, m0 + m0s);
                 std::printf("Error: %.3f%%\n", m0s / m0 * 100.0);
@@ -646,7 +724,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
         p.dt = c->sampling_time;
         p.Rcut = c->Rcut;
-        p.ppf = c->ppf;
+        p.ppf = ppf;
         p.L = (int)c->max_trajectory_length;
         p.halves = HH_HALVES_BOTH;
         p.zimmermann = c->use_zimmermann_trick ? 1 : 0;
@@ -725,6 +803,8 @@ int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
         if (rc == HH_OK) rc = run_processing(e, conf, o.quiet != 0);
     } else if (conf->calculation_type == HH_CALC_PROCESSING) {
         rc = run_processing(e, conf, o.quiet != 0);
+    } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M0) {
+        rc = run_phase_space_m0(e, conf, o);
     } else {
         rc = fail(HH_E_INVALID, "CALCULATION_TYPE is not set");
     }

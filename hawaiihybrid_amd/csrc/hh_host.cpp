@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -270,6 +271,10 @@ int hh_engine_create(int device, hh_engine** out)
     }
     // a blocking stream: the engine's work is ordered after the caller's copies and kernels on
     // the legacy default stream (hipMemcpy of pageable memory can return before its DMA ends)
+    // the graph-captured slices and the launch-shape tuner are opt-in until a hang seen with both
+    // enabled is explained (hh_pool.cpp)
+    if (const char* g = std::getenv("HH_GRAPHS")) e->use_graphs = g[0] && g[0] != '0';
+    if (const char* t = std::getenv("HH_TUNE")) e->use_tuner = t[0] && t[0] != '0';
     if (hipStreamCreateWithFlags(&e->stream, hipStreamDefault) != hipSuccess) {
         delete e;
         return fail(HH_E_DEVICE, "hipStreamCreate failed");
@@ -312,6 +317,7 @@ void hh_engine_destroy(hh_engine* e)
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->capture) (void)hipStreamDestroy(e->capture);
     delete e;
 }
 
@@ -343,10 +349,17 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
 
     if (p->engine == HH_ENGINE_BATCHED) {
         // resident lane pool: all jobs are admitted at once and streamed through time slices
-        int block = p->block;
-        if (block != 64 && block != 128 && block != 256) block = 256;
+        int block = p->block, waves = 2;
+        if (block == 0 && e->use_tuner) {
+            rc = hh::pool_tune(e, *sys, *p, ic_dev, ntraj, njobs, &block, &waves);
+            if (rc < 0) return rc;
+        } else if (block == 0) {
+            block = 256;
+        } else if (block != 64 && block != 128 && block != 256) {
+            return fail(HH_E_INVALID, "block must be 0 (tuned), 64, 128 or 256");
+        }
         hh::LanePool* pl = nullptr;
-        rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, block, &pl);
+        rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, block, &pl, waves);
         if (rc < 0) return rc;
         if (!pl->inflight.empty()) return fail(HH_E_BUSY, "lane pool of this class is in use");
         if (njobs > pl->ring_cap) return fail(HH_E_INVALID, "at most %d jobs per call", pl->ring_cap);

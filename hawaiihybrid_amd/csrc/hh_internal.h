@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
@@ -24,6 +25,10 @@ hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int n
 hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl,
                     const LaneSlots& sl, double* ws, hipStream_t st);
 hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st);
+hipError_t run_clamp_next(unsigned long long* next, unsigned long long total, hipStream_t st);
+// right-hand sides spent on the job in ring slot `ring_slot` (added to *out)
+hipError_t run_pool_work(const LaneSlots& sl, int ring_slot, const int* item_nrhs, long long n_items,
+                         unsigned long long* out, hipStream_t st);
 hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st);
 hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long long n, double* out, hipStream_t st);
 hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st);
@@ -107,6 +112,7 @@ struct LanePool {
     hh_system system{};         // mu and I of every job in the pool
     bool mixed = false;
     int block = 256;
+    int waves = 2;                  // resident wavefronts per SIMD
     long long S = 0;
     int ring_cap = 1024;
     DevBuf slots[2];            // double-buffered for repacking
@@ -118,6 +124,28 @@ struct LanePool {
     bool drained_repacked = false;
     long long unordered_seq = 0;    // first job not yet ordered (pool_order_group)
     LaneSlots view(int which) const { return slots_view(sys, slots[which].p, S); }
+    // one slice as a captured graph: [repack (when the queue has drained), clamp of the queue
+    // head, the pool kernel, copies of the done counters and the head into pinned memory]
+    struct SliceGraph {
+        bool repack = false;
+        int cur = 0;
+        PoolCtl ctl{};
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<SliceGraph> graphs;   // a few variants (repack or not, buffer, admission state)
+    bool repack_pending = false;
+    int* h_done = nullptr;            // pinned copies read by the host after each slice
+    unsigned long long* h_next = nullptr;
+    LanePool() = default;
+    LanePool(const LanePool&) = delete;
+    LanePool& operator=(const LanePool&) = delete;
+    ~LanePool()
+    {
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (h_done) (void)hipHostFree(h_done);
+        if (h_next) (void)hipHostFree(h_next);
+    }
 };
 
 }  // namespace hh
@@ -127,6 +155,9 @@ struct hh_session;
 struct hh_engine {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t capture = nullptr;      // non-blocking stream used only to capture slice graphs
+    bool use_graphs = false;            // HH_GRAPHS=1: slices as captured graphs (off: see hh_pool.cpp)
+    bool use_tuner = false;             // HH_TUNE=1: block = 0 measures the launch shape (off: see hh_pool.cpp)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     int wall_khz = 100000;        // s_memrealtime frequency
@@ -137,6 +168,7 @@ struct hh_engine {
     std::vector<hh::JobDesc> last_jobs;
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
     std::vector<std::unique_ptr<hh::LanePool>> pools;
+    std::vector<std::array<int, 4>> tuned;    // (system, mixed, block, waves) chosen by pool_tune
     std::mutex mu;
 };
 
@@ -145,7 +177,12 @@ namespace hh {
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;
 // the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
-int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out);
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out,
+             int waves = 2);
+// launch shape (workgroup size, resident wavefronts per SIMD) of the batched engine for this
+// system and precision: measured once per engine on the caller's own trajectories, then cached
+int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const double* const* ic_dev,
+              const long* ntraj, int njobs, int* block, int* waves);
 // device bytes a job of this shape holds while it runs
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj);
 // admit a correlation-function job: allocates its buffers, uploads its descriptor into the

@@ -716,6 +716,31 @@ __global__ void __launch_bounds__(256) repack_move_kernel(LaneSlots src, LaneSlo
     dst.local[d] = src.local[s];
 }
 
+// right-hand sides spent so far on one job: its items still in lanes (slot counters) plus its
+// finished items (per-item counters, zeroed at admission); one partial per block
+__global__ void __launch_bounds__(256) pool_work_kernel(LaneSlots sl, int ring_slot, const int* __restrict__ item_nrhs,
+                                                         long long n_items, unsigned long long* __restrict__ out)
+{
+    __shared__ unsigned long long part[4];
+    const long long S = sl.S;
+    unsigned long long w = 0;
+    for (long long s = (long long)blockIdx.x * 256 + threadIdx.x; s < S; s += (long long)gridDim.x * 256)
+        if (sl.i[s] == ring_slot) w += (unsigned long long)sl.i[4 * S + s];
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n_items; t += (long long)gridDim.x * 256)
+        w += (unsigned long long)item_nrhs[t];
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+hipError_t run_pool_work(const LaneSlots& sl, int ring_slot, const int* item_nrhs, long long n_items,
+                         unsigned long long* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(pool_work_kernel, dim3(256), dim3(256), 0, st, sl, ring_slot, item_nrhs, n_items, out);
+    return hipGetLastError();
+}
+
 __global__ void __launch_bounds__(256) slots_clear_kernel(int* __restrict__ job, long long S)
 {
     const long long s = (long long)blockIdx.x * 256 + threadIdx.x;
@@ -838,6 +863,19 @@ hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const Jo
     if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128);
     HH_LAUNCH(SYS_CO2AR, double, 256);
 #undef HH_LAUNCH
+}
+
+// lanes that found the queue empty advanced the head past the end; none of those indices was
+// handed out, so the head goes back to the first unadmitted item
+__global__ void clamp_next_kernel(unsigned long long* next, unsigned long long total)
+{
+    if (threadIdx.x == 0 && *next > total) *next = total;
+}
+
+hipError_t run_clamp_next(unsigned long long* next, unsigned long long total, hipStream_t st)
+{
+    hipLaunchKernelGGL(clamp_next_kernel, dim3(1), dim3(64), 0, st, next, total);
+    return hipGetLastError();
 }
 
 hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st)

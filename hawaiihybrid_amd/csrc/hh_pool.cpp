@@ -26,11 +26,11 @@ namespace {
 const int kZero = 0;
 }  // namespace
 
-int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out)
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out, int waves)
 {
     for (auto& p : set)
-        if (p->sys == sys.system && p->mixed == mixed && p->block == block && p->system.mu == sys.mu &&
-            p->system.I == sys.I) {
+        if (p->sys == sys.system && p->mixed == mixed && p->block == block && p->waves == waves &&
+            p->system.mu == sys.mu && p->system.I == sys.I) {
             *out = p.get();
             return HH_OK;
         }
@@ -39,7 +39,9 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
     pl->system = sys;
     pl->mixed = mixed;
     pl->block = block;
-    const long long grid = (long long)e->prop.multiProcessorCount * 2 * (256 / block);   // two waves per SIMD
+    pl->waves = waves;
+    // `waves` wavefronts per SIMD, four SIMDs per CU
+    const long long grid = (long long)e->prop.multiProcessorCount * waves * (256 / block);
     pl->S = grid * block;
     const size_t sb = slots_bytes(pl->sys, pl->S);
     const size_t arena = (size_t)arena_doubles_per_lane(pl->sys) * sizeof(double) * (size_t)pl->S;
@@ -158,6 +160,142 @@ int pool_order_group(hh_engine* e, LanePool& pl)
     return HH_OK;
 }
 
+// Both the tuner and the graph-captured slices below are opt-in (HH_TUNE=1, HH_GRAPHS=1): the
+// first GPU run with both enabled by default went silent in the .conf driver's batched run
+// (killed after 180 s, GPU responsive) and the cause is not established, so the default path
+// is the direct launch at 256 x 2 waves/SIMD that the rest of the tests exercise.
+int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const double* const* ic_dev,
+              const long* ntraj, int njobs, int* block, int* waves)
+{
+    const bool mixed = p.precision == HH_PREC_MIXED;
+    for (const auto& t : e->tuned)
+        if (t[0] == sys.system && t[1] == (int)mixed) {
+            *block = t[2];
+            *waves = t[3];
+            return HH_OK;
+        }
+    // candidates: workgroup size at two wavefronts per SIMD (the register budget allows two), and
+    // one wavefront per SIMD (half the lanes, half the arena traffic)
+    static const int kCand[][2] = {{256, 2}, {128, 2}, {64, 2}, {256, 1}};
+    constexpr double kSliceMs = 10.0;
+    DevBuf cf_tmp, work;
+    HH_CHECK(cf_tmp.ensure((size_t)p.L * sizeof(double)));
+    HH_CHECK(work.ensure(sizeof(unsigned long long)));
+    double best = -1.0;
+    int bb = 256, bw = 2;
+    for (const auto& c : kCand) {
+        PoolSet tmp;
+        LanePool* pl = nullptr;
+        int rc = pool_get(e, tmp, sys, mixed, c[0], &pl, c[1]);
+        if (rc < 0) return rc;
+        std::vector<PoolJob*> jobs;
+        for (int j = 0; j < njobs && j < pl->ring_cap; ++j) {
+            PoolJob* pj = nullptr;
+            rc = pool_admit(e, *pl, p, ic_dev[j], ntraj[j], cf_tmp.as<double>(), 0, &pj);
+            if (rc < 0) return rc;
+            HH_CHECK(hipMemsetAsync(pj->jd.nrhs, 0, (size_t)pj->jd.n_items * sizeof(int), e->stream));
+            jobs.push_back(pj);
+        }
+        rc = pool_order_group(e, *pl);
+        if (rc < 0) return rc;
+        PoolCtl ctl{};
+        ctl.next = pl->next.as<unsigned long long>();
+        ctl.total = pl->total;
+        ctl.first_seq = pl->inflight.front()->seq;
+        ctl.last_seq = pl->inflight.back()->seq;
+        ctl.ticks = (long long)(kSliceMs * e->wall_khz);
+        ctl.ring = pl->ring_cap;
+        SysConst sc{sys.mu, sys.I, (mixed && sys.system == HH_SYS_CO2AR) ? e->tab32.p : e->tab64.p};
+        HH_CHECK(hipEventRecord(e->ev[0], e->stream));
+        HH_CHECK(run_pool(pl->sys, mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
+                          pl->arena.as<double>(), e->stream));
+        HH_CHECK(hipEventRecord(e->ev[1], e->stream));
+        HH_CHECK(hipMemsetAsync(work.p, 0, sizeof(unsigned long long), e->stream));
+        for (PoolJob* pj : jobs)
+            HH_CHECK(run_pool_work(pl->view(pl->cur), (int)(pj->seq % pl->ring_cap), pj->jd.nrhs, pj->jd.n_items,
+                                   work.as<unsigned long long>(), e->stream));
+        unsigned long long w = 0;
+        HH_CHECK(hipMemcpyAsync(&w, work.p, sizeof w, hipMemcpyDeviceToHost, e->stream));
+        HH_CHECK(hipStreamSynchronize(e->stream));
+        float ms = 0.0f;
+        HH_CHECK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
+        const double rate = ms > 0.0f ? (double)w / ms : 0.0;
+        if (std::getenv("HH_POOL_TRACE"))
+            std::fprintf(stderr, "[hh tune] block %d, %d waves/SIMD: %.3g right-hand sides/ms\n", c[0], c[1], rate);
+        if (rate > best) {
+            best = rate;
+            bb = c[0];
+            bw = c[1];
+        }
+    }
+    e->tuned.push_back({sys.system, (int)mixed, bb, bw});
+    *block = bb;
+    *waves = bw;
+    return HH_OK;
+}
+
+namespace {
+
+bool same_ctl(const PoolCtl& a, const PoolCtl& b)
+{
+    return a.next == b.next && a.total == b.total && a.first_seq == b.first_seq && a.last_seq == b.last_seq &&
+           a.ticks == b.ticks && a.ring == b.ring;
+}
+
+// The slice of one pool as a graph launched on the engine stream; captured on the engine's
+// capture stream the first time a (repack, buffer, admission state) combination is needed.
+int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const SysConst& sc)
+{
+    if (!e->capture) HH_CHECK(hipStreamCreateWithFlags(&e->capture, hipStreamNonBlocking));
+    if (!pl.h_done) HH_CHECK(hipHostMalloc((void**)&pl.h_done, pl.ring_cap * sizeof(int), hipHostMallocDefault));
+    if (!pl.h_next) HH_CHECK(hipHostMalloc((void**)&pl.h_next, sizeof(unsigned long long), hipHostMallocDefault));
+    const bool repack = pl.repack_pending;
+    LanePool::SliceGraph* g = nullptr;
+    for (auto& c : pl.graphs)
+        if (c.repack == repack && c.cur == pl.cur && same_ctl(c.ctl, ctl)) g = &c;
+    if (!g) {
+        if (pl.graphs.size() >= 4) {
+            (void)hipGraphExecDestroy(pl.graphs.front().exec);
+            pl.graphs.erase(pl.graphs.begin());
+        }
+        const int src = pl.cur, dst = repack ? 1 - pl.cur : pl.cur;
+        hipGraph_t graph = nullptr;
+        HH_CHECK(hipStreamBeginCapture(e->capture, hipStreamCaptureModeThreadLocal));
+        hipError_t err = hipSuccess;
+        if (repack) err = run_repack(pl.sys, pl.view(src), pl.view(dst), pl.scratch.as<int>(), e->capture);
+        if (err == hipSuccess) err = run_clamp_next(pl.next.as<unsigned long long>(), pl.total, e->capture);
+        if (err == hipSuccess)
+            err = run_pool(pl.sys, pl.mixed, pl.block, sc, pl.ring.as<JobDesc>(), ctl, pl.view(dst),
+                           pl.arena.as<double>(), e->capture);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(pl.h_done, pl.ring_done.p, pl.ring_cap * sizeof(int), hipMemcpyDeviceToHost,
+                                 e->capture);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(pl.h_next, pl.next.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->capture);
+        const hipError_t end = hipStreamEndCapture(e->capture, &graph);
+        if (err == hipSuccess) err = end;
+        hipGraphExec_t exec = nullptr;
+        if (err == hipSuccess) err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        HH_CHECK(err);
+        LanePool::SliceGraph ng;
+        ng.repack = repack;
+        ng.cur = pl.cur;
+        ng.ctl = ctl;
+        ng.exec = exec;
+        pl.graphs.push_back(ng);
+        g = &pl.graphs.back();
+    }
+    HH_CHECK(hipGraphLaunch(g->exec, e->stream));
+    if (repack) {
+        pl.cur = 1 - pl.cur;
+        pl.repack_pending = false;
+    }
+    return HH_OK;
+}
+
+}  // namespace
+
 int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms, int* launches)
 {
     // 1. one slice of every pool that still integrates
@@ -180,8 +318,13 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             ctl.ring = pl->ring_cap;
             const bool mixed = pl->mixed && pl->sys == HH_SYS_CO2AR;
             SysConst sc{pl->system.mu, pl->system.I, mixed ? e->tab32.p : e->tab64.p};
-            HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
-                              pl->arena.as<double>(), e->stream));
+            if (e->use_graphs) {
+                const int rc = launch_slice_graph(e, *pl, ctl, sc);
+                if (rc != HH_OK) return rc;
+            } else {
+                HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
+                                  pl->arena.as<double>(), e->stream));
+            }
             if (launches) ++*launches;
         }
         HH_CHECK(hipEventRecord(e->ev[1], e->stream));
@@ -197,14 +340,21 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
     HH_CHECK(hipEventRecord(e->ev[2], e->stream));
     for (LanePool* pl : run) {
         std::vector<int> done(pl->ring_cap);
-        HH_CHECK(hipMemcpy(done.data(), pl->ring_done.p, pl->ring_cap * sizeof(int), hipMemcpyDeviceToHost));
         unsigned long long next = 0;
-        HH_CHECK(hipMemcpy(&next, pl->next.p, sizeof next, hipMemcpyDeviceToHost));
-        if (next > pl->total) {
-            // lanes that found the queue empty advanced the head past the end; none of those
-            // indices was handed out, so the head goes back to the first unadmitted item
-            next = pl->total;
-            HH_CHECK(hipMemcpy(pl->next.p, &next, sizeof next, hipMemcpyHostToDevice));
+        if (e->use_graphs) {
+            // the graph copied the counters into pinned memory at the end of the slice
+            std::memcpy(done.data(), pl->h_done, pl->ring_cap * sizeof(int));
+            next = *pl->h_next;
+            if (next > pl->total) next = pl->total;   // the next graph clamps the device copy
+        } else {
+            HH_CHECK(hipMemcpy(done.data(), pl->ring_done.p, pl->ring_cap * sizeof(int), hipMemcpyDeviceToHost));
+            HH_CHECK(hipMemcpy(&next, pl->next.p, sizeof next, hipMemcpyDeviceToHost));
+            if (next > pl->total) {
+                // lanes that found the queue empty advanced the head past the end; none of those
+                // indices was handed out, so the head goes back to the first unadmitted item
+                next = pl->total;
+                HH_CHECK(hipMemcpy(pl->next.p, &next, sizeof next, hipMemcpyHostToDevice));
+            }
         }
         bool integrating = false;
         for (auto& jp : pl->inflight) {
@@ -219,7 +369,13 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             j.state = 1;
             launched = true;
         }
-        if (integrating && next >= pl->total) {
+        if (integrating && next >= pl->total && e->use_graphs) {
+            // queue drained, items left: the next slice's graph spreads them over the wavefronts
+            pl->repack_pending = true;
+            if (trace)
+                std::fprintf(stderr, "[hh pool] sys %d %s: queue drained, repack scheduled\n", pl->sys,
+                             pl->mixed ? "mixed" : "fp64");
+        } else if (integrating && next >= pl->total) {
             // queue drained, items left: spread them over the pool's wavefronts
             HH_CHECK(run_repack(pl->sys, pl->view(pl->cur), pl->view(1 - pl->cur), pl->scratch.as<int>(), e->stream));
             pl->cur = 1 - pl->cur;

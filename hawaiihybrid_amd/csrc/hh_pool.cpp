@@ -332,6 +332,7 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
         if (kernel_ms) *kernel_ms += ms;
+        if (std::getenv("HH_POOL_TRACE")) std::fprintf(stderr, "[hh pool] slice of %zu pools: %.2f ms\n", run.size(), ms);
     }
 
     // 2. finished jobs -> reductions; drained pools -> repack

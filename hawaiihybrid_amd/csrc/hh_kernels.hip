@@ -56,6 +56,11 @@ struct Arena {
     {
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(hh_v2u, v), rsrc, lane8, (slot * N + i) * 512, 0);
     }
+    // element (slot, i) of another lane of the wavefront
+    __device__ __forceinline__ double ld_lane(int lane, int slot, int i) const
+    {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane * 8, (slot * N + i) * 512, 0));
+    }
 };
 
 template <int N>
@@ -265,23 +270,76 @@ __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, c
     }
 }
 
-// One attempted DOP853 step for a lane.  Returns true when the lane's item has ended.
-template <int SYS, typename P, int N>
-__device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* __restrict__ jobs, const Arena<N>& ar,
-                                            Lane<N>& ln)
+// Dense output of coordinate c (state index 2c) at x = (t_k - t) / h of an accepted step:
+// DOP853's 7th-order continuous extension (Hairer's contd8) in Horner form.
+template <int NQ>
+__device__ __forceinline__ double dense_coord(const double (&Fc)[7][NQ], int c, double y0, double x)
 {
-    const JobDesc& jb = jobs[ln.job];
+    const double omx = 1.0 - x;
+    double v = Fc[5][c] + x * Fc[6][c];
+    v = Fc[4][c] + omx * v;
+    v = Fc[3][c] + x * v;
+    v = Fc[2][c] + omx * v;
+    v = Fc[1][c] + x * v;
+    v = Fc[0][c] + omx * v;
+    return y0 + x * v;
+}
+
+// CF value mu(0) . mu(t_k) of a sample
+template <typename P>
+__device__ __forceinline__ double cf_value(const double* mu0, const P (&m)[3])
+{
+    return mu0[0] * (double)m[0] + mu0[1] * (double)m[1] + mu0[2] * (double)m[2];
+}
+
+// Checks of one sample in the reference's order (src/hawaii.c:2181-2236): a NaN dipole, a
+// huge CF value, a jump by more than 1e10 from the previous sample (from sample 2 on).
+template <typename P>
+__device__ __forceinline__ int sample_status(const P (&m)[3], double curr, double prev, int k)
+{
+    if (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2])) return ITEM_FAIL_DIPOLE_NAN;
+    if (fabs(curr) > 1e100) return ITEM_FAIL_CF_HUGE;
+    if (k > 1 && fabs(curr / prev) > 1e10) return ITEM_FAIL_CF_JUMP;
+    return ITEM_OK;
+}
+
+// sample k of an item whose row starts at `row`: mu(0) . mu(t_k), or the three dipole
+// components (rows [3][L]) when the job records the series
+template <typename P>
+__device__ __forceinline__ void store_sample(double* row, int L, int record, int k, const P (&m)[3], double curr)
+{
+    if (record) {
+        row[k] = (double)m[0];
+        row[L + k] = (double)m[1];
+        row[2 * (long long)L + k] = (double)m[2];
+    } else {
+        row[k] = curr;
+    }
+}
+
+enum StepOutcome : int { STEP_ENDED = 0, STEP_REJECTED = 1, STEP_ACCEPTED = 2 };
+
+// One attempted DOP853 step [t, t + h] of a lane: stages, error estimate, step-size control.
+// STEP_ACCEPTED: yn holds the new state, h the step taken and fac11 the error factor; when a
+// sample time falls in (t, t + h] (*dense) the dense-output coefficients Fc of the coordinates
+// are computed as well.  STEP_ENDED: the item stopped (finish_item was called).
+template <int SYS, typename P, int N>
+__device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc& jb, const Arena<N>& ar, Lane<N>& ln,
+                                              double (&yn)[N], double& h_taken, double& fac11_out, bool& dense,
+                                              double (&Fc)[7][N / 2])
+{
     const double tend = (jb.L - 1) * jb.dt;
+    dense = false;
     // CVode's mxstep test (src/trajectory.c:15, 62): max_steps accepted steps without reaching
     // the next sample time; kMaxTotalSteps bounds the step attempts of the item's whole life,
     // so every lane's loop ends.
     if (ln.steps >= jb.max_steps || ++ln.total_steps > kMaxTotalSteps) {
         finish_item<N>(jb, ln, ITEM_FAIL_STEPS, ln.k - 1, 0.0f);
-        return true;
+        return STEP_ENDED;
     }
     if (0.1 * fabs(ln.h) <= fabs(ln.t) * kEPS) {
         finish_item<N>(jb, ln, ITEM_FAIL_STEPSIZE, ln.k - 1, 0.0f);
-        return true;
+        return STEP_ENDED;
     }
     double h = ln.h;
     if (ln.t + 1.01 * h - tend > 0.0) h = tend - ln.t;
@@ -293,7 +351,6 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     arena_fence();
     // y_new (B row) and the two error estimates (E5, E3 rows) share their columns
     static_assert(DOP853_ROW_E5 == DOP853_ROW_B + 1 && DOP853_ROW_E3 == DOP853_ROW_B + 2, "tableau row order");
-    double yn[N];
     double err = 0.0, err2 = 0.0;
     {
         double be[3][N];
@@ -316,22 +373,19 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     const double fac11 = sqrt(sqrt(sqrt(err)));
     if (!(err == err) || !(h == h)) {   // non-finite state: stop the item
         finish_item<N>(jb, ln, ITEM_FAIL_STEPSIZE, ln.k - 1, 0.0f);
-        return true;
+        return STEP_ENDED;
     }
-
     if (err > 1.0) {
         ln.h = h / fmin(1.0 / 0.333, fac11 / 0.9);
         ln.reject = 1;
-        return false;
+        return STEP_REJECTED;
     }
-
-    // accepted step [t, t + h]: record every sample t_k in (t, t + h]
     ++ln.steps;
+    h_taken = h;
+    fac11_out = fac11;
     const double tnew = ln.t + h;
-    bool ended = false;
-    int status = ITEM_OK;
-    float drift = 0.0f;
     if (ln.k < jb.L && ln.k * jb.dt <= tnew) {
+        dense = true;
         double nz_dense;
         stages<SYS, P, N>(sc, ar, ln.y, h, 13, 15, nz_dense);
         ln.nrhs += 3;
@@ -340,7 +394,6 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
         // coordinates only (even state indices: the dipole and the R > Rcut test need no
         // momenta), kept in registers for all samples of the step
         constexpr int NQ = N / 2;
-        double Fc[7][NQ];
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
             const int i = 2 * c;
@@ -350,14 +403,59 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
             Fc[1][c] = h * k0 - dy;
             Fc[2][c] = 2.0 * dy - h * (ar.ld(12, i) + k0);
         }
-        {
-            double f[4][N];
-            rows_dot<N, 4, 3, 2>(ar, DOP853_ROW_D, f);
+        double f[4][N];
+        rows_dot<N, 4, 3, 2>(ar, DOP853_ROW_D, f);
 #pragma unroll
-            for (int d = 0; d < 4; ++d)
+        for (int d = 0; d < 4; ++d)
 #pragma unroll
-                for (int c = 0; c < NQ; ++c) Fc[3 + d][c] = h * f[d][2 * c];
-        }
+            for (int c = 0; c < NQ; ++c) Fc[3 + d][c] = h * f[d][2 * c];
+    }
+    return STEP_ACCEPTED;
+}
+
+// After an accepted step that did not end the item: y <- y_new, f(y) <- K12
+// (first-same-as-last), t <- t + h and the next step size.
+template <int N>
+__device__ __forceinline__ void advance(const JobDesc& jb, const Arena<N>& ar, Lane<N>& ln, const double (&yn)[N],
+                                        double h, double fac11)
+{
+    const double tend = (jb.L - 1) * jb.dt;
+    arena_fence();
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        ln.y[i] = yn[i];
+        ar.st(0, i, ar.ld(12, i));
+    }
+    ln.t = ln.t + h;
+    const double fac = fmax(1.0 / 6.0, fmin(1.0 / 0.333, fac11 / 0.9));
+    double hnew = h / fac;
+    if (fabs(hnew) > tend) hnew = tend;
+    if (ln.reject) {
+        hnew = fmin(fabs(hnew), fabs(h));
+        ln.reject = 0;
+    }
+    ln.h = hnew;
+}
+
+// One attempted DOP853 step for a lane that records its own samples (the plain engine: one
+// item per lane for its whole life).  Returns true when the lane's item has ended.
+template <int SYS, typename P, int N>
+__device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* __restrict__ jobs, const Arena<N>& ar,
+                                            Lane<N>& ln)
+{
+    constexpr int NQ = N / 2;
+    const JobDesc& jb = jobs[ln.job];
+    double yn[N], Fc[7][NQ], h, fac11;
+    bool dense;
+    const int outcome = dop853_attempt<SYS, P, N>(sc, jb, ar, ln, yn, h, fac11, dense, Fc);
+    if (outcome == STEP_ENDED) return true;
+    if (outcome == STEP_REJECTED) return false;
+    // accepted step [t, t + h]: record every sample t_k in (t, t + h]
+    const double tnew = ln.t + h;
+    bool ended = false;
+    int status = ITEM_OK;
+    float drift = 0.0f;
+    if (dense) {
         double* row = jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1);
         while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
             const double tk = ln.k * jb.dt;
@@ -366,46 +464,23 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
 #pragma unroll
                 for (int i = 0; i < N; ++i) ys[i] = yn[i];
             } else {
-                const double x = (tk - ln.t) / h, omx = 1.0 - x;
+                const double x = (tk - ln.t) / h;
 #pragma unroll
                 for (int i = 1; i < N; i += 2) ys[i] = 0.0;
 #pragma unroll
-                for (int c = 0; c < NQ; ++c) {
-                    double v = Fc[5][c] + x * Fc[6][c];
-                    v = Fc[4][c] + omx * v;
-                    v = Fc[3][c] + x * v;
-                    v = Fc[2][c] + omx * v;
-                    v = Fc[1][c] + x * v;
-                    v = Fc[0][c] + omx * v;
-                    ys[2 * c] = ln.y[2 * c] + x * v;
-                }
+                for (int c = 0; c < NQ; ++c) ys[2 * c] = dense_coord<NQ>(Fc, c, ln.y[2 * c], x);
             }
             P m[3];
             dipole<SYS, P>(sc, ys, m);
-            if (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2])) {
-                status = ITEM_FAIL_DIPOLE_NAN;
-                ended = true;
-                break;
-            }
+            const double curr = cf_value<P>(ln.mu0, m);
+            status = sample_status<P>(m, curr, ln.curr, ln.k);
             ln.prev = ln.curr;
-            ln.curr = ln.mu0[0] * (double)m[0] + ln.mu0[1] * (double)m[1] + ln.mu0[2] * (double)m[2];
-            if (fabs(ln.curr) > 1e100) {
-                status = ITEM_FAIL_CF_HUGE;
+            ln.curr = curr;
+            if (status != ITEM_OK) {
                 ended = true;
                 break;
             }
-            if (ln.k > 1 && fabs(ln.curr / ln.prev) > 1e10) {
-                status = ITEM_FAIL_CF_JUMP;
-                ended = true;
-                break;
-            }
-            if (jb.record) {
-                row[ln.k] = (double)m[0];
-                row[jb.L + ln.k] = (double)m[1];
-                row[2 * (long long)jb.L + ln.k] = (double)m[2];
-            } else {
-                row[ln.k] = ln.curr;
-            }
+            store_sample<P>(row, jb.L, jb.record, ln.k, m, curr);
             ln.steps = 0;
             if (ys[4] > jb.Rcut || ln.k + 1 >= jb.L) {
                 // energy check of make_step (src/trajectory.c:167-180), at the end of the step
@@ -422,23 +497,221 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
         finish_item<N>(jb, ln, status, status == ITEM_OK ? ln.k : ln.k - 1, drift);
         return true;
     }
-    // advance: y <- y_new, f(y) <- K12 (first-same-as-last)
-    arena_fence();
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        ln.y[i] = yn[i];
-        ar.st(0, i, ar.ld(12, i));
-    }
-    ln.t = tnew;
-    const double fac = fmax(1.0 / 6.0, fmin(1.0 / 0.333, fac11 / 0.9));
-    double hnew = h / fac;
-    if (fabs(hnew) > tend) hnew = tend;
-    if (ln.reject) {
-        hnew = fmin(fabs(hnew), fabs(h));
-        ln.reject = 0;
-    }
-    ln.h = hnew;
+    advance<N>(jb, ar, ln, yn, h, fac11);
     return false;
+}
+
+// ------------------------------------------------------------------ wavefront-shared sampling
+// In the batched engine the samples of a step are evaluated by the whole wavefront.  A lane's
+// step covers a lane-dependent number of sample times (none inside a collision, dozens in free
+// flight), so a lane-serial loop would run as many dipole passes as the wavefront's busiest
+// lane while the others idle.  Instead every lane with samples publishes its step (state,
+// dense-output coefficients, CF context) in arena slots 1.. (free once the step's stages are
+// combined) and the wavefront evaluates all pending samples 64 at a time: sample g of the
+// wavefront's list belongs to the lane whose exclusive prefix count is the largest <= g.
+// Values, checks and the stored samples are those of the lane-serial loop: sample k compares
+// with sample k - 1 (the previous pass's last lane, or the lane's CF value at the step start)
+// and only samples before the first failing one are stored.
+struct WaveShare {
+    int pre[65];      // exclusive prefix of the lanes' sample counts; pre[64] = total
+    int fail[64];     // (index in the step << 3) | status of the lane's first failing sample
+    double last[64];  // CF value of the lane's last sample
+};
+
+template <int NQ>
+struct StepRecord {
+    static constexpr int Y = 0, YN = NQ, FC = 2 * NQ, T = 9 * NQ, H = T + 1, TNEW = T + 2, MU0 = T + 3,
+                         CURR = T + 6, K0 = T + 7, ROW = T + 8, L = T + 9, DT = T + 10, RECORD = T + 11,
+                         SIZE = T + 12;
+};
+
+template <int N>
+__device__ __forceinline__ void rec_store(const Arena<N>& ar, int f, double v)
+{
+    ar.st(1 + f / N, f % N, v);
+}
+
+template <int N>
+__device__ __forceinline__ double rec_load(const Arena<N>& ar, int owner, int f)
+{
+    return ar.ld_lane(owner, 1 + f / N, f % N);
+}
+
+// Memory written by one lane (LDS, the wavefront's arena block) is visible to the others
+// after this; every lane of the wavefront executes it.  A wavefront's own memory operations
+// are performed in order and its lanes share one vector L1, so the wavefront-scope fence only
+// keeps the compiler from moving accesses across it (it emits no wait).
+__device__ __forceinline__ void wave_share_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Evaluate, check and store the n pending samples of every lane (n = 0: none).  Called by all
+// 64 lanes of the wavefront with the same control flow.
+template <int SYS, typename P, int N>
+__device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N>& ar, WaveShare& ws, int n)
+{
+    constexpr int NQ = N / 2;
+    using R = StepRecord<NQ>;
+    const int lane = (int)__lane_id();
+    int incl = n;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (total == 0) return;
+    ws.pre[lane] = incl - n;
+    if (lane == 0) ws.pre[64] = total;
+    ws.fail[lane] = 0x7fffffff;
+    wave_share_sync();
+    double carry = 0.0;   // CF value of the previous pass's last sample
+#pragma unroll 1
+    for (int base = 0; base < total; base += 64) {
+        const int g = base + lane;
+        const bool valid = g < total;
+        int owner = 0, q = 0, k = 0, cnt = 0;
+        double curr = 0.0, prev_start = 0.0;
+        P m[3] = {P(0), P(0), P(0)};
+        if (valid) {
+            int lo = 0, hi = 63;
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (ws.pre[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            owner = lo;
+            q = g - ws.pre[owner];
+            cnt = ws.pre[owner + 1] - ws.pre[owner];
+            k = (int)rec_load<N>(ar, owner, R::K0) + q;
+            const double t = rec_load<N>(ar, owner, R::T), h = rec_load<N>(ar, owner, R::H);
+            const double tnew = rec_load<N>(ar, owner, R::TNEW);
+            const double tk = k * rec_load<N>(ar, owner, R::DT);
+            double ys[N];
+#pragma unroll
+            for (int i = 1; i < N; i += 2) ys[i] = 0.0;
+            if (tk == tnew) {
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) ys[2 * c] = rec_load<N>(ar, owner, R::YN + c);
+            } else {
+                double Fc[7][NQ];
+#pragma unroll
+                for (int d = 0; d < 7; ++d)
+#pragma unroll
+                    for (int c = 0; c < NQ; ++c) Fc[d][c] = rec_load<N>(ar, owner, R::FC + d * NQ + c);
+                const double x = (tk - t) / h;
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) ys[2 * c] = dense_coord<NQ>(Fc, c, rec_load<N>(ar, owner, R::Y + c), x);
+            }
+            dipole<SYS, P>(sc, ys, m);
+            const double mu0[3] = {rec_load<N>(ar, owner, R::MU0), rec_load<N>(ar, owner, R::MU0 + 1),
+                                   rec_load<N>(ar, owner, R::MU0 + 2)};
+            curr = cf_value<P>(mu0, m);
+            prev_start = rec_load<N>(ar, owner, R::CURR);
+        }
+        const double up = __shfl_up(curr, 1, 64);
+        const double prev = q == 0 ? prev_start : (lane > 0 ? up : carry);
+        carry = __shfl(curr, 63, 64);
+        if (valid) {
+            const int st = sample_status<P>(m, curr, prev, k);
+            if (st != ITEM_OK) atomicMin(&ws.fail[owner], (q << 3) | st);
+        }
+        wave_share_sync();
+        if (valid) {
+            if (q < (ws.fail[owner] >> 3)) {
+                double* row = __builtin_bit_cast(double*, rec_load<N>(ar, owner, R::ROW));
+                store_sample<P>(row, (int)rec_load<N>(ar, owner, R::L), (int)rec_load<N>(ar, owner, R::RECORD), k, m,
+                                curr);
+            }
+            if (q == cnt - 1) ws.last[owner] = curr;
+        }
+    }
+    wave_share_sync();
+}
+
+// One attempted DOP853 step of every lane of a wavefront (the batched engine); lanes without
+// an item take part in the sample evaluation only.
+template <int SYS, typename P, int N>
+__device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const JobDesc* __restrict__ ring,
+                                                   const Arena<N>& ar, Lane<N>& ln, WaveShare& ws)
+{
+    constexpr int NQ = N / 2;
+    using R = StepRecord<NQ>;
+    static_assert(1 + (R::SIZE - 1) / N < 12, "the step record must leave K0 and K12 alone");
+    double yn[N], h = 0.0, fac11 = 0.0;
+    int outcome = STEP_ENDED, n = 0;
+    bool end = false;
+    if (ln.active) {
+        const JobDesc& jb = ring[ln.job];
+        double Fc[7][NQ];
+        bool dense;
+        outcome = dop853_attempt<SYS, P, N>(sc, jb, ar, ln, yn, h, fac11, dense, Fc);
+        if (outcome == STEP_ACCEPTED && dense) {
+            // the samples of this step, up to the one that ends the item (R > Rcut on the
+            // dense output, or the last sample index)
+            const double tnew = ln.t + h;
+            for (int k = ln.k; k < jb.L && k * jb.dt <= tnew; ++k) {
+                ++n;
+                const double tk = k * jb.dt;
+                const double Rk = tk == tnew ? yn[4] : dense_coord<NQ>(Fc, 2, ln.y[4], (tk - ln.t) / h);
+                if (Rk > jb.Rcut || k + 1 >= jb.L) {
+                    end = true;
+                    break;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                rec_store<N>(ar, R::Y + c, ln.y[2 * c]);
+                rec_store<N>(ar, R::YN + c, yn[2 * c]);
+            }
+#pragma unroll
+            for (int d = 0; d < 7; ++d)
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) rec_store<N>(ar, R::FC + d * NQ + c, Fc[d][c]);
+            rec_store<N>(ar, R::T, ln.t);
+            rec_store<N>(ar, R::H, h);
+            rec_store<N>(ar, R::TNEW, tnew);
+            rec_store<N>(ar, R::MU0, ln.mu0[0]);
+            rec_store<N>(ar, R::MU0 + 1, ln.mu0[1]);
+            rec_store<N>(ar, R::MU0 + 2, ln.mu0[2]);
+            rec_store<N>(ar, R::CURR, ln.curr);
+            rec_store<N>(ar, R::K0, (double)ln.k);
+            rec_store<N>(ar, R::ROW,
+                         __builtin_bit_cast(double, jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1)));
+            rec_store<N>(ar, R::L, (double)jb.L);
+            rec_store<N>(ar, R::DT, jb.dt);
+            rec_store<N>(ar, R::RECORD, (double)jb.record);
+        }
+    }
+    shared_samples<SYS, P, N>(sc, ar, ws, n);
+    if (outcome != STEP_ACCEPTED) return;
+    const JobDesc& jb = ring[ln.job];
+    if (n > 0) {
+        const int lane = (int)__lane_id();
+        const int f = ws.fail[lane];
+        if (f != 0x7fffffff) {
+            // a failed check stops before storing its sample
+            finish_item<N>(jb, ln, f & 7, ln.k + (f >> 3) - 1, 0.0f);
+            return;
+        }
+        ln.steps = 0;
+        if (end) {
+            // energy check of make_step (src/trajectory.c:167-180), at the end of the step
+            // that holds the last sample
+            const float drift = (float)fabs(hamiltonian<SYS, P>(sc, yn) - ln.E0);
+            finish_item<N>(jb, ln, ITEM_OK, ln.k + n - 1, drift);
+            return;
+        }
+        ln.k += n;
+        ln.prev = ln.curr;
+        ln.curr = ws.last[lane];
+    }
+    advance<N>(jb, ar, ln, yn, h, fac11);
 }
 
 // ------------------------------------------------------------------ kernels
@@ -555,24 +828,28 @@ __device__ __forceinline__ int ring_job(const JobDesc* __restrict__ ring, const 
 // One slice of the batched engine.  Every lane slot restores its item (if any), integrates
 // and refills from the queue until the slice's time is up or no work is left, and saves its
 // state at a step boundary, so slices can be separated by admission of new jobs, reductions
-// of finished ones and repacking of the slots.  Each lane's loop ends: the deadline and the
-// queue bound it, and every item ends within kMaxTotalSteps attempts.
+// of finished ones and repacking of the slots.  The loop is wavefront-uniform (lanes without
+// an item stay in it to evaluate samples of the others) and ends: the deadline and the queue
+// bound it, and every item ends within kMaxTotalSteps attempts.
 template <int SYS, typename P, int BS>
 __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, const JobDesc* __restrict__ ring,
                                                                    PoolCtl ctl, LaneSlots sl, double* __restrict__ ws)
 {
     constexpr int N = SysTraits<SYS>::QP;
     __shared__ Co2ArTables<P> s_tab;
+    __shared__ WaveShare s_share[BS / 64];
     const SysConst scl = stage_tables<SYS, P>(sc, &s_tab);
     const long long wave = (long long)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
     const Arena<N> ar = wave_arena<N>(ws, wave);
+    WaveShare& share = s_share[threadIdx.x >> 6];
     const long long slot = (long long)blockIdx.x * BS + threadIdx.x;
     const long long t_end = wall_clock64() + ctl.ticks;
     Lane<N> ln;
     restore_lane<N>(sl, slot, ar, ln);
+    bool queue_open = true;   // the queue only grows between slices
     for (;;) {
         if (wall_clock64() >= t_end) break;
-        if (!ln.active) {
+        if (!ln.active && queue_open) {
             bool got = false;
             while (!got) {
                 if (__hip_atomic_load(ctl.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= ctl.total) break;
@@ -587,9 +864,10 @@ __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, 
                 }
                 got = init_item<SYS, P, N>(scl, ring[j], j, (long long)item - ring[j].item_begin, ar, ln);
             }
-            if (!got) break;
+            queue_open = got;
         }
-        dop853_step<SYS, P, N>(scl, ring, ar, ln);
+        if (__ballot(ln.active) == 0) break;
+        dop853_step_shared<SYS, P, N>(scl, ring, ar, ln, share);
     }
     save_lane<N>(sl, slot, ar, ln);
 }

@@ -194,54 +194,74 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
     return true;
 }
 
-// out[j][i] = sum over the non-zeros q of tableau row r0 + j of A[r0 + j][q] * K_{col q}[i] for
+// out[j][i] = sum over the non-zeros q of tableau row R0 + j of A[R0 + j][q] * K_{col q}[i] for
 // NR consecutive rows that share one column list (rows 16-18: B, E5, E3; rows 19-22: the dense
-// output D rows).  Columns are loaded G at a time and each loaded column serves all NR rows, so
-// the row set costs ceil(nnz / G) memory round trips instead of NR * nnz.  Components i with
-// i % STRIDE == 0 only (STRIDE 2: the coordinates).  Per row the sum runs in the same order as
-// row_dot, so the results are identical.
-template <int N, int NR, int G, int STRIDE>
-__device__ __forceinline__ void rows_dot(const Arena<N>& ar, int r0, double (&out)[NR][N])
+// output D rows), components i with i % STRIDE == 0 only (STRIDE 2: the coordinates).  The row
+// is a compile-time constant, so columns, coefficients and arena offsets are immediates.
+// Columns are loaded G at a time (a compiler fence after each group bounds the registers held)
+// and each loaded column serves all NR rows; per row the sum runs over q in increasing order.
+__device__ __forceinline__ void arena_fence();
+
+template <int N, int R0, int NR, int G, int STRIDE>
+__device__ __forceinline__ void rows_dot(const Arena<N>& ar, double (&out)[NR][N])
 {
+    constexpr int nnz = DOP853_SP_NNZ[R0];
 #pragma unroll
     for (int j = 0; j < NR; ++j)
 #pragma unroll
         for (int i = 0; i < N; ++i) out[j][i] = 0.0;
-    const int nnz = DOP853_SP_NNZ[r0];
-#pragma unroll 1
+#pragma unroll
     for (int q0 = 0; q0 < nnz; q0 += G) {
         double v[G][N];
 #pragma unroll
-        for (int g = 0; g < G; ++g) {
-            if (q0 + g < nnz) {
-                const int m = DOP853_SP_COL[r0][q0 + g];
-#pragma unroll
-                for (int i = 0; i < N; i += STRIDE) v[g][i] = ar.ld(m, i);
-            }
-        }
-#pragma unroll
-        for (int g = 0; g < G; ++g) {
+        for (int g = 0; g < G; ++g)
             if (q0 + g < nnz) {
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    const double a = DOP853_SP_VAL[r0 + j][q0 + g];
-#pragma unroll
-                    for (int i = 0; i < N; i += STRIDE) out[j][i] += a * v[g][i];
-                }
+                for (int i = 0; i < N; i += STRIDE) v[g][i] = ar.ld(DOP853_SP_COL[R0][q0 + g], i);
             }
-        }
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (q0 + g < nnz) {
+#pragma unroll
+                for (int j = 0; j < NR; ++j)
+#pragma unroll
+                    for (int i = 0; i < N; i += STRIDE) out[j][i] += DOP853_SP_VAL[R0 + j][q0 + g] * v[g][i];
+            }
+        arena_fence();
     }
 }
 
-// out = sum over the non-zero entries (m, a) of sparse tableau row r of a * K_m (arena slots).
-// The row index is uniform across the wavefront, so the coefficients are scalar loads.
+template <int N, int R>
+__device__ __forceinline__ void row_dot_c(const Arena<N>& ar, double* out)
+{
+    double o[1][N];
+    rows_dot<N, R, 1, 3, 1>(ar, o);
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = o[0][i];
+}
+
+// out = sum over the non-zero entries (m, a) of tableau row r (1..15, uniform across the
+// wavefront) of a * K_m: one compile-time row per case
 template <int N>
 __device__ __forceinline__ void row_dot(const Arena<N>& ar, int r, double* out)
 {
-    double o[1][N];
-    rows_dot<N, 1, 3, 1>(ar, r, o);
-#pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = o[0][i];
+    switch (r) {
+    case 1: row_dot_c<N, 1>(ar, out); break;
+    case 2: row_dot_c<N, 2>(ar, out); break;
+    case 3: row_dot_c<N, 3>(ar, out); break;
+    case 4: row_dot_c<N, 4>(ar, out); break;
+    case 5: row_dot_c<N, 5>(ar, out); break;
+    case 6: row_dot_c<N, 6>(ar, out); break;
+    case 7: row_dot_c<N, 7>(ar, out); break;
+    case 8: row_dot_c<N, 8>(ar, out); break;
+    case 9: row_dot_c<N, 9>(ar, out); break;
+    case 10: row_dot_c<N, 10>(ar, out); break;
+    case 11: row_dot_c<N, 11>(ar, out); break;
+    case 12: row_dot_c<N, 12>(ar, out); break;
+    case 13: row_dot_c<N, 13>(ar, out); break;
+    case 14: row_dot_c<N, 14>(ar, out); break;
+    default: row_dot_c<N, 15>(ar, out); break;
+    }
 }
 
 // Compiler-only barrier: stage vectors are re-read from the arena after it instead of being
@@ -354,7 +374,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
     double err = 0.0, err2 = 0.0;
     {
         double be[3][N];
-        rows_dot<N, 3, 2, 1>(ar, DOP853_ROW_B, be);
+        rows_dot<N, DOP853_ROW_B, 3, 2, 1>(ar, be);
         // FP32 surfaces: the momentum tolerance is raised by the step's force rounding noise
         // times h, so the controller does not shrink the step to resolve FP32 noise (zero and
         // without effect in FP64)
@@ -404,7 +424,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
             Fc[2][c] = 2.0 * dy - h * (ar.ld(12, i) + k0);
         }
         double f[4][N];
-        rows_dot<N, 4, 3, 2>(ar, DOP853_ROW_D, f);
+        rows_dot<N, DOP853_ROW_D, 4, 3, 2>(ar, f);
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll

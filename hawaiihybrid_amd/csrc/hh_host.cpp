@@ -353,10 +353,11 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         if (block == 0 && e->use_tuner) {
             rc = hh::pool_tune(e, *sys, *p, ic_dev, ntraj, njobs, &block, &waves);
             if (rc < 0) return rc;
-        } else if (block == 0) {
-            block = 256;
-        } else if (block != 64 && block != 128 && block != 256) {
+        } else if (block != 0 && block != 64 && block != 128 && block != 256) {
             return fail(HH_E_INVALID, "block must be 0 (tuned), 64, 128 or 256");
+        } else {
+            if (block == 0) block = 256;
+            waves = hh::pool_waves(block);
         }
         hh::LanePool* pl = nullptr;
         rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, block, &pl, waves);

@@ -22,8 +22,8 @@ namespace hh {
 int arena_doubles_per_lane(int sys);
 hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
                      hipStream_t st);
-hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl,
-                    const LaneSlots& sl, double* ws, hipStream_t st);
+hipError_t run_pool(int sys, bool mixed, int block, int waves, const SysConst& sc, const JobDesc* ring,
+                    const PoolCtl& ctl, const LaneSlots& sl, double* ws, hipStream_t st);
 hipError_t run_slots_clear(const LaneSlots& sl, hipStream_t st);
 hipError_t run_clamp_next(unsigned long long* next, unsigned long long total, hipStream_t st);
 // right-hand sides spent on the job in ring slot `ring_slot` (added to *out)
@@ -176,6 +176,9 @@ namespace hh {
 // the ensemble CF of a finished job into cf_out (on the engine stream)
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;
+// resident wavefronts per SIMD of the default launch shape for a workgroup size: 256-lane
+// workgroups run one wavefront per SIMD (whole tableau rows in registers), 64 and 128 two
+inline int pool_waves(int block) { return block == 256 ? 1 : 2; }
 // the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
 int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out,
              int waves = 2);

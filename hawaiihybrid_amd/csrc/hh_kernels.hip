@@ -231,36 +231,44 @@ __device__ __forceinline__ void rows_dot(const Arena<N>& ar, double (&out)[NR][N
     }
 }
 
-template <int N, int R>
+// Column groups of the tableau rows for a kernel resident at W wavefronts per SIMD: at two
+// the register budget (256) holds a few columns at a time; at one (512) whole rows, one memory
+// round trip per row.
+template <int W>
+struct Groups {
+    static constexpr int stage = W == 1 ? 16 : 3, be = W == 1 ? 8 : 2, dense = W == 1 ? 12 : 3;
+};
+
+template <int N, int R, int G>
 __device__ __forceinline__ void row_dot_c(const Arena<N>& ar, double* out)
 {
     double o[1][N];
-    rows_dot<N, R, 1, 3, 1>(ar, o);
+    rows_dot<N, R, 1, G, 1>(ar, o);
 #pragma unroll
     for (int i = 0; i < N; ++i) out[i] = o[0][i];
 }
 
 // out = sum over the non-zero entries (m, a) of tableau row r (1..15, uniform across the
 // wavefront) of a * K_m: one compile-time row per case
-template <int N>
+template <int N, int G>
 __device__ __forceinline__ void row_dot(const Arena<N>& ar, int r, double* out)
 {
     switch (r) {
-    case 1: row_dot_c<N, 1>(ar, out); break;
-    case 2: row_dot_c<N, 2>(ar, out); break;
-    case 3: row_dot_c<N, 3>(ar, out); break;
-    case 4: row_dot_c<N, 4>(ar, out); break;
-    case 5: row_dot_c<N, 5>(ar, out); break;
-    case 6: row_dot_c<N, 6>(ar, out); break;
-    case 7: row_dot_c<N, 7>(ar, out); break;
-    case 8: row_dot_c<N, 8>(ar, out); break;
-    case 9: row_dot_c<N, 9>(ar, out); break;
-    case 10: row_dot_c<N, 10>(ar, out); break;
-    case 11: row_dot_c<N, 11>(ar, out); break;
-    case 12: row_dot_c<N, 12>(ar, out); break;
-    case 13: row_dot_c<N, 13>(ar, out); break;
-    case 14: row_dot_c<N, 14>(ar, out); break;
-    default: row_dot_c<N, 15>(ar, out); break;
+    case 1: row_dot_c<N, 1, G>(ar, out); break;
+    case 2: row_dot_c<N, 2, G>(ar, out); break;
+    case 3: row_dot_c<N, 3, G>(ar, out); break;
+    case 4: row_dot_c<N, 4, G>(ar, out); break;
+    case 5: row_dot_c<N, 5, G>(ar, out); break;
+    case 6: row_dot_c<N, 6, G>(ar, out); break;
+    case 7: row_dot_c<N, 7, G>(ar, out); break;
+    case 8: row_dot_c<N, 8, G>(ar, out); break;
+    case 9: row_dot_c<N, 9, G>(ar, out); break;
+    case 10: row_dot_c<N, 10, G>(ar, out); break;
+    case 11: row_dot_c<N, 11, G>(ar, out); break;
+    case 12: row_dot_c<N, 12, G>(ar, out); break;
+    case 13: row_dot_c<N, 13, G>(ar, out); break;
+    case 14: row_dot_c<N, 14, G>(ar, out); break;
+    default: row_dot_c<N, 15, G>(ar, out); break;
     }
 }
 
@@ -270,7 +278,7 @@ __device__ __forceinline__ void arena_fence() { __asm__ __volatile__("" ::: "mem
 
 // stages s0..s1 (inclusive): K_s = f(y + h sum_m A[s][m] K_m); noise: the largest FP32 noise
 // bound of the stages' momentum derivatives
-template <int SYS, typename P, int N>
+template <int SYS, typename P, int N, int W = 2>
 __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, const double* y, double h, int s0,
                                        int s1, double& noise)
 {
@@ -279,7 +287,7 @@ __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, c
     for (int s = s0; s <= s1; ++s) {
         arena_fence();
         double yt[N], k[N];
-        row_dot<N>(ar, s, yt);
+        row_dot<N, Groups<W>::stage>(ar, s, yt);
 #pragma unroll
         for (int i = 0; i < N; ++i) yt[i] = y[i] + h * yt[i];
         double nz;
@@ -343,7 +351,7 @@ enum StepOutcome : int { STEP_ENDED = 0, STEP_REJECTED = 1, STEP_ACCEPTED = 2 };
 // STEP_ACCEPTED: yn holds the new state, h the step taken and fac11 the error factor; when a
 // sample time falls in (t, t + h] (*dense) the dense-output coefficients Fc of the coordinates
 // are computed as well.  STEP_ENDED: the item stopped (finish_item was called).
-template <int SYS, typename P, int N>
+template <int SYS, typename P, int N, int W = 2>
 __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc& jb, const Arena<N>& ar, Lane<N>& ln,
                                               double (&yn)[N], double& h_taken, double& fac11_out, bool& dense,
                                               double (&Fc)[7][N / 2])
@@ -366,7 +374,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
 
     // stages 1..11 and stage 12 = f(t + h, y_new) (tableau row 12 is the B row)
     double noise;
-    stages<SYS, P, N>(sc, ar, ln.y, h, 1, 12, noise);
+    stages<SYS, P, N, W>(sc, ar, ln.y, h, 1, 12, noise);
     ln.nrhs += 12;
     arena_fence();
     // y_new (B row) and the two error estimates (E5, E3 rows) share their columns
@@ -374,7 +382,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
     double err = 0.0, err2 = 0.0;
     {
         double be[3][N];
-        rows_dot<N, DOP853_ROW_B, 3, 2, 1>(ar, be);
+        rows_dot<N, DOP853_ROW_B, 3, Groups<W>::be, 1>(ar, be);
         // FP32 surfaces: the momentum tolerance is raised by the step's force rounding noise
         // times h, so the controller does not shrink the step to resolve FP32 noise (zero and
         // without effect in FP64)
@@ -407,7 +415,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
     if (ln.k < jb.L && ln.k * jb.dt <= tnew) {
         dense = true;
         double nz_dense;
-        stages<SYS, P, N>(sc, ar, ln.y, h, 13, 15, nz_dense);
+        stages<SYS, P, N, W>(sc, ar, ln.y, h, 13, 15, nz_dense);
         ln.nrhs += 3;
         arena_fence();
         // F0 = dy, F1 = h K0 - dy, F2 = 2 dy - h (K12 + K0), F3..F6 = h (D K), for the
@@ -424,7 +432,7 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
             Fc[2][c] = 2.0 * dy - h * (ar.ld(12, i) + k0);
         }
         double f[4][N];
-        rows_dot<N, DOP853_ROW_D, 4, 3, 2>(ar, f);
+        rows_dot<N, DOP853_ROW_D, 4, Groups<W>::dense, 2>(ar, f);
 #pragma unroll
         for (int d = 0; d < 4; ++d)
 #pragma unroll
@@ -656,7 +664,7 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
 
 // One attempted DOP853 step of every lane of a wavefront (the batched engine); lanes without
 // an item take part in the sample evaluation only.
-template <int SYS, typename P, int N>
+template <int SYS, typename P, int N, int W>
 __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const JobDesc* __restrict__ ring,
                                                    const Arena<N>& ar, Lane<N>& ln, WaveShare& ws)
 {
@@ -670,7 +678,7 @@ __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const Job
         const JobDesc& jb = ring[ln.job];
         double Fc[7][NQ];
         bool dense;
-        outcome = dop853_attempt<SYS, P, N>(sc, jb, ar, ln, yn, h, fac11, dense, Fc);
+        outcome = dop853_attempt<SYS, P, N, W>(sc, jb, ar, ln, yn, h, fac11, dense, Fc);
         if (outcome == STEP_ACCEPTED && dense) {
             // the samples of this step, up to the one that ends the item (R > Rcut on the
             // dense output, or the last sample index)
@@ -851,8 +859,8 @@ __device__ __forceinline__ int ring_job(const JobDesc* __restrict__ ring, const 
 // of finished ones and repacking of the slots.  The loop is wavefront-uniform (lanes without
 // an item stay in it to evaluate samples of the others) and ends: the deadline and the queue
 // bound it, and every item ends within kMaxTotalSteps attempts.
-template <int SYS, typename P, int BS>
-__global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, const JobDesc* __restrict__ ring,
+template <int SYS, typename P, int BS, int W>
+__global__ void __launch_bounds__(BS, W) pool_cf_kernel(SysConst sc, const JobDesc* __restrict__ ring,
                                                                    PoolCtl ctl, LaneSlots sl, double* __restrict__ ws)
 {
     constexpr int N = SysTraits<SYS>::QP;
@@ -887,7 +895,7 @@ __global__ void __launch_bounds__(BS, HH_MIN_WAVES) pool_cf_kernel(SysConst sc, 
             queue_open = got;
         }
         if (__ballot(ln.active) == 0) break;
-        dop853_step_shared<SYS, P, N>(scl, ring, ar, ln, share);
+        dop853_step_shared<SYS, P, N, W>(scl, ring, ar, ln, share);
     }
     save_lane<N>(sl, slot, ar, ln);
 }
@@ -1114,12 +1122,12 @@ hipError_t launch_plain(const SysConst& sc, const JobDesc* jobs_dev, int njobs, 
     return hipGetLastError();
 }
 
-template <int SYS, typename P, int BS>
+template <int SYS, typename P, int BS, int W>
 hipError_t launch_pool(const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl, const LaneSlots& sl, double* ws,
                        hipStream_t st)
 {
     const unsigned grid = (unsigned)(sl.S / BS);
-    hipLaunchKernelGGL((pool_cf_kernel<SYS, P, BS>), dim3(grid), dim3(BS), 0, st, sc, ring, ctl, sl, ws);
+    hipLaunchKernelGGL((pool_cf_kernel<SYS, P, BS, W>), dim3(grid), dim3(BS), 0, st, sc, ring, ctl, sl, ws);
     return hipGetLastError();
 }
 
@@ -1136,30 +1144,40 @@ hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int n
                            : launch_plain<SYS_CO2AR>(sc, jobs_dev, njobs, total, ws, st);
 }
 
-// the pool holds sl.S = grid * block lane slots; block is 64, 128 or 256
-hipError_t run_pool(int sys, bool mixed, int block, const SysConst& sc, const JobDesc* ring, const PoolCtl& ctl,
-                    const LaneSlots& sl, double* ws, hipStream_t st)
+// the pool holds sl.S = grid * block lane slots; block is 64, 128 or 256 at two wavefronts per
+// SIMD, 256 at one
+hipError_t run_pool(int sys, bool mixed, int block, int waves, const SysConst& sc, const JobDesc* ring,
+                    const PoolCtl& ctl, const LaneSlots& sl, double* ws, hipStream_t st)
 {
-#define HH_LAUNCH(S, PT, B) return launch_pool<S, PT, B>(sc, ring, ctl, sl, ws, st)
-    if (sys == SYS_HEAR) {
+#define HH_LAUNCH(S, PT, B, W) return launch_pool<S, PT, B, W>(sc, ring, ctl, sl, ws, st)
+    if (waves == 1) {
+        if (block != 256) return hipErrorInvalidValue;
         // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
-        if (mixed) {
-            if (block == 64) HH_LAUNCH(SYS_HEAR, float, 64);
-            if (block == 128) HH_LAUNCH(SYS_HEAR, float, 128);
-            HH_LAUNCH(SYS_HEAR, float, 256);
+        if (sys == SYS_HEAR) {
+            if (mixed) HH_LAUNCH(SYS_HEAR, float, 256, 1);
+            HH_LAUNCH(SYS_HEAR, double, 256, 1);
         }
-        if (block == 64) HH_LAUNCH(SYS_HEAR, double, 64);
-        if (block == 128) HH_LAUNCH(SYS_HEAR, double, 128);
-        HH_LAUNCH(SYS_HEAR, double, 256);
+        if (mixed) HH_LAUNCH(SYS_CO2AR, float, 256, 1);
+        HH_LAUNCH(SYS_CO2AR, double, 256, 1);
+    }
+    if (sys == SYS_HEAR) {
+        if (mixed) {
+            if (block == 64) HH_LAUNCH(SYS_HEAR, float, 64, 2);
+            if (block == 128) HH_LAUNCH(SYS_HEAR, float, 128, 2);
+            HH_LAUNCH(SYS_HEAR, float, 256, 2);
+        }
+        if (block == 64) HH_LAUNCH(SYS_HEAR, double, 64, 2);
+        if (block == 128) HH_LAUNCH(SYS_HEAR, double, 128, 2);
+        HH_LAUNCH(SYS_HEAR, double, 256, 2);
     }
     if (mixed) {
-        if (block == 64) HH_LAUNCH(SYS_CO2AR, float, 64);
-        if (block == 128) HH_LAUNCH(SYS_CO2AR, float, 128);
-        HH_LAUNCH(SYS_CO2AR, float, 256);
+        if (block == 64) HH_LAUNCH(SYS_CO2AR, float, 64, 2);
+        if (block == 128) HH_LAUNCH(SYS_CO2AR, float, 128, 2);
+        HH_LAUNCH(SYS_CO2AR, float, 256, 2);
     }
-    if (block == 64) HH_LAUNCH(SYS_CO2AR, double, 64);
-    if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128);
-    HH_LAUNCH(SYS_CO2AR, double, 256);
+    if (block == 64) HH_LAUNCH(SYS_CO2AR, double, 64, 2);
+    if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128, 2);
+    HH_LAUNCH(SYS_CO2AR, double, 256, 2);
 #undef HH_LAUNCH
 }
 

@@ -207,7 +207,7 @@ int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const d
         ctl.ring = pl->ring_cap;
         SysConst sc{sys.mu, sys.I, (mixed && sys.system == HH_SYS_CO2AR) ? e->tab32.p : e->tab64.p};
         HH_CHECK(hipEventRecord(e->ev[0], e->stream));
-        HH_CHECK(run_pool(pl->sys, mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
+        HH_CHECK(run_pool(pl->sys, mixed, pl->block, pl->waves, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
                           pl->arena.as<double>(), e->stream));
         HH_CHECK(hipEventRecord(e->ev[1], e->stream));
         HH_CHECK(hipMemsetAsync(work.p, 0, sizeof(unsigned long long), e->stream));
@@ -265,7 +265,7 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
         if (repack) err = run_repack(pl.sys, pl.view(src), pl.view(dst), pl.scratch.as<int>(), e->capture);
         if (err == hipSuccess) err = run_clamp_next(pl.next.as<unsigned long long>(), pl.total, e->capture);
         if (err == hipSuccess)
-            err = run_pool(pl.sys, pl.mixed, pl.block, sc, pl.ring.as<JobDesc>(), ctl, pl.view(dst),
+            err = run_pool(pl.sys, pl.mixed, pl.block, pl.waves, sc, pl.ring.as<JobDesc>(), ctl, pl.view(dst),
                            pl.arena.as<double>(), e->capture);
         if (err == hipSuccess)
             err = hipMemcpyAsync(pl.h_done, pl.ring_done.p, pl.ring_cap * sizeof(int), hipMemcpyDeviceToHost,
@@ -322,7 +322,7 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
                 const int rc = launch_slice_graph(e, *pl, ctl, sc);
                 if (rc != HH_OK) return rc;
             } else {
-                HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
+                HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, pl->waves, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
                                   pl->arena.as<double>(), e->stream));
             }
             if (launches) ++*launches;

@@ -74,7 +74,8 @@ const int kZero = 0;
 int hh_session::admit(Request& r)
 {
     if (r.kind == HH_REQ_CF) {
-        int rc = hh::pool_get(e, pools, r.sys, r.p.precision == HH_PREC_MIXED, opts.block, &r.pool);
+        int rc = hh::pool_get(e, pools, r.sys, r.p.precision == HH_PREC_MIXED, opts.block, &r.pool,
+                              hh::pool_waves(opts.block));
         if (rc < 0) return rc;
         if (!hh::pool_has_room(*r.pool)) return HH_E_BUSY;
         r.kms0 = kms;

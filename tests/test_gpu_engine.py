@@ -104,6 +104,27 @@ def test_job_result_identical_alone_and_in_a_mix(eng):
     assert np.array_equal(alone[0].cpu().numpy(), mixed[1].cpu().numpy())
 
 
+@pytest.mark.parametrize("prec", [H.PREC_MIXED, H.PREC_FP64])
+def test_launch_shape_does_not_change_results(eng, prec):
+    """The 256-lane pool runs one wavefront per SIMD and combines whole tableau rows at once; 64-
+    and 128-lane pools run two per SIMD and combine rows a few columns at a time.  Sums keep their
+    order and samples are evaluated by the same code, so every shape (and the plain engine, one
+    item per lane with lane-serial sampling) gives the same bits."""
+    smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
+    ic = eng.to_device_soa(H.sample_ics(H.co2_ar(), smp, 9, 600)[0])
+    outs = []
+    for block in (0, 128, 64):
+        p = H.cf_params(512, engine=H.ENGINE_BATCHED, precision=prec, rtol=1e-10, block=block)
+        cf, st = eng.cf_run(H.co2_ar(), p, [ic])
+        print(f"  block {st.block} grid {st.grid}", flush=True)
+        outs.append(cf[0].cpu().numpy())
+    if prec == H.PREC_FP64:
+        cf, _ = eng.cf_run(H.co2_ar(), H.cf_params(512, engine=H.ENGINE_PLAIN, precision=prec, rtol=1e-10), [ic])
+        outs.append(cf[0].cpu().numpy())
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
 @pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
 def test_step_limit_drops_trajectory_like_cvode(eng, engine):
     """max_steps accepted steps without reaching the next sample ends a half with the step-limit

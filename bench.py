@@ -283,7 +283,7 @@ def main():
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
     roofline = {"bound": "mfma", "achieved": achieved, "peak": MI355X_FP64_VECTOR_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / MI355X_FP64_VECTOR_TFLOPS, "traffic": traffic,
-                "kernel": "pool_cf_kernel<CO2-Ar, float, 256>", "pipe": "FP64 vector ALU (no matrix work)",
+                "kernel": "pool_cf_kernel<CO2-Ar, float, 256 lanes, 1 wavefront/SIMD>", "pipe": "FP64 vector ALU (no matrix work)",
                 "launches": stm.launches, "avg_launch_ms": launch_ms,
                 "algorithmic_flop_per_launch": flops / max(stm.launches, 1),
                 "flop_model": "hawaiihybrid_amd/data/flops.json: nrhs*rhs + nsamples*(dip+sample) + steps*step"}

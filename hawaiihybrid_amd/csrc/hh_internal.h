@@ -177,7 +177,8 @@ namespace hh {
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;
 // resident wavefronts per SIMD of the default launch shape for a workgroup size: 256-lane
-// workgroups run one wavefront per SIMD (whole tableau rows in registers), 64 and 128 two
+// workgroups run one wavefront per SIMD (wider tableau column groups in registers), 64 and
+// 128 two
 inline int pool_waves(int block) { return block == 256 ? 1 : 2; }
 // the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
 int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out,

@@ -232,11 +232,12 @@ __device__ __forceinline__ void rows_dot(const Arena<N>& ar, double (&out)[NR][N
 }
 
 // Column groups of the tableau rows for a kernel resident at W wavefronts per SIMD: at two
-// the register budget (256) holds a few columns at a time; at one (512) whole rows, one memory
-// round trip per row.
+// (256 registers) three columns at a time; at one, six (at most two memory round trips per
+// row), which still fits the 256 architectural VGPRs without spilling into AGPRs and measured
+// faster than whole rows.
 template <int W>
 struct Groups {
-    static constexpr int stage = W == 1 ? 16 : 3, be = W == 1 ? 8 : 2, dense = W == 1 ? 12 : 3;
+    static constexpr int stage = W == 1 ? 6 : 3, be = W == 1 ? 4 : 2, dense = W == 1 ? 6 : 3;
 };
 
 template <int N, int R, int G>

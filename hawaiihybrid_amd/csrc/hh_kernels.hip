@@ -684,13 +684,29 @@ __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const Job
             // the samples of this step, up to the one that ends the item (R > Rcut on the
             // dense output, or the last sample index)
             const double tnew = ln.t + h;
-            for (int k = ln.k; k < jb.L && k * jb.dt <= tnew; ++k) {
-                ++n;
-                const double tk = k * jb.dt;
-                const double Rk = tk == tnew ? yn[4] : dense_coord<NQ>(Fc, 2, ln.y[4], (tk - ln.t) / h);
-                if (Rk > jb.Rcut || k + 1 >= jb.L) {
-                    end = true;
-                    break;
+            // On [t, tnew] the dense output of R stays within sum_d |Fc[d][2]| of R(t) (x and
+            // 1 - x lie in [0, 1]); when that bound, with slack for rounding, and R(tnew) are
+            // below Rcut no sample can end the item and the count follows from the sample times.
+            double span = 0.0;
+#pragma unroll
+            for (int d = 0; d < 7; ++d) span += fabs(Fc[d][2]);
+            const double rmax = ln.y[4] + span * (1.0 + 1e-12) + 1e-12 * fabs(ln.y[4]);
+            if (rmax < jb.Rcut && yn[4] < jb.Rcut) {
+                // last sample index kf <= L - 1 with kf dt <= tnew (the loop's own predicate)
+                int kf = (int)fmin(floor(tnew / jb.dt), (double)(jb.L - 1));
+                while (kf + 1 <= jb.L - 1 && (kf + 1) * jb.dt <= tnew) ++kf;
+                while (kf > ln.k && kf * jb.dt > tnew) --kf;
+                n = kf - ln.k + 1;
+                end = kf + 1 >= jb.L;
+            } else {
+                for (int k = ln.k; k < jb.L && k * jb.dt <= tnew; ++k) {
+                    ++n;
+                    const double tk = k * jb.dt;
+                    const double Rk = tk == tnew ? yn[4] : dense_coord<NQ>(Fc, 2, ln.y[4], (tk - ln.t) / h);
+                    if (Rk > jb.Rcut || k + 1 >= jb.L) {
+                        end = true;
+                        break;
+                    }
                 }
             }
 #pragma unroll

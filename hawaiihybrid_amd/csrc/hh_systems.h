@@ -202,6 +202,21 @@ template <typename P>
 __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb, const double* y,
                                            const Trig& tr, double* dV, double& noise)
 {
+    const double R = y[4];
+    const int b = co2ar_branch(R);
+    // FP32: the radial coefficients of the point's branch are read from LDS first, so their
+    // latency overlaps the angular part (one wait instead of one per slot)
+    P cf[6][8];
+    double cen = 0.0;
+    if constexpr (sizeof(P) == 4) {
+        const int bb = b > 0 ? b : 1;   // the exponential branch (b == 0) uses other constants
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cf[q][j] = tb->pes[bb][q][j];
+        cen = tb->center[bb];
+        __builtin_amdgcn_sched_barrier(0);
+    }
     P sP, cP, sp, cp;
     sincos_p<P>(y[0], &sP, &cP);
     sincos_p<P>(y[6], &sp, &cp);
@@ -218,8 +233,6 @@ __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb
     P Pl[11], dPl[11];
     legendre10<P>(c, Pl, dPl);
 
-    const double R = y[4];
-    const int b = co2ar_branch(R);
     P dVdR = P(0), dVdc = P(0);
     P mag = P(0);   // sum of |terms| (FP32 only)
     if (b == 0) {
@@ -233,12 +246,12 @@ __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb
             if constexpr (sizeof(P) == 4) mag += fabs(tR) + fabs(tc);
         }
     } else {
-        const P x = (P)(R - tb->center[b]);
+        const P x = (P)(R - (sizeof(P) == 4 ? cen : tb->center[b]));
         const P sinv = (P)(1.0 / R);
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             P v, dv;
-            slot_eval<P>(tb->pes[b][s], x, sinv, v, dv);
+            slot_eval<P>(sizeof(P) == 4 ? cf[s] : tb->pes[b][s], x, sinv, v, dv);
             const P tR = dv * Pl[2 * s], tc = v * dPl[2 * s];
             dVdR += tR;
             dVdc += tc;
@@ -286,6 +299,24 @@ __device__ __forceinline__ P co2ar_pes(const Co2ArTables<P>* __restrict__ tb, co
 template <typename P>
 __device__ __forceinline__ void co2ar_dipole(const Co2ArTables<P>* __restrict__ tb, const double* y, P* mu)
 {
+    const double R = y[4];
+    const int b = co2ar_branch(R);
+    // FP32: the branch's radial coefficients are read from LDS before the angular part (as in
+    // co2ar_dpes)
+    P cx[5][8], cz[6][8];
+    double cen = 0.0;
+    if constexpr (sizeof(P) == 4) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cx[q][j] = tb->dipx[b][q][j];
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) cz[q][j] = tb->dipz[b][q][j];
+        cen = tb->center[b];
+        __builtin_amdgcn_sched_barrier(0);
+    }
     P sT, cT, sP, cP, st, ct, sp, cp;
     sincos_p<P>(y[2], &sT, &cT);
     sincos_p<P>(y[0], &sP, &cP);
@@ -297,15 +328,13 @@ __device__ __forceinline__ void co2ar_dipole(const Co2ArTables<P>* __restrict__ 
     c = c > P(1) ? P(1) : (c < P(-1) ? P(-1) : c);
     P Pl[11], dPl[11];
     legendre10<P>(c, Pl, dPl);
-    const double R = y[4];
-    const int b = co2ar_branch(R);
-    const P x = (P)(R - tb->center[b]);
+    const P x = (P)(R - (sizeof(P) == 4 ? cen : tb->center[b]));
     const P sinv = (P)(1.0 / R);
     P gx = P(0), dz = P(0);
 #pragma unroll
-    for (int s = 0; s < 5; ++s) gx += slot_value<P>(tb->dipx[b][s], x, sinv) * dPl[2 * s + 2];
+    for (int s = 0; s < 5; ++s) gx += slot_value<P>(sizeof(P) == 4 ? cx[s] : tb->dipx[b][s], x, sinv) * dPl[2 * s + 2];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) dz += slot_value<P>(tb->dipz[b][s], x, sinv) * Pl[2 * s];
+    for (int s = 0; s < 6; ++s) dz += slot_value<P>(sizeof(P) == 4 ? cz[s] : tb->dipz[b][s], x, sinv) * Pl[2 * s];
     // mu = dz Rhat - gx (u - c Rhat)
     mu[0] = dz * Rx - gx * (ux - c * Rx);
     mu[1] = dz * Ry - gx * (uy - c * Ry);

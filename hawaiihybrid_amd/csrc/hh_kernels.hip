@@ -36,30 +36,46 @@ constexpr int kSlots = 16;               // K0..K15
 #define HH_MIN_WAVES 2   // occupancy target (waves per SIMD) of the trajectory kernels
 #endif
 
-// Wave-blocked arena: wavefront w owns kSlots*N*64 consecutive doubles; element (slot, i) of
-// lane l is at double offset (slot*N + i)*64 + l of the block.  Accesses go through a buffer
-// resource of the block (wave-uniform, scalar registers), a 32-bit lane offset and a scalar
-// slot offset: a coalesced 64-lane access with no per-lane 64-bit address kept live.  An
-// access outside the block is dropped (stores) or reads 0 (loads) by the hardware.
+// Wave-blocked arena: wavefront w owns kSlots*N*64 consecutive doubles; components are paired,
+// element (slot, i) of lane l at double offset (slot*N + (i & ~1))*64 + 2l + (i & 1) of the
+// block, so one 16-byte access moves components i, i+1 of a lane and a wavefront's access is
+// 1 KiB contiguous.  Accesses go through a buffer resource of the block (wave-uniform, scalar
+// registers), a 32-bit lane offset and a constant slot offset: no per-lane 64-bit address is
+// kept live.  An access outside the block is dropped (stores) or reads 0 (loads).
 typedef unsigned int hh_v2u __attribute__((ext_vector_type(2)));
+typedef unsigned int hh_v4u __attribute__((ext_vector_type(4)));
 constexpr int kBufferConfig = 0x00020000;   // raw 32-bit-element buffer, gfx9 word 3
 
 template <int N>
 struct Arena {
+    static_assert(N % 2 == 0, "components are stored in pairs");
     __amdgpu_buffer_rsrc_t rsrc;
-    int lane8;
+    int lane16;
+    static __device__ __forceinline__ int off(int slot, int i) { return (slot * N + (i & ~1)) * 512 + (i & 1) * 8; }
     __device__ __forceinline__ double ld(int slot, int i) const
     {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane8, (slot * N + i) * 512, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane16, off(slot, i), 0));
     }
     __device__ __forceinline__ void st(int slot, int i, double v) const
     {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(hh_v2u, v), rsrc, lane8, (slot * N + i) * 512, 0);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(hh_v2u, v), rsrc, lane16, off(slot, i), 0);
+    }
+    // components i, i + 1 (i even) in one access
+    __device__ __forceinline__ void ld2(int slot, int i, double& a, double& b) const
+    {
+        const hh_v4u v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, lane16, (slot * N + i) * 512, 0);
+        a = __builtin_bit_cast(double, hh_v2u{v.x, v.y});
+        b = __builtin_bit_cast(double, hh_v2u{v.z, v.w});
+    }
+    __device__ __forceinline__ void st2(int slot, int i, double a, double b) const
+    {
+        const hh_v2u x = __builtin_bit_cast(hh_v2u, a), y = __builtin_bit_cast(hh_v2u, b);
+        __builtin_amdgcn_raw_buffer_store_b128(hh_v4u{x.x, x.y, y.x, y.y}, rsrc, lane16, (slot * N + i) * 512, 0);
     }
     // element (slot, i) of another lane of the wavefront
     __device__ __forceinline__ double ld_lane(int lane, int slot, int i) const
     {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane * 8, (slot * N + i) * 512, 0));
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane * 16, off(slot, i), 0));
     }
 };
 
@@ -69,7 +85,7 @@ __device__ __forceinline__ Arena<N> wave_arena(double* ws, long long wave)
     const long long w = __builtin_amdgcn_readfirstlane((int)wave);
     double* block = ws + w * (long long)(kSlots * N * 64);
     return Arena<N>{__builtin_amdgcn_make_buffer_rsrc(block, 0, kSlots * N * 64 * 8, kBufferConfig),
-                    (int)(threadIdx.x & 63) * 8};
+                    (int)(threadIdx.x & 63) * 16};
 }
 
 template <int N>
@@ -187,7 +203,7 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
         double f[N];
         rhs<SYS, P>(sc, ln.y, f);
 #pragma unroll
-        for (int i = 0; i < N; ++i) ar.st(0, i, f[i]);
+        for (int i = 0; i < N; i += 2) ar.st2(0, i, f[i], f[i + 1]);
     }
     ln.nrhs = 1;
     ln.h = initial_step<SYS, P, N>(sc, ar, ln.y, jb.rtol, jb.atol, (jb.L - 1) * jb.dt, ln.nrhs);
@@ -216,8 +232,13 @@ __device__ __forceinline__ void rows_dot(const Arena<N>& ar, double (&out)[NR][N
 #pragma unroll
         for (int g = 0; g < G; ++g)
             if (q0 + g < nnz) {
+                if constexpr (STRIDE == 1) {
 #pragma unroll
-                for (int i = 0; i < N; i += STRIDE) v[g][i] = ar.ld(DOP853_SP_COL[R0][q0 + g], i);
+                    for (int i = 0; i < N; i += 2) ar.ld2(DOP853_SP_COL[R0][q0 + g], i, v[g][i], v[g][i + 1]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < N; i += STRIDE) v[g][i] = ar.ld(DOP853_SP_COL[R0][q0 + g], i);
+                }
             }
 #pragma unroll
         for (int g = 0; g < G; ++g)
@@ -295,7 +316,7 @@ __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, c
         rhs<SYS, P>(sc, yt, k, nz);
         noise = fmax(noise, nz);
 #pragma unroll
-        for (int i = 0; i < N; ++i) ar.st(s, i, k[i]);
+        for (int i = 0; i < N; i += 2) ar.st2(s, i, k[i], k[i + 1]);
     }
 }
 
@@ -451,9 +472,12 @@ __device__ __forceinline__ void advance(const JobDesc& jb, const Arena<N>& ar, L
     const double tend = (jb.L - 1) * jb.dt;
     arena_fence();
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
+    for (int i = 0; i < N; i += 2) {
         ln.y[i] = yn[i];
-        ar.st(0, i, ar.ld(12, i));
+        ln.y[i + 1] = yn[i + 1];
+        double a, b;
+        ar.ld2(12, i, a, b);
+        ar.st2(0, i, a, b);
     }
     ln.t = ln.t + h;
     const double fac = fmax(1.0 / 6.0, fmin(1.0 / 0.333, fac11 / 0.9));

@@ -72,10 +72,12 @@ struct Arena {
         const hh_v2u x = __builtin_bit_cast(hh_v2u, a), y = __builtin_bit_cast(hh_v2u, b);
         __builtin_amdgcn_raw_buffer_store_b128(hh_v4u{x.x, x.y, y.x, y.y}, rsrc, lane16, (slot * N + i) * 512, 0);
     }
-    // element (slot, i) of another lane of the wavefront
-    __device__ __forceinline__ double ld_lane(int lane, int slot, int i) const
+    // components i, i + 1 (i even) of another lane of the wavefront
+    __device__ __forceinline__ void ld2_lane(int lane, int slot, int i, double& a, double& b) const
     {
-        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, lane * 16, off(slot, i), 0));
+        const hh_v4u v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, lane * 16, (slot * N + i) * 512, 0);
+        a = __builtin_bit_cast(double, hh_v2u{v.x, v.y});
+        b = __builtin_bit_cast(double, hh_v2u{v.z, v.w});
     }
 };
 
@@ -322,17 +324,23 @@ __device__ __forceinline__ void stages(const SysConst& sc, const Arena<N>& ar, c
 
 // Dense output of coordinate c (state index 2c) at x = (t_k - t) / h of an accepted step:
 // DOP853's 7th-order continuous extension (Hairer's contd8) in Horner form.
+__device__ __forceinline__ double dense_value(const double (&F)[7], double y0, double x)
+{
+    const double omx = 1.0 - x;
+    double v = F[5] + x * F[6];
+    v = F[4] + omx * v;
+    v = F[3] + x * v;
+    v = F[2] + omx * v;
+    v = F[1] + x * v;
+    v = F[0] + omx * v;
+    return y0 + x * v;
+}
+
 template <int NQ>
 __device__ __forceinline__ double dense_coord(const double (&Fc)[7][NQ], int c, double y0, double x)
 {
-    const double omx = 1.0 - x;
-    double v = Fc[5][c] + x * Fc[6][c];
-    v = Fc[4][c] + omx * v;
-    v = Fc[3][c] + x * v;
-    v = Fc[2][c] + omx * v;
-    v = Fc[1][c] + x * v;
-    v = Fc[0][c] + omx * v;
-    return y0 + x * v;
+    const double F[7] = {Fc[0][c], Fc[1][c], Fc[2][c], Fc[3][c], Fc[4][c], Fc[5][c], Fc[6][c]};
+    return dense_value(F, y0, x);
 }
 
 // CF value mu(0) . mu(t_k) of a sample
@@ -571,23 +579,23 @@ struct WaveShare {
     double last[64];  // CF value of the lane's last sample
 };
 
+// Fields of a lane's step record (field f at arena slot 1 + f / N, component f % N), in pairs
+// that one 16-byte access moves: the sample-time scalars [0, CF0), the CF scalars [CF0, YN),
+// the coordinates at the step's end (used only for a sample at exactly t + h), then per
+// coordinate c its value at the step's start followed by its seven dense-output coefficients
+// (Y + 8c .. Y + 8c + 7).
 template <int NQ>
 struct StepRecord {
-    static constexpr int Y = 0, YN = NQ, FC = 2 * NQ, T = 9 * NQ, H = T + 1, TNEW = T + 2, MU0 = T + 3,
-                         CURR = T + 6, K0 = T + 7, ROW = T + 8, L = T + 9, DT = T + 10, RECORD = T + 11,
-                         SIZE = T + 12;
+    static constexpr int T = 0, H = 1, TNEW = 2, DT = 3, K0 = 4, CF0 = 6, MU0 = 6, CURR = 9, ROW = 10, L = 11,
+                         RECORD = 12, YN = 14, Y = YN + (NQ + 1) / 2 * 2, SIZE = Y + 8 * NQ;
 };
 
-template <int N>
-__device__ __forceinline__ void rec_store(const Arena<N>& ar, int f, double v)
+// fields [f0, f0 + n) of the record of lane `owner` (f0 and n even)
+template <int N, int NREC>
+__device__ __forceinline__ void rec_load_block(const Arena<N>& ar, int owner, int f0, int n, double (&r)[NREC])
 {
-    ar.st(1 + f / N, f % N, v);
-}
-
-template <int N>
-__device__ __forceinline__ double rec_load(const Arena<N>& ar, int owner, int f)
-{
-    return ar.ld_lane(owner, 1 + f / N, f % N);
+#pragma unroll
+    for (int f = f0; f < f0 + n; f += 2) ar.ld2_lane(owner, 1 + f / N, f % N, r[f], r[f + 1]);
 }
 
 // Memory written by one lane (LDS, the wavefront's arena block) is visible to the others
@@ -625,8 +633,9 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
     for (int base = 0; base < total; base += 64) {
         const int g = base + lane;
         const bool valid = g < total;
-        int owner = 0, q = 0, k = 0, cnt = 0;
+        int owner = 0, q = 0, k = 0, cnt = 0, rowL = 0, record = 0;
         double curr = 0.0, prev_start = 0.0;
+        double* row = nullptr;
         P m[3] = {P(0), P(0), P(0)};
         if (valid) {
             int lo = 0, hi = 63;
@@ -641,31 +650,36 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
             owner = lo;
             q = g - ws.pre[owner];
             cnt = ws.pre[owner + 1] - ws.pre[owner];
-            k = (int)rec_load<N>(ar, owner, R::K0) + q;
-            const double t = rec_load<N>(ar, owner, R::T), h = rec_load<N>(ar, owner, R::H);
-            const double tnew = rec_load<N>(ar, owner, R::TNEW);
-            const double tk = k * rec_load<N>(ar, owner, R::DT);
+            double r[R::SIZE];
+            rec_load_block<N>(ar, owner, 0, R::CF0, r);
+            k = (int)r[R::K0] + q;
+            const double tk = k * r[R::DT];
             double ys[N];
 #pragma unroll
             for (int i = 1; i < N; i += 2) ys[i] = 0.0;
-            if (tk == tnew) {
+            if (tk == r[R::TNEW]) {
+                rec_load_block<N>(ar, owner, R::YN, R::Y - R::YN, r);
 #pragma unroll
-                for (int c = 0; c < NQ; ++c) ys[2 * c] = rec_load<N>(ar, owner, R::YN + c);
+                for (int c = 0; c < NQ; ++c) ys[2 * c] = r[R::YN + c];
             } else {
-                double Fc[7][NQ];
+                const double x = (tk - r[R::T]) / r[R::H];
 #pragma unroll
-                for (int d = 0; d < 7; ++d)
-#pragma unroll
-                    for (int c = 0; c < NQ; ++c) Fc[d][c] = rec_load<N>(ar, owner, R::FC + d * NQ + c);
-                const double x = (tk - t) / h;
-#pragma unroll
-                for (int c = 0; c < NQ; ++c) ys[2 * c] = dense_coord<NQ>(Fc, c, rec_load<N>(ar, owner, R::Y + c), x);
+                for (int c = 0; c < NQ; ++c) {
+                    rec_load_block<N>(ar, owner, R::Y + 8 * c, 8, r);
+                    const double F[7] = {r[R::Y + 8 * c + 1], r[R::Y + 8 * c + 2], r[R::Y + 8 * c + 3],
+                                         r[R::Y + 8 * c + 4], r[R::Y + 8 * c + 5], r[R::Y + 8 * c + 6],
+                                         r[R::Y + 8 * c + 7]};
+                    ys[2 * c] = dense_value(F, r[R::Y + 8 * c], x);
+                }
             }
             dipole<SYS, P>(sc, ys, m);
-            const double mu0[3] = {rec_load<N>(ar, owner, R::MU0), rec_load<N>(ar, owner, R::MU0 + 1),
-                                   rec_load<N>(ar, owner, R::MU0 + 2)};
+            rec_load_block<N>(ar, owner, R::CF0, R::YN - R::CF0, r);
+            const double mu0[3] = {r[R::MU0], r[R::MU0 + 1], r[R::MU0 + 2]};
             curr = cf_value<P>(mu0, m);
-            prev_start = rec_load<N>(ar, owner, R::CURR);
+            prev_start = r[R::CURR];
+            row = __builtin_bit_cast(double*, r[R::ROW]);
+            rowL = (int)r[R::L];
+            record = (int)r[R::RECORD];
         }
         const double up = __shfl_up(curr, 1, 64);
         const double prev = q == 0 ? prev_start : (lane > 0 ? up : carry);
@@ -676,11 +690,7 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
         }
         wave_share_sync();
         if (valid) {
-            if (q < (ws.fail[owner] >> 3)) {
-                double* row = __builtin_bit_cast(double*, rec_load<N>(ar, owner, R::ROW));
-                store_sample<P>(row, (int)rec_load<N>(ar, owner, R::L), (int)rec_load<N>(ar, owner, R::RECORD), k, m,
-                                curr);
-            }
+            if (q < (ws.fail[owner] >> 3)) store_sample<P>(row, rowL, record, k, m, curr);
             if (q == cnt - 1) ws.last[owner] = curr;
         }
     }
@@ -733,28 +743,32 @@ __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const Job
                     }
                 }
             }
+            double r[R::SIZE];
+            r[R::T] = ln.t;
+            r[R::H] = h;
+            r[R::TNEW] = tnew;
+            r[R::DT] = jb.dt;
+            r[R::MU0] = ln.mu0[0];
+            r[R::MU0 + 1] = ln.mu0[1];
+            r[R::MU0 + 2] = ln.mu0[2];
+            r[R::CURR] = ln.curr;
+            r[R::K0] = (double)ln.k;
+            r[R::ROW] = __builtin_bit_cast(double, jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1));
+            r[R::L] = (double)jb.L;
+            r[R::RECORD] = (double)jb.record;
+            r[R::K0 + 1] = 0.0;
+            r[R::RECORD + 1] = 0.0;
+#pragma unroll
+            for (int f = R::YN + NQ; f < R::Y; ++f) r[f] = 0.0;
 #pragma unroll
             for (int c = 0; c < NQ; ++c) {
-                rec_store<N>(ar, R::Y + c, ln.y[2 * c]);
-                rec_store<N>(ar, R::YN + c, yn[2 * c]);
+                r[R::YN + c] = yn[2 * c];
+                r[R::Y + 8 * c] = ln.y[2 * c];
+#pragma unroll
+                for (int d = 0; d < 7; ++d) r[R::Y + 8 * c + 1 + d] = Fc[d][c];
             }
 #pragma unroll
-            for (int d = 0; d < 7; ++d)
-#pragma unroll
-                for (int c = 0; c < NQ; ++c) rec_store<N>(ar, R::FC + d * NQ + c, Fc[d][c]);
-            rec_store<N>(ar, R::T, ln.t);
-            rec_store<N>(ar, R::H, h);
-            rec_store<N>(ar, R::TNEW, tnew);
-            rec_store<N>(ar, R::MU0, ln.mu0[0]);
-            rec_store<N>(ar, R::MU0 + 1, ln.mu0[1]);
-            rec_store<N>(ar, R::MU0 + 2, ln.mu0[2]);
-            rec_store<N>(ar, R::CURR, ln.curr);
-            rec_store<N>(ar, R::K0, (double)ln.k);
-            rec_store<N>(ar, R::ROW,
-                         __builtin_bit_cast(double, jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1)));
-            rec_store<N>(ar, R::L, (double)jb.L);
-            rec_store<N>(ar, R::DT, jb.dt);
-            rec_store<N>(ar, R::RECORD, (double)jb.record);
+            for (int f = 0; f < R::SIZE; f += 2) ar.st2(1 + f / N, f % N, r[f], r[f + 1]);
         }
     }
     shared_samples<SYS, P, N>(sc, ar, ws, n);

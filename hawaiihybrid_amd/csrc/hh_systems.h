@@ -102,6 +102,28 @@ __device__ __forceinline__ P hear_dip(P R)
     return exp(P(0.47293181) - P(0.40100613) * R - P(0.10292726) * r2) - P(148.55391) * inv7;
 }
 
+// 1/x within an ulp: the hardware reciprocal estimate refined by two Newton steps (each at
+// least doubles the correct bits).  The engines with FP32 surfaces (P = float) use it in place
+// of the correctly rounded FP64 division, which costs twice the instructions and a longer
+// dependent chain; the FP64 engines divide.
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+template <typename P>
+__device__ __forceinline__ double recip(double x)
+{
+    if constexpr (sizeof(P) == 4)
+        return rcp_nr(x);
+    else
+        return 1.0 / x;
+}
+
 // ------------------------------------------------------------------ CO2-Ar
 __device__ __forceinline__ int co2ar_branch(double R)
 {
@@ -247,7 +269,7 @@ __device__ __forceinline__ void co2ar_dpes(const Co2ArTables<P>* __restrict__ tb
         }
     } else {
         const P x = (P)(R - (sizeof(P) == 4 ? cen : tb->center[b]));
-        const P sinv = (P)(1.0 / R);
+        const P sinv = (P)recip<P>(R);
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
             P v, dv;
@@ -329,7 +351,7 @@ __device__ __forceinline__ void co2ar_dipole(const Co2ArTables<P>* __restrict__ 
     P Pl[11], dPl[11];
     legendre10<P>(c, Pl, dPl);
     const P x = (P)(R - (sizeof(P) == 4 ? cen : tb->center[b]));
-    const P sinv = (P)(1.0 / R);
+    const P sinv = (P)recip<P>(R);
     P gx = P(0), dz = P(0);
 #pragma unroll
     for (int s = 0; s < 5; ++s) gx += slot_value<P>(sizeof(P) == 4 ? cx[s] : tb->dipx[b][s], x, sinv) * dPl[2 * s + 2];
@@ -374,8 +396,8 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
         sincos_p<double>(y[8], &tr.st, &tr.ct);
         co2ar_dpes<P>((const Co2ArTables<P>*)sc.tables, y, tr, dV, noise);
     }
-    const double iR = 1.0 / R, iR2 = iR * iR, imu = 1.0 / mu;
-    const double is = 1.0 / tr.sT, is2 = is * is;
+    const double iR = recip<P>(R), iR2 = iR * iR, imu = 1.0 / mu;
+    const double is = recip<P>(tr.sT), is2 = is * is;
     const double a = imu * iR2;               // 1/(mu R^2)
     yd[4] = pR * imu;
     yd[5] = (pTheta * pTheta + pPhi * pPhi * is2) * a * iR - dV[2];
@@ -384,7 +406,7 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
     yd[2] = pTheta * a;
     yd[3] = pPhi * pPhi * tr.cT * a * is2 * is - dV[1];
     if constexpr (SYS == SYS_CO2AR) {
-        const double iI = 1.0 / sc.I, ist = 1.0 / tr.st, ist2 = ist * ist;
+        const double iI = 1.0 / sc.I, ist = recip<P>(tr.st), ist2 = ist * ist;
         const double pphi = y[7], ptheta = y[9];
         yd[6] = pphi * iI * ist2;
         yd[7] = -dV[3];

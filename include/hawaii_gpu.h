@@ -116,7 +116,8 @@ typedef struct {
     int max_steps;       /* max integrator steps between two samples (<= 0: 50000) */
     int engine;          /* HH_ENGINE_* */
     int precision;       /* HH_PREC_*; the plain engine is FP64 only */
-    int block;           /* batched: workgroup size 64/128/256 (0: autotuned) */
+    int block;           /* batched: workgroup size 64/128/256; 256 runs one wavefront per SIMD, 64 and
+                            128 two (0: 256, or measured when HH_TUNE=1) */
     int grid;            /* batched: resident workgroups (0: autotuned) */
     int zimmermann;      /* use_zimmermann_trick: CF averaged over every time origin of the
                             trajectory's dipole series (correlation_eval_zimmerman_trick,
@@ -270,7 +271,7 @@ typedef struct hh_session hh_session;
 typedef struct {
     double slice_ms;          /* pool kernel time slice (<= 0: 10 ms) */
     double max_device_gib;    /* admission budget for request buffers (<= 0: 80% of the free HBM) */
-    int block;                /* lane-pool workgroup size 64 / 128 / 256 (0: 256) */
+    int block;                /* lane-pool workgroup size 64 / 128 / 256 (0: 256, one wavefront per SIMD) */
 } hh_session_opts;
 
 enum { HH_REQ_WAITING = 0, HH_REQ_RUNNING = 1, HH_REQ_DONE = 2, HH_REQ_FAILED = 3 };

@@ -421,8 +421,15 @@ __device__ __forceinline__ int dop853_attempt(const SysConst& sc, const JobDesc&
         for (int i = 0; i < N; ++i) {
             yn[i] = ln.y[i] + h * be[0][i];
             const double sk = jb.atol + jb.rtol * fmax(fabs(ln.y[i]), fabs(yn[i])) + ((i & 1) ? floor_p : 0.0);
-            err += (be[1][i] / sk) * (be[1][i] / sk);
-            err2 += (be[2][i] / sk) * (be[2][i] / sk);
+            if constexpr (sizeof(P) == 4) {
+                // FP32 surfaces: one Newton-refined reciprocal per component (see rcp_nr)
+                const double isk = rcp_nr(sk), e5 = be[1][i] * isk, e3 = be[2][i] * isk;
+                err += e5 * e5;
+                err2 += e3 * e3;
+            } else {
+                err += (be[1][i] / sk) * (be[1][i] / sk);
+                err2 += (be[2][i] / sk) * (be[2][i] / sk);
+            }
         }
     }
     double deno = err + 0.01 * err2;
@@ -662,7 +669,7 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
 #pragma unroll
                 for (int c = 0; c < NQ; ++c) ys[2 * c] = r[R::YN + c];
             } else {
-                const double x = (tk - r[R::T]) / r[R::H];
+                const double x = sizeof(P) == 4 ? (tk - r[R::T]) * rcp_nr(r[R::H]) : (tk - r[R::T]) / r[R::H];
 #pragma unroll
                 for (int c = 0; c < NQ; ++c) {
                     rec_load_block<N>(ar, owner, R::Y + 8 * c, 8, r);

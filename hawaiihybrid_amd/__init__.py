@@ -52,6 +52,12 @@ class CFParams(C.Structure):
                 ("zimmermann", C.c_int)]
 
 
+class CFArrayParams(C.Structure):
+    """hh_cf_array_params: satellite temperatures of a CORRELATION_ARRAY run."""
+    _fields_ = [("T_base", C.c_double), ("pair_state", C.c_int), ("ntemp", C.c_int), ("pesmin", C.c_double),
+                ("temps", C.c_void_p), ("ratios", C.c_void_p)]
+
+
 class CFStats(C.Structure):
     _fields_ = [("integrate_ms", C.c_double), ("reduce_ms", C.c_double), ("n_items", C.c_longlong),
                 ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
@@ -142,6 +148,7 @@ def lib():
             "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
             "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
             "hh_eval": (i, [v, P(System), i, v, C.c_long, v]),
+            "hh_cf_array": (i, [v, P(System), P(CFParams), P(CFArrayParams), v, C.c_long, v, v, P(CFStats)]),
             "hh_conf_parse": (i, [C.c_char_p, v]),
             "hh_session_create": (i, [v, P(SessionOpts), P(v)]),
             "hh_submit_cf": (i, [v, P(System), P(CFParams), v, C.c_long, v, P(C.c_uint64)]),
@@ -297,6 +304,24 @@ class Engine:
         _check(lib().hh_eval(self.h, C.byref(system), what, C.c_void_p(y_dev.data_ptr()), n,
                              C.c_void_p(out.data_ptr())))
         return out
+
+    def cf_array(self, system, params, ic_dev, T_base, pair_state, pesmin, temps, ratios=None):
+        """CORRELATION_ARRAY (hh_cf_array): the CF of the ensemble ic_dev (SoA [QP][ntraj], sampled
+        at T_base) reweighted to each of `temps`.  Returns (CFs [ntemp][L] device tensor, nstar
+        [ntemp] numpy, stats)."""
+        torch = self.torch
+        t = np.ascontiguousarray(temps, dtype=np.float64)
+        r = None if ratios is None else np.ascontiguousarray(ratios, dtype=np.float64)
+        ap = CFArrayParams(T_base, pair_state, len(t), pesmin, t.ctypes.data_as(C.c_void_p),
+                           r.ctypes.data_as(C.c_void_p) if r is not None else None)
+        out = torch.zeros((len(t), params.L), dtype=torch.float64, device=f"cuda:{self.device}")
+        nstar = np.zeros(len(t))
+        st = CFStats()
+        torch.cuda.synchronize(self.device)
+        _check(lib().hh_cf_array(self.h, C.byref(system), C.byref(params), C.byref(ap), C.c_void_p(ic_dev.data_ptr()),
+                                 ic_dev.shape[1], C.c_void_p(out.data_ptr()), nstar.ctypes.data_as(C.c_void_p),
+                                 C.byref(st)))
+        return out, nstar, st
 
     def m0(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
         m, q, ms = C.c_double(), C.c_double(), C.c_double()

@@ -507,6 +507,90 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
     return HH_OK;
 }
 
+int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const hh_cf_array_params* a,
+                const double* ic_dev, long ntraj, double* cf_dev, double* nstar, hh_cf_stats* stats)
+{
+    if (!e || !sys || !p || !a || !ic_dev || ntraj <= 0 || !cf_dev || !nstar)
+        return fail(HH_E_INVALID, "hh_cf_array: null argument or ntraj <= 0");
+    if (a->ntemp < 1 || a->ntemp > 64 || !a->temps) return fail(HH_E_INVALID, "hh_cf_array: 1..64 temperatures");
+    if (!(a->T_base > 0)) return fail(HH_E_INVALID, "hh_cf_array: T_base must be positive");
+    for (int st = 0; st < a->ntemp; ++st)
+        if (!(a->temps[st] > 0)) return fail(HH_E_INVALID, "hh_cf_array: temperature %d is not positive", st);
+    // src/hawaii.c:2657-2659: only free (and metastable) and bound pairs have weights
+    if (a->pair_state != HH_PAIR_FREE && a->pair_state != HH_PAIR_BOUND)
+        return fail(HH_E_INVALID, "hh_cf_array: pair state must be free or bound");
+    if (p->zimmermann) return fail(HH_E_INVALID, "hh_cf_array: Zimmermann jobs are not supported");
+    // one CF job over the ensemble (its per-item rows stay with the engine)
+    hh_cf_params q = *p;
+    q.ppf = 1.0;
+    hh::DevBuf cf_tmp;
+    HH_CHECK(cf_tmp.ensure((size_t)p->L * sizeof(double)));
+    const double* icp[1] = {ic_dev};
+    const long nt[1] = {ntraj};
+    double* cfp[1] = {cf_tmp.as<double>()};
+    int rc = hh_cf_run(e, sys, &q, 1, icp, nt, cfp, stats);
+    if (rc < 0) return rc;
+
+    std::lock_guard<std::mutex> lock(e->mu);
+    HH_CHECK(hipSetDevice(e->device));
+    if (e->last_jobs.size() != 1) return fail(HH_E_INVALID, "hh_cf_array: the engine lost the job's items");
+    const hh::JobDesc jd = e->last_jobs[0];
+    // energies at the initial points (Hamiltonian(ms) right after p_generator, src/hawaii.c:2626)
+    const int M = a->ntemp;
+    hh::DevBuf en, wr_dev, ns_dev;
+    HH_CHECK(en.ensure((size_t)ntraj * sizeof(double)));
+    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    HH_CHECK(hh::run_eval(sys->system, sc, HH_EVAL_HAMILTONIAN, ic_dev, ntraj, en.as<double>(), e->stream));
+    std::vector<double> E((size_t)ntraj);
+    std::vector<int> status((size_t)jd.n_items);
+    HH_CHECK(hipMemcpyAsync(E.data(), en.p, E.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HH_CHECK(hipMemcpyAsync(status.data(), jd.status, status.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    HH_CHECK(hipStreamSynchronize(e->stream));
+    // The array loop skips a trajectory whose last integrator call ended on the step limit
+    // (`if (status == -1) continue;`, src/hawaii.c:2637): no weight, not in nstar.  A dipole
+    // failure ends correlation_eval first (status 1) and counts with a zero CF.
+    auto counted = [&](long t) {
+        if (jd.halves == 2) {
+            const int f = status[2 * t], b = status[2 * t + 1];
+            if (hh::dipole_failure(f) || hh::dipole_failure(b)) return true;
+            return b != hh::ITEM_FAIL_STEPS;
+        }
+        return hh::dipole_failure(status[t]) || status[t] != hh::ITEM_FAIL_STEPS;
+    };
+    // weights in the reference's operation order, nstar summed in trajectory order
+    constexpr double kHkT = 4.3597447222071e-18 / 1.380649e-23;   // Hartree / Boltzmann (src/constants.h:24)
+    const double Tb = a->T_base;
+    std::vector<double> wr((size_t)ntraj * M, 0.0), ns(M, 0.0);
+    for (long t = 0; t < ntraj; ++t) {
+        if (!counted(t)) continue;
+        for (int st = 0; st < M; ++st) {
+            const double Ts = a->temps[st];
+            double WTT = 1.0;
+            if (a->pair_state == HH_PAIR_BOUND && Ts < Tb) WTT = std::exp(-a->pesmin * kHkT / Tb / Ts * (Tb - Ts));
+            const double weight = std::exp(-E[t] * kHkT / Tb / Ts * (Tb - Ts)) / WTT;
+            if (a->pair_state == HH_PAIR_FREE && Ts > Tb) continue;
+            const double ratio = a->ratios ? a->ratios[st] : 1.0;
+            wr[(size_t)t * M + st] = weight * ratio;
+            ns[st] += weight;
+        }
+    }
+    HH_CHECK(wr_dev.ensure(wr.size() * sizeof(double)));
+    HH_CHECK(ns_dev.ensure(ns.size() * sizeof(double)));
+    HH_CHECK(hipMemcpyAsync(wr_dev.p, wr.data(), wr.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipMemcpyAsync(ns_dev.p, ns.data(), ns.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    // at most 256 chunks of trajectories per temperature
+    const long long chunk = std::max(256LL, (jd.ntraj + 255) / 256);
+    const int nchunks = (int)((jd.ntraj + chunk - 1) / chunk);
+    const size_t pw = (size_t)M * nchunks * jd.L * sizeof(double);
+    HH_CHECK(e->psum.ensure(pw));
+    HH_CHECK(e->pcomp.ensure(pw));
+    HH_CHECK(hh::run_reduce_weighted(jd, M, wr_dev.as<double>(), ns_dev.as<double>(), e->psum.as<double>(),
+                                     e->pcomp.as<double>(), chunk, nchunks, cf_dev, e->stream));
+    HH_CHECK(hipStreamSynchronize(e->stream));
+    for (int st = 0; st < M; ++st) nstar[st] = ns[st];
+    return HH_OK;
+}
+
 int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev)
 {
     if (!e || !sys || !y_dev || !out_dev || n <= 0) return fail(HH_E_INVALID, "hh_eval: bad argument");

@@ -36,6 +36,10 @@ size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
                       double* cf_out, hipStream_t st);
+// CORRELATION_ARRAY: per-temperature weighted sums of a job's items, normalised by nstar
+// (device [ntemp]); wr is device [ntraj][ntemp], psum/pcomp hold ntemp * nchunks * L doubles
+hipError_t run_reduce_weighted(const JobDesc& jb, int ntemp, const double* wr, const double* nstar, double* psum,
+                               double* pcomp, long long chunk, int nchunks, double* cf_out, hipStream_t st);
 // Zimmermann's trick (hh_zimm.hip)
 int zimm_log_points(int L);
 size_t zimm_workspace_doubles(int L);

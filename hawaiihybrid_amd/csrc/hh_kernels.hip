@@ -1173,6 +1173,60 @@ __global__ void __launch_bounds__(256) cf_reduce_final_kernel(int L, int nchunks
     cf_out[k] = (s + cc) / (double)ntraj;
 }
 
+// CORRELATION_ARRAY: the same sums with a weight per trajectory and satellite temperature,
+// wr[t * ntemp + st] (Boltzmann reweight times the temperature's partition-function ratio);
+// blockIdx.z is the temperature.  Partial sums psum[(st * nchunks + c) * L + k].
+__global__ void __launch_bounds__(256) cf_reduce_weighted_partial_kernel(JobDesc jb, long long chunk, int ntemp,
+                                                                         const double* __restrict__ wr,
+                                                                         double* __restrict__ psum,
+                                                                         double* __restrict__ pcomp)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y, st = blockIdx.z;
+    if (k >= jb.L) return;
+    const long long t0 = (long long)c * chunk;
+    long long t1 = t0 + chunk;
+    if (t1 > jb.ntraj) t1 = jb.ntraj;
+    double s = 0.0, cc = 0.0;
+    for (long long t = t0; t < t1; ++t) {
+        double v;
+        if (jb.halves == 2) {
+            const long long a = 2 * t, b = 2 * t + 1;
+            if (dropped(jb.status[a], jb.status[b])) continue;
+            const double cf = k < jb.len[a] ? jb.rows[a * jb.L + k] : 0.0;
+            const double cb = k < jb.len[b] ? jb.rows[b * jb.L + k] : 0.0;
+            v = 0.5 * (cf + cb) * kALU * kALU * kALU;
+        } else {
+            if (dropped(jb.status[t], jb.status[t])) continue;
+            const double cf = k < jb.len[t] ? jb.rows[t * jb.L + k] : 0.0;
+            v = cf * kALU * kALU * kALU;
+        }
+        neumaier_add(s, cc, wr[t * ntemp + st] * v);
+    }
+    const long long o = ((long long)st * gridDim.y + c) * jb.L + k;
+    psum[o] = s;
+    pcomp[o] = cc;
+}
+
+// cf_out[st * L + k] = sum over chunks / nstar[st] (0 where nstar is 0: no trajectory carries
+// weight at that temperature)
+__global__ void __launch_bounds__(256) cf_reduce_weighted_final_kernel(int L, int nchunks, const double* __restrict__ nstar,
+                                                                       const double* __restrict__ psum,
+                                                                       const double* __restrict__ pcomp,
+                                                                       double* __restrict__ cf_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int st = blockIdx.y;
+    if (k >= L) return;
+    double s = 0.0, cc = 0.0;
+    for (int c = 0; c < nchunks; ++c) {
+        const long long o = ((long long)st * nchunks + c) * L + k;
+        neumaier_add(s, cc, psum[o]);
+        cc += pcomp[o];
+    }
+    cf_out[(long long)st * L + k] = nstar[st] > 0.0 ? (s + cc) / nstar[st] : 0.0;
+}
+
 // ------------------------------------------------------------------ host-side launchers
 template <int SYS>
 hipError_t launch_plain(const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
@@ -1316,6 +1370,18 @@ LaneSlots slots_view(int sys, void* base, long long S)
     sl.local = reinterpret_cast<long long*>(sl.d + (size_t)(2 * N + 8) * S);
     sl.i = reinterpret_cast<int*>(sl.local + S);
     return sl;
+}
+
+hipError_t run_reduce_weighted(const JobDesc& jb, int ntemp, const double* wr, const double* nstar, double* psum,
+                               double* pcomp, long long chunk, int nchunks, double* cf_out, hipStream_t st)
+{
+    dim3 g1((jb.L + 255) / 256, nchunks, ntemp);
+    hipLaunchKernelGGL(cf_reduce_weighted_partial_kernel, g1, dim3(256), 0, st, jb, chunk, ntemp, wr, psum, pcomp);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(cf_reduce_weighted_final_kernel, dim3((jb.L + 255) / 256, ntemp), dim3(256), 0, st, jb.L,
+                       nchunks, nstar, psum, pcomp, cf_out);
+    return hipGetLastError();
 }
 
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,

@@ -162,6 +162,29 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
 enum { HH_EVAL_PES = 0, HH_EVAL_DIPOLE = 1, HH_EVAL_HAMILTONIAN = 2, HH_EVAL_RHS = 3 };
 int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev);
 
+/* CORRELATION_ARRAY: calculate_correlation_array_and_save's accumulation (src/hawaii.c:2361-2793,
+ * weights src/hawaii.c:2639-2674).  The ensemble ic_dev was sampled at T_base (hh_sample_ics with
+ * the same pair state); hh_cf_run integrates it once (p->ppf is ignored) and every trajectory's
+ * CF is accumulated at each satellite temperature T_s with weight
+ *     w = exp(-E HkT / T_base / T_s (T_base - T_s)) / WTT,     E = H at the initial point,
+ * WTT = exp(-pesmin HkT / T_base / T_s (T_base - T_s)) for bound pairs with T_s < T_base, else 1;
+ * free pairs take no weight above T_base.  Outputs, per temperature: cf_dev[st * L + k] =
+ * sum(w ratio_st crln) / nstar_st and nstar[st] = sum(w) (the "trajectories" count the reference
+ * writes).  A dipole failure adds its weight and a zero CF; a trajectory whose last half ended on
+ * the step limit adds nothing (the array loop's `continue`, src/hawaii.c:2637, where hh_cf_run
+ * counts it in ntraj as calculate_correlation_and_save does).  Zimmermann jobs are not accepted. */
+typedef struct {
+    double T_base;        /* K: the ensemble's sampling temperature */
+    int pair_state;       /* HH_PAIR_FREE or HH_PAIR_BOUND */
+    int ntemp;            /* 1..64 satellite temperatures */
+    double pesmin;        /* Hartree: the bound-pair weight cap */
+    const double* temps;  /* [ntemp] K, host */
+    const double* ratios; /* [ntemp] partial partition function ratios, host (NULL: all 1) */
+} hh_cf_array_params;
+
+int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const hh_cf_array_params* a,
+                const double* ic_dev, long ntraj, double* cf_dev, double* nstar, hh_cf_stats* stats);
+
 /* ------------------------------------------------------------------ CF -> SF on the device */
 /* Connes apodization in place on cf_dev, Makhoul IDCT through a 2n-point complex FFT,
  * unit scaling: nu_dev[n] in cm^-1, sf_dev[n] (idct_cf_to_sf).  n must be a power of two. */

@@ -47,3 +47,37 @@ int orc_cf_moment(const double *cf, size_t len, double dt, double ntraj, int n, 
     *out = -SECOND_COEFF * (double)(d2 / ((long double)dt * dt)) * scale;
     return 0;
 }
+
+/*
+ * CORRELATION_ARRAY weights (calculate_correlation_array_and_save, src/hawaii.c:2639-2674).
+ * For trajectory t of energy E[t] (H at its initial point) and satellite temperature Ts:
+ *   WTT = exp(-pesmin HkT / Tb / Ts (Tb - Ts)) for bound pairs with Ts < Tb, else 1,
+ *   weight = exp(-E HkT / Tb / Ts (Tb - Ts)) / WTT,
+ * free pairs (pair_state 1) take no weight when Ts > Tb.  w[t * M + st] = weight * ratio[st]
+ * (ratios NULL: 1) and nstar[st] = sum of the weights in trajectory order.  A trajectory with
+ * counted[t] == 0 takes no weight at all: the array loop skips a trajectory whose last
+ * integrator call ended on the step limit (`if (status == -1) continue;`, src/hawaii.c:2637),
+ * unlike calculate_correlation_and_save, which still counts it (counted NULL: all counted).
+ */
+#define HKT (4.3597447222071e-18 / 1.380649e-23) /* src/constants.h:24 */
+
+void orc_cf_array_weights(const double *E, long n, double Tb, int pair_state, double pesmin, int M,
+                          const double *temps, const double *ratios, const int *counted, double *w, double *nstar)
+{
+    long t;
+    int st;
+    for (st = 0; st < M; ++st) nstar[st] = 0.0;
+    for (t = 0; t < n; ++t) {
+        for (st = 0; st < M; ++st) {
+            const double Ts = temps[st];
+            double WTT = 1.0, weight;
+            w[t * M + st] = 0.0;
+            if (counted && !counted[t]) continue;
+            if (pair_state == 2 && Ts < Tb) WTT = exp(-pesmin * HKT / Tb / Ts * (Tb - Ts));
+            weight = exp(-E[t] * HKT / Tb / Ts * (Tb - Ts)) / WTT;
+            if (pair_state == 1 && Ts > Tb) continue;
+            w[t * M + st] = weight * (ratios ? ratios[st] : 1.0);
+            nstar[st] += weight;
+        }
+    }
+}

@@ -158,6 +158,9 @@ double orc_pf_analytic(const orc_system *s, double T);
 /* compute_Mn_from_cf_using_classical_detailed_balance (src/hawaii.c:5945-5979), n = 0 or 2;
  * ntraj > 0 divides (an unnormalised CF).  Returns 0, or -1 when the CF is too short. */
 int orc_cf_moment(const double *cf, size_t len, double dt, double ntraj, int n, double *out);
+/* CORRELATION_ARRAY weights per trajectory and satellite temperature (moments.c) */
+void orc_cf_array_weights(const double *E, long n, double Tb, int pair_state, double pesmin, int M,
+                          const double *temps, const double *ratios, const int *counted, double *w, double *nstar);
 
 /* ---------------------------------------------------------------- N2-Ar PIP-NN surfaces */
 /* A multilayer perceptron as the reference's MLPES / qm::MLP (PES-IDS/mlp.hpp:178-209,

@@ -142,6 +142,7 @@ def lib():
             "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
             "orc_pf_analytic": (d, [P(System), d]),
             "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
+            "orc_cf_array_weights": (None, [v, C.c_long, d, i, d, i, v, v, v, v, v]),
             "orc_zimmermann_series_cf": (None, [v, i, v]),
             "orc_mlp_forward": (d, [P(OrcMlp), v, v]),
             "orc_pip_n2ar_pes": (None, [v, v]),
@@ -201,6 +202,43 @@ def ensemble_from_rows(crln, ppf=1.0):
     for row in crln:
         acc += ppf * row
     return acc / crln.shape[0]
+
+
+def hamiltonian(system, ic):
+    """H at each initial point (rows of ic)."""
+    L = lib()
+    return np.array([L.orc_hamiltonian(C.byref(system), ptr(np.ascontiguousarray(r, dtype=np.float64))) for r in ic])
+
+
+def cf_array_weights(E, T_base, pair_state, pesmin, temps, ratios=None, counted=None):
+    """CORRELATION_ARRAY weights w [n][M] (weight x ratio) and nstar [M] (oracle/moments.c);
+    counted[t] == 0 marks a trajectory the array loop skips (step limit)."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    t = np.ascontiguousarray(temps, dtype=np.float64)
+    r = None if ratios is None else np.ascontiguousarray(ratios, dtype=np.float64)
+    c = None if counted is None else np.ascontiguousarray(counted, dtype=np.int32)
+    w = np.zeros((len(E), len(t)))
+    ns = np.zeros(len(t))
+    lib().orc_cf_array_weights(ptr(E), len(E), T_base, pair_state, pesmin, len(t), ptr(t),
+                               ptr(r) if r is not None else None, ptr(c) if c is not None else None, ptr(w), ptr(ns))
+    return w, ns
+
+
+def array_counted(status, halves_both=True):
+    """1 where the array loop counts a trajectory: all but those whose last integrator call
+    ended on the step limit without an earlier dipole failure (oracle orc_traj_cf's status)."""
+    return (np.asarray(status) != 4).astype(np.int32)
+
+
+def ensemble_array_from_rows(crln, w, nstar):
+    """Per temperature: sum over trajectories, in order, of w[t][st] crln[t], / nstar[st]
+    (src/hawaii.c:2666-2670, the writer divides by the weight sum)."""
+    M = w.shape[1]
+    acc = np.zeros((M, crln.shape[1]))
+    for t, row in enumerate(crln):
+        for st in range(M):
+            acc[st] += w[t, st] * row
+    return np.array([acc[st] / nstar[st] if nstar[st] > 0 else np.zeros(crln.shape[1]) for st in range(M)])
 
 
 def idct(v):

@@ -127,8 +127,7 @@ class Lexer {
 };
 
 // keywords of the reference that steer parts outside this engine: accepted, ignored
-const char* kIgnored[] = {"SATELLITE_TEMPERATURES", "INITIALM2_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS",
-                          "CF_FILENAMES", "PARTIAL_PARTITION_FUNCTION_RATIOS",
+const char* kIgnored[] = {"INITIALM2_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS",
                           "APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
                           "AVERAGE_TIME_BETWEEN_COLLISIONS", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
                           "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
@@ -221,10 +220,12 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
         if (eq.text != "=") return err(line, "expected '=' after %s", key);
         Tok v = lx.next();
         std::vector<double> list;
+        std::vector<std::string> slist;
         if (v.text == "{") {
             Tok a = lx.next();
             while (a.kind != Tok::END && a.text != "}") {
                 if (a.kind == Tok::NUMBER) list.push_back(a.num);
+                if (a.kind == Tok::STRING) slist.push_back(a.text);
                 a = lx.next();
             }
         }
@@ -265,6 +266,8 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 c->calculation_type = HH_CALC_PROCESSING;
             else if (w == "PHASE_SPACE_M0")
                 c->calculation_type = HH_CALC_PHASE_SPACE_M0;
+            else if (w == "CORRELATION_ARRAY")
+                c->calculation_type = HH_CALC_CORRELATION_ARRAY;
             else
                 return err(line, "CALCULATION_TYPE %s is not part of this engine", w);
         } else if (key == "PAIR_STATE") {
@@ -328,6 +331,18 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             if (list.empty() || list.size() > 64) return err(line, "%s expects {T1, T2, ...} (at most 64)", key);
             c->ntemperatures = (int)list.size();
             for (size_t k = 0; k < list.size(); ++k) c->temperatures[k] = list[k];
+        } else if (key == "SATELLITE_TEMPERATURES") {
+            if (list.empty() || list.size() > 64) return err(line, "%s expects {T1, T2, ...} (at most 64)", key);
+            c->nsatellite = (int)list.size();
+            for (size_t k = 0; k < list.size(); ++k) c->satellite_temperatures[k] = list[k];
+        } else if (key == "PARTIAL_PARTITION_FUNCTION_RATIOS") {
+            if (list.empty() || list.size() > 64) return err(line, "%s expects {r1, r2, ...} (at most 64)", key);
+            c->nppf_ratios = (int)list.size();
+            for (size_t k = 0; k < list.size(); ++k) c->ppf_ratios[k] = list[k];
+        } else if (key == "CF_FILENAMES") {
+            if (slist.empty() || slist.size() > 64) return err(line, "%s expects {\"a\", \"b\", ...} (at most 64)", key);
+            c->ncf_filenames = (int)slist.size();
+            for (size_t k = 0; k < slist.size(); ++k) copy_str(c->cf_filenames[k], slist[k]);
         } else if (key == "USE_ZIMMERMANN_TRICK") {
             const std::string w = Lexer::upper(v.text);
             c->use_zimmermann_trick = (w == "TRUE" || (v.kind == Tok::NUMBER && v.num != 0));
@@ -776,6 +791,168 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     return code;
 }
 
+// CALCULATION_TYPE = CORRELATION_ARRAY: calculate_correlation_array_and_save
+// (src/hawaii.c:2361-2793).  One ensemble sampled at TEMPERATURE (the base temperature) per
+// iteration, integrated once and accumulated at every satellite temperature with its Boltzmann
+// weight (hh_cf_array); after each iteration every CF file is rewritten with its weight sum as
+// the trajectory count.
+int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+{
+    // the reference's checks (src/hawaii.c:2367-2378)
+    if (c->max_trajectory_length <= 0 || c->Rcut <= 0 || c->sampling_time <= 0 || c->total_trajectories <= 0 ||
+        c->cvode_tolerance <= 0 || c->niterations < 1 || c->nsatellite < 1 || c->ncf_filenames != c->nsatellite)
+        return fail(HH_E_INVALID, "MaxTrajectoryLength, Rcut, sampling_time, total_trajectories, cvode_tolerance, "
+                                  "niterations, satellite_temperatures and one cf_filename per satellite "
+                                  "temperature must be set");
+    if (c->nppf_ratios != 0 && c->nppf_ratios != c->nsatellite)
+        return fail(HH_E_INVALID, "partial_partition_function_ratios needs one ratio per satellite temperature");
+    if (c->use_zimmermann_trick)
+        return fail(HH_E_INVALID, "Zimmermann's trick is not part of this engine's CORRELATION_ARRAY");
+    const int ps = c->pair_state ? c->pair_state : HH_PAIR_FREE;
+    if (ps != HH_PAIR_FREE && ps != HH_PAIR_BOUND)
+        return fail(HH_E_INVALID, "CORRELATION_ARRAY needs PAIR_STATE FREE_AND_METASTABLE or BOUND");
+    if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
+        return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
+    hh_system sys{};
+    {
+        const int rc0 = system_of(c, &sys);
+        if (rc0 < 0) return rc0;
+    }
+    const int M = c->nsatellite;
+    const double Tb = c->temperature;
+    // partition-function ratios (and M0) by VEGAS at each satellite temperature when not given
+    // (src/hawaii.c:2388-2437)
+    std::vector<double> ratios(M, 0.0), hep_m0(M, 0.0);
+    if (c->nppf_ratios == M) {
+        for (int st = 0; st < M; ++st) ratios[st] = c->ppf_ratios[st];
+    } else {
+        if (!o.quiet)
+            std::printf("Ratios of partial partition functions to full (analytic) partial partition function are not "
+                        "provided.\nConducting calculations using adaptive Monte Carlo method.\n");
+        for (int st = 0; st < M; ++st) {
+            const double T = c->satellite_temperatures[st];
+            const double pf_an = hh_pf_analytic(&sys, T);
+            if (pf_an < 0) return (int)HH_E_INVALID;
+            hh_vegas_params vp{HH_INTEGRAND_PF, ps, T, c->sampler_Rmin, c->sampler_Rmax,
+                               (int)(c->hep_ppf_niterations > 0 ? c->hep_ppf_niterations : 12),
+                               c->hep_ppf_npoints > 0 ? c->hep_ppf_npoints : 1000000L, (o.seed ? o.seed : 1u) + st};
+            hh_vegas_result vr{};
+            int vrc = hh_vegas(e, &sys, &vp, &vr);
+            if (vrc < 0) return vrc;
+            ratios[st] = vr.value / pf_an;
+            vp.integrand = HH_INTEGRAND_M0;
+            vp.niterations = (int)(c->hep_m0_niterations > 0 ? c->hep_m0_niterations : 12);
+            vp.npoints = c->hep_m0_npoints > 0 ? c->hep_m0_npoints : 1000000L;
+            vrc = hh_vegas(e, &sys, &vp, &vr);
+            if (vrc < 0) return vrc;
+            hep_m0[st] = vr.value * kZeroCoeff / pf_an;
+            if (!o.quiet) {
+                std::printf("Satellite temperature = %.2e\n", T);
+                std::printf("Analytic partition function divided by V: %.5e\n", pf_an);
+                std::printf("T = %.2e => PPF ratio: %.5e\n", T, ratios[st]);
+                std::printf("T = %.2e => M0: %.5e\n", T, hep_m0[st]);
+            }
+        }
+    }
+
+    const int qp = hh_qp_size(sys.system);
+    const long n_iter = c->total_trajectories / c->niterations;
+    if (n_iter <= 0) return fail(HH_E_INVALID, "total_trajectories < niterations");
+    const uint32_t seed = o.seed ? o.seed : good_seed();
+    hh_pes_fn pes = nullptr;
+    void* so = nullptr;
+    if (c->so_potential[0]) {
+        so = dlopen(c->so_potential, RTLD_LAZY);
+        if (so) {
+            pes = reinterpret_cast<hh_pes_fn>(dlsym(so, "pes_lab"));
+            if (!pes) {
+                dlclose(so);
+                return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
+            }
+            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
+        } else if (!o.quiet) {
+            std::printf("note: %s not loadable; the engine's built-in surface drives the sampler\n", c->so_potential);
+        }
+    }
+    hh_sampler smp{HH_IC_REJECTION, ps, Tb, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, 0.0};
+    const long total = n_iter * c->niterations;
+    std::vector<double> ic((size_t)total * qp);
+    long nd = 0;
+    if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
+    long rc = hh_sample_ics(&sys, &smp, seed, total, pes, ic.data(), &nd);
+    if (so) dlclose(so);
+    if (rc < 0) return (int)rc;
+
+    hh_cf_params p{};
+    const bool mixed = o.precision == HH_PREC_MIXED;
+    p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+    p.dt = c->sampling_time;
+    p.Rcut = c->Rcut;
+    p.ppf = 1.0;
+    p.L = (int)c->max_trajectory_length;
+    p.halves = HH_HALVES_BOTH;
+    p.max_steps = 50000;
+    p.engine = HH_ENGINE_BATCHED;
+    p.precision = mixed ? HH_PREC_MIXED : HH_PREC_FP64;
+    hh_cf_array_params ap{Tb, ps, M, c->pesmin, c->satellite_temperatures, ratios.data()};
+
+    double *ic_dev = nullptr, *cf_dev = nullptr;
+    int code = HH_OK;
+    std::vector<double> soa((size_t)n_iter * qp), it((size_t)M * p.L), ns(M);
+    std::vector<double> sum((size_t)M * p.L, 0.0), nstar(M, 0.0);
+    do {
+        if (hipMalloc(&ic_dev, (size_t)n_iter * qp * sizeof(double)) != hipSuccess ||
+            hipMalloc(&cf_dev, (size_t)M * p.L * sizeof(double)) != hipSuccess) {
+            code = fail(HH_E_NOMEM, "device allocation failed");
+            break;
+        }
+        for (long iter = 0; iter < c->niterations && code == HH_OK; ++iter) {
+            const long first = iter * n_iter;
+            for (long t = 0; t < n_iter; ++t)
+                for (int i = 0; i < qp; ++i) soa[(size_t)i * n_iter + t] = ic[(size_t)(first + t) * qp + i];
+            if (hipMemcpy(ic_dev, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "upload of initial conditions failed");
+                break;
+            }
+            hh_cf_stats st;
+            if ((code = hh_cf_array(e, &sys, &p, &ap, ic_dev, n_iter, cf_dev, ns.data(), &st)) < 0) break;
+            if (hipMemcpy(it.data(), cf_dev, it.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            // the reference accumulates unnormalised sums and their weight sums (src/hawaii.c:2681-2697)
+            for (int s2 = 0; s2 < M; ++s2) {
+                for (int k = 0; k < p.L; ++k) sum[(size_t)s2 * p.L + k] += it[(size_t)s2 * p.L + k] * ns[s2];
+                nstar[s2] += ns[s2];
+            }
+            if (!o.quiet)
+                std::printf("\nITERATION %ld/%ld: accumulated %ld trajectories. Saving the temporary results\n",
+                            iter + 1, c->niterations, (iter + 1) * n_iter);
+            for (int s2 = 0; s2 < M && code == HH_OK; ++s2) {
+                Series cf;
+                cf.T = c->satellite_temperatures[s2];
+                cf.ntraj = nstar[s2];
+                cf.x.resize(p.L);
+                cf.y.assign(p.L, 0.0);
+                for (int k = 0; k < p.L; ++k) {
+                    cf.x[k] = c->sampling_time * k;
+                    cf.y[k] = nstar[s2] > 0 ? sum[(size_t)s2 * p.L + k] / nstar[s2] : 0.0;
+                }
+                if (!o.quiet) {
+                    const double m0cf = cf.y[0] * kZeroCoeff / kALU / kALU / kALU;
+                    std::printf("T = %.2f: NSTAR = %.2f, M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e\n",
+                                cf.T, nstar[s2], m0cf, hep_m0[s2]);
+                }
+                code = write_series(c->cf_filenames[s2], cf);
+            }
+            if (!o.quiet) std::fflush(stdout);
+        }
+    } while (false);
+    if (ic_dev) (void)hipFree(ic_dev);
+    if (cf_dev) (void)hipFree(cf_dev);
+    return code;
+}
+
 }  // namespace
 
 extern "C" {
@@ -805,6 +982,8 @@ int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
         rc = run_processing(e, conf, o.quiet != 0);
     } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M0) {
         rc = run_phase_space_m0(e, conf, o);
+    } else if (conf->calculation_type == HH_CALC_CORRELATION_ARRAY) {
+        rc = run_correlation_array(e, conf, o);
     } else {
         rc = fail(HH_E_INVALID, "CALCULATION_TYPE is not set");
     }

@@ -33,7 +33,8 @@
 extern "C" {
 #endif
 
-enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3 };
+enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3,
+       HH_CALC_CORRELATION_ARRAY = 4 };
 enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF };
@@ -75,6 +76,13 @@ typedef struct {
     long hep_m0_niterations, hep_m0_npoints;
     int ntemperatures;                           /* TEMPERATURES = {...} (PHASE_SPACE_M0) */
     double temperatures[64];
+    /* CORRELATION_ARRAY (src/driver.c:1170-1247) */
+    int nsatellite;                              /* SATELLITE_TEMPERATURES = {...} */
+    double satellite_temperatures[64];
+    int nppf_ratios;                             /* PARTIAL_PARTITION_FUNCTION_RATIOS = {...} (0: VEGAS) */
+    double ppf_ratios[64];
+    int ncf_filenames;                           /* CF_FILENAMES = {"...", ...} */
+    char cf_filenames[64][HH_CONF_PATH];
 } hh_conf;
 
 /* Parse a configuration file (no device use).  Unknown keywords and unsupported PROCESSING

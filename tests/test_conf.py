@@ -27,7 +27,10 @@ class Conf(C.Structure):
                 ("nproc", C.c_int), ("proc", Proc * 128),
                 ("hep_ppf_niterations", C.c_long), ("hep_ppf_npoints", C.c_long),
                 ("hep_m0_niterations", C.c_long), ("hep_m0_npoints", C.c_long),
-                ("ntemperatures", C.c_int), ("temperatures", C.c_double * 64)]
+                ("ntemperatures", C.c_int), ("temperatures", C.c_double * 64),
+                ("nsatellite", C.c_int), ("satellite_temperatures", C.c_double * 64),
+                ("nppf_ratios", C.c_int), ("ppf_ratios", C.c_double * 64),
+                ("ncf_filenames", C.c_int), ("cf_filenames", (C.c_char * 512) * 64)]
 
 
 def parse_text(tmp_path, text):
@@ -66,6 +69,19 @@ def test_parse_reference_example_configs(name):
     assert rc == 0, H.lib().hh_last_error()
     assert c.calculation_type == 1 and c.n_monomers == 2
     assert c.total_trajectories > 0 and c.max_trajectory_length > 0 and c.Rcut > 0
+
+
+def test_parse_reference_correlation_array_config():
+    """The reference's CORRELATION_ARRAY example (six satellite temperatures of bound He-Ar)."""
+    path = os.path.join(REF_CONFIGS, "CF-B-He-Ar.conf")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    rc, c = parse(path)
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 4 and c.pair_state == 2 and c.temperature == 100.0
+    assert c.nsatellite == 6 and list(c.satellite_temperatures[:6]) == [50.0, 60.0, 70.0, 80.0, 90.0, 100.0]
+    assert c.ncf_filenames == 6 and c.cf_filenames[0].value == b"CF-B-He-Ar-50.0.txt"
+    assert c.nppf_ratios == 0
 
 
 def test_parse_linear_molecule_and_processing(tmp_path):
@@ -135,3 +151,25 @@ def test_parse_phase_space_m0(tmp_path):
     assert c.calculation_type == 3
     assert c.ntemperatures == 3 and list(c.temperatures[:3]) == [100.0, 200.0, 300.0]
     assert c.hep_m0_niterations == 10 and c.hep_m0_npoints == 200000
+
+
+def test_parse_correlation_array(tmp_path):
+    """CORRELATION_ARRAY's lists: satellite temperatures, one CF file name each and, optionally,
+    the partition-function ratios (src/driver.c:1170-1247)."""
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = CORRELATION_ARRAY
+  PAIR_STATE = BOUND
+  TEMPERATURE = 100.0
+  satellite_temperatures = {50.0, 60.0, 70.0}
+  partial_partition_function_ratios = {1.5, 1.2, 1.0}
+  cf_filenames = { "CF-50.txt",
+                   "CF-60.txt", "CF-70.txt" }
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 4 and c.pair_state == 2
+    assert c.nsatellite == 3 and list(c.satellite_temperatures[:3]) == [50.0, 60.0, 70.0]
+    assert c.nppf_ratios == 3 and list(c.ppf_ratios[:3]) == [1.5, 1.2, 1.0]
+    assert c.ncf_filenames == 3 and [c.cf_filenames[k].value for k in range(3)] == [b"CF-50.txt", b"CF-60.txt",
+                                                                                     b"CF-70.txt"]

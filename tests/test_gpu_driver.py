@@ -53,6 +53,30 @@ def test_driver_correlation_single_matches_oracle(tmp_path):
     assert sf.shape == (256, 2)
 
 
+def test_driver_correlation_array_matches_oracle(tmp_path):
+    """CALCULATION_TYPE = CORRELATION_ARRAY: one file per satellite temperature, each with its
+    weight sum as the trajectory count, against the oracle's rows of the same ensemble weighted
+    with the oracle's restatement of src/hawaii.c:2639-2674."""
+    conf = os.path.join(HERE, "conf", "cf-array-hear-bound-small.conf")
+    r = run_driver([conf, "--seed", "42"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "ITERATION 2/2: accumulated 256 trajectories" in r.stdout
+    temps, ratios = [60.0, 100.0, 150.0], [1.25, 1.0, 0.75]
+    ic, _, _ = O.sample_ics(O.hear(), O.Sampler(O.IC_REJECTION, 2, 100.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0), 42, 256)
+    ref = O.cf_rows(O.hear(), O.CFParams(1e-12, 1e-12, 200.0, 256, 40.0, O.HALVES_BOTH, 50000, 0), ic, 8)
+    w, ns = O.cf_array_weights(O.hamiltonian(O.hear(), ic), 100.0, 2, O.PESMIN_HEAR, temps, ratios,
+                               counted=O.array_counted(ref["status"]))
+    want = O.ensemble_array_from_rows(ref["crln"], w, ns)
+    for k, T in enumerate(temps):
+        head, cf = read_columns(tmp_path / f"CF-B-hear-{T:.1f}.txt")
+        assert f"# TEMPERATURE: {T:.2f}" in head
+        assert f"# AVERAGE OVER {ns[k]:.2f} TRAJECTORIES" in head
+        assert cf.shape == (256, 2)
+        err = np.linalg.norm(cf[:, 1] - want[k]) / np.linalg.norm(want[k])
+        print(f"  T={T}: nstar {ns[k]:.3f} rel L2 {err:.2e}", flush=True)
+        assert err < 1e-8, (T, err)
+
+
 def write_ref_format(path, x, y, T, ntraj, sf):
     with open(path, "w") as f:
         f.write(f"# HAWAII HYBRID v0.1\n# TEMPERATURE: {T:.2f}\n# AVERAGE OVER {ntraj:.2f} TRAJECTORIES\n")

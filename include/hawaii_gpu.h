@@ -18,6 +18,9 @@
  *                        (src/hawaii.c:2969-3114) with correlation_eval
  *                        (src/hawaii.c:2116-2347) and the CVODE propagation it drives
  *                        (src/trajectory.c:146-200)
+ *   hh_cf_array          the trajectory loop of calculate_correlation_array_and_save
+ *                        (src/hawaii.c:2361-2793): one ensemble, CFs at satellite
+ *                        temperatures by Boltzmann reweighting (src/hawaii.c:2639-2674)
  *   hh_cf_to_sf          idct_cf_to_sf (src/hawaii.c:5569-5601)
  *   hh_m0                calculate_M0 / mpi_calculate_M0 (src/hawaii.c:1423-1491,
  *                        src/hawaii.c:1620-1738)

@@ -18,7 +18,7 @@ PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
 ENGINE_PLAIN, ENGINE_BATCHED = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 HALVES_BOTH, HALVES_FORWARD = 0, 1
-EVAL_PES, EVAL_DIPOLE, EVAL_HAMILTONIAN, EVAL_RHS = 0, 1, 2, 3
+EVAL_PES, EVAL_DIPOLE, EVAL_HAMILTONIAN, EVAL_RHS, EVAL_DIPOLE_RATE = 0, 1, 2, 3, 4
 
 # example_configs/CF-F-He-Ar.conf:5, example_configs/CF-F-CO2-Ar.conf:5,27,42
 MU_HEAR = 6.632039160324215e3
@@ -91,6 +91,7 @@ class VegasResult(C.Structure):
 
 INTEGRAND_M0, INTEGRAND_M2, INTEGRAND_PF = 0, 1, 2
 ZERO_COEFF = 0.00361479637 / (4.0 * 3.141592653589793)   # src/constants.h:29
+SECOND_COEFF = 13856114.29344114                          # src/constants.h:30
 
 REQ_WAITING, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 REQ_CF, REQ_M0 = 0, 1
@@ -147,6 +148,7 @@ def lib():
             "hh_cf_items": (i, [v, i, v, v, v, v, v]),
             "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
             "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
+            "hh_m2": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
             "hh_eval": (i, [v, P(System), i, v, C.c_long, v]),
             "hh_cf_array": (i, [v, P(System), P(CFParams), P(CFArrayParams), v, C.c_long, v, v, P(CFStats)]),
             "hh_conf_parse": (i, [C.c_char_p, v]),
@@ -296,9 +298,9 @@ class Engine:
 
     def eval(self, system, what, y_dev):
         """Device surfaces at phase points y_dev (SoA [QP][n]): EVAL_PES -> [n], EVAL_DIPOLE ->
-        [3][n], EVAL_HAMILTONIAN -> [n], EVAL_RHS -> [QP][n]."""
+        [3][n], EVAL_HAMILTONIAN -> [n], EVAL_RHS -> [QP][n], EVAL_DIPOLE_RATE (|d mu/dt|^2) -> [n]."""
         n = y_dev.shape[1]
-        rows = {EVAL_PES: 1, EVAL_DIPOLE: 3, EVAL_HAMILTONIAN: 1, EVAL_RHS: y_dev.shape[0]}[what]
+        rows = {EVAL_PES: 1, EVAL_DIPOLE: 3, EVAL_HAMILTONIAN: 1, EVAL_RHS: y_dev.shape[0], EVAL_DIPOLE_RATE: 1}[what]
         out = self.torch.empty((rows, n), dtype=self.torch.float64, device=y_dev.device)
         self.torch.cuda.synchronize(self.device)
         _check(lib().hh_eval(self.h, C.byref(system), what, C.c_void_p(y_dev.data_ptr()), n,
@@ -327,6 +329,14 @@ class Engine:
         m, q, ms = C.c_double(), C.c_double(), C.c_double()
         self.torch.cuda.synchronize(self.device)
         _check(lib().hh_m0(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf, C.byref(m),
+                           C.byref(q), C.byref(ms)))
+        return m.value, q.value, ms.value
+
+    def m2(self, system, sampler, seed, npoints, nstreams, ppf=1.0):
+        """Rejection-sampler M2 (hh_m2): the hh_m0 streams averaging |d mu/dt|^2, times SecondCoeff."""
+        m, q, ms = C.c_double(), C.c_double(), C.c_double()
+        self.torch.cuda.synchronize(self.device)
+        _check(lib().hh_m2(self.h, C.byref(system), C.byref(sampler), seed, npoints, nstreams, ppf, C.byref(m),
                            C.byref(q), C.byref(ms)))
         return m.value, q.value, ms.value
 

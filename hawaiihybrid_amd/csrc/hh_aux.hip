@@ -11,6 +11,7 @@
 // (mt_seed32 seeding, state interleaved across lanes so equal draw indices coalesce) and
 // runs the reference's candidate loop (q_generator, p_generator, Hamiltonian, reject, pair
 // filter, dipole, Welford running mean / sum of squared deviations) over its quota.
+// M2 (calculate_M2, src/hawaii.c:1524-1617) is the same loop with |d mu/dt|^2 as the value.
 #include <hip/hip_runtime.h>
 
 #include "hh_engine.h"
@@ -199,7 +200,8 @@ __device__ double m0_normal(LaneMt& mt)
 
 // One launch draws candidates for every stream until it has its quota, hits the candidate
 // limit, or (ticks > 0) the launch has run `ticks` wall-clock ticks; `init` seeds the streams.
-template <int SYS>
+// ORDER 0 averages |mu|^2 (M0), ORDER 2 |d mu/dt|^2 (M2).
+template <int SYS, int ORDER>
 __global__ void __launch_bounds__(256) m0_kernel(SysConst sc, M0Params p, uint32_t seed, long nstreams, long npoints,
                                                  uint32_t* __restrict__ state, M0Acc acc, int init, long long ticks)
 {
@@ -262,9 +264,14 @@ __global__ void __launch_bounds__(256) m0_kernel(SysConst sc, M0Params p, uint32
         if (weight < u) continue;
         if (p.pair_state == 1 && energy < 0.0) continue;
         if (p.pair_state == 2 && energy > 0.0) continue;
-        double mu[3];
-        dipole<SYS, double>(sc, y, mu);
-        const double f = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+        double f;
+        if constexpr (ORDER == 2) {
+            f = dipole_rate_sq<SYS>(sc, y);
+        } else {
+            double mu[3];
+            dipole<SYS, double>(sc, y, mu);
+            f = mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
+        }
         const double diff = f - m;
         m += diff / (n + 1.0);
         q += diff * diff * (n / (n + 1.0));
@@ -279,15 +286,12 @@ __global__ void __launch_bounds__(256) m0_kernel(SysConst sc, M0Params p, uint32
 }
 
 hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
-                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st)
+                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st, int order)
 {
     const unsigned grid = (unsigned)((nstreams + 255) / 256);
-    if (sys == SYS_HEAR)
-        hipLaunchKernelGGL(m0_kernel<SYS_HEAR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state, acc,
-                           init, ticks);
-    else
-        hipLaunchKernelGGL(m0_kernel<SYS_CO2AR>, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state,
-                           acc, init, ticks);
+    auto k = sys == SYS_HEAR ? (order == 2 ? m0_kernel<SYS_HEAR, 2> : m0_kernel<SYS_HEAR, 0>)
+                             : (order == 2 ? m0_kernel<SYS_CO2AR, 2> : m0_kernel<SYS_CO2AR, 0>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, st, sc, p, seed, nstreams, npoints, state, acc, init, ticks);
     return hipGetLastError();
 }
 

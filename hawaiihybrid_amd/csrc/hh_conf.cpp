@@ -27,6 +27,7 @@ using hh::fail;
 
 constexpr double kALU = 5.29177210903e-11;                    // src/constants.h:8
 constexpr double kZeroCoeff = 0.00361479637 / (4.0 * M_PI);   // src/constants.h:29
+constexpr double kSecondCoeff = 13856114.29344114;              // src/constants.h:30
 
 struct Tok {
     enum Kind { END, IDENT, NUMBER, STRING, SYM, BLOCK } kind = END;
@@ -127,8 +128,7 @@ class Lexer {
 };
 
 // keywords of the reference that steer parts outside this engine: accepted, ignored
-const char* kIgnored[] = {"INITIALM2_NPOINTS", "HEP_M2_NITERATIONS", "HEP_M2_NPOINTS",
-                          "APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
+const char* kIgnored[] = {"APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
                           "AVERAGE_TIME_BETWEEN_COLLISIONS", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
                           "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
                           "JINI_HISTOGRAM_BINS", "JINI_HISTOGRAM_MAX", "JINI_HISTOGRAM_FILENAME",
@@ -264,8 +264,10 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 c->calculation_type = HH_CALC_CORRELATION_SINGLE;
             else if (w == "PROCESSING")
                 c->calculation_type = HH_CALC_PROCESSING;
-            else if (w == "PHASE_SPACE_M0")
+            else if (w == "CALCULATE_PHASE_SPACE_M0")
                 c->calculation_type = HH_CALC_PHASE_SPACE_M0;
+            else if (w == "CALCULATE_PHASE_SPACE_M2")
+                c->calculation_type = HH_CALC_PHASE_SPACE_M2;
             else if (w == "CORRELATION_ARRAY")
                 c->calculation_type = HH_CALC_CORRELATION_ARRAY;
             else
@@ -327,6 +329,12 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             rc = inum(c->hep_m0_niterations);
         } else if (key == "HEP_M0_NPOINTS") {
             rc = inum(c->hep_m0_npoints);
+        } else if (key == "INITIALM2_NPOINTS") {
+            rc = inum(c->initialM2_npoints);
+        } else if (key == "HEP_M2_NITERATIONS") {
+            rc = inum(c->hep_m2_niterations);
+        } else if (key == "HEP_M2_NPOINTS") {
+            rc = inum(c->hep_m2_npoints);
         } else if (key == "TEMPERATURES") {
             if (list.empty() || list.size() > 64) return err(line, "%s expects {T1, T2, ...} (at most 64)", key);
             c->ntemperatures = (int)list.size();
@@ -582,6 +590,24 @@ int system_of(const hh_conf* c, hh_system* sys)
     return HH_OK;
 }
 
+// The M2 estimates from the accumulated CF after each iteration (src/hawaii.c:3070-3090,
+// 2735-2751): the 9-point stencil over the first 5 samples, the 21-point one over 11, each
+// beside the VEGAS M2 (0 where none was integrated).
+void print_m2_from_cf(const Series& cf, double hep, const char* prefix)
+{
+    const long len = (long)cf.y.size();
+    const double dt = cf.x.size() > 1 ? cf.x[1] - cf.x[0] : 0.0;
+    double m2 = 0.0;
+    if (len >= 5 && hh_cf_moment(cf.y.data(), 5, dt, 0.0, 2, &m2) == HH_OK)
+        std::printf("%sM2 ESTIMATE FROM CF (9-point): %.5e, PRELIMINARY M2 ESTIMATE: %.5e, diff: %.3f%%\n", prefix, m2,
+                    hep, (m2 - hep) / hep * 100.0);
+    else
+        std::printf("%sTrajectory is too short to estimate M2\n", prefix);
+    if (len >= 11 && hh_cf_moment(cf.y.data(), len, dt, 0.0, 2, &m2) == HH_OK)
+        std::printf("%sM2 ESTIMATE FROM CF (21-point): %.5e, PRELIMINARY M2 ESTIMATE: %.5e, diff: %.3f%%\n", prefix,
+                    m2, hep, (m2 - hep) / hep * 100.0);
+}
+
 // CALCULATION_PHASE_SPACE_M0 (src/driver.c:3389-3445): VEGAS M0 at each temperature
 int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
 {
@@ -592,7 +618,7 @@ int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
     if (rc < 0) return rc;
     std::vector<double> Ts(c->temperatures, c->temperatures + c->ntemperatures);
     if (Ts.empty() && c->temperature > 0) Ts.push_back(c->temperature);
-    if (Ts.empty()) return fail(HH_E_INVALID, "no temperature is provided to run PHASE_SPACE_M0");
+    if (Ts.empty()) return fail(HH_E_INVALID, "no temperature is provided to run CALCULATION_PHASE_SPACE_M0");
     for (double T : Ts) {
         const double pf_an = hh_pf_analytic(&sys, T);
         if (pf_an < 0) return (int)HH_E_INVALID;
@@ -605,6 +631,47 @@ int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
         if (rc < 0) return rc;
         std::printf("T = %.2e => M0: %.5e +- %.5e\n", T, vr.value * kZeroCoeff / pf_an, vr.error * kZeroCoeff / pf_an);
     }
+    return HH_OK;
+}
+
+// The VEGAS M2 at T (c_mpi_perform_integration(INTEGRAND_M2), scaled by SecondCoeff / pf_analytic
+// as src/driver.c:3377-3381 and src/hawaii.c:2867-2880 do), with HEP_M2_NITERATIONS x
+// HEP_M2_NPOINTS (default 12 x 10^6).
+int hep_m2(hh_engine* e, const hh_system& sys, const hh_conf* c, int pair_state, double T, uint32_t seed, double* m2,
+           double* m2_err)
+{
+    const double pf_an = hh_pf_analytic(&sys, T);
+    if (pf_an < 0) return (int)HH_E_INVALID;
+    hh_vegas_params vp{HH_INTEGRAND_M2, pair_state, T, c->sampler_Rmin, c->sampler_Rmax,
+                       (int)(c->hep_m2_niterations > 0 ? c->hep_m2_niterations : 12),
+                       c->hep_m2_npoints > 0 ? c->hep_m2_npoints : 1000000L, seed};
+    hh_vegas_result vr{};
+    const int rc = hh_vegas(e, &sys, &vp, &vr);
+    if (rc < 0) return rc;
+    *m2 = vr.value * kSecondCoeff / pf_an;
+    if (m2_err) *m2_err = vr.error * kSecondCoeff / pf_an;
+    return HH_OK;
+}
+
+// CALCULATION_PHASE_SPACE_M2 (src/driver.c:3344-3388): VEGAS M2 at TEMPERATURE
+int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+{
+    if (!(c->sampler_Rmin > 0))
+        return fail(HH_E_INVALID, "minimum value for R throught integration (sampler_Rmin) must be set");
+    if (!(c->sampler_Rmax > 0))
+        return fail(HH_E_INVALID, "maximum value for R throught integration (sampler_Rmax) must be set");
+    if (!(c->temperature > 0)) return fail(HH_E_INVALID, "no temperature is provided to run CALCULATION_PHASE_SPACE_M2");
+    hh_system sys{};
+    int rc = system_of(c, &sys);
+    if (rc < 0) return rc;
+    const double pf_an = hh_pf_analytic(&sys, c->temperature);
+    if (pf_an < 0) return (int)HH_E_INVALID;
+    if (!o.quiet) std::printf("Analytic partition function divided by V: %.5e\n", pf_an);
+    double m2 = 0.0, m2_err = 0.0;
+    rc = hep_m2(e, sys, c, c->pair_state ? c->pair_state : HH_PAIR_FREE, c->temperature, o.seed ? o.seed : 1u, &m2,
+                &m2_err);
+    if (rc < 0) return rc;
+    std::printf("T = %.2e => M2: %.5e +- %.5e\n", c->temperature, m2, m2_err);
     return HH_OK;
 }
 
@@ -625,8 +692,10 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         const int rc0 = system_of(c, &sys);
         if (rc0 < 0) return rc0;
     }
-    // partition-function ratio and M0 by VEGAS when the ratio is not given (src/hawaii.c:2829-2880)
+    // partition-function ratio, M0 and M2 by VEGAS when the ratio is not given (src/hawaii.c:2829-2880);
+    // otherwise the M2 the CF estimates are compared with stays 0 (src/hawaii.c:2867)
     double ppf = c->ppf;
+    double hep_m0 = 0.0, prelim_m2 = 0.0;
     if (ppf <= 0) {
         const double pf_an = hh_pf_analytic(&sys, c->temperature);
         if (pf_an < 0) return (int)HH_E_INVALID;
@@ -649,10 +718,11 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         vp.npoints = c->hep_m0_npoints > 0 ? c->hep_m0_npoints : 1000000L;
         vrc = hh_vegas(e, &sys, &vp, &vr);
         if (vrc < 0) return vrc;
-        if (!o.quiet) {
-            std::printf("T = %.2e => M0: %.5e\n", c->temperature, vr.value * kZeroCoeff / pf_an);
-            std::printf("note: the VEGAS M2 integral is not part of this engine\n");
-        }
+        hep_m0 = vr.value * kZeroCoeff / pf_an;
+        if (!o.quiet) std::printf("T = %.2e => M0: %.5e\n", c->temperature, hep_m0);
+        vrc = hep_m2(e, sys, c, vp.pair_state, c->temperature, vp.seed, &prelim_m2, nullptr);
+        if (vrc < 0) return vrc;
+        if (!o.quiet) std::printf("T = %.2e => M2: %.5e\n", c->temperature, prelim_m2);
     }
 
     const int qp = hh_qp_size(sys.system);
@@ -679,8 +749,12 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
 
     hh_sampler smp{HH_IC_REJECTION, c->pair_state ? c->pair_state : HH_PAIR_FREE, c->temperature, c->sampler_Rmin,
                    c->sampler_Rmax, c->pesmin, 0.0};
+    // the stream: initialM0_npoints accepted points for the preliminary M0, then
+    // initialM2_npoints for the preliminary M2, then the trajectories (src/hawaii.c:2947-2965)
     const long m0n = c->initialM0_npoints > 1 ? c->initialM0_npoints : 0;
-    const long total = m0n + n_iter * c->niterations;
+    const long m2n = c->initialM2_npoints > 1 ? c->initialM2_npoints : 0;
+    const long first_traj = m0n + m2n;
+    const long total = first_traj + n_iter * c->niterations;
     std::vector<double> ic((size_t)total * qp);
     long nd = 0;
     if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
@@ -691,7 +765,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     }
 
     // device buffers: one iteration of initial conditions (SoA), the CF, the M0 dipoles
-    const long nmax = std::max(n_iter, m0n);
+    const long nmax = std::max(n_iter, std::max(m0n, m2n));
     double *ic_dev = nullptr, *cf_dev = nullptr, *out_dev = nullptr;
     int code = HH_OK;
     std::vector<double> soa((size_t)nmax * qp);
@@ -705,7 +779,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     do {
         if (hipMalloc(&ic_dev, (size_t)nmax * qp * sizeof(double)) != hipSuccess ||
             hipMalloc(&cf_dev, c->max_trajectory_length * sizeof(double)) != hipSuccess ||
-            hipMalloc(&out_dev, (size_t)3 * std::max(m0n, 1L) * sizeof(double)) != hipSuccess) {
+            hipMalloc(&out_dev, (size_t)3 * std::max(std::max(m0n, m2n), 1L) * sizeof(double)) != hipSuccess) {
             code = fail(HH_E_NOMEM, "device allocation failed");
             break;
         }
@@ -734,6 +808,29 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
                 std::printf("Error: %.3f%%\n", m0s / m0 * 100.0);
             }
         }
+        // preliminary M2 over the next initialM2_npoints accepted points (calculate_M2,
+        // src/hawaii.c:1524-1617): |d mu/dt|^2 on the device, the running mean on the host
+        if (m2n > 0) {
+            if ((code = upload(m0n, m2n)) < 0) break;
+            if ((code = hh_eval(e, &sys, HH_EVAL_DIPOLE_RATE, ic_dev, m2n, out_dev)) < 0) break;
+            std::vector<double> f((size_t)m2n);
+            if (hipMemcpy(f.data(), out_dev, f.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            double m = 0.0, q = 0.0;
+            for (long k = 0; k < m2n; ++k) {
+                const double diff = f[k] - m;
+                m += diff / (k + 1.0);
+                q += diff * diff * (k / (k + 1.0));
+            }
+            const double m2 = m * kSecondCoeff * ppf;
+            const double m2s = std::sqrt(q / m2n / (m2n - 1.0)) * kSecondCoeff * ppf;
+            if (!o.quiet) {
+                std::printf("M2 = %.10e +/- %.10e [%.10e ... %.10e]\n", m2, m2s, m2 - m2s, m2 + m2s);
+                std::printf("Error: %.3f%%\n", m2s / m2 * 100.0);
+            }
+        }
         hh_cf_params p{};
         const bool mixed = o.precision == HH_PREC_MIXED;
         p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
@@ -754,7 +851,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         std::vector<double> sum(p.L, 0.0), it(p.L);
         double ntraj = 0.0;
         for (long iter = 0; iter < c->niterations && code == HH_OK; ++iter) {
-            if ((code = upload(m0n + iter * n_iter, n_iter)) < 0) break;
+            if ((code = upload(first_traj + iter * n_iter, n_iter)) < 0) break;
             const double* icp[1] = {ic_dev};
             double* cfp[1] = {cf_dev};
             const long nt[1] = {n_iter};
@@ -772,7 +869,9 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
                 std::printf("ITERATION %ld/%ld: accumulated %.0f trajectories. Saving the temporary result to '%s'\n",
                             iter + 1, c->niterations, ntraj, c->cf_filename);
                 const double m0cf = cf.y[0] * kZeroCoeff / kALU / kALU / kALU;
-                std::printf("M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e\n", m0cf, m0);
+                std::printf("M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e, diff: %.3f%%\n", m0cf, hep_m0,
+                            (m0cf - hep_m0) / hep_m0 * 100.0);
+                print_m2_from_cf(cf, prelim_m2, "");
                 std::printf("device: %.1f ms integration, %lld RHS, %lld failed, %lld step-limited items\n",
                             st.integrate_ms, st.nrhs, st.n_failed, st.n_stepfail);
                 std::fflush(stdout);
@@ -822,7 +921,7 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
     const double Tb = c->temperature;
     // partition-function ratios (and M0) by VEGAS at each satellite temperature when not given
     // (src/hawaii.c:2388-2437)
-    std::vector<double> ratios(M, 0.0), hep_m0(M, 0.0);
+    std::vector<double> ratios(M, 0.0), hep_m0(M, 0.0), hep_m2s(M, 0.0);
     if (c->nppf_ratios == M) {
         for (int st = 0; st < M; ++st) ratios[st] = c->ppf_ratios[st];
     } else {
@@ -846,11 +945,14 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
             vrc = hh_vegas(e, &sys, &vp, &vr);
             if (vrc < 0) return vrc;
             hep_m0[st] = vr.value * kZeroCoeff / pf_an;
+            vrc = hep_m2(e, sys, c, ps, T, vp.seed, &hep_m2s[st], nullptr);
+            if (vrc < 0) return vrc;
             if (!o.quiet) {
                 std::printf("Satellite temperature = %.2e\n", T);
                 std::printf("Analytic partition function divided by V: %.5e\n", pf_an);
                 std::printf("T = %.2e => PPF ratio: %.5e\n", T, ratios[st]);
                 std::printf("T = %.2e => M0: %.5e\n", T, hep_m0[st]);
+                std::printf("T = %.2e => M2: %.5e\n", T, hep_m2s[st]);
             }
         }
     }
@@ -940,8 +1042,10 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
                 }
                 if (!o.quiet) {
                     const double m0cf = cf.y[0] * kZeroCoeff / kALU / kALU / kALU;
-                    std::printf("T = %.2f: NSTAR = %.2f, M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e\n",
-                                cf.T, nstar[s2], m0cf, hep_m0[s2]);
+                    std::printf("T = %.2f: NSTAR = %.2f, M0 ESTIMATE FROM CF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e, "
+                                "diff: %.3f%%\n", cf.T, nstar[s2], m0cf, hep_m0[s2],
+                                (m0cf - hep_m0[s2]) / hep_m0[s2] * 100.0);
+                    print_m2_from_cf(cf, hep_m2s[s2], "");
                 }
                 code = write_series(c->cf_filenames[s2], cf);
             }
@@ -982,6 +1086,8 @@ int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
         rc = run_processing(e, conf, o.quiet != 0);
     } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M0) {
         rc = run_phase_space_m0(e, conf, o);
+    } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M2) {
+        rc = run_phase_space_m2(e, conf, o);
     } else if (conf->calculation_type == HH_CALC_CORRELATION_ARRAY) {
         rc = run_correlation_array(e, conf, o);
     } else {

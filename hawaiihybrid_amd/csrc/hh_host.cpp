@@ -183,7 +183,7 @@ M0Acc m0_view(void* base, long ns)
     return a;
 }
 
-int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q)
+int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q, int order)
 {
     std::vector<double> hm(nstreams), hq(nstreams);
     std::vector<long> hn(nstreams);
@@ -193,20 +193,23 @@ int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double
     for (long s = 0; s < nstreams; ++s) {
         const long quota = npoints / nstreams + (s < npoints % nstreams ? 1 : 0);
         if (hn[s] != quota)
-            return fail(HH_E_INVALID, "M0: stream %ld accepted %ld of %ld points within the candidate limit "
-                                      "(check pesmin, T and the pair state)", s, hn[s], quota);
+            return fail(HH_E_INVALID, "M%d: stream %ld accepted %ld of %ld points within the candidate limit "
+                                      "(check pesmin, T and the pair state)", order, s, hn[s], quota);
     }
-    // combine as mpi_calculate_M0 does across ranks (src/hawaii.c:1722-1733): mean of the
-    // per-stream means and of the per-stream q, q normalised by the per-stream count
+    // combine as mpi_calculate_M0 / mpi_calculate_M2 do across ranks (src/hawaii.c:1722-1733,
+    // 1850-1854): mean of the per-stream means and of the per-stream q, q normalised by the
+    // per-stream count
     double sm = 0.0, sq = 0.0;
     for (long s = 0; s < nstreams; ++s) {
         sm += hm[s];
         sq += hq[s];
     }
     const double ZeroCoeff = 0.00361479637 / (4.0 * M_PI);   // src/constants.h:29
+    const double SecondCoeff = 13856114.29344114;            // src/constants.h:30
+    const double coeff = order == 2 ? SecondCoeff : ZeroCoeff;
     const double n0 = (double)hn[0];
-    *m = (sm / nstreams) * ZeroCoeff * ppf;
-    *q = std::sqrt((sq / nstreams) / n0 / (n0 - 1)) * ZeroCoeff * ppf;
+    *m = (sm / nstreams) * coeff * ppf;
+    *q = std::sqrt((sq / nstreams) / n0 / (n0 - 1)) * coeff * ppf;
     return HH_OK;
 }
 
@@ -214,6 +217,8 @@ int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double
 
 namespace {
 const int& kZeroInt = hh::kZeroIntStorage;
+int moment_run(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
+               double ppf, int order, double* m, double* q, double* ms);
 }
 
 extern "C" {
@@ -596,7 +601,8 @@ int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, l
     if (!e || !sys || !y_dev || !out_dev || n <= 0) return fail(HH_E_INVALID, "hh_eval: bad argument");
     if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
         return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
-    if (what < HH_EVAL_PES || what > HH_EVAL_RHS) return fail(HH_E_INVALID, "hh_eval: unknown quantity %d", what);
+    if (what < HH_EVAL_PES || what > HH_EVAL_DIPOLE_RATE)
+        return fail(HH_E_INVALID, "hh_eval: unknown quantity %d", what);
     std::lock_guard<std::mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
@@ -631,7 +637,24 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
 int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
           double ppf, double* m, double* q, double* ms)
 {
-    if (!e || !sys || !smp || !m || !q) return fail(HH_E_INVALID, "hh_m0: null argument");
+    return moment_run(e, sys, smp, seed, npoints, nstreams, ppf, 0, m, q, ms);
+}
+
+int hh_m2(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
+          double ppf, double* m, double* q, double* ms)
+{
+    return moment_run(e, sys, smp, seed, npoints, nstreams, ppf, 2, m, q, ms);
+}
+
+}  // extern "C"
+
+namespace {
+
+// the rejection-sampler moments of hh_m0 (order 0) and hh_m2 (order 2)
+int moment_run(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
+               double ppf, int order, double* m, double* q, double* ms)
+{
+    if (!e || !sys || !smp || !m || !q) return fail(HH_E_INVALID, "hh_m%d: null argument", order);
     int rc = hh::m0_check(sys, smp, npoints, nstreams);
     if (rc < 0) return rc;
     std::lock_guard<std::mutex> lock(e->mu);
@@ -643,10 +666,10 @@ int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t se
     hh::M0Params mp{smp->T, smp->Rmin, smp->Rmax, smp->pesmin, smp->pair_state, 0};
     HH_CHECK(hipEventRecord(e->ev[0], e->stream));
     HH_CHECK(hh::run_m0(sys->system, sc, mp, seed, nstreams, npoints, reinterpret_cast<uint32_t*>(e->m0_state.p), acc, 1,
-                        0, e->stream));
+                        0, e->stream, order));
     HH_CHECK(hipEventRecord(e->ev[1], e->stream));
     HH_CHECK(hipStreamSynchronize(e->stream));
-    rc = hh::m0_combine(acc, npoints, nstreams, ppf, m, q);
+    rc = hh::m0_combine(acc, npoints, nstreams, ppf, m, q, order);
     if (rc < 0) return rc;
     if (ms) {
         float a = 0;
@@ -656,4 +679,4 @@ int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t se
     return HH_OK;
 }
 
-}  // extern "C"
+}  // namespace

@@ -49,12 +49,13 @@ hipError_t run_zimm_reduce(const JobDesc& jb, double ppf, const double2* tw, dou
 hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2* work_b, double* nu, double* sf,
                         hipStream_t st, bool lds_ok);
 hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
-                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st);
+                  uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st, int order = 0);
 int m0_check(const hh_system* sys, const hh_sampler* smp, long npoints, int nstreams);
 size_t m0_bytes(long nstreams);
 M0Acc m0_view(void* base, long nstreams);
-// M0 of one request: combine the per-stream results as mpi_calculate_M0 combines ranks
-int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q);
+// M0 (order 0) or M2 (order 2) of one request: combine the per-stream results as
+// mpi_calculate_M0 / mpi_calculate_M2 combine ranks
+int m0_combine(const M0Acc& acc, long npoints, long nstreams, double ppf, double* m, double* q, int order = 0);
 
 // error reporting: thread-local message, negative code
 int fail(int code, const char* fmt, ...);

@@ -1014,6 +1014,8 @@ __global__ void __launch_bounds__(256) eval_kernel(SysConst sc, int what, const 
         out[2 * n + i] = m[2];
     } else if (what == 2) {
         out[i] = hamiltonian<SYS, double>(sc, v);
+    } else if (what == 4) {
+        out[i] = dipole_rate_sq<SYS>(sc, v);
     } else {
         double f[N];
         rhs<SYS, double>(sc, v, f);

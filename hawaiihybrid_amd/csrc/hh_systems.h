@@ -482,4 +482,46 @@ __device__ __forceinline__ double hamiltonian(const SysConst& sc, const double* 
     return K + V;
 }
 
+// |d mu/dt|^2 at phase point y, the M2 integrand (calculate_M2, src/hawaii.c:1564-1594;
+// integrand_M2, src/hep_hawaii.cpp:296-325): the lab dipole's derivative in every coordinate by
+// central differences with h = 1e-3, contracted with dH/dp (the coordinate velocities, the even
+// components of Hamilton's equations, compute_dHdp src/hawaii.c:1493-1522), summed over the
+// coordinates in state order.  One dipole call site in a rolled loop over the 2 x (QP/2)
+// displaced states keeps the CO2-Ar surface inlined once (no register spill); the displaced
+// state is built with selects so it stays in registers.
+template <int SYS>
+__device__ __forceinline__ double dipole_rate_sq(const SysConst& sc, const double* y)
+{
+    constexpr int QP = SysTraits<SYS>::QP;
+    constexpr double h = 1.0e-3;
+    double yd[QP];
+    rhs<SYS, double>(sc, y, yd);
+    double md0 = 0.0, md1 = 0.0, md2 = 0.0;
+    double dp0 = 0.0, dp1 = 0.0, dp2 = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < QP; ++j) {
+        const int c = j & ~1;   // the displaced coordinate: + h at even j, - h at odd j
+        const double step = (j & 1) ? -h : h;
+        double yy[QP];
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < QP; ++k) {
+            yy[k] = k == c ? y[k] + step : y[k];
+            v = k == c ? yd[k] : v;
+        }
+        double d[3];
+        dipole<SYS, double>(sc, yy, d);
+        if (j & 1) {
+            md0 += v * ((dp0 - d[0]) / (2.0 * h));
+            md1 += v * ((dp1 - d[1]) / (2.0 * h));
+            md2 += v * ((dp2 - d[2]) / (2.0 * h));
+        } else {
+            dp0 = d[0];
+            dp1 = d[1];
+            dp2 = d[2];
+        }
+    }
+    return md0 * md0 + md1 * md1 + md2 * md2;
+}
+
 }  // namespace hh

@@ -8,7 +8,9 @@
 // the same x gives the same phase point (the integral is unaffected: Phi only enters through
 // the relative orientation, which phi1 covers uniformly).  Outside [Rmin, Rmax], or off the pair
 // state (FREE: E >= 0, BOUND: E <= 0), the integrand is zero; else
-//     PF: J exp(-E HkT / T),      M0: J |mu|^2 exp(-E HkT / T).
+//     PF: J exp(-E HkT / T),      M0: J |mu|^2 exp(-E HkT / T),
+//     M2: J |d mu/dt|^2 exp(-E HkT / T) (integrand_M2, src/hep_hawaii.cpp:228-330; the dipole's
+//         coordinate derivatives by central differences, h = 1e-3, contracted with dH/dp).
 //
 // The integrator follows G. P. Lepage, J. Comput. Phys. 27 (1978) 192: a separable grid of
 // kVegasBins bins per dimension, each iteration samples N points uniformly in grid space
@@ -63,7 +65,8 @@ struct VegasParams {
     long long npoints;
 };
 
-template <int SYS>
+// M2 selects the M2 integrand at compile time, so the M0 and PF kernels keep their registers
+template <int SYS, bool M2>
 __device__ __forceinline__ double vegas_integrand(const SysConst& sc, const VegasParams& p, const double* x)
 {
     constexpr int QP = SysTraits<SYS>::QP;
@@ -98,7 +101,9 @@ __device__ __forceinline__ double vegas_integrand(const SysConst& sc, const Vega
     if (p.pair_state == HH_PAIR_FREE && energy < 0.0) return 0.0;
     if (p.pair_state == HH_PAIR_BOUND && energy > 0.0) return 0.0;
     double w = jac * exp(-energy * kVegasHkT / p.T);
-    if (p.integrand == HH_INTEGRAND_M0) {
+    if constexpr (M2) {
+        w *= dipole_rate_sq<SYS>(sc, y);
+    } else if (p.integrand == HH_INTEGRAND_M0) {
         double mu[3];
         dipole<SYS, double>(sc, y, mu);
         w *= mu[0] * mu[0] + mu[1] * mu[1] + mu[2] * mu[2];
@@ -108,7 +113,7 @@ __device__ __forceinline__ double vegas_integrand(const SysConst& sc, const Vega
 
 // Every block writes sum f, sum f^2 (f = integrand x grid Jacobian), the count of non-finite
 // values and, per dimension and bin, sum f^2 of its points.
-template <int SYS>
+template <int SYS, bool M2>
 __global__ __launch_bounds__(kVegasBlock) void vegas_kernel(SysConst sc, VegasParams p, const double* __restrict__ grid,
                                                             double* __restrict__ block_sums,
                                                             double* __restrict__ block_bins)
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(kVegasBlock) void vegas_kernel(SysConst sc, VegasPa
                 b[d] = bi;
             }
         }
-        double f = vegas_integrand<SYS>(sc, p, x) * jac;
+        double f = vegas_integrand<SYS, M2>(sc, p, x) * jac;
         if (!isfinite(f)) {
             f = 0.0;
             nnf += 1.0;
@@ -246,8 +251,8 @@ extern "C" int hh_vegas(hh_engine* e, const hh_system* sys, const hh_vegas_param
     if (!e || !sys || !vp || !out) return fail(HH_E_INVALID, "hh_vegas: null argument");
     if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
         return fail(HH_E_UNSUPPORTED, "hh_vegas: system %d has no device surfaces", sys->system);
-    if (vp->integrand != HH_INTEGRAND_M0 && vp->integrand != HH_INTEGRAND_PF)
-        return fail(HH_E_UNSUPPORTED, "hh_vegas: integrand %d not provided (M0, PF)", vp->integrand);
+    if (vp->integrand != HH_INTEGRAND_M0 && vp->integrand != HH_INTEGRAND_M2 && vp->integrand != HH_INTEGRAND_PF)
+        return fail(HH_E_INVALID, "hh_vegas: unknown integrand %d", vp->integrand);
     if (!(vp->T > 0) || !(vp->Rmin > 0) || !(vp->Rmax > vp->Rmin))
         return fail(HH_E_INVALID, "hh_vegas: T > 0 and 0 < sampler_Rmin < sampler_Rmax required");
     if (vp->niterations < 2 || vp->npoints < 2)
@@ -275,12 +280,11 @@ extern "C" int hh_vegas(hh_engine* e, const hh_system* sys, const hh_vegas_param
     for (int it = 0; it < vp->niterations; ++it) {
         p.iteration = it;
         HH_CHECK(hipMemcpyAsync(gridb.p, grid.data(), grid.size() * sizeof(double), hipMemcpyHostToDevice, e->stream));
-        if (sys->system == HH_SYS_HEAR)
-            hipLaunchKernelGGL(vegas_kernel<SYS_HEAR>, dim3(nblocks), dim3(kVegasBlock), 0, e->stream, sc, p,
-                               gridb.as<double>(), sums.as<double>(), binsb.as<double>());
-        else
-            hipLaunchKernelGGL(vegas_kernel<SYS_CO2AR>, dim3(nblocks), dim3(kVegasBlock), 0, e->stream, sc, p,
-                               gridb.as<double>(), sums.as<double>(), binsb.as<double>());
+        const bool m2 = vp->integrand == HH_INTEGRAND_M2;
+        auto k = sys->system == HH_SYS_HEAR ? (m2 ? vegas_kernel<SYS_HEAR, true> : vegas_kernel<SYS_HEAR, false>)
+                                            : (m2 ? vegas_kernel<SYS_CO2AR, true> : vegas_kernel<SYS_CO2AR, false>);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(kVegasBlock), 0, e->stream, sc, p, gridb.as<double>(),
+                           sums.as<double>(), binsb.as<double>());
         HH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(vegas_sum_kernel, dim3(1), dim3(256), 0, e->stream, sums.as<double>(), nblocks, 3,
                            outb.as<double>());

@@ -5,22 +5,27 @@
  * &MONOMER and &PROCESSING blocks, `key = value` lines, `!` comments, case-insensitive
  * keywords) and runs the part of it that is this engine's path:
  *   CALCULATION_TYPE = CORRELATION_SINGLE   calculate_correlation_and_save
- *                                           (src/hawaii.c:2795-3117): preliminary M0 from the
+ *                                           (src/hawaii.c:2795-3117): preliminary M0 and M2
+ *                                           (INITIALM0_NPOINTS, INITIALM2_NPOINTS) from the
  *                                           rejection sampler, NITERATIONS iterations of
  *                                           TOTAL_TRAJECTORIES/NITERATIONS trajectories from the
  *                                           seeded stream, the CF file rewritten after each
  *                                           iteration (write_correlation_function_ext,
  *                                           src/hawaii.c:4150-4212)
- *   CALCULATION_TYPE = PHASE_SPACE_M0       the VEGAS M0 integral at TEMPERATURE or each of
- *                                           TEMPERATURES (src/driver.c:3389-3445)
+ *   CALCULATION_TYPE = CORRELATION_ARRAY    calculate_correlation_array_and_save
+ *                                           (src/hawaii.c:2361-2793): one ensemble per
+ *                                           iteration reweighted to SATELLITE_TEMPERATURES
+ *   CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M0   the VEGAS M0 integral at TEMPERATURE or each
+ *                                           of TEMPERATURES (src/driver.c:3389-3445)
+ *   CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M2   the VEGAS M2 integral at TEMPERATURE
+ *                                           (src/driver.c:3344-3388)
  *   CALCULATION_TYPE = PROCESSING           the stack machine of run_processing
  *                                           (src/driver.c:2956-3081) restricted to READ_CF,
  *                                           READ_SF, CF_TO_SF, CMP, DUP, DUP2, DROP, DROP2,
  *                                           WRITE_CF, WRITE_SF
  * Systems: two ATOM monomers (He-Ar surfaces) or LINEAR_MOLECULE + ATOM (CO2-Ar surfaces).
  * Without PARTIAL_PARTITION_FUNCTION_RATIO the ratio is integrated by VEGAS as the reference does
- * (src/hawaii.c:2826-2850), and the VEGAS M0 follows; the M2 integrals are not part of this
- * engine.  A plugin named by
+ * (src/hawaii.c:2826-2880), and the VEGAS M0 and M2 follow.  A plugin named by
  * so_potential is dlopen'ed and its pes_lab drives the host rejection sampler, so the sampled
  * phase points follow the reference's draws; the device integrates with its built-in surfaces.
  */
@@ -34,7 +39,7 @@ extern "C" {
 #endif
 
 enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3,
-       HH_CALC_CORRELATION_ARRAY = 4 };
+       HH_CALC_CORRELATION_ARRAY = 4, HH_CALC_PHASE_SPACE_M2 = 5 };
 enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF };
@@ -74,7 +79,7 @@ typedef struct {
     hh_conf_proc proc[HH_CONF_MAX_PROC];
     long hep_ppf_niterations, hep_ppf_npoints;   /* VEGAS settings (0: the reference's 12 x 10^6) */
     long hep_m0_niterations, hep_m0_npoints;
-    int ntemperatures;                           /* TEMPERATURES = {...} (PHASE_SPACE_M0) */
+    int ntemperatures;                           /* TEMPERATURES = {...} (CALCULATE_PHASE_SPACE_M0) */
     double temperatures[64];
     /* CORRELATION_ARRAY (src/driver.c:1170-1247) */
     int nsatellite;                              /* SATELLITE_TEMPERATURES = {...} */
@@ -83,11 +88,13 @@ typedef struct {
     double ppf_ratios[64];
     int ncf_filenames;                           /* CF_FILENAMES = {"...", ...} */
     char cf_filenames[64][HH_CONF_PATH];
+    long initialM2_npoints;                      /* rejection-sampler M2 before the trajectories */
+    long hep_m2_niterations, hep_m2_npoints;     /* VEGAS M2 (0: the reference's 12 x 10^6) */
 } hh_conf;
 
 /* Parse a configuration file (no device use).  Unknown keywords and unsupported PROCESSING
  * functions are errors naming the line; keywords of the reference that only steer parts
- * outside this engine (VEGAS point counts, M2, histograms) are accepted and ignored. */
+ * outside this engine (the PR/mu path's histograms and spin weights) are accepted and ignored. */
 int hh_conf_parse(const char* path, hh_conf* out);
 
 typedef struct {

@@ -24,6 +24,10 @@
  *   hh_cf_to_sf          idct_cf_to_sf (src/hawaii.c:5569-5601)
  *   hh_m0                calculate_M0 / mpi_calculate_M0 (src/hawaii.c:1423-1491,
  *                        src/hawaii.c:1620-1738)
+ *   hh_m2                calculate_M2 / mpi_calculate_M2 (src/hawaii.c:1524-1617,
+ *                        src/hawaii.c:1740-1858)
+ *   hh_vegas             c_mpi_perform_integration with integrand_M0 / integrand_M2 /
+ *                        integrand_pf (src/hep_hawaii.cpp:116-380)
  *   hh_pf_analytic       analytic_full_partition_function_by_V (src/hawaii.c:5071-5090)
  *   hh_cf_moment         compute_Mn_from_cf_using_classical_detailed_balance
  *                        (src/hawaii.c:5945-5979)
@@ -161,8 +165,10 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
  *   HH_EVAL_DIPOLE     out[3][n]    lab-frame dipole (a.u.)        dipole_lab
  *   HH_EVAL_HAMILTONIAN out[n]      H = K + V                      Hamiltonian (src/hawaii.c:1190-1204)
  *   HH_EVAL_RHS        out[QP][n]   Hamilton's equations           rhs (src/hawaii.c:553-615)
+ *   HH_EVAL_DIPOLE_RATE out[n]      |d mu/dt|^2 (a.u.), central    calculate_M2's per-point value
+ *                                   differences h = 1e-3 . dH/dp   (src/hawaii.c:1564-1594)
  * The batched device form of a plugin's entry points. */
-enum { HH_EVAL_PES = 0, HH_EVAL_DIPOLE = 1, HH_EVAL_HAMILTONIAN = 2, HH_EVAL_RHS = 3 };
+enum { HH_EVAL_PES = 0, HH_EVAL_DIPOLE = 1, HH_EVAL_HAMILTONIAN = 2, HH_EVAL_RHS = 3, HH_EVAL_DIPOLE_RATE = 4 };
 int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev);
 
 /* CORRELATION_ARRAY: calculate_correlation_array_and_save's accumulation (src/hawaii.c:2361-2793,
@@ -200,17 +206,24 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
 int hh_m0(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
           double ppf, double* m, double* q, double* ms);
 
+/* Rejection-sampler M2 (calculate_M2 / mpi_calculate_M2, src/hawaii.c:1524-1617, 1740-1858):
+ * the same streams and candidate loop as hh_m0, averaging |d mu/dt|^2 (HH_EVAL_DIPOLE_RATE) over
+ * the accepted points; m, q scaled by SecondCoeff (src/constants.h:30) times ppf. */
+int hh_m2(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32_t seed, long npoints, int nstreams,
+          double ppf, double* m, double* q, double* ms);
+
 /* ------------------------------------------------------------------ VEGAS phase-space integrals */
 /* c_mpi_perform_integration / mpi_perform_integration (src/hep_hawaii.cpp:332-380) with the
- * integrands integrand_M0 and integrand_pf (src/hep_hawaii.cpp:116-201) over the phase space of
- * transform_variables (src/hep_hawaii.cpp:8-114): adaptive Monte Carlo (Lepage's VEGAS) on the
- * device, niterations of npoints, the first iteration discarded.  M0 = value * ZeroCoeff /
- * hh_pf_analytic; the partition-function ratio = value(PF) / hh_pf_analytic.  The estimate
+ * integrands integrand_M0, integrand_M2 and integrand_pf (src/hep_hawaii.cpp:116-330) over the
+ * phase space of transform_variables (src/hep_hawaii.cpp:8-114): adaptive Monte Carlo (Lepage's
+ * VEGAS) on the device, niterations of npoints, the first iteration discarded.  M0 = value *
+ * ZeroCoeff / hh_pf_analytic; M2 = value * SecondCoeff / hh_pf_analytic (src/driver.c:3377-3381);
+ * the partition-function ratio = value(PF) / hh_pf_analytic.  The estimate
  * agrees with the reference's hep-mc run statistically (same integrand, own sampler). */
 enum { HH_INTEGRAND_M0 = 0, HH_INTEGRAND_M2 = 1, HH_INTEGRAND_PF = 2 };   /* IntegrandType order */
 
 typedef struct {
-    int integrand;          /* HH_INTEGRAND_M0 or HH_INTEGRAND_PF */
+    int integrand;          /* HH_INTEGRAND_M0, HH_INTEGRAND_M2 or HH_INTEGRAND_PF */
     int pair_state;         /* HH_PAIR_FREE (E >= 0), HH_PAIR_BOUND (E <= 0), HH_PAIR_ALL */
     double T;               /* K */
     double Rmin, Rmax;      /* sampler_Rmin, sampler_Rmax */

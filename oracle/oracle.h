@@ -152,6 +152,16 @@ void orc_cf_to_sf(const double *t, double *cf, size_t n, double *nu, double *sf)
 void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
             double *m, double *q);
 
+/* ---------------------------------------------------------------- M2 */
+/* |d mu/dt|^2 at the full state y (calculate_M2's per-point value, src/hawaii.c:1564-1594):
+ * central differences (h = 1e-3) of the dipole in every coordinate, contracted with dH/dp (the
+ * even components of orc_rhs, compute_dHdp src/hawaii.c:1493-1522). */
+double orc_dipole_rate_sq(const orc_system *s, const double *y);
+/* Rejection-sampler M2 (calculate_M2, src/hawaii.c:1524-1617): the orc_m0 loop with
+ * orc_dipole_rate_sq as the value, scaled by SecondCoeff * ppf. */
+void orc_m2(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+            double *m, double *q);
+
 /* analytic_full_partition_function_by_V (src/hawaii.c:5071-5090) */
 double orc_pf_analytic(const orc_system *s, double T);
 

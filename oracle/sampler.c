@@ -8,6 +8,7 @@
  *   reject           src/hawaii.c:1382-1421  (u drawn first; weight = desired/proposal/M < u)
  *   candidate loop   src/hawaii.c:2969-2993  (Hamiltonian before reject; FREE keeps E >= 0)
  *   R0 injection     src/hawaii.c:3486-3512  (q_generator, R := R0, p_generator, pR := -|pR|)
+ *   calculate_M0     src/hawaii.c:1423-1491, calculate_M2 src/hawaii.c:1524-1617 (same loop)
  */
 #include <math.h>
 #include "oracle.h"
@@ -124,10 +125,37 @@ long orc_sample_ics(const orc_system *s, const orc_sampler *sp, uint32_t seed,
     return candidates;
 }
 
-void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
-            double *m, double *q)
+double orc_dipole_rate_sq(const orc_system *s, const double *y)
+{
+    const double h = 1.0e-3;
+    int i, k, nq = orc_qp_size(s) / 2;
+    double ydot[10], qv[5], D[5][3], md[3];
+    orc_rhs(s, y, ydot);
+    for (i = 0; i < nq; ++i) qv[i] = y[2 * i];
+    for (i = 0; i < nq; ++i) {
+        double dp[3], dm[3], tmp = qv[i];
+        qv[i] = tmp + h;
+        orc_dipole(s, qv, dp);
+        qv[i] = tmp - h;
+        orc_dipole(s, qv, dm);
+        for (k = 0; k < 3; ++k) D[i][k] = (dp[k] - dm[k]) / (2.0 * h);
+        qv[i] = tmp;
+    }
+    /* dip_lab_dot = dHdp (1 x nq) . D (nq x 3), summed over the coordinates in order */
+    for (k = 0; k < 3; ++k) {
+        md[k] = 0.0;
+        for (i = 0; i < nq; ++i) md[k] += ydot[2 * i] * D[i][k];
+    }
+    return md[0] * md[0] + md[1] * md[1] + md[2] * md[2];
+}
+
+/* the candidate loop shared by calculate_M0 and calculate_M2 (order 0 or 2) */
+static void moment_loop(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+                        int order, double *m, double *q)
 {
     const double ZeroCoeff = 0.00361479637 / (4.0 * M_PI);   /* src/constants.h:29 */
+    const double SecondCoeff = 13856114.29344114;            /* src/constants.h:30 */
+    const double coeff = order == 2 ? SecondCoeff : ZeroCoeff;
     rng r;
     long n = 0;
     int i, nq = orc_qp_size(s) / 2;
@@ -144,14 +172,30 @@ void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoi
         if (reject(s, y, sp->T, sp->pesmin, &r)) continue;
         if (sp->pair_state == ORC_PAIR_FREE && energy < 0.0) continue;
         if (sp->pair_state == ORC_PAIR_BOUND && energy > 0.0) continue;
-        for (i = 0; i < nq; ++i) qv[i] = y[2 * i];
-        orc_dipole(s, qv, d);
-        fval = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (order == 2) {
+            fval = orc_dipole_rate_sq(s, y);
+        } else {
+            for (i = 0; i < nq; ++i) qv[i] = y[2 * i];
+            orc_dipole(s, qv, d);
+            fval = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        }
         diff = fval - mm;
         mm += diff / (n + 1.0);
         qq += diff * diff * (n / (n + 1.0));
         n++;
     }
-    *m = mm * ZeroCoeff * ppf;
-    *q = sqrt(qq / n / (n - 1)) * ZeroCoeff * ppf;
+    *m = mm * coeff * ppf;
+    *q = sqrt(qq / n / (n - 1)) * coeff * ppf;
+}
+
+void orc_m0(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+            double *m, double *q)
+{
+    moment_loop(s, sp, seed, npoints, ppf, 0, m, q);
+}
+
+void orc_m2(const orc_system *s, const orc_sampler *sp, uint32_t seed, long npoints, double ppf,
+            double *m, double *q)
+{
+    moment_loop(s, sp, seed, npoints, ppf, 2, m, q);
 }

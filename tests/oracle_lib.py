@@ -140,6 +140,8 @@ def lib():
             "orc_connes": (None, [v, sz, d]),
             "orc_cf_to_sf": (None, [v, v, sz, v, v]),
             "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
+            "orc_m2": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
+            "orc_dipole_rate_sq": (d, [P(System), v]),
             "orc_pf_analytic": (d, [P(System), d]),
             "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
             "orc_cf_array_weights": (None, [v, C.c_long, d, i, d, i, v, v, v, v, v]),
@@ -269,6 +271,20 @@ def m0(system, sampler, seed, npoints, ppf=1.0):
     q = C.c_double(0)
     lib().orc_m0(C.byref(system), C.byref(sampler), seed, npoints, ppf, C.byref(m), C.byref(q))
     return m.value, q.value
+
+
+def m2(system, sampler, seed, npoints, ppf=1.0):
+    m = C.c_double(0)
+    q = C.c_double(0)
+    lib().orc_m2(C.byref(system), C.byref(sampler), seed, npoints, ppf, C.byref(m), C.byref(q))
+    return m.value, q.value
+
+
+def dipole_rate_sq(system, y):
+    """|d mu/dt|^2 at each full state (rows of y)."""
+    L = lib()
+    return np.array([L.orc_dipole_rate_sq(C.byref(system), ptr(np.ascontiguousarray(r, dtype=np.float64)))
+                     for r in np.atleast_2d(y)])
 
 
 def ref_lib(name):

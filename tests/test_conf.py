@@ -30,7 +30,8 @@ class Conf(C.Structure):
                 ("ntemperatures", C.c_int), ("temperatures", C.c_double * 64),
                 ("nsatellite", C.c_int), ("satellite_temperatures", C.c_double * 64),
                 ("nppf_ratios", C.c_int), ("ppf_ratios", C.c_double * 64),
-                ("ncf_filenames", C.c_int), ("cf_filenames", (C.c_char * 512) * 64)]
+                ("ncf_filenames", C.c_int), ("cf_filenames", (C.c_char * 512) * 64),
+                ("initialM2_npoints", C.c_long), ("hep_m2_niterations", C.c_long), ("hep_m2_npoints", C.c_long)]
 
 
 def parse_text(tmp_path, text):
@@ -69,6 +70,35 @@ def test_parse_reference_example_configs(name):
     assert rc == 0, H.lib().hh_last_error()
     assert c.calculation_type == 1 and c.n_monomers == 2
     assert c.total_trajectories > 0 and c.max_trajectory_length > 0 and c.Rcut > 0
+
+
+def test_parse_reference_m2_settings():
+    """CF-F-He-Ar.conf asks for the preliminary M2 over 10^6 points (src/driver.c:1193)."""
+    path = os.path.join(REF_CONFIGS, "CF-F-He-Ar.conf")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    rc, c = parse(path)
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.initialM0_npoints == 1000000 and c.initialM2_npoints == 1000000
+
+
+def test_parse_phase_space_m2(tmp_path):
+    """CALCULATE_PHASE_SPACE_M2 and the VEGAS M2 settings (src/driver.c:1106-1107, 1196-1197); the
+    reference spells the phase-space types with CALCULATE_, a bare PHASE_SPACE_M0 is unknown."""
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M2
+  TEMPERATURE = 100.0
+  HEP_M2_NITERATIONS = 8
+  HEP_M2_NPOINTS = 500_000
+  INITIALM2_NPOINTS = 1000
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 5 and c.hep_m2_niterations == 8 and c.hep_m2_npoints == 500000
+    assert c.initialM2_npoints == 1000
+    rc, _ = parse_text(tmp_path, "&INPUT\n  CALCULATION_TYPE = PHASE_SPACE_M0\n&END\n")
+    assert rc < 0 and b"PHASE_SPACE_M0 is not part of this engine" in H.lib().hh_last_error()
 
 
 def test_parse_reference_correlation_array_config():
@@ -131,7 +161,7 @@ def test_parse_errors_name_the_problem(tmp_path, text, needle):
 def test_parse_phase_space_m0(tmp_path):
     rc, c = parse_text(tmp_path, """
 &INPUT
-  CALCULATION_TYPE = PHASE_SPACE_M0
+  CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M0
   PAIR_STATE = FREE_AND_METASTABLE
   pair_reduced_mass = 6.632039160324215e3
   TEMPERATURES = {100.0, 200.0, 300.0}

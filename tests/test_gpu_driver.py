@@ -53,6 +53,34 @@ def test_driver_correlation_single_matches_oracle(tmp_path):
     assert sf.shape == (256, 2)
 
 
+def test_driver_preliminary_m2_and_stream_order(tmp_path):
+    """initialM2_npoints: the preliminary M2 (calculate_M2) averages |d mu/dt|^2 over the accepted
+    points after the preliminary M0's, and the trajectories continue the stream after both
+    (src/hawaii.c:2947-2965)."""
+    conf = open(os.path.join(HERE, "conf", "cf-hear-small.conf")).read()
+    conf = conf.replace("initialM2_npoints = 0", "initialM2_npoints = 200")
+    conf = conf.replace("NITERATIONS         = 2", "NITERATIONS         = 1")
+    conf = conf.replace("TOTAL_TRAJECTORIES  = 512", "TOTAL_TRAJECTORIES  = 128")
+    conf = conf.replace("MaxTrajectoryLength = 256", "MaxTrajectoryLength = 64")
+    conf = conf.replace('sf_filename = "SF-hear-small.txt"', "")
+    (tmp_path / "m2.conf").write_text(conf)
+    r = run_driver(["m2.conf", "--seed", "42"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    import re
+    m = re.search(r"^M2 = ([0-9.e+-]+) \+/- ([0-9.e+-]+)", r.stdout, re.M)
+    assert m, r.stdout[-2000:]
+    ic, _, _ = O.sample_ics(O.hear(), O.Sampler(O.IC_REJECTION, 1, 300.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0), 42,
+                            100 + 200 + 128)
+    f = O.dipole_rate_sq(O.hear(), ic[100:300])
+    assert float(m.group(1)) == pytest.approx(f.mean() * H.SECOND_COEFF, rel=1e-8)
+    assert float(m.group(2)) == pytest.approx(f.std(ddof=1) / np.sqrt(200) * H.SECOND_COEFF, rel=1e-6)
+    assert "M2 ESTIMATE FROM CF (9-point)" in r.stdout and "M2 ESTIMATE FROM CF (21-point)" in r.stdout
+    _, cf = read_columns(tmp_path / "CF-hear-small.txt")
+    p = O.CFParams(1e-12, 1e-12, 200.0, 64, 40.0, O.HALVES_BOTH, 50000)
+    cf_ref = O.cf_rows(O.hear(), p, ic[300:], 8)["crln"].sum(0) / 128
+    assert np.linalg.norm(cf[:, 1] - cf_ref) / np.linalg.norm(cf_ref) < 1e-8
+
+
 def test_driver_correlation_array_matches_oracle(tmp_path):
     """CALCULATION_TYPE = CORRELATION_ARRAY: one file per satellite temperature, each with its
     weight sum as the trajectory count, against the oracle's rows of the same ensemble weighted
@@ -144,11 +172,11 @@ def test_driver_cmp(tmp_path, golden, perturb):
 
 
 def test_driver_phase_space_m0_reproduces_reference_log(tmp_path):
-    """CALCULATION_TYPE = PHASE_SPACE_M0 for CO2-Ar at 100 K (FREE, R in [4, 40]) against the
+    """CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M0 for CO2-Ar at 100 K (FREE, R in [4, 40]) against the
     reference's VEGAS M0 6.25651e-4 (+-0.070%, example_configs/CF-F-CO2-Ar.conf.out:186-189),
     within 4 combined standard errors."""
     (tmp_path / "m0.conf").write_text("""&INPUT
-  CALCULATION_TYPE = PHASE_SPACE_M0
+  CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M0
   PAIR_STATE = FREE_AND_METASTABLE
   TEMPERATURE = 100.0
   pair_reduced_mass = 3.817083069753919e+04
@@ -173,6 +201,36 @@ def test_driver_phase_space_m0_reproduces_reference_log(tmp_path):
     assert abs(val - ref) < 4 * np.hypot(err, ref * 7.04919e-4), (val, err)
 
 
+def test_driver_phase_space_m2_reproduces_reference_log(tmp_path):
+    """CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M2 for CO2-Ar at 100 K (FREE, R in [4, 40]) against
+    the reference's VEGAS M2 0.441296 (+-0.076%, example_configs/CF-F-CO2-Ar.conf.out:235-238),
+    within 4 combined standard errors."""
+    (tmp_path / "m2.conf").write_text("""&INPUT
+  CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M2
+  PAIR_STATE = FREE_AND_METASTABLE
+  TEMPERATURE = 100.0
+  pair_reduced_mass = 3.817083069753919e+04
+  sampler_Rmin = 4.0
+  sampler_Rmax = 40.0
+&END
+&MONOMER
+  MONOMER_TYPE = LINEAR_MOLECULE
+  II = {2.819827114123505e+05, 2.819827114123505e+05, 0.0}
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+""")
+    r = run_driver(["m2.conf", "--seed", "9"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    import re
+    m = re.search(r"T = 1\.00e\+02 => M2: ([0-9.e+-]+) \+- ([0-9.e+-]+)", r.stdout)
+    assert m, r.stdout[-2000:]
+    val, err = float(m.group(1)), float(m.group(2))
+    ref = 0.441296
+    assert abs(val - ref) < 4 * np.hypot(err, ref * 7.61738e-4), (val, err)
+
+
 def test_driver_correlation_single_integrates_ppf_when_not_given(tmp_path):
     """Without PARTIAL_PARTITION_FUNCTION_RATIO the driver integrates it by VEGAS as the reference
     does; for He-Ar at 300 K the printed ratio is used for the CF and equals the PF integral over
@@ -185,4 +243,4 @@ def test_driver_correlation_single_integrates_ppf_when_not_given(tmp_path):
     import re
     m = re.search(r"=> PPF ratio: ([0-9.e+-]+)", r.stdout)
     assert m and float(m.group(1)) > 0, r.stdout[-2000:]
-    assert "=> M0:" in r.stdout
+    assert "=> M0:" in r.stdout and "=> M2:" in r.stdout

@@ -179,6 +179,42 @@ def test_m0_matches_oracle_streams(eng):
     assert q > 0
 
 
+@pytest.mark.parametrize("which", ["hear", "co2ar"])
+def test_m2_matches_oracle_streams(eng, which):
+    """M2 (calculate_M2) over 64 streams seeded seed+s equals the oracle's per-stream M2
+    combined as mpi_calculate_M2 combines ranks.  The per-point value differences the dipole
+    with h = 1e-3, which scales the surfaces' last-bit differences by ~1/h: 1e-8 relative."""
+    if which == "hear":
+        sh, so, T, ps, pesmin = H.he_ar(), O.hear(), 300.0, H.PAIR_FREE, H.PESMIN_HEAR
+    else:
+        sh, so, T, ps, pesmin = H.co2_ar(), O.co2ar(), 100.0, H.PAIR_BOUND, H.PESMIN_CO2AR
+    smp = H.Sampler(H.IC_REJECTION, ps, T, 4.0, 40.0, pesmin, 0.0)
+    m, q, _ = eng.m2(sh, smp, 7, 64 * 100, 64, 1.5)
+    res = [O.m2(so, O.Sampler(0, ps, T, 4.0, 40.0, pesmin, 0.0), 7 + s, 100, 1.5) for s in range(64)]
+    mo = np.mean([r[0] for r in res])
+    # q: the per-stream sums of squared deviations averaged, as the ranks' are
+    qo = 1.5 * H.SECOND_COEFF * np.sqrt(np.mean([(r[1] / (1.5 * H.SECOND_COEFF)) ** 2 * 100 * 99 for r in res])
+                                        / 100 / 99)
+    assert abs(m - mo) < 1e-8 * abs(mo)
+    assert abs(q - qo) < 1e-7 * qo
+
+
+@pytest.mark.parametrize("which", ["hear", "co2ar"])
+def test_dipole_rate_matches_oracle(eng, which):
+    """hh_eval(HH_EVAL_DIPOLE_RATE), |d mu/dt|^2 at sampled phase points, against the oracle."""
+    if which == "hear":
+        sh, so = H.he_ar(), O.hear()
+        smp = H.Sampler(H.IC_REJECTION, 3, 300.0, 3.5, 40.0, H.PESMIN_HEAR, 0.0)
+    else:
+        sh, so = H.co2_ar(), O.co2ar()
+        smp = H.Sampler(H.IC_REJECTION, 3, 300.0, 4.0, 40.0, H.PESMIN_CO2AR, 0.0)
+    y, _, _ = H.sample_ics(sh, smp, 23, 400)
+    f = eng.eval(sh, H.EVAL_DIPOLE_RATE, eng.to_device_soa(y)).cpu().numpy()[0]
+    fo = O.dipole_rate_sq(so, y)
+    assert np.all(fo > 0)
+    np.testing.assert_allclose(f, fo, rtol=1e-8)
+
+
 def test_m0_rejects_unreachable_sampler(eng):
     bad = H.Sampler(H.IC_REJECTION, H.PAIR_BOUND, 300.0, 39.0, 40.0, H.PESMIN_HEAR, 0.0)
     with pytest.raises(H.HHError):

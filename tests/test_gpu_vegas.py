@@ -8,9 +8,11 @@ estimates over 12 iterations x 10^6 points of the same integrands at T = 100 K:
       (+-0.085%), M0 4.07505e-4 (CF-B-CO2-Ar.conf.out:138-189)
   He-Ar FREE_AND_METASTABLE, R in [3, 40]:  M0 integral 0.347622 (+-0.018%), M0 2.08597e-6
       (CF-F-He-Ar.conf.out:186-190)
+and of integrand_M2 (the same runs, lines 190-239 of each log): CO2-Ar FREE 0.148636 (+-0.076%),
+CO2-Ar BOUND 0.0338833 (+-0.117%), He-Ar FREE 4.65447e-8 (+-0.018%).
 The device estimate must agree within 4 combined standard errors.  A second pin ties VEGAS to the
 rejection sampler (bit-exact with the reference's draws): M0-integral / PF-integral = <mu^2>,
-which hh_m0 estimates independently.
+which hh_m0 estimates independently; likewise M2-integral / PF-integral = <|d mu/dt|^2> of hh_m2.
 """
 import math
 
@@ -37,6 +39,9 @@ def eng():
     ("co2ar", H.PAIR_BOUND, 4.0, H.INTEGRAND_PF, 1.53022e10, 6.82507e-4),
     ("co2ar", H.PAIR_BOUND, 4.0, H.INTEGRAND_M0, 6.6114e6, 8.46597e-4),
     ("hear", H.PAIR_FREE, 3.0, H.INTEGRAND_M0, 0.347622, 1.79001e-4),
+    ("co2ar", H.PAIR_FREE, 4.0, H.INTEGRAND_M2, 0.148636, 7.61738e-4),
+    ("co2ar", H.PAIR_BOUND, 4.0, H.INTEGRAND_M2, 0.0338833, 1.16809e-3),
+    ("hear", H.PAIR_FREE, 3.0, H.INTEGRAND_M2, 4.65447e-08, 1.76591e-4),
 ])
 def test_vegas_matches_reference_logs(eng, case):
     which, pair, rmin, integrand, ref, ref_rel = case
@@ -69,8 +74,26 @@ def test_vegas_ratio_matches_rejection_sampler(eng):
     assert abs(z) < 4.0
 
 
+def test_vegas_m2_ratio_matches_rejection_sampler(eng):
+    """<|d mu/dt|^2> two ways for He-Ar at 300 K FREE: VEGAS M2/PF, and hh_m2."""
+    sysd = H.he_ar()
+    T, rmin, rmax = 300.0, 4.0, 40.0
+    m2 = eng.vegas(sysd, H.INTEGRAND_M2, T, rmin, rmax, niterations=12, npoints=1000000, seed=5)
+    pf = eng.vegas(sysd, H.INTEGRAND_PF, T, rmin, rmax, niterations=12, npoints=1000000, seed=4)
+    ratio = m2.value / pf.value
+    rerr = ratio * math.hypot(m2.error / m2.value, pf.error / pf.value)
+    smp = H.Sampler(H.IC_REJECTION, H.PAIR_FREE, T, rmin, rmax, H.PESMIN_HEAR, 0.0)
+    nstreams = 1024
+    m, q, _ = eng.m2(sysd, smp, 13, 2_000_000, nstreams, 1.0)
+    rate, rate_err = m / H.SECOND_COEFF, q / H.SECOND_COEFF / math.sqrt(nstreams)
+    z = (ratio - rate) / math.hypot(rerr, rate_err)
+    print(f"VEGAS <|mudot|^2> {ratio:.6e} +- {rerr:.1e}; rejection {rate:.6e} +- {rate_err:.1e}; z = {z:+.2f}",
+          flush=True)
+    assert abs(z) < 4.0
+
+
 def test_vegas_rejects_bad_input(eng):
     with pytest.raises(H.HHError):
-        eng.vegas(H.he_ar(), H.INTEGRAND_M2, 100.0, 3.0, 40.0)
+        eng.vegas(H.he_ar(), 7, 100.0, 3.0, 40.0)
     with pytest.raises(H.HHError):
         eng.vegas(H.he_ar(), H.INTEGRAND_PF, 100.0, 40.0, 3.0)

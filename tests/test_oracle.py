@@ -407,3 +407,68 @@ def test_zimmermann_constant_dipole_is_flat():
     crln = np.zeros(L)
     O.lib().orc_zimmermann_series_cf(O.ptr(np.ascontiguousarray(d)), L, O.ptr(crln))
     np.testing.assert_allclose(crln, 0.14 * 5.29177210903e-11 ** 3, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- M2 (src/hawaii.c:1524-1617)
+def test_m2_point_hear_closed_form():
+    """For a central dipole mu(R) r_hat, |d mu/dt|^2 = mu'(R)^2 Rdot^2 + mu(R)^2 (Thetadot^2 +
+    sin^2 Theta Phidot^2), with the velocities of the kinetic energy (Rdot = pR/m, Thetadot =
+    pTheta/(m R^2), Phidot = pPhi/(m R^2 sin^2 Theta)) and mu' by a 6th-order difference.  The
+    oracle's central differences (h = 1e-3) agree to O(h^2)."""
+    s = O.hear()
+    L = O.lib()
+    m = O.MU_HEAR
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        R = rng.uniform(4.0, 15.0)
+        Th = rng.uniform(0.2, math.pi - 0.2)
+        Ph = rng.uniform(0, 2 * math.pi)
+        pPh, pTh, pR = rng.normal(0, 30.0), rng.normal(0, 30.0), rng.normal(0, 5.0)
+        y = np.array([Ph, pPh, Th, pTh, R, pR])
+        f = O.dipole_rate_sq(s, y)[0]
+        k = 1e-4
+        dmu = (-L.orc_hear_dip(R + 3 * k) + 9 * L.orc_hear_dip(R + 2 * k) - 45 * L.orc_hear_dip(R + k)
+               + 45 * L.orc_hear_dip(R - k) - 9 * L.orc_hear_dip(R - 2 * k) + L.orc_hear_dip(R - 3 * k)) / (-60 * k)
+        mu = L.orc_hear_dip(R)
+        Rd, Thd, Phd = pR / m, pTh / (m * R * R), pPh / (m * R * R * math.sin(Th) ** 2)
+        expect = dmu ** 2 * Rd ** 2 + mu ** 2 * (Thd ** 2 + math.sin(Th) ** 2 * Phd ** 2)
+        assert f == pytest.approx(expect, rel=1e-5)
+
+
+def test_m2_point_co2ar_is_derivative_along_the_flow():
+    """d mu/dt = the derivative of mu(q(t)) along Hamilton's flow: a Richardson-extrapolated
+    directional difference of the dipole along qdot (coordinates only) at sampled states."""
+    s = O.co2ar()
+    L = O.lib()
+    smp = O.Sampler(O.IC_REJECTION, O.PAIR_FREE, 200.0, 4.0, 20.0, O.PESMIN_CO2AR, 0.0)
+    ic, _, _ = O.sample_ics(s, smp, 17, 12)
+    for y in ic:
+        yd = np.zeros(10)
+        L.orc_rhs(C.byref(s), O.ptr(y), O.ptr(yd))
+        qdot = yd[0::2]
+        eps = 2e-3 / np.abs(qdot).max()
+
+        def mu_at(t):
+            q = np.ascontiguousarray(y[0::2] + t * qdot)
+            out = np.zeros(3)
+            L.orc_dipole(C.byref(s), O.ptr(q), O.ptr(out))
+            return out
+
+        d1 = (mu_at(eps) - mu_at(-eps)) / (2 * eps)
+        d2 = (mu_at(eps / 2) - mu_at(-eps / 2)) / eps
+        mudot = (4 * d2 - d1) / 3
+        f = O.dipole_rate_sq(s, y)[0]
+        assert f == pytest.approx(float(mudot @ mudot), rel=1e-5)
+
+
+def test_m2_rejection_sampler_averages_the_accepted_ensemble():
+    """calculate_M2 averages |d mu/dt|^2 over the same accepted points as the trajectory
+    sampler draws (the candidate loop of src/hawaii.c:1546-1560), times SecondCoeff."""
+    for s, smp in ((O.hear(), O.Sampler(O.IC_REJECTION, O.PAIR_FREE, 300.0, 4.0, 40.0, O.PESMIN_HEAR, 0.0)),
+                   (O.co2ar(), O.Sampler(O.IC_REJECTION, O.PAIR_BOUND, 100.0, 4.0, 40.0, O.PESMIN_CO2AR, 0.0))):
+        m, q = O.m2(s, smp, 9, 3000, 2.5)
+        ic, _, _ = O.sample_ics(s, smp, 9, 3000)
+        f = O.dipole_rate_sq(s, ic)
+        SecondCoeff = 13856114.29344114
+        assert m == pytest.approx(f.mean() * SecondCoeff * 2.5, rel=1e-12)
+        assert q == pytest.approx(f.std(ddof=1) / math.sqrt(f.size) * SecondCoeff * 2.5, rel=1e-9)

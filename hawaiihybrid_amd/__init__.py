@@ -58,6 +58,13 @@ class CFArrayParams(C.Structure):
                 ("temps", C.c_void_p), ("ratios", C.c_void_p)]
 
 
+class PrmuParams(C.Structure):
+    """hh_prmu_params: the PR/mu spectral function's trajectory and spectrum settings."""
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("R0", C.c_double),
+                ("L", C.c_int), ("nfreq", C.c_int), ("precision", C.c_int), ("max_steps", C.c_int),
+                ("allow_truncating", C.c_int)]
+
+
 class CFStats(C.Structure):
     _fields_ = [("integrate_ms", C.c_double), ("reduce_ms", C.c_double), ("n_items", C.c_longlong),
                 ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
@@ -151,6 +158,7 @@ def lib():
             "hh_m2": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
             "hh_eval": (i, [v, P(System), i, v, C.c_long, v]),
             "hh_cf_array": (i, [v, P(System), P(CFParams), P(CFArrayParams), v, C.c_long, v, v, P(CFStats)]),
+            "hh_prmu": (i, [v, P(System), P(PrmuParams), v, C.c_long, v, v, P(C.c_long), P(CFStats)]),
             "hh_conf_parse": (i, [C.c_char_p, v]),
             "hh_session_create": (i, [v, P(SessionOpts), P(v)]),
             "hh_submit_cf": (i, [v, P(System), P(CFParams), v, C.c_long, v, P(C.c_uint64)]),
@@ -306,6 +314,23 @@ class Engine:
         _check(lib().hh_eval(self.h, C.byref(system), what, C.c_void_p(y_dev.data_ptr()), n,
                              C.c_void_p(out.data_ptr())))
         return out
+
+    def prmu(self, system, params, ic_dev, sf_dev=None):
+        """PR/mu spectrum (hh_prmu) of the R0-injected ensemble ic_dev (SoA [QP][ntraj]): adds
+        every counted trajectory's weighted power spectrum into sf_dev ([nfreq], zeros when None).
+        Returns (sf_dev, counted [ntraj] numpy, ncounted, stats)."""
+        torch = self.torch
+        if sf_dev is None:
+            sf_dev = torch.zeros(params.nfreq, dtype=torch.float64, device=f"cuda:{self.device}")
+        n = ic_dev.shape[1]
+        counted = np.zeros(n, dtype=np.int32)
+        nc = C.c_long(0)
+        st = CFStats()
+        torch.cuda.synchronize(self.device)
+        _check(lib().hh_prmu(self.h, C.byref(system), C.byref(params), C.c_void_p(ic_dev.data_ptr()), n,
+                             C.c_void_p(sf_dev.data_ptr()), counted.ctypes.data_as(C.c_void_p), C.byref(nc),
+                             C.byref(st)))
+        return sf_dev, counted, nc.value, st
 
     def cf_array(self, system, params, ic_dev, T_base, pair_state, pesmin, temps, ratios=None):
         """CORRELATION_ARRAY (hh_cf_array): the CF of the ensemble ic_dev (SoA [QP][ntraj], sampled

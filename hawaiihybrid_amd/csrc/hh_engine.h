@@ -49,6 +49,7 @@ struct JobDesc {
     int L;                   // MaxTrajectoryLength
     int max_steps;           // max accepted steps between two samples
     int record;              // 0: rows hold mu(0).mu(t_k) [L]; 1: the dipole series mu_x, mu_y, mu_z [3][L]
+                             // (Zimmermann); 2: the same series for PR/mu, dipole check only, no CF reduce
     double dt, Rcut, rtol, atol;
 };
 

@@ -114,6 +114,7 @@ int check_cf_args(const hh_system* sys, const hh_cf_params* p)
 // Zimmermann series the FFT cross-correlation over every time origin.
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out)
 {
+    if (jd.record == 2) return HH_OK;   // PR/mu series: the spectrum is formed by its caller
     if (jd.record) {
         const size_t ws = zimm_workspace_doubles(jd.L);
         HH_CHECK(e->psum.ensure(ws * sizeof(double)));
@@ -339,6 +340,15 @@ int hh_engine_info(hh_engine* e, char* buf, size_t len)
 int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njobs, const double* const* ic_dev,
               const long* ntraj, double* const* cf_dev, hh_cf_stats* stats)
 {
+    return hh::cf_run(e, sys, p, njobs, ic_dev, ntraj, cf_dev, stats, -1);
+}
+
+}  // extern "C"
+
+int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njobs, const double* const* ic_dev,
+               const long* ntraj, double* const* cf_dev, hh_cf_stats* stats, int record)
+{
+    if (record < 0) record = p && p->zimmermann ? 1 : 0;
     if (!e || !sys || !p || njobs <= 0 || !ic_dev || !ntraj || !cf_dev)
         return fail(HH_E_INVALID, "hh_cf_run: null argument or njobs <= 0");
     int rc = hh::check_cf_args(sys, p);
@@ -372,7 +382,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         std::vector<hh::PoolJob*> mine;
         for (int j = 0; j < njobs; ++j) {
             hh::PoolJob* pj = nullptr;
-            rc = hh::pool_admit(e, *pl, *p, ic_dev[j], ntraj[j], cf_dev[j], 0, &pj);
+            rc = hh::pool_admit(e, *pl, *p, ic_dev[j], ntraj[j], cf_dev[j], 0, &pj, record);
             if (rc < 0) {
                 while (!pl->inflight.empty()) pl->inflight.pop_back();
                 return rc;
@@ -426,7 +436,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         jobs[j].ntraj = ntraj[j];
         total += jobs[j].n_items;
     }
-    const int rec = p->zimmermann ? 3 : 1;
+    const int rec = record ? 3 : 1;
     const size_t rows_bytes = (size_t)total * (size_t)p->L * rec * sizeof(double);
     if (e->rows.ensure(rows_bytes) != hipSuccess)
         return fail(HH_E_NOMEM, "cannot allocate %.2f GiB of per-trajectory CF rows", rows_bytes / 1073741824.0);
@@ -440,7 +450,7 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
         const long long off = jb.item_begin;
         jb.ic = ic_dev[j];
         jb.rows = e->rows.as<double>() + off * p->L * rec;
-        jb.record = p->zimmermann ? 1 : 0;
+        jb.record = record;
         jb.len = e->len.as<int>() + off;
         jb.status = e->status.as<int>() + off;
         jb.nrhs = e->nrhs.as<int>() + off;
@@ -484,6 +494,8 @@ int hh_cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njo
     }
     return HH_OK;
 }
+
+extern "C" {
 
 int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* e_drift, double* rows)
 {

@@ -169,6 +169,7 @@ struct hh_engine {
     hh::DevBuf tab64, tab32;      // Co2ArTables<double>, Co2ArTables<float>
     hh::DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, arena;
     hh::DevBuf zimm_tw;           // twiddles e^{2 pi i j/P} of the Zimmermann transforms
+    hh::DevBuf prmu_a, prmu_b, prmu_w;   // PR/mu: FFT ping-pong series, weights and spectra
     int zimm_logP = 0;
     std::vector<hh::JobDesc> last_jobs;
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
@@ -193,11 +194,15 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
 int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const double* const* ic_dev,
               const long* ntraj, int njobs, int* block, int* waves);
 // device bytes a job of this shape holds while it runs
-size_t pool_job_bytes(const hh_cf_params& p, long ntraj);
+size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record = -1);
 // admit a correlation-function job: allocates its buffers, uploads its descriptor into the
-// pool's ring and appends its items to the pool's queue; *out (optional) sees the job
+// pool's ring and appends its items to the pool's queue; *out (optional) sees the job.
+// record: the JobDesc row mode (-1: 1 with p.zimmermann, else 0; 2: a PR/mu dipole series)
 int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* ic_dev, long ntraj, double* cf_dev,
-               uint64_t request, PoolJob** out);
+               uint64_t request, PoolJob** out, int record = -1);
+// hh_cf_run with the row mode of every job (-1: from p->zimmermann; 2: PR/mu series, no reduce)
+int cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njobs, const double* const* ic_dev,
+           const long* ntraj, double* const* cf_dev, hh_cf_stats* stats, int record);
 // hand out the items of the jobs admitted since the last call (a group) longest-expected-first
 // across the whole group; call after admitting them and before the next pool_step
 int pool_order_group(hh_engine* e, LanePool& pl);

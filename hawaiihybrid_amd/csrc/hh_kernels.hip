@@ -351,11 +351,13 @@ __device__ __forceinline__ double cf_value(const double* mu0, const P (&m)[3])
 }
 
 // Checks of one sample in the reference's order (src/hawaii.c:2181-2236): a NaN dipole, a
-// huge CF value, a jump by more than 1e10 from the previous sample (from sample 2 on).
+// huge CF value, a jump by more than 1e10 from the previous sample (from sample 2 on).  A
+// PR/mu series (record 2) checks the dipole only (src/hawaii.c:3659-3671).
 template <typename P>
-__device__ __forceinline__ int sample_status(const P (&m)[3], double curr, double prev, int k)
+__device__ __forceinline__ int sample_status(const P (&m)[3], double curr, double prev, int k, int record)
 {
     if (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2])) return ITEM_FAIL_DIPOLE_NAN;
+    if (record == 2) return ITEM_OK;   // the PR/mu series: the dipole check only
     if (fabs(curr) > 1e100) return ITEM_FAIL_CF_HUGE;
     if (k > 1 && fabs(curr / prev) > 1e10) return ITEM_FAIL_CF_JUMP;
     return ITEM_OK;
@@ -541,7 +543,7 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
             P m[3];
             dipole<SYS, P>(sc, ys, m);
             const double curr = cf_value<P>(ln.mu0, m);
-            status = sample_status<P>(m, curr, ln.curr, ln.k);
+            status = sample_status<P>(m, curr, ln.curr, ln.k, jb.record);
             ln.prev = ln.curr;
             ln.curr = curr;
             if (status != ITEM_OK) {
@@ -692,7 +694,7 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
         const double prev = q == 0 ? prev_start : (lane > 0 ? up : carry);
         carry = __shfl(curr, 63, 64);
         if (valid) {
-            const int st = sample_status<P>(m, curr, prev, k);
+            const int st = sample_status<P>(m, curr, prev, k, record);
             if (st != ITEM_OK) atomicMin(&ws.fail[owner], (q << 3) | st);
         }
         wave_share_sync();

@@ -61,12 +61,13 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
     return HH_OK;
 }
 
-size_t pool_job_bytes(const hh_cf_params& p, long ntraj)
+size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record)
 {
     const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
-    // rows ([3][L] per item for the Zimmermann series), len, status, nrhs, e_drift, and the job's
+    if (record < 0) record = p.zimmermann ? 1 : 0;
+    // rows ([3][L] per item for a dipole series), len, status, nrhs, e_drift, and the job's
     // share of its group's order
-    const size_t row = (size_t)p.L * (p.zimmermann ? 3 : 1);
+    const size_t row = (size_t)p.L * (record ? 3 : 1);
     return (size_t)items * (row * sizeof(double) + 4 * sizeof(int) + sizeof(float));
 }
 
@@ -76,13 +77,14 @@ bool pool_has_room(const LanePool& pl)
 }
 
 int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* ic_dev, long ntraj, double* cf_dev,
-               uint64_t request, PoolJob** out)
+               uint64_t request, PoolJob** out, int record)
 {
+    if (record < 0) record = p.zimmermann ? 1 : 0;
     if (!pool_has_room(pl)) return fail(HH_E_BUSY, "lane pool ring full (%d jobs in flight)", pl.ring_cap);
     auto job = std::make_unique<PoolJob>();
     const int halves = p.halves == HH_HALVES_BOTH ? 2 : 1;
     const long long n_items = (long long)ntraj * halves;
-    const size_t bytes = pool_job_bytes(p, ntraj);
+    const size_t bytes = pool_job_bytes(p, ntraj, record);
     if (job->buf.ensure(bytes) != hipSuccess)
         return fail(HH_E_NOMEM, "cannot allocate %.2f GiB for a job of %ld trajectories", bytes / 1073741824.0, ntraj);
     job->bytes = bytes;
@@ -94,7 +96,7 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     JobDesc& jd = job->jd;
     jd.ic = ic_dev;
     jd.rows = job->buf.as<double>();
-    jd.record = p.zimmermann ? 1 : 0;
+    jd.record = record;
     jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L * (jd.record ? 3 : 1));
     jd.status = jd.len + n_items;
     jd.nrhs = jd.status + n_items;

@@ -1,0 +1,199 @@
+// PR/mu spectral function (calculate_spectral_function_using_prmu_representation_and_save,
+// src/hawaii.c:3148-3977) for pairs without requantised rotors: trajectories injected at R0
+// with an inward momentum (hh_sample_ics, HH_IC_R0_INJECT), integrated by the batched engine
+// with the dipole recorded every sampling_time until the first sample beyond R0 (rows of
+// record mode 2), then per trajectory
+//     S_t(nu_k) = SF_COEFF * pr_mu * sum_c |FFT(Connes(mu_c))_k|^2,   pr_mu = -pR(0) / m,
+// SF_COEFF = Hartree ALU^6 ATU R0^2 dt^2 (src/hawaii.c:3395), summed over the trajectories.
+//
+// The reference's series starts at t = dt (step 0 of its loop, src/hawaii.c:3578) and holds L
+// samples, zero after the trajectory's end; the engine's row also holds mu(0) at index 0, so
+// sample i of the series is row index i + 1, and the engine runs L + 2 samples: a row shorter
+// than L + 2 ended beyond R0 within the reference's L samples, a row of L + 2 did not (the
+// reference drops such a trajectory and draws the next one, src/hawaii.c:3701-3708).
+//
+// On the device the three components of a trajectory go through two complex FFTs of length L:
+// z1 = x + i y gives |X_k|^2 + |Y_k|^2 = (|Z1_k|^2 + |Z1_{L-k}|^2) / 2, z2 = z gives |Z_k|^2
+// (the sign convention of the transform does not enter a power spectrum).  The FFT is a batched
+// radix-2 Stockham transform over global memory, one launch per stage for every series of a
+// chunk of trajectories (coalesced: consecutive threads take consecutive butterflies of one
+// series).  The per-trajectory spectra are weighted and summed in trajectory order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hawaii_gpu.h"
+#include "hh_internal.h"
+
+namespace hh {
+namespace {
+
+// series s = 2t (x + i y) and 2t + 1 (z) of trajectory t of the chunk, Connes-apodised
+// (connes_apodization, src/hawaii.c:5401-5410, over the whole L-sample array)
+__global__ void __launch_bounds__(256) prmu_pack_kernel(const double* __restrict__ rows, const int* __restrict__ len,
+                                                        const int* __restrict__ keep, long long t0, long long nt,
+                                                        int Lrow, long long L, double dt, double2* __restrict__ z)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nt * L) return;
+    const long long tl = g / L, i = g % L, t = t0 + tl;
+    double x = 0.0, y = 0.0, w = 0.0;
+    if (keep[t] && i + 1 < (long long)len[t]) {
+        const double* row = rows + t * 3 * (long long)Lrow;
+        const double tmax = dt * (double)L, tc = dt * (double)i, r = tc / tmax;
+        const double f = (1.0 - r * r) * (1.0 - r * r);
+        x = row[i + 1] * f;
+        y = row[Lrow + i + 1] * f;
+        w = row[2 * (long long)Lrow + i + 1] * f;
+    }
+    z[(2 * tl) * L + i] = make_double2(x, y);
+    z[(2 * tl + 1) * L + i] = make_double2(w, 0.0);
+}
+
+// one radix-2 Stockham stage (sub-transforms of 2 half_len points) over nseries series of L
+__global__ void __launch_bounds__(256) prmu_fft_stage_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                             long long L, long long half_len, long long nseries)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nb = L / 2;
+    if (g >= nseries * nb) return;
+    const long long s = g / nb, j = g % nb;
+    const double2* a_in = in + s * L;
+    double2* o = out + s * L;
+    const long long k = j % half_len, grp = j / half_len;
+    const double2 a = a_in[j], b = a_in[j + nb];
+    double sn, cs;
+    sincospi((double)k / (double)half_len, &sn, &cs);
+    const double2 wb = make_double2(b.x * cs - b.y * sn, b.x * sn + b.y * cs);
+    o[grp * 2 * half_len + k] = make_double2(a.x + wb.x, a.y + wb.y);
+    o[grp * 2 * half_len + k + half_len] = make_double2(a.x - wb.x, a.y - wb.y);
+}
+
+// pw[t][k] = w[t] * (|X_k|^2 + |Y_k|^2 + |Z_k|^2), k < nfreq (gsl_fft_square, src/hawaii.c:2349-2356)
+__global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restrict__ Z, long long L, int nfreq,
+                                                         long long t0, long long nt, const double* __restrict__ w,
+                                                         double* __restrict__ pw)
+{
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nt * nfreq) return;
+    const long long tl = g / nfreq, k = g % nfreq;
+    const double2* z1 = Z + (2 * tl) * L;
+    const double2* z2 = Z + (2 * tl + 1) * L;
+    const double2 a = z1[k], b = z1[(L - k) & (L - 1)], c = z2[k];
+    const double p = 0.5 * ((a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y)) + (c.x * c.x + c.y * c.y);
+    pw[tl * nfreq + k] = w[t0 + tl] * p;
+}
+
+// sf[k] += sum over the chunk's trajectories, in trajectory order
+__global__ void __launch_bounds__(256) prmu_accumulate_kernel(const double* __restrict__ pw, long long nt, int nfreq,
+                                                              double* __restrict__ sf)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nfreq) return;
+    double acc = sf[k];
+    for (long long t = 0; t < nt; ++t) acc += pw[t * nfreq + k];
+    sf[k] = acc;
+}
+
+constexpr double kSfHartree = 4.3597447222071e-18;   // src/constants.h:10
+constexpr double kSfALU = 5.29177210903e-11;       // src/constants.h:8
+constexpr double kSfATU = 2.4188843265857e-17;       // src/constants.h:12
+
+}  // namespace
+}  // namespace hh
+
+using hh::fail;
+
+extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params* p, const double* ic_dev, long ntraj,
+                       double* sf_dev, int* counted, long* ncounted, hh_cf_stats* stats)
+{
+    using namespace hh;
+    if (!e || !sys || !p || !ic_dev || !sf_dev || ntraj <= 0) return fail(HH_E_INVALID, "hh_prmu: bad argument");
+    if (p->L < 4 || p->L > (1 << 20) || (p->L & (p->L - 1)))
+        return fail(HH_E_INVALID, "hh_prmu: MaxTrajectoryLength must be a power of two in [4, 2^20] (radix-2 FFT)");
+    if (p->nfreq < 1 || p->nfreq > p->L / 2 + 1)
+        return fail(HH_E_INVALID, "hh_prmu: nfreq must lie in [1, L/2 + 1]");
+    if (!(p->dt > 0) || !(p->R0 > 0) || !(p->rtol > 0) || !(p->atol > 0))
+        return fail(HH_E_INVALID, "hh_prmu: sampling_time, R0 and tolerances must be positive");
+    hh_cf_params cp{};
+    cp.rtol = p->rtol;
+    cp.atol = p->atol;
+    cp.dt = p->dt;
+    cp.Rcut = p->R0;
+    cp.ppf = 1.0;
+    cp.L = p->L + 2;
+    cp.halves = HH_HALVES_FORWARD;
+    cp.max_steps = p->max_steps;
+    cp.engine = HH_ENGINE_BATCHED;
+    cp.precision = p->precision;
+    double* dummy[1] = {sf_dev};   // no CF is reduced for a series job
+    const double* icp[1] = {ic_dev};
+    const long nt[1] = {ntraj};
+    int rc = cf_run(e, sys, &cp, 1, icp, nt, dummy, stats, 2);
+    if (rc < 0) return rc;
+
+    std::lock_guard<std::mutex> lock(e->mu);
+    HH_CHECK(hipSetDevice(e->device));
+    const JobDesc jd = e->last_jobs[0];
+    std::vector<int> len(ntraj), status(ntraj), keep(ntraj);
+    std::vector<double> pR(ntraj), w(ntraj);
+    HH_CHECK(hipMemcpy(len.data(), jd.len, ntraj * sizeof(int), hipMemcpyDeviceToHost));
+    HH_CHECK(hipMemcpy(status.data(), jd.status, ntraj * sizeof(int), hipMemcpyDeviceToHost));
+    HH_CHECK(hipMemcpy(pR.data(), ic_dev + 5 * ntraj, ntraj * sizeof(double), hipMemcpyDeviceToHost));
+    const double SF_COEFF = kSfHartree * std::pow(kSfALU, 6) * kSfATU * p->R0 * p->R0 * p->dt * p->dt;
+    long n_keep = 0;
+    for (long t = 0; t < ntraj; ++t) {
+        // ended beyond R0 within L samples (or kept truncated), without a failed dipole or step
+        keep[t] = status[t] == 0 && (len[t] <= p->L + 1 || p->allow_truncating) ? 1 : 0;
+        const double pr_mu = -pR[t] / sys->mu;
+        w[t] = keep[t] ? SF_COEFF * pr_mu : 0.0;
+        n_keep += keep[t];
+        if (counted) counted[t] = keep[t];
+    }
+    if (ncounted) *ncounted = n_keep;
+    const long long L = p->L;
+    // chunks of trajectories whose two series per trajectory fit 1 GiB per buffer
+    const long long chunk = std::max<long long>(1, std::min<long long>(ntraj, (1LL << 30) / (2 * L * 16)));
+    HH_CHECK(e->prmu_a.ensure((size_t)chunk * 2 * L * sizeof(double2)));
+    HH_CHECK(e->prmu_b.ensure((size_t)chunk * 2 * L * sizeof(double2)));
+    HH_CHECK(e->prmu_w.ensure((size_t)ntraj * (sizeof(double) + sizeof(int)) + sizeof(int) +
+                              (size_t)chunk * p->nfreq * sizeof(double)));
+    double* wd = e->prmu_w.as<double>();
+    int* keepd = reinterpret_cast<int*>(wd + ntraj);
+    double* pw = reinterpret_cast<double*>(keepd + ntraj + (ntraj & 1));
+    HH_CHECK(hipMemcpyAsync(wd, w.data(), ntraj * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipMemcpyAsync(keepd, keep.data(), ntraj * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipEventRecord(e->ev[0], e->stream));
+    for (long long t0 = 0; t0 < ntraj; t0 += chunk) {
+        const long long n = std::min<long long>(chunk, ntraj - t0);
+        double2* a = e->prmu_a.as<double2>();
+        double2* b = e->prmu_b.as<double2>();
+        hipLaunchKernelGGL(prmu_pack_kernel, dim3((unsigned)((n * L + 255) / 256)), dim3(256), 0, e->stream, jd.rows,
+                           jd.len, keepd, t0, n, cp.L, L, p->dt, a);
+        HH_CHECK(hipGetLastError());
+        for (long long half = 1; half < L; half <<= 1) {
+            hipLaunchKernelGGL(prmu_fft_stage_kernel, dim3((unsigned)((2 * n * (L / 2) + 255) / 256)), dim3(256), 0,
+                               e->stream, a, b, L, half, 2 * n);
+            HH_CHECK(hipGetLastError());
+            std::swap(a, b);
+        }
+        hipLaunchKernelGGL(prmu_power_kernel, dim3((unsigned)((n * p->nfreq + 255) / 256)), dim3(256), 0, e->stream, a,
+                           L, p->nfreq, t0, n, wd, pw);
+        HH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(prmu_accumulate_kernel, dim3((unsigned)((p->nfreq + 255) / 256)), dim3(256), 0, e->stream, pw,
+                           n, p->nfreq, sf_dev);
+        HH_CHECK(hipGetLastError());
+    }
+    HH_CHECK(hipEventRecord(e->ev[1], e->stream));
+    HH_CHECK(hipStreamSynchronize(e->stream));
+    if (stats) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
+        stats->reduce_ms += ms;
+    }
+    return HH_OK;
+}

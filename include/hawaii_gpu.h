@@ -21,6 +21,9 @@
  *   hh_cf_array          the trajectory loop of calculate_correlation_array_and_save
  *                        (src/hawaii.c:2361-2793): one ensemble, CFs at satellite
  *                        temperatures by Boltzmann reweighting (src/hawaii.c:2639-2674)
+ *   hh_prmu              the trajectory loop and spectrum accumulation of
+ *                        calculate_spectral_function_using_prmu_representation_and_save
+ *                        (src/hawaii.c:3148-3977), without requantised rotors
  *   hh_cf_to_sf          idct_cf_to_sf (src/hawaii.c:5569-5601)
  *   hh_m0                calculate_M0 / mpi_calculate_M0 (src/hawaii.c:1423-1491,
  *                        src/hawaii.c:1620-1738)
@@ -193,6 +196,33 @@ typedef struct {
 
 int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const hh_cf_array_params* a,
                 const double* ic_dev, long ntraj, double* cf_dev, double* nstar, hh_cf_stats* stats);
+
+/* ------------------------------------------------------------------ PR/mu spectral function */
+/* calculate_spectral_function_using_prmu_representation_and_save (src/hawaii.c:3148-3977) for
+ * pairs without requantised rotors (ATOM + ATOM, LINEAR_MOLECULE + ATOM), without the Poisson
+ * free-path cut-off and spin weights (both need requantised rotors in the reference).  ic_dev
+ * (SoA [QP][ntraj]) holds R0-injected initial conditions (hh_sample_ics, HH_IC_R0_INJECT).
+ * Each trajectory is integrated forward with its lab dipole sampled at t = dt, 2 dt, ... up to
+ * and including the first sample beyond R0; a trajectory still inside R0 after L samples, or
+ * one whose dipole failed, is not counted (counted[t] = 0: the reference draws another one in
+ * its place).  For every counted trajectory
+ *     sf_dev[k] += SF_COEFF * (-pR / mu) * sum_c |FFT_L(Connes(mu_c))_k|^2,  k < nfreq,
+ * SF_COEFF = Hartree ALU^6 ATU R0^2 dt^2 (src/hawaii.c:3395, 3806-3822); the caller divides
+ * by the number of counted trajectories.  L is the FFT length (a power of two). */
+typedef struct {
+    double rtol, atol;      /* CVODE_TOLERANCE */
+    double dt;              /* sampling_time */
+    double R0;              /* injection distance; a trajectory ends at the first sample beyond it */
+    int L;                  /* MaxTrajectoryLength: samples per trajectory and FFT length */
+    int nfreq;              /* spectral points kept (frequency_array_length), <= L/2 + 1 */
+    int precision;          /* HH_PREC_FP64 or HH_PREC_MIXED */
+    int max_steps;          /* max integrator steps between two samples (<= 0: 50000) */
+    int allow_truncating;   /* count trajectories still inside R0 after L samples
+                               (allow_truncating_trajectories_at_length_limit) */
+} hh_prmu_params;
+
+int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params* p, const double* ic_dev, long ntraj,
+            double* sf_dev, int* counted, long* ncounted, hh_cf_stats* stats);
 
 /* ------------------------------------------------------------------ CF -> SF on the device */
 /* Connes apodization in place on cf_dev, Makhoul IDCT through a 2n-point complex FFT,

@@ -114,3 +114,23 @@ void orc_cf_to_sf(const double *t, double *cf, size_t n, double *nu, double *sf)
         sf[i] = sf[i] * tmax / M_PI * Yscale;
     }
 }
+
+/* One PR/mu trajectory's spectrum (src/hawaii.c:3806-3822): each component's series of L
+ * samples (x[c * L + i], zero after the trajectory) Connes-apodised, transformed (a real radix-2
+ * FFT of length L), squared as gsl_fft_square does (|X_k|^2), summed over the components:
+ * out[k], k < nfreq. */
+void orc_prmu_power(const double *x, size_t L, double dt, size_t nfreq, double *out)
+{
+    double *re = malloc(L * sizeof(double)), *im = malloc(L * sizeof(double));
+    size_t c, k;
+    for (k = 0; k < nfreq; ++k) out[k] = 0.0;
+    for (c = 0; c < 3; ++c) {
+        memcpy(re, x + c * L, L * sizeof(double));
+        memset(im, 0, L * sizeof(double));
+        orc_connes(re, L, dt);
+        orc_fft_radix2(re, im, L, -1);
+        for (k = 0; k < nfreq; ++k) out[k] += re[k] * re[k] + im[k] * im[k];
+    }
+    free(re);
+    free(im);
+}

@@ -74,9 +74,10 @@ static void dipole_of(const orc_system *s, const double *y, double mu[3])
     orc_dipole(s, q, mu);
 }
 
-/* vec (optional): the dipole at every recorded sample, vec[3k + i] */
+/* vec (optional): the dipole at every recorded sample, vec[3k + i]; cf_checks = 0 keeps only
+ * the dipole check (the PR/mu loop, src/hawaii.c:3659-3671) */
 static int half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *c, double *vec, int *len,
-                   orc_traj_stats *st)
+                   orc_traj_stats *st, int cf_checks)
 {
     const int n = orc_qp_size(s);
     fctx fc = {s, 0};
@@ -212,8 +213,8 @@ static int half_cf(const orc_system *s, const orc_cf_params *p, const double *y0
                 if (isnan(mut[0]) || isnan(mut[1]) || isnan(mut[2])) { status = ORC_FAIL_DIPOLE_NAN; goto done; }
                 prev = curr;
                 curr = mu0[0] * mut[0] + mu0[1] * mut[1] + mu0[2] * mut[2];
-                if (fabs(curr) > 1e100) { status = ORC_FAIL_CF_HUGE; goto done; }
-                if (k > 1 && fabs(curr / prev) > 1e10) { status = ORC_FAIL_CF_JUMP; goto done; }
+                if (cf_checks && fabs(curr) > 1e100) { status = ORC_FAIL_CF_HUGE; goto done; }
+                if (cf_checks && k > 1 && fabs(curr / prev) > 1e10) { status = ORC_FAIL_CF_JUMP; goto done; }
                 c[k] = curr;
                 if (vec) {
                     vec[3 * k] = mut[0];
@@ -248,7 +249,19 @@ done:
 int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0, double *c, int *len,
                 orc_traj_stats *st)
 {
-    return half_cf(s, p, y0, c, NULL, len, st);
+    return half_cf(s, p, y0, c, NULL, len, st, 1);
+}
+
+/* The PR/mu trajectory (src/hawaii.c:3578-3672): from y0 forward, the dipole recorded at
+ * t = 0, dt, 2 dt, ... up to and including the first sample with R > p->Rcut, at most p->L
+ * samples; vec[3k + i] holds component i at t = k dt (the reference's series is k >= 1). */
+int orc_prmu_series(const orc_system *s, const orc_cf_params *p, const double *y0, double *vec, int *len,
+                    orc_traj_stats *st)
+{
+    double *c = calloc(p->L, sizeof(double));
+    int status = half_cf(s, p, y0, c, vec, len, st, 0);
+    free(c);
+    return status;
 }
 
 void orc_zimmermann_series_cf(const double *d, int L, double *crln)
@@ -275,10 +288,10 @@ static int traj_cf_zimmermann(const orc_system *s, const orc_cf_params *p, const
     int n = orc_qp_size(s), i, k, status;
     *len_b = 0;
     memset(crln, 0, sizeof(double) * p->L);
-    status = half_cf(s, p, y0, c, vf, len_f, st);
+    status = half_cf(s, p, y0, c, vf, len_f, st, 1);
     if (status == ORC_FAIL_DIPOLE_NAN || status == ORC_FAIL_CF_HUGE || status == ORC_FAIL_CF_JUMP) goto out;
     for (i = 0; i < n; ++i) yb[i] = (i % 2 == 1) ? -y0[i] : y0[i];
-    status = half_cf(s, p, yb, c, vb, len_b, st);
+    status = half_cf(s, p, yb, c, vb, len_b, st, 1);
     if (status == ORC_FAIL_DIPOLE_NAN || status == ORC_FAIL_CF_HUGE || status == ORC_FAIL_CF_JUMP) goto out;
     if (status == ORC_FAIL_STEPS) goto out;
     for (k = 0; k < *len_f; ++k)

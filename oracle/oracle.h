@@ -124,6 +124,11 @@ int orc_half_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
 int orc_traj_cf(const orc_system *s, const orc_cf_params *p, const double *y0,
                 double *crln, int *len_f, int *len_b, orc_traj_stats *st);
 
+/* The PR/mu trajectory: the dipole series vec[3k + i] at t = k dt from y0 (forward) up to and
+ * including the first sample with R > p->Rcut, at most p->L samples; dipole check only. */
+int orc_prmu_series(const orc_system *s, const orc_cf_params *p, const double *y0, double *vec, int *len,
+                    orc_traj_stats *st);
+
 /* The Zimmermann sum over time origins of one dipole series d[3 * (2L - 1)] (d[3c + i] is
  * component i at series index c, mu(0) at c = L - 1):
  * crln[s] = ALU^3 / L sum_{c=0}^{L-1} d(c) . d(c + s)  (src/hawaii.c:2089-2094). */
@@ -145,6 +150,8 @@ void orc_dct(const double *v, size_t n, double *out);
 void orc_connes(double *a, size_t n, double dt);
 /* t[n], cf[n] (apodized in place as the reference does) -> nu[n] (cm^-1), sf[n] */
 void orc_cf_to_sf(const double *t, double *cf, size_t n, double *nu, double *sf);
+/* one PR/mu spectrum: x[3][L] series -> sum_c |FFT(Connes(x_c))_k|^2, k < nfreq */
+void orc_prmu_power(const double *x, size_t L, double dt, size_t nfreq, double *out);
 
 /* ---------------------------------------------------------------- M0 */
 /* Rejection-sampler M0 (calculate_M0, src/hawaii.c:1423-1491): m = mean(mu.mu)*ZeroCoeff*ppf

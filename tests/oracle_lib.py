@@ -142,6 +142,8 @@ def lib():
             "orc_m0": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
             "orc_m2": (None, [P(System), P(Sampler), u32, C.c_long, d, P(C.c_double), P(C.c_double)]),
             "orc_dipole_rate_sq": (d, [P(System), v]),
+            "orc_prmu_series": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(TrajStats)]),
+            "orc_prmu_power": (None, [v, sz, d, sz, v]),
             "orc_pf_analytic": (d, [P(System), d]),
             "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
             "orc_cf_array_weights": (None, [v, C.c_long, d, i, d, i, v, v, v, v, v]),
@@ -285,6 +287,26 @@ def dipole_rate_sq(system, y):
     L = lib()
     return np.array([L.orc_dipole_rate_sq(C.byref(system), ptr(np.ascontiguousarray(r, dtype=np.float64)))
                      for r in np.atleast_2d(y)])
+
+
+def prmu_series(system, y0, dt, L, R0, rtol=1e-12):
+    """The PR/mu dipole series of one trajectory: (status, len, vec [len][3]) with vec[k] the
+    dipole at t = k dt, up to the first sample beyond R0, at most L samples."""
+    p = CFParams(rtol, rtol, dt, L, R0, HALVES_FORWARD, 50000, 0)
+    vec = np.zeros(3 * L)
+    n = C.c_int(0)
+    st = TrajStats()
+    status = lib().orc_prmu_series(C.byref(system), C.byref(p), ptr(np.ascontiguousarray(y0, dtype=np.float64)),
+                                   ptr(vec), C.byref(n), C.byref(st))
+    return status, n.value, vec.reshape(L, 3)[:n.value].copy()
+
+
+def prmu_power(x, dt, nfreq):
+    """sum_c |FFT(Connes(x_c))_k|^2 for x [3][L]."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(nfreq)
+    lib().orc_prmu_power(ptr(x), x.shape[1], dt, nfreq, ptr(out))
+    return out
 
 
 def ref_lib(name):

@@ -148,6 +148,9 @@ def lib():
             "hh_qp_size": (i, [i]),
             "hh_mt_fill": (None, [C.c_uint32, v, C.c_size_t]),
             "hh_sample_ics": (C.c_long, [P(System), P(Sampler), C.c_uint32, C.c_long, v, v, P(C.c_long)]),
+            "hh_sample_ics_from": (C.c_long, [P(System), P(Sampler), C.c_uint32, C.c_long, C.c_long, v, v,
+                                              P(C.c_long)]),
+            "hh_sf_moment": (i, [v, v, C.c_long, d, i, P(d)]),
             "hh_engine_create": (i, [i, P(v)]),
             "hh_engine_destroy": (None, [v]),
             "hh_engine_info": (i, [v, C.c_char_p, C.c_size_t]),
@@ -200,6 +203,26 @@ def sample_ics(system, sampler, seed, ntraj, pes=None):
     n = _check(lib().hh_sample_ics(C.byref(system), C.byref(sampler), seed, ntraj, pes,
                                    ic.ctypes.data_as(C.c_void_p), C.byref(nd)))
     return ic, n, nd.value
+
+
+def sample_ics_from(system, sampler, seed, skip_draws, ntraj, pes=None):
+    """sample_ics continuing the stream after its first skip_draws draws; the returned draw
+    count includes the skipped ones."""
+    ic = np.zeros((ntraj, qp_size(system.system)), dtype=np.float64)
+    nd = C.c_long(0)
+    n = _check(lib().hh_sample_ics_from(C.byref(system), C.byref(sampler), seed, skip_draws, ntraj, pes,
+                                        ic.ctypes.data_as(C.c_void_p), C.byref(nd)))
+    return ic, n, nd.value
+
+
+def sf_moment(nu, sf, n, ntraj=0.0):
+    """compute_Mn_from_sf_using_classical_detailed_balance (hh_sf_moment)."""
+    nu = np.ascontiguousarray(nu, dtype=np.float64)
+    sf = np.ascontiguousarray(sf, dtype=np.float64)
+    out = C.c_double(0)
+    _check(lib().hh_sf_moment(nu.ctypes.data_as(C.c_void_p), sf.ctypes.data_as(C.c_void_p), nu.size, float(ntraj), n,
+                              C.byref(out)))
+    return out.value
 
 
 def pf_analytic(system, T):

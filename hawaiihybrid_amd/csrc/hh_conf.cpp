@@ -128,8 +128,7 @@ class Lexer {
 };
 
 // keywords of the reference that steer parts outside this engine: accepted, ignored
-const char* kIgnored[] = {"APPROXIMATEFREQUENCYMAX", "ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT",
-                          "AVERAGE_TIME_BETWEEN_COLLISIONS", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
+const char* kIgnored[] = {"ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
                           "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
                           "JINI_HISTOGRAM_BINS", "JINI_HISTOGRAM_MAX", "JINI_HISTOGRAM_FILENAME",
                           "JFIN_HISTOGRAM_BINS", "JFIN_HISTOGRAM_MAX", "JFIN_HISTOGRAM_FILENAME",
@@ -268,6 +267,8 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 c->calculation_type = HH_CALC_PHASE_SPACE_M0;
             else if (w == "CALCULATE_PHASE_SPACE_M2")
                 c->calculation_type = HH_CALC_PHASE_SPACE_M2;
+            else if (w == "PR_MU")
+                c->calculation_type = HH_CALC_PR_MU;
             else if (w == "CORRELATION_ARRAY")
                 c->calculation_type = HH_CALC_CORRELATION_ARRAY;
             else
@@ -329,6 +330,10 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             rc = inum(c->hep_m0_niterations);
         } else if (key == "HEP_M0_NPOINTS") {
             rc = inum(c->hep_m0_npoints);
+        } else if (key == "APPROXIMATEFREQUENCYMAX") {
+            rc = num(c->approx_frequency_max);
+        } else if (key == "AVERAGE_TIME_BETWEEN_COLLISIONS") {
+            rc = num(c->average_time_between_collisions);
         } else if (key == "INITIALM2_NPOINTS") {
             rc = inum(c->initialM2_npoints);
         } else if (key == "HEP_M2_NITERATIONS") {
@@ -1057,6 +1062,212 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
     return code;
 }
 
+// Welford mean and its standard error over f[0..n) (calculate_M0 / calculate_M2's running sums),
+// times coeff
+void welford(const std::vector<double>& f, double coeff, double* m_out, double* s_out)
+{
+    double m = 0.0, q = 0.0;
+    const long n = (long)f.size();
+    for (long k = 0; k < n; ++k) {
+        const double diff = f[k] - m;
+        m += diff / (k + 1.0);
+        q += diff * diff * (k / (k + 1.0));
+    }
+    *m_out = m * coeff;
+    *s_out = std::sqrt(q / n / (n - 1.0)) * coeff;
+}
+
+// CALCULATION_TYPE = PR_MU: calculate_spectral_function_using_prmu_representation_and_save
+// (src/hawaii.c:3148-3977) without requantised rotors.  The stream serves the preliminary M0
+// (initialM0_npoints) and M2 (initialM2_npoints) first, then each iteration draws R0-injected
+// initial conditions until TOTAL_TRAJECTORIES / NITERATIONS of them have ended beyond R0 within
+// MaxTrajectoryLength samples (the reference draws a new one in place of each that did not);
+// the SF file is rewritten after every iteration.
+int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+{
+    // the reference's checks (src/hawaii.c:3160-3168, 1529-1530)
+    if (c->max_trajectory_length <= 0 || c->sampling_time <= 0 || c->cvode_tolerance <= 0 ||
+        c->total_trajectories <= 0 || c->niterations < 1 || !(c->R0 > 0) || !(c->approx_frequency_max > 0) ||
+        !c->sf_filename[0])
+        return fail(HH_E_INVALID, "MaxTrajectoryLength, sampling_time, cvode_tolerance, total_trajectories, niterations, "
+                                  "R0, ApproximateFrequencyMax and sf_filename must be set");
+    if (c->average_time_between_collisions > 0)
+        return fail(HH_E_UNSUPPORTED, "the Poisson free-path cut-off (average_time_between_collisions) needs a "
+                                      "requantised rotor, which is not part of this engine");
+    if (c->initialM0_npoints < 2 || c->initialM2_npoints < 2)
+        return fail(HH_E_INVALID, "PR_MU runs the preliminary M0 and M2: initialM0_npoints and initialM2_npoints "
+                                  "must be set");
+    if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
+        return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
+    hh_system sys{};
+    int rc = system_of(c, &sys);
+    if (rc < 0) return rc;
+    const long L = c->max_trajectory_length;
+    const double dt = c->sampling_time, T = c->temperature;
+    const double ATU = 2.4188843265857e-17, LightSpeed = 2.99792458e8, LightSpeed_cm = 2.99792458e10;   // src/constants.h
+    const double frequency_step = 1.0 / (dt * ATU) / LightSpeed_cm / (double)L;
+    const double theoretical_frequency_max = 1.0 / (LightSpeed * 100.0 * dt * ATU) / 2.0;
+    if (c->approx_frequency_max > theoretical_frequency_max)
+        return fail(HH_E_INVALID, "Requested maximum frequency (%.3e) should be less than 1/2 of the maximum signal "
+                                  "frequency, which is (%.3e)", c->approx_frequency_max, theoretical_frequency_max);
+    const int nfreq = (int)(c->approx_frequency_max / frequency_step) + 1;
+    const double max_frequency = frequency_step * (nfreq - 1);
+    Series sf;
+    sf.sf = true;
+    sf.T = T;
+    sf.x.resize(nfreq);
+    sf.y.assign(nfreq, 0.0);
+    const double nu_step = nfreq > 1 ? (max_frequency - 0.0) / (nfreq - 1) : 0.0;   // linspace (src/hawaii.c:3994-4012)
+    for (int k = 0; k < nfreq; ++k) sf.x[k] = 0.0 + nu_step * k;
+    if (!o.quiet) {
+        std::printf("Calculating spectral function using pr/mu representation at T = %.2f\n", T);
+        std::printf("Theoretical maximum frequency with provided parameters: %.3e cm-1\n", theoretical_frequency_max);
+        std::printf("The frequency step with provided parameters:            %.3e cm-1\n", frequency_step);
+        std::printf("The spectral function will be calculated in the range [0 .. %.3e cm-1] for %d points\n",
+                    max_frequency, nfreq);
+    }
+    const int ps = c->pair_state ? c->pair_state : HH_PAIR_FREE;
+    double ppf = c->ppf;
+    if (ppf <= 0) {
+        // src/hawaii.c:3424-3444: 15 iterations of 2e6 points
+        const double pf_an = hh_pf_analytic(&sys, T);
+        if (pf_an < 0) return (int)HH_E_INVALID;
+        hh_vegas_params vp{HH_INTEGRAND_PF, ps, T, c->sampler_Rmin, c->sampler_Rmax, 15, 2000000L, o.seed ? o.seed : 1u};
+        hh_vegas_result vr{};
+        rc = hh_vegas(e, &sys, &vp, &vr);
+        if (rc < 0) return rc;
+        ppf = vr.value / pf_an;
+        if (!o.quiet) {
+            std::printf("Full partition function Q/V = %.12lf\n", pf_an);
+            std::printf("Partial partition function Qp/V = %.12lf\n", vr.value);
+            std::printf("Ratio of partial partition to full partition function: %.5e\n\n", ppf);
+        }
+    }
+    const uint32_t seed = o.seed ? o.seed : good_seed();
+    if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
+    const int qp = hh_qp_size(sys.system);
+    const long m0n = c->initialM0_npoints, m2n = c->initialM2_npoints;
+    const long n_iter = c->total_trajectories / c->niterations;
+    if (n_iter <= 0) return fail(HH_E_INVALID, "total_trajectories < niterations");
+    // preliminary moments from the rejection sampler's points
+    hh_sampler smp{HH_IC_REJECTION, ps, T, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, 0.0};
+    std::vector<double> ic((size_t)std::max(m0n + m2n, n_iter) * qp);
+    long draws = 0;
+    long lrc = hh_sample_ics_from(&sys, &smp, seed, 0, m0n + m2n, nullptr, ic.data(), &draws);
+    if (lrc < 0) return (int)lrc;
+    const long nmax = std::max(m0n + m2n, n_iter);
+    double *ic_dev = nullptr, *out_dev = nullptr, *sf_dev = nullptr;
+    std::vector<double> soa((size_t)nmax * qp);
+    auto upload = [&](long first, long n) -> int {
+        for (long t = 0; t < n; ++t)
+            for (int i = 0; i < qp; ++i) soa[(size_t)i * n + t] = ic[(size_t)(first + t) * qp + i];
+        if (hipMemcpy(ic_dev, soa.data(), (size_t)n * qp * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(HH_E_DEVICE, "upload of initial conditions failed");
+        return (int)HH_OK;
+    };
+    int code = HH_OK;
+    do {
+        if (hipMalloc(&ic_dev, (size_t)nmax * qp * sizeof(double)) != hipSuccess ||
+            hipMalloc(&out_dev, (size_t)3 * std::max(m0n, m2n) * sizeof(double)) != hipSuccess ||
+            hipMalloc(&sf_dev, (size_t)nfreq * sizeof(double)) != hipSuccess ||
+            hipMemset(sf_dev, 0, (size_t)nfreq * sizeof(double)) != hipSuccess) {
+            code = fail(HH_E_NOMEM, "device allocation failed");
+            break;
+        }
+        double prelim_m0 = 0.0, prelim_m0s = 0.0, prelim_m2 = 0.0, prelim_m2s = 0.0;
+        {
+            if ((code = upload(0, m0n)) < 0) break;
+            if ((code = hh_eval(e, &sys, HH_EVAL_DIPOLE, ic_dev, m0n, out_dev)) < 0) break;
+            std::vector<double> d((size_t)3 * m0n), f((size_t)m0n);
+            if (hipMemcpy(d.data(), out_dev, d.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            for (long k = 0; k < m0n; ++k) f[k] = d[k] * d[k] + d[m0n + k] * d[m0n + k] + d[2 * m0n + k] * d[2 * m0n + k];
+            welford(f, kZeroCoeff * ppf, &prelim_m0, &prelim_m0s);
+            if ((code = upload(m0n, m2n)) < 0) break;
+            if ((code = hh_eval(e, &sys, HH_EVAL_DIPOLE_RATE, ic_dev, m2n, out_dev)) < 0) break;
+            f.resize(m2n);
+            if (hipMemcpy(f.data(), out_dev, f.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            welford(f, kSecondCoeff * ppf, &prelim_m2, &prelim_m2s);
+            if (!o.quiet) {
+                std::printf("M0 = %.10e +/- %.10e [%.10e ... %.10e]\n", prelim_m0, prelim_m0s, prelim_m0 - prelim_m0s,
+                            prelim_m0 + prelim_m0s);
+                std::printf("Error: %.3f%%\n\n", prelim_m0s / prelim_m0 * 100.0);
+                std::printf("M2 = %.10e +/- %.10e [%.10e ... %.10e]\n", prelim_m2, prelim_m2s, prelim_m2 - prelim_m2s,
+                            prelim_m2 + prelim_m2s);
+                std::printf("Error: %.3f%%\n\n", prelim_m2s / prelim_m2 * 100.0);
+            }
+        }
+        hh_prmu_params pp{};
+        const bool mixed = o.precision == HH_PREC_MIXED;
+        pp.rtol = pp.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+        pp.dt = dt;
+        pp.R0 = c->R0;
+        pp.L = (int)L;
+        pp.nfreq = nfreq;
+        pp.precision = mixed ? HH_PREC_MIXED : HH_PREC_FP64;
+        pp.max_steps = 50000;
+        hh_sampler inj{HH_IC_R0_INJECT, ps, T, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, c->R0};
+        std::vector<int> counted((size_t)n_iter);
+        double ntraj = 0.0;
+        for (long iter = 0; iter < c->niterations && code == HH_OK; ++iter) {
+            long need = n_iter;
+            int idle = 0;
+            while (need > 0 && code == HH_OK) {
+                lrc = hh_sample_ics_from(&sys, &inj, seed, draws, need, nullptr, ic.data(), &draws);
+                if (lrc < 0) {
+                    code = (int)lrc;
+                    break;
+                }
+                if ((code = upload(0, need)) < 0) break;
+                long nc = 0;
+                hh_cf_stats st;
+                if ((code = hh_prmu(e, &sys, &pp, ic_dev, need, sf_dev, counted.data(), &nc, &st)) < 0) break;
+                if (nc == 0 && ++idle >= 100) {
+                    code = fail(HH_E_INVALID, "no trajectory ended beyond R0 within %ld samples in 100 draws; "
+                                              "increase MaxTrajectoryLength", L);
+                    break;
+                }
+                if (nc < need && !o.quiet)
+                    std::printf("INFO: %ld trajectories exceed maximum length; drawing new ones in their place\n",
+                                need - nc);
+                need -= nc;
+            }
+            if (code != HH_OK) break;
+            ntraj += (double)n_iter;
+            if (hipMemcpy(sf.y.data(), sf_dev, nfreq * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+                code = fail(HH_E_DEVICE, "download failed");
+                break;
+            }
+            double m0_est = 0.0, m2_est = 0.0;
+            (void)hh_sf_moment(sf.x.data(), sf.y.data(), nfreq, ntraj, 0, &m0_est);
+            (void)hh_sf_moment(sf.x.data(), sf.y.data(), nfreq, ntraj, 2, &m2_est);
+            if (!o.quiet) {
+                std::printf("ITERATION %ld/%ld: accumulated %d trajectories. Saving temporary result to '%s'\n", iter + 1,
+                            c->niterations, (int)ntraj, c->sf_filename);
+                std::printf("M0 ESTIMATE FROM SF: %.5e, PRELIMINARY M0 ESTIMATE: %.5e, diff: %.3f%%\n", m0_est,
+                            prelim_m0, (m0_est - prelim_m0) / prelim_m0 * 100.0);
+                std::printf("M2 ESTIMATE FROM SF: %.5e, PRELIMINARY M2 ESTIMATE: %.5e, diff: %.3f%%\n\n", m2_est,
+                            prelim_m2, (m2_est - prelim_m2) / prelim_m2 * 100.0);
+                std::fflush(stdout);
+            }
+            // written normalised by the trajectory count (write_spectral_function_ext, src/hawaii.c:4230-4265)
+            Series out = sf;
+            out.ntraj = ntraj;
+            for (int k = 0; k < nfreq; ++k) out.y[k] = sf.y[k] / ntraj;
+            code = write_series(c->sf_filename, out);
+        }
+    } while (false);
+    if (ic_dev) (void)hipFree(ic_dev);
+    if (out_dev) (void)hipFree(out_dev);
+    if (sf_dev) (void)hipFree(sf_dev);
+    return code;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1088,6 +1299,8 @@ int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
         rc = run_phase_space_m0(e, conf, o);
     } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M2) {
         rc = run_phase_space_m2(e, conf, o);
+    } else if (conf->calculation_type == HH_CALC_PR_MU) {
+        rc = run_prmu(e, conf, o);
     } else if (conf->calculation_type == HH_CALC_CORRELATION_ARRAY) {
         rc = run_correlation_array(e, conf, o);
     } else {

@@ -16,8 +16,8 @@
 #include "hh_internal.h"
 
 extern "C" __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp,
-                                                                        uint32_t seed, long ntraj, hh_pes_fn pes,
-                                                                        double* ic, long* n_draws);
+                                                                        uint32_t seed, long skip, long ntraj,
+                                                                        hh_pes_fn pes, double* ic, long* n_draws);
 
 namespace hh {
 namespace {
@@ -240,7 +240,14 @@ int hh_qp_size(int system)
 long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
                    double* ic, long* n_draws)
 {
+    return hh_sample_ics_from(sys, smp, seed, 0, ntraj, pes, ic, n_draws);
+}
+
+long hh_sample_ics_from(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long skip_draws, long ntraj,
+                        hh_pes_fn pes, double* ic, long* n_draws)
+{
     if (!sys || !smp || !ic || ntraj < 0) return fail(HH_E_INVALID, "hh_sample_ics: null argument or ntraj < 0");
+    if (skip_draws < 0) return fail(HH_E_INVALID, "hh_sample_ics_from: skip_draws < 0");
     if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_N2AR)
         return fail(HH_E_UNSUPPORTED, "hh_sample_ics: system %d has no sampler", sys->system);
     if (smp->mode != HH_IC_REJECTION && smp->mode != HH_IC_R0_INJECT && smp->mode != HH_IC_PHASE_SPACE)
@@ -253,7 +260,7 @@ long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, l
         return fail(HH_E_INVALID, "sampler_Rmin/sampler_Rmax must satisfy 0 < Rmin < Rmax");
     if (smp->mode == HH_IC_REJECTION && !(smp->pesmin < 0 || smp->pesmin > 0))
         return fail(HH_E_INVALID, "pesmin must be non-zero in rejection mode");
-    return hh_sample_ics_impl(sys, smp, seed, ntraj, pes, ic, n_draws);
+    return hh_sample_ics_impl(sys, smp, seed, skip_draws, ntraj, pes, ic, n_draws);
 }
 
 int hh_engine_create(int device, hh_engine** out)

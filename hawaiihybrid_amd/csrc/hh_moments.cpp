@@ -93,3 +93,25 @@ int hh_cf_moment(const double* cf, long len, double dt, double ntraj, int n, dou
     *out = -kSecondCoeff * (double)(d2 / ((long double)dt * dt)) * scale;
     return HH_OK;
 }
+
+// compute_Mn_from_sf_using_classical_detailed_balance (src/hawaii.c:5981-6013):
+// Mn = 2 Moment_SF_Coeff * integral of sf(nu) nu^n d nu (cm^-1) by the composite Simpson 3/8
+// weights of integrate_composite_simpson (src/hawaii.c:5926-5943): 1, 3, 3, 2, 3, 3, 2, ..., 1.
+int hh_sf_moment(const double* nu, const double* sf, long len, double ntraj, int n, double* out)
+{
+    if (!nu || !sf || !out || len < 2) return hh::fail(HH_E_INVALID, "hh_sf_moment: nu, sf, out and len >= 2 required");
+    if (n < 0) return hh::fail(HH_E_INVALID, "hh_sf_moment: order must be non-negative");
+    if (n % 2 == 1) {
+        *out = 0.0;
+        return HH_OK;
+    }
+    const double NL = 2.686780111e25;          // src/constants.h:22
+    const double HBar = 1.054571817e-34;       // src/constants.h:21
+    const double coeff = (8.0 * M_PI * M_PI * M_PI) * NL * NL / 3.0 / HBar;   // Moment_SF_Coeff
+    const double h = nu[1] - nu[0];
+    double sum = sf[0] * std::pow(nu[0], n) + sf[len - 1] * std::pow(nu[len - 1], n);
+    for (long j = 1; j < len - 1; ++j) sum += (j % 3 == 0 ? 2.0 : 3.0) * (sf[j] * std::pow(nu[j], n));
+    *out = 2.0 * coeff * (sum * 3.0 * h / 8.0);
+    if (ntraj > 0) *out /= ntraj;
+    return HH_OK;
+}

@@ -216,12 +216,13 @@ void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n)
     for (size_t i = 0; i < n; ++i) out[i] = mt.next();
 }
 
-__attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
-                        double* ic, long* n_draws)
+__attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long skip,
+                                                              long ntraj, hh_pes_fn pes, double* ic, long* n_draws)
 {
     const Sys s{sys->system, sys->mu, sys->I, pes};
     const int qp = s.linear() ? 10 : 6;
     Mt mt(seed);
+    for (long i = 0; i < skip; ++i) (void)mt.drand();   // one word per draw: the stream after `skip` draws
     long accepted = 0, candidates = 0;
     double y[10];
     while (accepted < ntraj) {

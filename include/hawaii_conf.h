@@ -19,6 +19,11 @@
  *                                           of TEMPERATURES (src/driver.c:3389-3445)
  *   CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M2   the VEGAS M2 integral at TEMPERATURE
  *                                           (src/driver.c:3344-3388)
+ *   CALCULATION_TYPE = PR_MU                calculate_spectral_function_using_prmu_...
+ *                                           (src/hawaii.c:3148-3977) for ATOM + ATOM and
+ *                                           LINEAR_MOLECULE + ATOM: preliminary M0 and M2, then
+ *                                           R0-injected trajectories, SF_FILENAME rewritten after
+ *                                           each iteration
  *   CALCULATION_TYPE = PROCESSING           the stack machine of run_processing
  *                                           (src/driver.c:2956-3081) restricted to READ_CF,
  *                                           READ_SF, CF_TO_SF, CMP, DUP, DUP2, DROP, DROP2,
@@ -39,7 +44,7 @@ extern "C" {
 #endif
 
 enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3,
-       HH_CALC_CORRELATION_ARRAY = 4, HH_CALC_PHASE_SPACE_M2 = 5 };
+       HH_CALC_CORRELATION_ARRAY = 4, HH_CALC_PHASE_SPACE_M2 = 5, HH_CALC_PR_MU = 6 };
 enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF };
@@ -90,6 +95,9 @@ typedef struct {
     char cf_filenames[64][HH_CONF_PATH];
     long initialM2_npoints;                      /* rejection-sampler M2 before the trajectories */
     long hep_m2_niterations, hep_m2_npoints;     /* VEGAS M2 (0: the reference's 12 x 10^6) */
+    /* PR_MU (src/driver.c:1180-1190) */
+    double approx_frequency_max;                 /* APPROXIMATEFREQUENCYMAX, cm^-1 */
+    double average_time_between_collisions;      /* a.t.u.; > 0 (Poisson cut-off) is not provided */
 } hh_conf;
 
 /* Parse a configuration file (no device use).  Unknown keywords and unsupported PROCESSING

@@ -34,6 +34,8 @@
  *   hh_pf_analytic       analytic_full_partition_function_by_V (src/hawaii.c:5071-5090)
  *   hh_cf_moment         compute_Mn_from_cf_using_classical_detailed_balance
  *                        (src/hawaii.c:5945-5979)
+ *   hh_sf_moment         compute_Mn_from_sf_using_classical_detailed_balance
+ *                        (src/hawaii.c:5981-6013)
  *   hh_pipnn_*           the N2-Ar PIP-NN plugin surfaces (PES-IDS/ai_pes_n2_ar_pip_nn.cpp,
  *                        PES-IDS/ai_ids_n2_ar_pip_nn.cpp) as batched device evaluations
  * The .conf driver front end is declared in hawaii_conf.h.
@@ -104,6 +106,13 @@ typedef struct {
  * host surface.  Returns the number of candidates drawn (>= ntraj) or a negative code. */
 long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long ntraj, hh_pes_fn pes,
                    double* ic_rowmajor, long* n_draws);
+
+/* The same stream continued: the first skip_draws draws of the seeded stream are passed over
+ * (the draws of an earlier call, its *n_draws), so consecutive calls sample as one loop would,
+ * the preliminary moments' points first and then the trajectories' (src/hawaii.c:3443-3490).
+ * *n_draws counts from the start of the stream, skipped draws included. */
+long hh_sample_ics_from(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long skip_draws, long ntraj,
+                        hh_pes_fn pes, double* ic_rowmajor, long* n_draws);
 
 /* ------------------------------------------------------------------ engine */
 typedef struct hh_engine hh_engine;
@@ -284,6 +293,12 @@ double hh_pf_analytic(const hh_system* sys, double T);
  * difference over 21 points (len >= 11) or 9 points (len >= 5), cf even in t, sample step dt.
  * ntraj > 0 marks an unnormalised CF (divided by ntraj); ntraj <= 0 a normalised one. */
 int hh_cf_moment(const double* cf, long len, double dt, double ntraj, int n, double* out);
+
+/* compute_Mn_from_sf_using_classical_detailed_balance (src/hawaii.c:5981-6013) on a host SF:
+ * Mn = 2 Moment_SF_Coeff * int sf nu^n d nu over the uniform grid nu[len] (cm^-1) by the composite
+ * Simpson 3/8 weights of integrate_composite_simpson (src/hawaii.c:5926-5943); odd n give 0;
+ * ntraj > 0 divides (an unnormalised SF, as the PR/mu loop's estimates). */
+int hh_sf_moment(const double* nu, const double* sf, long len, double ntraj, int n, double* out);
 
 /* ------------------------------------------------------------------ PIP-NN surfaces (N2-Ar) */
 /* The N2-Ar permutation-invariant-polynomial neural-network surfaces

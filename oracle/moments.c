@@ -81,3 +81,16 @@ void orc_cf_array_weights(const double *E, long n, double Tb, int pair_state, do
         }
     }
 }
+
+/* compute_Mn_from_sf_using_classical_detailed_balance (src/hawaii.c:5981-6013) with
+ * integrate_composite_simpson (src/hawaii.c:5926-5943) */
+double orc_sf_moment(const double *nu, const double *sf, size_t len, double ntraj, int n)
+{
+    const double coeff = (8.0 * M_PI * M_PI * M_PI) * 2.686780111e25 * 2.686780111e25 / 3.0 / 1.054571817e-34;
+    double h = nu[1] - nu[0], sum;
+    size_t j;
+    if (n % 2 == 1) return 0.0;
+    sum = sf[0] * pow(nu[0], n) + sf[len - 1] * pow(nu[len - 1], n);
+    for (j = 1; j < len - 1; ++j) sum += (j % 3 == 0 ? 2.0 : 3.0) * (sf[j] * pow(nu[j], n));
+    return 2.0 * coeff * (sum * 3.0 * h / 8.0) / (ntraj > 0 ? ntraj : 1.0);
+}

@@ -175,6 +175,8 @@ double orc_pf_analytic(const orc_system *s, double T);
 /* compute_Mn_from_cf_using_classical_detailed_balance (src/hawaii.c:5945-5979), n = 0 or 2;
  * ntraj > 0 divides (an unnormalised CF).  Returns 0, or -1 when the CF is too short. */
 int orc_cf_moment(const double *cf, size_t len, double dt, double ntraj, int n, double *out);
+/* compute_Mn_from_sf_using_classical_detailed_balance (src/hawaii.c:5981-6013); ntraj > 0 divides */
+double orc_sf_moment(const double *nu, const double *sf, size_t len, double ntraj, int n);
 /* CORRELATION_ARRAY weights per trajectory and satellite temperature (moments.c) */
 void orc_cf_array_weights(const double *E, long n, double Tb, int pair_state, double pesmin, int M,
                           const double *temps, const double *ratios, const int *counted, double *w, double *nstar);

@@ -144,6 +144,7 @@ def lib():
             "orc_dipole_rate_sq": (d, [P(System), v]),
             "orc_prmu_series": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(TrajStats)]),
             "orc_prmu_power": (None, [v, sz, d, sz, v]),
+            "orc_sf_moment": (d, [v, v, sz, d, i]),
             "orc_pf_analytic": (d, [P(System), d]),
             "orc_cf_moment": (i, [v, sz, d, d, i, P(C.c_double)]),
             "orc_cf_array_weights": (None, [v, C.c_long, d, i, d, i, v, v, v, v, v]),

@@ -31,7 +31,8 @@ class Conf(C.Structure):
                 ("nsatellite", C.c_int), ("satellite_temperatures", C.c_double * 64),
                 ("nppf_ratios", C.c_int), ("ppf_ratios", C.c_double * 64),
                 ("ncf_filenames", C.c_int), ("cf_filenames", (C.c_char * 512) * 64),
-                ("initialM2_npoints", C.c_long), ("hep_m2_niterations", C.c_long), ("hep_m2_npoints", C.c_long)]
+                ("initialM2_npoints", C.c_long), ("hep_m2_niterations", C.c_long), ("hep_m2_npoints", C.c_long),
+                ("approx_frequency_max", C.c_double), ("average_time_between_collisions", C.c_double)]
 
 
 def parse_text(tmp_path, text):
@@ -101,6 +102,20 @@ def test_parse_phase_space_m2(tmp_path):
     assert rc < 0 and b"PHASE_SPACE_M0 is not part of this engine" in H.lib().hh_last_error()
 
 
+def test_parse_reference_prmu_config():
+    """The reference's PR_MU example (CO-Ar with a requantised rotor): the PR/mu settings parse;
+    the rotor type is one this engine does not run."""
+    path = os.path.join(REF_CONFIGS, "prmu_co_ar_req+poisson.conf")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    rc, c = parse(path)
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.calculation_type == 6 and c.R0 == 40.0 and c.approx_frequency_max == 150.0
+    assert c.average_time_between_collisions == 2.596e6 and c.max_trajectory_length == 65536
+    assert c.sf_filename == b"SF-PRMU-CO-Ar-300.0.txt" and c.initialM2_npoints == 10000
+    assert c.monomer_type[0] == 3 and c.monomer_type[1] == 1   # HH_MONOMER_OTHER + ATOM
+
+
 def test_parse_reference_correlation_array_config():
     """The reference's CORRELATION_ARRAY example (six satellite temperatures of bound He-Ar)."""
     path = os.path.join(REF_CONFIGS, "CF-B-He-Ar.conf")
@@ -146,7 +161,7 @@ def test_parse_linear_molecule_and_processing(tmp_path):
 
 @pytest.mark.parametrize("text,needle", [
     ("&INPUT\n  FOO = 1\n&END\n", b"x.conf:2: unknown keyword FOO"),
-    ("&INPUT\n  CALCULATION_TYPE = PR_MU\n&END\n", b"PR_MU is not part of this engine"),
+    ("&INPUT\n  CALCULATION_TYPE = PR_MU_ARRAY\n&END\n", b"PR_MU_ARRAY is not part of this engine"),
     ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  FIT_BASELINE()\n&END\n",
      b"FIT_BASELINE is not part of this engine"),
     ("&INPUT\n  TEMPERATURE = \"hot\"\n&END\n", b"TEMPERATURE expects a number"),

@@ -244,3 +244,71 @@ def test_driver_correlation_single_integrates_ppf_when_not_given(tmp_path):
     m = re.search(r"=> PPF ratio: ([0-9.e+-]+)", r.stdout)
     assert m and float(m.group(1)) > 0, r.stdout[-2000:]
     assert "=> M0:" in r.stdout and "=> M2:" in r.stdout
+
+
+def test_driver_prmu_matches_oracle(tmp_path):
+    """CALCULATION_TYPE = PR_MU (He-Ar, R0 = 12): the preliminary M0 and M2 over the stream's
+    first 2 x 100 rejection-sampled points, then R0-injected trajectories from where the stream
+    left off, those still inside R0 after L samples replaced by the next ones; the SF file
+    against the oracle's series and spectra of the same trajectories."""
+    L, dt, R0, n_iter = 1024, 200.0, 12.0, 32
+    (tmp_path / "prmu.conf").write_text(f"""&INPUT
+  CALCULATION_TYPE = PR_MU
+  PAIR_STATE = FREE_AND_METASTABLE
+  TEMPERATURE = 300.0
+  pair_reduced_mass = 6.632039160324215e+03
+  NITERATIONS = 2
+  TOTAL_TRAJECTORIES = {2 * n_iter}
+  CVODE_TOLERANCE = 1e-12
+  sampling_time = {dt}
+  MaxTrajectoryLength = {L}
+  pesmin = -9.424412560638486e-05
+  partial_partition_function_ratio = 1.0
+  sampler_Rmin = 4.0
+  sampler_Rmax = 40.0
+  initialM0_npoints = 100
+  initialM2_npoints = 100
+  R0 = {R0}
+  sf_filename = "SF-prmu.txt"
+  ApproximateFrequencyMax = 300.0
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+&MONOMER
+  MONOMER_TYPE = ATOM
+&END
+""")
+    r = run_driver(["prmu.conf", "--seed", "11"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "ITERATION 2/2: accumulated 64 trajectories" in r.stdout
+    assert "M0 ESTIMATE FROM SF" in r.stdout and "M2 ESTIMATE FROM SF" in r.stdout
+    sh, so = H.he_ar(), O.hear()
+    rej = H.Sampler(H.IC_REJECTION, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 0.0)
+    pts, _, draws = H.sample_ics_from(sh, rej, 11, 0, 200)
+    import re
+    m = re.search(r"^M0 = ([0-9.e+-]+)", r.stdout, re.M)
+    d = np.array([np.dot(v, v) for v in (np.array([O.lib().orc_hear_dip(R)]) for R in pts[:100, 4])])
+    assert float(m.group(1)) == pytest.approx(d.mean() * H.ZERO_COEFF, rel=1e-9)
+    inj = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_HEAR, R0)
+    ic, _, _ = H.sample_ics_from(sh, inj, 11, draws, 200)
+    coeff = 4.3597447222071e-18 * 5.29177210903e-11 ** 6 * 2.4188843265857e-17 * R0 * R0 * dt * dt
+    head, sf = read_columns(tmp_path / "SF-prmu.txt")
+    nfreq = sf.shape[0]
+    assert f"# SPECTRAL FUNCTION LENGTH: {nfreq}" in head and "# AVERAGE OVER 64.00 TRAJECTORIES" in head
+    want, used = np.zeros(nfreq), 0
+    for y0 in ic:
+        status, n, vec = O.prmu_series(so, y0, dt, L + 2, R0)
+        if status != 0 or n > L + 1:
+            continue
+        x = np.zeros((3, L))
+        x[:, :n - 1] = vec[1:n].T
+        want += coeff * (-y0[5] / O.MU_HEAR) * O.prmu_power(x, dt, nfreq)
+        used += 1
+        if used == 2 * n_iter:
+            break
+    assert used == 2 * n_iter
+    want /= used
+    step = 1.0 / (dt * 2.4188843265857e-17) / 2.99792458e10 / L
+    assert np.allclose(sf[:, 0], step * np.arange(nfreq), rtol=1e-9, atol=1e-9)
+    assert np.linalg.norm(sf[:, 1] - want) / np.linalg.norm(want) < 1e-8

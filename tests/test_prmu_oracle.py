@@ -65,3 +65,40 @@ def test_prmu_series_matches_independent_integration():
             mu = np.zeros(3)
             L.orc_dipole(C.byref(s), O.ptr(np.ascontiguousarray(ys[0::2, k])), O.ptr(mu))
             np.testing.assert_allclose(vec[k], mu, rtol=1e-6, atol=1e-9 * np.abs(vec).max())
+
+
+def test_sf_moment_simpson_exact_on_cubics():
+    """integrate_composite_simpson is Simpson's 3/8 rule: exact for nu^2 (and nu^3) when the grid
+    has 3m + 1 points; the engine's and the oracle's restatements agree, odd orders give 0."""
+    import hawaiihybrid_amd as H
+    coeff = (8.0 * np.pi ** 3) * 2.686780111e25 ** 2 / 3.0 / 1.054571817e-34
+    nu = np.linspace(0.0, 150.0, 3 * 40 + 1)
+    sf = np.ones_like(nu)
+    m2 = H.sf_moment(nu, sf, 2)
+    assert m2 == pytest.approx(2 * coeff * 150.0 ** 3 / 3, rel=1e-13)
+    m0 = H.sf_moment(nu, nu.copy(), 0, ntraj=4.0)      # int nu d nu / 4
+    assert m0 == pytest.approx(2 * coeff * 150.0 ** 2 / 2 / 4.0, rel=1e-13)
+    assert H.sf_moment(nu, sf, 1) == 0.0
+    rng = np.random.default_rng(2)
+    y = rng.random(301)
+    x = np.linspace(0.0, 200.0, 301)
+    L = O.lib()
+    for n in (0, 2):
+        assert H.sf_moment(x, y, n, 7.0) == pytest.approx(L.orc_sf_moment(O.ptr(x), O.ptr(y), 301, 7.0, n), rel=1e-14)
+
+
+def test_sample_ics_from_continues_the_stream():
+    """hh_sample_ics_from(skip = the draws of a first call) returns what one longer call returns
+    after the first call's points (R0 injection draws a fixed number of words per point)."""
+    import hawaiihybrid_amd as H
+    sh = H.co2_ar()
+    smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_CO2AR, 20.0)
+    whole, _, _ = H.sample_ics(sh, smp, 77, 50)
+    first, _, d1 = H.sample_ics_from(sh, smp, 77, 0, 20)
+    rest, _, d2 = H.sample_ics_from(sh, smp, 77, d1, 30)
+    assert np.array_equal(first, whole[:20]) and np.array_equal(rest, whole[20:])
+    rej = H.Sampler(H.IC_REJECTION, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_CO2AR, 0.0)
+    a, _, da = H.sample_ics(sh, rej, 5, 40)
+    b, _, db = H.sample_ics_from(sh, rej, 5, 0, 25)
+    c, _, dc = H.sample_ics_from(sh, rej, 5, db, 15)
+    assert np.array_equal(np.vstack([b, c]), a) and dc == da

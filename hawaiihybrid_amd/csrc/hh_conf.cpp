@@ -131,8 +131,7 @@ class Lexer {
 const char* kIgnored[] = {"ODD_J_SPIN_WEIGHT", "EVEN_J_SPIN_WEIGHT", "DJ", "INITIAL_J", "TORQUE_CACHE_LEN", "TORQUE_LIMIT",
                           "NSWITCH_HISTOGRAM_BINS", "NSWITCH_HISTOGRAM_MAX", "NSWITCH_HISTOGRAM_FILENAME",
                           "JINI_HISTOGRAM_BINS", "JINI_HISTOGRAM_MAX", "JINI_HISTOGRAM_FILENAME",
-                          "JFIN_HISTOGRAM_BINS", "JFIN_HISTOGRAM_MAX", "JFIN_HISTOGRAM_FILENAME",
-                          "SPECTRUM_FREQUENCY_MAX"};
+                          "JFIN_HISTOGRAM_BINS", "JFIN_HISTOGRAM_MAX", "JFIN_HISTOGRAM_FILENAME"};
 
 bool ignored(const std::string& k)
 {
@@ -181,8 +180,13 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             // NAME(arg, ...), or a setting KEY = value of the processing stack
             Tok open = lx.next();
             if (open.text == "=") {
-                (void)lx.next();
-                if (!ignored(key)) return err(line, "unknown PROCESSING setting %s", key);
+                const Tok val = lx.next();
+                if (key == "SPECTRUM_FREQUENCY_MAX") {
+                    if (val.kind != Tok::NUMBER) return err(line, "%s expects a number", key);
+                    c->spectrum_frequency_max = val.num;
+                } else if (!ignored(key)) {
+                    return err(line, "unknown PROCESSING setting %s", key);
+                }
                 t = lx.next();
                 continue;
             }
@@ -199,7 +203,9 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 {"READ_CF", HH_PROC_READ_CF, 1}, {"READ_SF", HH_PROC_READ_SF, 1}, {"CF_TO_SF", HH_PROC_CF_TO_SF, 0},
                 {"CMP", HH_PROC_CMP, 0},          {"DUP", HH_PROC_DUP, 0},         {"DUP2", HH_PROC_DUP2, 0},
                 {"DROP", HH_PROC_DROP, 0},        {"DROP2", HH_PROC_DROP2, 0},     {"WRITE_CF", HH_PROC_WRITE_CF, 1},
-                {"WRITE_SF", HH_PROC_WRITE_SF, 1}};
+                {"WRITE_SF", HH_PROC_WRITE_SF, 1}, {"READ_SPECTRUM", HH_PROC_READ_SPECTRUM, 1},
+                {"WRITE_SPECTRUM", HH_PROC_WRITE_SPECTRUM, 1}, {"ALPHA", HH_PROC_ALPHA, 0}, {"D1", HH_PROC_D1, 0},
+                {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0}};
             int found = -1;
             for (const auto& o : ops)
                 if (key == o.name) {
@@ -372,6 +378,7 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
 // ------------------------------------------------------------------ CF / SF files
 struct Series {
     bool sf = false;
+    bool spectrum = false;      // a Spectrum (ALPHA's output, cm^-1 Amagat^-2): x in cm^-1
     std::vector<double> x, y;   // t or nu; normalised values
     double T = 0.0, ntraj = 0.0;
 };
@@ -395,7 +402,10 @@ int write_series(const char* path, const Series& s)
     if (!fp) return fail(HH_E_INVALID, "could not open '%s' for writing", path);
     std::fprintf(fp, "# HAWAII HYBRID v0.1\n# Saved on %s\n# TEMPERATURE: %.2f\n# AVERAGE OVER %.2f TRAJECTORIES\n",
                  now_stamp().c_str(), s.T, s.ntraj);
-    if (s.sf) {
+    if (s.spectrum) {
+        // write_spectrum (src/hawaii.c:4070-4115)
+        for (size_t i = 0; i < s.x.size(); ++i) std::fprintf(fp, "%.10f %.10e\n", s.x[i], s.y[i]);
+    } else if (s.sf) {
         std::fprintf(fp, "# SPECTRAL FUNCTION LENGTH: %zu\n", s.x.size());
         for (size_t i = 0; i < s.x.size(); ++i) std::fprintf(fp, "%.10f   %.10e\n", s.x[i], s.y[i]);
     } else {
@@ -439,14 +449,14 @@ int read_series(const char* path, bool sf, Series& s)
 bool cmp_series(const Series& a, const Series& b, std::string& why)
 {
     char buf[256];
-    if (a.sf != b.sf) return why = "incompatible element types", false;
+    if (a.sf != b.sf || a.spectrum != b.spectrum) return why = "incompatible element types", false;
     if (a.x.size() != b.x.size()) {
         snprintf(buf, sizeof buf, "'len' differs (%zu and %zu)", a.x.size(), b.x.size());
         return why = buf, false;
     }
     if (std::fabs(a.T - b.T) > 1e-2) return why = "'Temperature' differs", false;
     if (std::fabs(a.ntraj - b.ntraj) > 1e-3) return why = "'ntraj' differs", false;
-    const double xtol = a.sf ? 1e-6 : 1e-3;
+    const double xtol = (a.sf || a.spectrum) ? 1e-6 : 1e-3;
     for (size_t i = 0; i < a.x.size(); ++i) {
         if (std::fabs(a.x[i] - b.x[i]) > xtol) {
             snprintf(buf, sizeof buf, "abscissae at index %zu differ (%.6e and %.6e)", i, a.x[i], b.x[i]);
@@ -487,6 +497,117 @@ int cf_to_sf(hh_engine* e, const Series& cf, Series& sf)
     (void)hipFree(nu);
     (void)hipFree(s);
     return rc;
+}
+
+// ------------------------------------------------------------------ desymmetrizations, ALPHA
+constexpr double kPlanck = 6.62607015e-34;       // src/constants.h:20
+constexpr double kBoltzmann = 1.380649e-23;      // src/constants.h:9
+constexpr double kLightSpeedCm = 2.99792458e10;  // src/constants.h:14
+constexpr double kHBarSI = 1.054571817e-34;      // src/constants.h:21
+constexpr double kATUs = 2.4188843265857e-17;    // src/constants.h:12
+
+double h_nu_over_kT(double nu, double T) { return kPlanck * kLightSpeedCm * nu / (kBoltzmann * T); }
+
+// D1 (desymmetrize_d1, src/hawaii.c:5718-5738), D2 (desymmetrize_d2, src/hawaii.c:5740-5765),
+// D3 (desymmetrize_schofield, src/hawaii.c:5696-5716) of an SF
+Series desymmetrize(const Series& sf, int which)
+{
+    Series out = sf;
+    for (size_t i = 0; i < sf.x.size(); ++i) {
+        const double h = h_nu_over_kT(sf.x[i], sf.T);
+        if (which == 1) {
+            out.y[i] = sf.y[i] * 2.0 / (1.0 + std::exp(-h));
+        } else if (which == 2) {
+            out.y[i] = sf.x[i] < 1e-10 ? 0.0 : sf.y[i] * h / (1.0 - std::exp(-h));
+        } else {
+            out.y[i] = sf.y[i] * std::exp(h / 2.0);
+        }
+    }
+    return out;
+}
+
+// Natural cubic spline through (x, y) (the gsl_interp_cspline the reference interpolates with):
+// second derivatives m from the tridiagonal system with m_0 = m_{n-1} = 0.
+struct NaturalSpline {
+    const std::vector<double>& x;
+    const std::vector<double>& y;
+    std::vector<double> m;
+    NaturalSpline(const std::vector<double>& xs, const std::vector<double>& ys) : x(xs), y(ys), m(xs.size(), 0.0)
+    {
+        const size_t n = x.size();
+        if (n < 3) return;
+        std::vector<double> diag(n, 0.0), rhs(n, 0.0), up(n, 0.0);
+        for (size_t i = 1; i + 1 < n; ++i) {
+            const double h0 = x[i] - x[i - 1], h1 = x[i + 1] - x[i];
+            diag[i] = 2.0 * (h0 + h1);
+            up[i] = h1;
+            rhs[i] = 6.0 * ((y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0);
+        }
+        // Thomas algorithm on rows 1 .. n-2 (sub-diagonal h0 of row i = x[i] - x[i-1])
+        for (size_t i = 2; i + 1 < n; ++i) {
+            const double w = (x[i] - x[i - 1]) / diag[i - 1];
+            diag[i] -= w * up[i - 1];
+            rhs[i] -= w * rhs[i - 1];
+        }
+        for (size_t i = n - 2; i >= 1; --i) {
+            m[i] = (rhs[i] - (i + 2 < n ? up[i] * m[i + 1] : 0.0)) / diag[i];
+            if (i == 1) break;
+        }
+    }
+    double operator()(double t) const
+    {
+        const size_t n = x.size();
+        size_t i = (size_t)(std::upper_bound(x.begin(), x.end(), t) - x.begin());
+        i = i == 0 ? 0 : std::min(i - 1, n - 2);
+        const double h = x[i + 1] - x[i], A = (x[i + 1] - t) / h, B = 1.0 - A;
+        return A * y[i] + B * y[i + 1] + ((A * A * A - A) * m[i] + (B * B * B - B) * m[i + 1]) * h * h / 6.0;
+    }
+};
+
+// D4 / D4a: Egelstaff's time t -> sqrt(t^2 + (hbar / 2kT)^2) on the CF (egelstaff_time_transform,
+// src/hawaii.c:5767-5818; D4a renormalised to the CF's value at t = 0, Frommhold), the kept samples
+// zero-padded to a power of two (pad_to_power_of_two, src/hawaii.c:5555-5566), transformed as
+// CF_TO_SF and desymmetrized as D3 (desymmetrize_egelstaff_from_cf / _frommhold_from_cf,
+// src/hawaii.c:5841-5881).
+int egelstaff_sf(hh_engine* e, const Series& cf, bool frommhold, Series& out)
+{
+    if (!(cf.T > 0)) return fail(HH_E_INVALID, "D4/D4a: the CF needs a temperature");
+    const double cc = kHBarSI / kBoltzmann / cf.T / kATUs / 2.0;
+    const size_t n = cf.x.size();
+    if (n < 3 || !(cc < cf.x[n - 1])) return fail(HH_E_INVALID, "D4/D4a: the CF is shorter than hbar/2kT");
+    const NaturalSpline sp(cf.x, cf.y);
+    const double norm = frommhold ? sp(0.0) / sp(cc) : 1.0;
+    Series eg;
+    eg.T = cf.T;
+    eg.ntraj = cf.ntraj;
+    for (size_t i = 0; i < n; ++i) {
+        const double te = std::sqrt(cf.x[i] * cf.x[i] + cc * cc);
+        if (te <= cf.x[n - 1]) {
+            eg.x.push_back(cf.x[i]);
+            eg.y.push_back(norm * sp(te));
+        }
+    }
+    size_t p2 = 1;
+    while (p2 < eg.x.size()) p2 <<= 1;
+    eg.x.resize(p2, 0.0);
+    eg.y.resize(p2, 0.0);
+    Series sf;
+    const int rc = cf_to_sf(e, eg, sf);
+    if (rc < 0) return rc;
+    out = desymmetrize(sf, 3);
+    return HH_OK;
+}
+
+// ALPHA: the absorption coefficient (compute_alpha, src/hawaii.c:5904-5924)
+Series alpha(const Series& sf)
+{
+    const double coeff = (8.0 * M_PI * M_PI * M_PI) * 2.686780111e25 * 2.686780111e25 / 3.0 / kHBarSI;   // Moment_SF_Coeff
+    Series sp = sf;
+    sp.sf = false;
+    sp.spectrum = true;
+    for (size_t i = 0; i < sf.x.size(); ++i)
+        sp.y[i] = coeff * sp.x[i] * (1.0 - std::exp(-h_nu_over_kT(sp.x[i], sf.T))) * sf.y[i];
+    return sp;
 }
 
 int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
@@ -545,10 +666,65 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
             case HH_PROC_DROP2:
                 if ((rc = need(2)) == HH_OK) st.resize(st.size() - 2);
                 break;
+            case HH_PROC_READ_SPECTRUM: {
+                Series s;
+                rc = read_series(p.arg, false, s);
+                s.spectrum = true;
+                if (rc == HH_OK) st.push_back(std::move(s));
+                break;
+            }
+            case HH_PROC_D1:
+            case HH_PROC_D2:
+            case HH_PROC_D3:
+            case HH_PROC_ALPHA: {
+                if ((rc = need(1)) < 0) break;
+                if (!st.back().sf || st.back().spectrum) {
+                    rc = fail(HH_E_INVALID, "PROCESSING call %d expects a spectral function on the stack", i + 1);
+                    break;
+                }
+                if (!(st.back().T > 0)) {
+                    rc = fail(HH_E_INVALID, "PROCESSING call %d: the spectral function has no temperature", i + 1);
+                    break;
+                }
+                st.back() = p.op == HH_PROC_ALPHA ? alpha(st.back())
+                                                  : desymmetrize(st.back(), p.op == HH_PROC_D1 ? 1 : p.op == HH_PROC_D2 ? 2 : 3);
+                break;
+            }
+            case HH_PROC_D4:
+            case HH_PROC_D4A: {
+                if ((rc = need(1)) < 0) break;
+                if (st.back().sf || st.back().spectrum) {
+                    rc = fail(HH_E_INVALID, "D4/D4a expect a correlation function on the stack");
+                    break;
+                }
+                Series sf;
+                rc = egelstaff_sf(e, st.back(), p.op == HH_PROC_D4A, sf);
+                if (rc == HH_OK) st.back() = std::move(sf);
+                break;
+            }
+            case HH_PROC_WRITE_SPECTRUM: {
+                if ((rc = need(1)) < 0) break;
+                if (!st.back().spectrum) {
+                    rc = fail(HH_E_INVALID, "WRITE_SPECTRUM: wrong element type on the stack");
+                    break;
+                }
+                Series sp = st.back();
+                // SPECTRUM_FREQUENCY_MAX truncation (execute_write_spectrum, src/driver.c:2831-2838)
+                if (c->spectrum_frequency_max > 0 && sp.x.size() > 1) {
+                    const size_t keep = (size_t)(c->spectrum_frequency_max / (sp.x[1] - sp.x[0]));
+                    if (sp.x.size() > keep) {
+                        sp.x.resize(keep);
+                        sp.y.resize(keep);
+                    }
+                }
+                rc = write_series(p.arg, sp);
+                st.pop_back();
+                break;
+            }
             case HH_PROC_WRITE_CF:
             case HH_PROC_WRITE_SF:
                 if ((rc = need(1)) < 0) break;
-                if (st.back().sf != (p.op == HH_PROC_WRITE_SF)) {
+                if (st.back().spectrum || st.back().sf != (p.op == HH_PROC_WRITE_SF)) {
                     rc = fail(HH_E_INVALID, "WRITE_%s: wrong element type on the stack", p.op == HH_PROC_WRITE_SF ? "SF" : "CF");
                     break;
                 }

@@ -13,6 +13,10 @@
   cf_sf_hear_1000.npz  the data columns of the reference's own test fixtures
                        tests/CF-F-He-Ar-1000.0.txt and tests/SF-F-He-Ar-1000.0.txt
                        (the CF_TO_SF known-answer pair of tests/convert-cf-to-sf.conf)
+  spectra_hear_1000.npz  every 4th point of the expected spectra of the reference's
+                       tests/desymmetrizations.conf and tests/compute-alpha.conf
+                       (SPCL, SPD1, SPD2, SPD3, SPD4, SPD4a-F-He-Ar-1000.0.txt: ALPHA of the
+                       CF_TO_SF of CF-F-He-Ar-1000.0 without and with each desymmetrization)
 
 Build the reference checkers first: make -C oracle ref
 """
@@ -130,9 +134,19 @@ def cf_sf():
                         sf_nu=sf[:, 0], sf=sf[:, 1], T=np.array([1000.0]), ntraj=np.array([5e8]))
 
 
+def spectra():
+    out = {}
+    for name in ("SPCL", "SPD1", "SPD2", "SPD3", "SPD4", "SPD4a"):
+        sp = read_two_columns(os.path.join(REF, "tests", name + "-F-He-Ar-1000.0.txt"))
+        out["nu"] = sp[::4, 0]
+        out[name] = sp[::4, 1]
+    np.savez_compressed(os.path.join(HERE, "spectra_hear_1000.npz"), step=np.array([4]), **out)
+
+
 if __name__ == "__main__":
     mt_streams()
     hear()
     co2ar()
     cf_sf()
+    spectra()
     print("golden fixtures written to", HERE)

@@ -32,7 +32,8 @@ class Conf(C.Structure):
                 ("nppf_ratios", C.c_int), ("ppf_ratios", C.c_double * 64),
                 ("ncf_filenames", C.c_int), ("cf_filenames", (C.c_char * 512) * 64),
                 ("initialM2_npoints", C.c_long), ("hep_m2_niterations", C.c_long), ("hep_m2_npoints", C.c_long),
-                ("approx_frequency_max", C.c_double), ("average_time_between_collisions", C.c_double)]
+                ("approx_frequency_max", C.c_double), ("average_time_between_collisions", C.c_double),
+                ("spectrum_frequency_max", C.c_double)]
 
 
 def parse_text(tmp_path, text):
@@ -157,6 +158,26 @@ def test_parse_linear_molecule_and_processing(tmp_path):
     assert c.II[0][0] == 2.819827114123505e5 and c.II[0][2] == 0.0
     assert c.nproc == 4 and [c.proc[i].op for i in range(4)] == [0, 4, 2, 9]
     assert c.proc[0].arg == b"a.txt" and c.proc[3].arg == b"b.txt"
+
+
+def test_parse_processing_spectra(tmp_path):
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  SPECTRUM_FREQUENCY_MAX = 500.0
+  READ_CF("a.txt")
+  D4a()
+  ALPHA()
+  WRITE_SPECTRUM("b.txt")
+  READ_SPECTRUM("b.txt")
+  D1()
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    assert c.spectrum_frequency_max == 500.0
+    assert [c.proc[i].op for i in range(c.nproc)] == [0, 17, 12, 11, 10, 13]
 
 
 @pytest.mark.parametrize("text,needle", [

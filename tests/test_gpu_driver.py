@@ -312,3 +312,80 @@ def test_driver_prmu_matches_oracle(tmp_path):
     step = 1.0 / (dt * 2.4188843265857e-17) / 2.99792458e10 / L
     assert np.allclose(sf[:, 0], step * np.arange(nfreq), rtol=1e-9, atol=1e-9)
     assert np.linalg.norm(sf[:, 1] - want) / np.linalg.norm(want) < 1e-8
+
+
+def test_driver_desymmetrizations_and_alpha_known_answers(tmp_path, golden):
+    """The reference's tests/desymmetrizations.conf and tests/compute-alpha.conf through the
+    driver: ALPHA of the CF_TO_SF of CF-F-He-Ar-1000.0 without and with D1, D2, D3, and D4, D4a
+    from the CF, written with WRITE_SPECTRUM and compared with the reference's expected spectra
+    (every 4th point): 1e-9 of the peak everywhere, 1e-4 relative above 1e-6 of the peak (the
+    SF's far tail, where two FFTs of different rounding agree to a few digits only, is raised by
+    D2 and D3's factors growing with frequency).  SPECTRUM_FREQUENCY_MAX truncates the written
+    spectrum."""
+    g = golden("cf_sf_hear_1000.npz")
+    s = golden("spectra_hear_1000.npz")
+    write_ref_format(tmp_path / "cf.txt", g["cf_t"], g["cf"], float(g["T"][0]), float(g["ntraj"][0]), False)
+    (tmp_path / "d.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  DUP()
+    CF_TO_SF()
+    DUP()
+      ALPHA()
+      WRITE_SPECTRUM("SPCL.txt")
+    DUP()
+      D1()
+      ALPHA()
+      WRITE_SPECTRUM("SPD1.txt")
+    DUP()
+      D2()
+      ALPHA()
+      WRITE_SPECTRUM("SPD2.txt")
+    D3()
+    ALPHA()
+    WRITE_SPECTRUM("SPD3.txt")
+  DUP()
+    D4()
+    ALPHA()
+    WRITE_SPECTRUM("SPD4.txt")
+  D4a()
+  ALPHA()
+  DUP()
+    WRITE_SPECTRUM("SPD4a.txt")
+  READ_SPECTRUM("SPD4a.txt")
+  CMP()
+&END
+""")
+    r = run_driver(["d.conf"], str(tmp_path))
+    assert r.returncode == 0 and "CMP: equal" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    for name in ("SPCL", "SPD1", "SPD2", "SPD3", "SPD4", "SPD4a"):
+        head, sp = read_columns(tmp_path / f"{name}.txt")
+        assert "# TEMPERATURE: 1000.00" in head and sp.shape == (65536, 2)
+        got, want = sp[::4, 1], s[name]
+        assert np.max(np.abs(sp[::4, 0] - s["nu"])) < 1e-6
+        peak = np.abs(want).max()
+        assert np.max(np.abs(got - want)) < 1e-9 * peak, name
+        big = np.abs(want) > 1e-6 * peak
+        assert np.max(np.abs(got[big] / want[big] - 1.0)) < 1e-4, name
+    (tmp_path / "d4a.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  SPECTRUM_FREQUENCY_MAX = 1000.0
+  READ_CF("cf.txt")
+  D4a()
+  ALPHA()
+  WRITE_SPECTRUM("SPD4a-cut.txt")
+&END
+""")
+    r = run_driver(["d4a.conf"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _, sp = read_columns(tmp_path / "SPD4a-cut.txt")
+    n = int(1000.0 / (s["nu"][1] / 4))
+    assert sp.shape == (n, 2)
+    want = s["SPD4a"][: (n + 3) // 4]
+    got = sp[::4, 1]
+    peak = np.abs(s["SPD4a"]).max()
+    assert np.max(np.abs(got - want)) < 1e-9 * peak

@@ -205,7 +205,8 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 {"DROP", HH_PROC_DROP, 0},        {"DROP2", HH_PROC_DROP2, 0},     {"WRITE_CF", HH_PROC_WRITE_CF, 1},
                 {"WRITE_SF", HH_PROC_WRITE_SF, 1}, {"READ_SPECTRUM", HH_PROC_READ_SPECTRUM, 1},
                 {"WRITE_SPECTRUM", HH_PROC_WRITE_SPECTRUM, 1}, {"ALPHA", HH_PROC_ALPHA, 0}, {"D1", HH_PROC_D1, 0},
-                {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0}};
+                {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0},
+                {"AVERAGE_CFS", HH_PROC_AVERAGE_CFS, 0}};
             int found = -1;
             for (const auto& o : ops)
                 if (key == o.name) {
@@ -700,6 +701,43 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
                 Series sf;
                 rc = egelstaff_sf(e, st.back(), p.op == HH_PROC_D4A, sf);
                 if (rc == HH_OK) st.back() = std::move(sf);
+                break;
+            }
+            case HH_PROC_AVERAGE_CFS: {
+                // every CF on the stack, weighted by its trajectory count (average_correlation_functions,
+                // src/hawaii.c:4599-4673); the average replaces the whole stack
+                if ((rc = need(1)) < 0) break;
+                Series avg;
+                double wsum = 0.0;
+                for (size_t k = 0; k < st.size() && rc == HH_OK; ++k) {
+                    const Series& cf = st[k];
+                    if (cf.sf || cf.spectrum) {
+                        rc = fail(HH_E_INVALID, "AVERAGE_CFS expects only correlation functions on the stack");
+                    } else if (!(cf.ntraj > 0)) {
+                        rc = fail(HH_E_INVALID, "# of trajectories for correlation function (%zu) is invalid: %.4e", k,
+                                  cf.ntraj);
+                    } else if (k == 0) {
+                        if (cf.x.size() < 2 || !(cf.x[1] - cf.x[0] > 0.0) || !(cf.T > 0))
+                            rc = fail(HH_E_INVALID, "AVERAGE_CFS: the first CF needs two samples and a temperature");
+                        avg = cf;
+                        for (double& v : avg.y) v = 0.0;
+                    } else if (cf.x.size() != avg.x.size()) {
+                        rc = fail(HH_E_INVALID, "expected correlation functions to be of equal length, however cf.len = "
+                                                "%zu and expected len = %zu", cf.x.size(), avg.x.size());
+                    } else if (std::fabs(cf.T - avg.T) > 1e-10) {
+                        rc = fail(HH_E_INVALID, "AVERAGE_CFS: mismatching temperatures %.2e and %.2e", cf.T, avg.T);
+                    } else if (std::fabs((cf.x[1] - cf.x[0]) - (avg.x[1] - avg.x[0])) > 1e-12) {
+                        rc = fail(HH_E_INVALID, "AVERAGE_CFS: the time steps of the correlation functions differ");
+                    }
+                    if (rc != HH_OK) break;
+                    for (size_t j = 0; j < avg.y.size(); ++j) avg.y[j] += cf.ntraj * cf.y[j];
+                    wsum += cf.ntraj;
+                }
+                if (rc != HH_OK) break;
+                for (double& v : avg.y) v /= wsum;
+                avg.ntraj = wsum;
+                st.clear();
+                st.push_back(std::move(avg));
                 break;
             }
             case HH_PROC_WRITE_SPECTRUM: {

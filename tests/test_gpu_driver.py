@@ -389,3 +389,37 @@ def test_driver_desymmetrizations_and_alpha_known_answers(tmp_path, golden):
     got = sp[::4, 1]
     peak = np.abs(s["SPD4a"]).max()
     assert np.max(np.abs(got - want)) < 1e-9 * peak
+
+
+CF4 = [(0.0, 1.0762846557e-33), (200.0, 1.0734463146e-33), (400.0, 1.0650019293e-33), (600.0, 1.0511598707e-33)]
+
+
+def test_driver_average_cfs_known_answer(tmp_path):
+    """The reference's tests/average-cfs.conf: two copies of tests/cf.txt (T 100 K, 3.4e6
+    trajectories; its four rows are CF4) average to tests/cf-average.txt (the same rows over
+    6.8e6 trajectories); a CF without a trajectory count is an error."""
+    x, y = np.array([r[0] for r in CF4]), np.array([r[1] for r in CF4])
+    write_ref_format(tmp_path / "cf.txt", x, y, 100.0, 3.4e6, False)
+    write_ref_format(tmp_path / "cf-average.txt", x, y, 100.0, 6.8e6, False)
+    (tmp_path / "nontraj.txt").write_text("# HAWAII HYBRID v0.1\n# TEMPERATURE: 100.00\n" +
+                                          "".join(f"{a:.2f} {b:.10e}\n" for a, b in CF4))
+    (tmp_path / "avg.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  READ_CF("cf.txt")
+  AVERAGE_CFS()
+  DUP()
+    WRITE_CF("out.txt")
+  READ_CF("cf-average.txt")
+  CMP()
+  READ_CF("cf.txt")
+  READ_CF("nontraj.txt")
+  AVERAGE_CFS()
+&END
+""")
+    r = run_driver(["avg.conf"], str(tmp_path))
+    assert r.returncode != 0 and "CMP: equal" in r.stdout and "is invalid" in r.stderr, r.stdout + r.stderr
+    head, cf = read_columns(tmp_path / "out.txt")
+    assert "# AVERAGE OVER 6800000.00 TRAJECTORIES" in head and np.allclose(cf[:, 1], y, rtol=1e-12)

@@ -207,6 +207,27 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 {"WRITE_SPECTRUM", HH_PROC_WRITE_SPECTRUM, 1}, {"ALPHA", HH_PROC_ALPHA, 0}, {"D1", HH_PROC_D1, 0},
                 {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0},
                 {"AVERAGE_CFS", HH_PROC_AVERAGE_CFS, 0}};
+            if (key == "ADD_SPECTRA") {
+                // named arguments only (n = <int>, wavenumber_step = <float>), kept as "n;step"
+                long n = -1;
+                double step = 0.0;
+                for (size_t k = 0; k < args.size(); k += 3) {
+                    if (k + 2 >= args.size() || args[k + 1].text != "=" || args[k + 2].kind != Tok::NUMBER)
+                        return err(line, "function call %s accepts only named arguments", key);
+                    const std::string name = Lexer::upper(args[k].text);
+                    if (name == "N")
+                        n = (long)args[k + 2].num;
+                    else if (name == "WAVENUMBER_STEP")
+                        step = args[k + 2].num;
+                    else
+                        return err(line, "function call ADD_SPECTRA got unexpected named argument %s", args[k].text);
+                }
+                snprintf(pr.arg, sizeof pr.arg, "%ld;%.17g", n, step);
+                pr.op = HH_PROC_ADD_SPECTRA;
+                ++c->nproc;
+                t = lx.next();
+                continue;
+            }
             int found = -1;
             for (const auto& o : ops)
                 if (key == o.name) {
@@ -738,6 +759,51 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
                 avg.ntraj = wsum;
                 st.clear();
                 st.push_back(std::move(avg));
+                break;
+            }
+            case HH_PROC_ADD_SPECTRA: {
+                // execute_add_spectra (src/driver.c:2474-2621): the top n spectra (all when n < 0)
+                // resampled by natural cubic splines onto 0, step, ... up to the smallest maximum
+                // frequency (step: the largest input step unless given) and summed
+                long n = -1;
+                double step = 0.0;
+                std::sscanf(p.arg, "%ld;%lf", &n, &step);
+                const bool step_set = step > 0.0;
+                const size_t cnt = n < 0 ? st.size() : (size_t)n;
+                if (cnt == 0 || cnt > st.size()) {
+                    rc = fail(HH_E_INVALID, "requested to add %zu spectra but only %zu found on stack", cnt, st.size());
+                    break;
+                }
+                double fmax = 3.4e38;
+                for (size_t k = st.size() - cnt; k < st.size() && rc == HH_OK; ++k) {
+                    const Series& sp = st[k];
+                    if (!sp.spectrum || sp.x.size() < 2) {
+                        rc = fail(HH_E_INVALID, "ADD_SPECTRA expects spectra on the stack");
+                        break;
+                    }
+                    const double s = (sp.x.back() - sp.x.front()) / (double)(sp.x.size() - 1);
+                    const double fb = s * (double)(sp.x.size() - 1);
+                    if (!step_set && s > step) step = s;
+                    if (fmax > fb) fmax = fb;
+                    if (k > st.size() - cnt && sp.T != st[k - 1].T)
+                        rc = fail(HH_E_INVALID, "temperature mismatch detected at spectrum (index: %zu)", k);
+                }
+                if (rc != HH_OK) break;
+                size_t len = (size_t)std::ceil(fmax / step) + 1;
+                while ((double)(len - 1) * step > fmax) --len;
+                Series sum;
+                sum.spectrum = true;
+                sum.T = st.back().T;
+                sum.x.resize(len);
+                sum.y.assign(len, 0.0);
+                for (size_t j = 0; j < len; ++j) sum.x[j] = step * (double)j;   // linspace(0, step (len - 1), len)
+                for (size_t k = st.size() - cnt; k < st.size(); ++k) {
+                    const NaturalSpline spl(st[k].x, st[k].y);
+                    for (size_t j = 0; j < len; ++j) sum.y[j] += spl((double)j * step);
+                    sum.ntraj += st[k].ntraj;
+                }
+                st.resize(st.size() - cnt);
+                st.push_back(std::move(sum));
                 break;
             }
             case HH_PROC_WRITE_SPECTRUM: {

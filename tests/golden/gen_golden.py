@@ -17,6 +17,9 @@
                        tests/desymmetrizations.conf and tests/compute-alpha.conf
                        (SPCL, SPD1, SPD2, SPD3, SPD4, SPD4a-F-He-Ar-1000.0.txt: ALPHA of the
                        CF_TO_SF of CF-F-He-Ar-1000.0 without and with each desymmetrization)
+  add_spectra_hear_100.npz  the data columns of the reference's tests/add-spectra.conf inputs
+                       (SPCL-B- and SPCL-F-He-Ar-100.0-raw.txt) and expected sums
+                       (SPCL-T-He-Ar-100.0-raw.txt, ...-raw-wn-step=0.05.txt)
 
 Build the reference checkers first: make -C oracle ref
 """
@@ -143,10 +146,20 @@ def spectra():
     np.savez_compressed(os.path.join(HERE, "spectra_hear_1000.npz"), step=np.array([4]), **out)
 
 
+def add_spectra():
+    out = {}
+    for key, name in (("B", "SPCL-B-He-Ar-100.0-raw"), ("F", "SPCL-F-He-Ar-100.0-raw"),
+                      ("T", "SPCL-T-He-Ar-100.0-raw"), ("T005", "SPCL-T-He-Ar-100.0-raw-wn-step=0.05")):
+        sp = read_two_columns(os.path.join(REF, "tests", name + ".txt"))
+        out[key + "_nu"], out[key] = sp[:, 0], sp[:, 1]
+    np.savez_compressed(os.path.join(HERE, "add_spectra_hear_100.npz"), **out)
+
+
 if __name__ == "__main__":
     mt_streams()
     hear()
     co2ar()
     cf_sf()
     spectra()
+    add_spectra()
     print("golden fixtures written to", HERE)

@@ -423,3 +423,36 @@ def test_driver_average_cfs_known_answer(tmp_path):
     assert r.returncode != 0 and "CMP: equal" in r.stdout and "is invalid" in r.stderr, r.stdout + r.stderr
     head, cf = read_columns(tmp_path / "out.txt")
     assert "# AVERAGE OVER 6800000.00 TRAJECTORIES" in head and np.allclose(cf[:, 1], y, rtol=1e-12)
+
+
+def test_driver_add_spectra_known_answer(tmp_path, golden):
+    """The reference's tests/add-spectra.conf: the bound and free He-Ar spectra at 100 K summed by
+    ADD_SPECTRA(n=2) and ADD_SPECTRA(wavenumber_step=0.05) (natural cubic spline resampling onto
+    the common grid) against its expected SPCL-T files."""
+    g = golden("add_spectra_hear_100.npz")
+    for key, nt in (("B", 2808000.0), ("F", 500000000.0)):
+        with open(tmp_path / f"{key}.txt", "w") as f:
+            f.write(f"# HAWAII HYBRID v0.1\n# TEMPERATURE: 100.00\n# AVERAGE OVER {nt:.2f} TRAJECTORIES\n")
+            f.writelines(f"{a:.3f} {b:.10e}\n" for a, b in zip(g[key + "_nu"], g[key]))
+    (tmp_path / "add.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_SPECTRUM("B.txt")
+  READ_SPECTRUM("F.txt")
+  DUP2()
+    ADD_SPECTRA(n=2)
+    WRITE_SPECTRUM("T.txt")
+  ADD_SPECTRA(wavenumber_step=0.05)
+  WRITE_SPECTRUM("T005.txt")
+&END
+""")
+    r = run_driver(["add.conf"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    for out, key in (("T.txt", "T"), ("T005.txt", "T005")):
+        head, sp = read_columns(tmp_path / out)
+        assert "# AVERAGE OVER 502808000.00 TRAJECTORIES" in head
+        assert sp.shape[0] == g[key].size
+        assert np.max(np.abs(sp[:, 0] - g[key + "_nu"])) < 1e-9
+        peak = np.abs(g[key]).max()
+        assert np.max(np.abs(sp[:, 1] - g[key])) < 1e-9 * peak, out

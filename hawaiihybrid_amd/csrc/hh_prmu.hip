@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) prmu_accumulate_kernel(const double* __re
 }
 
 constexpr double kSfHartree = 4.3597447222071e-18;   // src/constants.h:10
-constexpr double kSfALU = 5.29177210903e-11;       // src/constants.h:8
+constexpr double kSfALU = 5.29177210903e-11;         // src/constants.h:8
 constexpr double kSfATU = 2.4188843265857e-17;       // src/constants.h:12
 
 }  // namespace

@@ -207,6 +207,19 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 {"WRITE_SPECTRUM", HH_PROC_WRITE_SPECTRUM, 1}, {"ALPHA", HH_PROC_ALPHA, 0}, {"D1", HH_PROC_D1, 0},
                 {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0},
                 {"AVERAGE_CFS", HH_PROC_AVERAGE_CFS, 0}};
+            if (key == "COMPUTE_MN_CLASSICAL_DETAILED_BALANCE" || key == "COMPUTE_MN_QUANTUM_DETAILED_BALANCE") {
+                // one integer argument, positional or named n (src/driver.c:2304-2330)
+                const bool named = args.size() == 3 && Lexer::upper(args[0].text) == "N" && args[1].text == "=";
+                if (!(named || args.size() == 1)) return err(line, "%s expects one argument n", key);
+                const Tok& v = args[named ? 2 : 0];
+                if (v.kind != Tok::NUMBER || v.num != (double)(long)v.num)
+                    return err(line, "%s expects an integer order n", key);
+                snprintf(pr.arg, sizeof pr.arg, "%ld", (long)v.num);
+                pr.op = key == "COMPUTE_MN_QUANTUM_DETAILED_BALANCE" ? HH_PROC_MN_QUANTUM : HH_PROC_MN_CLASSICAL;
+                ++c->nproc;
+                t = lx.next();
+                continue;
+            }
             if (key == "ADD_SPECTRA") {
                 // named arguments only (n = <int>, wavenumber_step = <float>), kept as "n;step"
                 long n = -1;
@@ -759,6 +772,52 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
                 avg.ntraj = wsum;
                 st.clear();
                 st.push_back(std::move(avg));
+                break;
+            }
+            case HH_PROC_MN_CLASSICAL:
+            case HH_PROC_MN_QUANTUM: {
+                // execute_compute_mn_{classical,quantum}_detailed_balance (src/driver.c:2304-2421):
+                // consumes the CF or SF on top; an SF is cut at SPECTRUM_FREQUENCY_MAX first
+                if ((rc = need(1)) < 0) break;
+                const int n = std::atoi(p.arg);
+                Series it = std::move(st.back());
+                st.pop_back();
+                const bool quantum = p.op == HH_PROC_MN_QUANTUM;
+                if (it.spectrum || (quantum && !it.sf)) {
+                    rc = fail(HH_E_INVALID, "spectral moments are computed from %s", quantum ? "an SF" : "a CF or an SF");
+                    break;
+                }
+                double Mn = 0.0;
+                if (it.sf) {
+                    if (c->spectrum_frequency_max > 0 && it.x.size() > 1) {
+                        const size_t keep = (size_t)(c->spectrum_frequency_max / (it.x[1] - it.x[0]));
+                        if (it.x.size() > keep) {
+                            it.x.resize(keep);
+                            it.y.resize(keep);
+                        }
+                    }
+                    if (quantum) {
+                        // compute_Mn_from_sf_using_quantum_detailed_balance (src/hawaii.c:6011-6046)
+                        const double coeff = (8.0 * M_PI * M_PI * M_PI) * 2.686780111e25 * 2.686780111e25 / 3.0 / kHBarSI;
+                        std::vector<double> y(it.x.size());
+                        for (size_t k = 0; k < y.size(); ++k) {
+                            const double e = std::exp(-h_nu_over_kT(it.x[k], it.T));
+                            y[k] = it.y[k] * std::pow(it.x[k], n) * (n % 2 == 0 ? 1.0 + e : 1.0 - e);
+                        }
+                        const double h = it.x[1] - it.x[0];
+                        double sum = y.front() + y.back();
+                        for (size_t k = 1; k + 1 < y.size(); ++k) sum += (k % 3 == 0 ? 2.0 : 3.0) * y[k];
+                        Mn = coeff * (sum * 3.0 * h / 8.0);
+                    } else {
+                        rc = hh_sf_moment(it.x.data(), it.y.data(), (long)it.x.size(), 0.0, n, &Mn);
+                    }
+                } else {
+                    rc = hh_cf_moment(it.y.data(), (long)it.y.size(), it.x.size() > 1 ? it.x[1] - it.x[0] : 0.0, 0.0, n,
+                                      &Mn);
+                }
+                if (rc == HH_OK && !quiet)
+                    std::printf("Spectral moment using %s detailed balance [based on %s] = %.5e\n",
+                                quantum ? "quantum" : "classical", it.sf ? "SF" : "CF", Mn);
                 break;
             }
             case HH_PROC_ADD_SPECTRA: {

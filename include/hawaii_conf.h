@@ -28,7 +28,8 @@
  *                                           (src/driver.c:2956-3081) restricted to READ_CF,
  *                                           READ_SF, READ_SPECTRUM, CF_TO_SF, CMP, DUP, DUP2,
  *                                           DROP, DROP2, WRITE_CF, WRITE_SF, WRITE_SPECTRUM,
- *                                           ALPHA, AVERAGE_CFS, ADD_SPECTRA and the desymmetrizations D1, D2,
+ *                                           ALPHA, AVERAGE_CFS, ADD_SPECTRA, COMPUTE_Mn_CLASSICAL_/
+ *                                           QUANTUM_DETAILED_BALANCE and the desymmetrizations D1, D2,
  *                                           D3, D4, D4a
  * Systems: two ATOM monomers (He-Ar surfaces) or LINEAR_MOLECULE + ATOM (CO2-Ar surfaces).
  * Without PARTIAL_PARTITION_FUNCTION_RATIO the ratio is integrated by VEGAS as the reference does
@@ -51,7 +52,7 @@ enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2,
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF, HH_PROC_READ_SPECTRUM, HH_PROC_WRITE_SPECTRUM,
        HH_PROC_ALPHA, HH_PROC_D1, HH_PROC_D2, HH_PROC_D3, HH_PROC_D4, HH_PROC_D4A, HH_PROC_AVERAGE_CFS,
-       HH_PROC_ADD_SPECTRA };
+       HH_PROC_ADD_SPECTRA, HH_PROC_MN_CLASSICAL, HH_PROC_MN_QUANTUM };
 
 #define HH_CONF_PATH 512
 #define HH_CONF_MAX_PROC 128

@@ -456,3 +456,35 @@ def test_driver_add_spectra_known_answer(tmp_path, golden):
         assert np.max(np.abs(sp[:, 0] - g[key + "_nu"])) < 1e-9
         peak = np.abs(g[key]).max()
         assert np.max(np.abs(sp[:, 1] - g[key])) < 1e-9 * peak, out
+
+
+def test_driver_spectral_moments(tmp_path, golden):
+    """The reference's tests/compute-classical-moments.conf and compute-quantum-moments.conf
+    sequence: M0 from the CF and from its SF agree (CF(0) ZeroCoeff / ALU^3 against the SF's
+    Simpson integral), odd classical orders are 0, and the quantum M0 of the SF (int SF (1 +
+    exp(-h nu / kT))) is printed; n as a named or positional argument."""
+    g = golden("cf_sf_hear_1000.npz")
+    write_ref_format(tmp_path / "cf.txt", g["cf_t"], g["cf"], float(g["T"][0]), float(g["ntraj"][0]), False)
+    (tmp_path / "mn.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  DUP()
+    COMPUTE_Mn_CLASSICAL_DETAILED_BALANCE(n=0)
+  CF_TO_SF()
+  DUP()
+    COMPUTE_Mn_CLASSICAL_DETAILED_BALANCE(0)
+  DUP()
+    COMPUTE_Mn_CLASSICAL_DETAILED_BALANCE(1)
+  COMPUTE_Mn_QUANTUM_DETAILED_BALANCE(n=0)
+&END
+""")
+    r = run_driver(["mn.conf"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    import re
+    vals = re.findall(r"Spectral moment using (\w+) detailed balance \[based on (\w+)\] = ([0-9.e+-]+)", r.stdout)
+    assert [v[:2] for v in vals] == [("classical", "CF"), ("classical", "SF"), ("classical", "SF"), ("quantum", "SF")]
+    m0_cf, m0_sf, m1_sf, q0 = (float(v[2]) for v in vals)
+    assert m0_cf == pytest.approx(float(g["cf"][0]) * H.ZERO_COEFF / 5.29177210903e-11 ** 3, rel=1e-5)
+    assert m0_sf == pytest.approx(m0_cf, rel=1e-3) and m1_sf == 0.0 and q0 > 0

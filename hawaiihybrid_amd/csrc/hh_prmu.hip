@@ -14,10 +14,14 @@
 //
 // On the device the three components of a trajectory go through two complex FFTs of length L:
 // z1 = x + i y gives |X_k|^2 + |Y_k|^2 = (|Z1_k|^2 + |Z1_{L-k}|^2) / 2, z2 = z gives |Z_k|^2
-// (the sign convention of the transform does not enter a power spectrum).  The FFT is a batched
-// radix-2 Stockham transform over global memory, one launch per stage for every series of a
-// chunk of trajectories (coalesced: consecutive threads take consecutive butterflies of one
-// series).  The per-trajectory spectra are weighted and summed in trajectory order.
+// (the sign convention of the transform does not enter a power spectrum).  The FFT is the
+// four-step (Bailey) factorisation L = L1 L2, both factors resident in LDS: with n = n1 L2 + n2
+// and k = k1 + L1 k2,
+//     y[k1][n2] = w_L^(n2 k1) FFT_L1(x[. L2 + n2])[k1],   X[k1 + L1 k2] = FFT_L2(y[k1][.])[k2],
+// one launch for the columns (tiles of adjacent columns, so the strided reads stay in 64 B
+// segments) and one for the rows, two passes over HBM instead of log2(L).  The spectrum is read
+// through the index map k -> (k mod L1) L2 + k / L1.  The per-trajectory spectra are weighted
+// and summed in trajectory order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,36 +58,93 @@ __global__ void __launch_bounds__(256) prmu_pack_kernel(const double* __restrict
     z[(2 * tl + 1) * L + i] = make_double2(w, 0.0);
 }
 
-// one radix-2 Stockham stage (sub-transforms of 2 half_len points) over nseries series of L
-__global__ void __launch_bounds__(256) prmu_fft_stage_kernel(const double2* __restrict__ in, double2* __restrict__ out,
-                                                             long long L, long long half_len, long long nseries)
+constexpr int kFftTile = 4096;   // complex values per workgroup tile: 64 KiB of LDS
+
+// In place in LDS: `cnt` contiguous transforms of length n (a power of two), X_k = sum_j x_j
+// e^{+2 pi i jk/n}: bit reversal, then radix-2 decimation-in-time butterflies.
+__device__ void lds_fft_many(double2* a, int n, int cnt)
 {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long nb = L / 2;
-    if (g >= nseries * nb) return;
-    const long long s = g / nb, j = g % nb;
-    const double2* a_in = in + s * L;
-    double2* o = out + s * L;
-    const long long k = j % half_len, grp = j / half_len;
-    const double2 a = a_in[j], b = a_in[j + nb];
-    double sn, cs;
-    sincospi((double)k / (double)half_len, &sn, &cs);
-    const double2 wb = make_double2(b.x * cs - b.y * sn, b.x * sn + b.y * cs);
-    o[grp * 2 * half_len + k] = make_double2(a.x + wb.x, a.y + wb.y);
-    o[grp * 2 * half_len + k + half_len] = make_double2(a.x - wb.x, a.y - wb.y);
+    int logn = 0;
+    while ((1 << logn) < n) ++logn;
+    for (int i = threadIdx.x; i < n * cnt; i += blockDim.x) {
+        const int c = i >> logn, e = i & (n - 1);
+        int r = 0;
+        for (int b = 0, x = e; b < logn; ++b, x >>= 1) r = (r << 1) | (x & 1);
+        if (r > e) {
+            const double2 t = a[c * n + e];
+            a[c * n + e] = a[c * n + r];
+            a[c * n + r] = t;
+        }
+    }
+    __syncthreads();
+    const int hn = n >> 1;
+    for (int half = 1; half < n; half <<= 1) {
+        for (int j = threadIdx.x; j < hn * cnt; j += blockDim.x) {
+            const int c = j / hn, jj = j % hn, k = jj % half;
+            const int i0 = c * n + (jj / half) * 2 * half + k, i1 = i0 + half;
+            double sn, cs;
+            sincospi((double)k / (double)half, &sn, &cs);
+            const double2 u = a[i0], v = a[i1];
+            const double2 w = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+            a[i0] = make_double2(u.x + w.x, u.y + w.y);
+            a[i1] = make_double2(u.x - w.x, u.y - w.y);
+        }
+        __syncthreads();
+    }
+}
+
+// step 1: C adjacent columns n2 of one series per workgroup; out[k1 L2 + n2] = twiddled FFT_L1
+__global__ void __launch_bounds__(256) prmu_fft_cols_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                            int L1, int L2, int C, long long L)
+{
+    extern __shared__ double2 tile[];   // [C][L1]
+    const int tiles = L2 / C;
+    const long long s = blockIdx.x / tiles;
+    const int n2_0 = (int)(blockIdx.x % tiles) * C;
+    const double2* x = in + s * L;
+    for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
+        const int n1 = i / C, c = i % C;
+        tile[c * L1 + n1] = x[(long long)n1 * L2 + n2_0 + c];
+    }
+    __syncthreads();
+    lds_fft_many(tile, L1, C);
+    double2* y = out + s * L;
+    for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
+        const int k1 = i / C, c = i % C, n2 = n2_0 + c;
+        const double2 v = tile[c * L1 + k1];
+        double sn, cs;
+        sincospi(2.0 * (double)((long long)n2 * k1) / (double)L, &sn, &cs);
+        y[(long long)k1 * L2 + n2] = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+    }
+}
+
+// step 2: R adjacent rows k1 (contiguous) of one series per workgroup, FFT_L2 in place
+__global__ void __launch_bounds__(256) prmu_fft_rows_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                            int L1, int L2, int R, long long L)
+{
+    extern __shared__ double2 tile[];   // [R][L2]
+    const int tiles = L1 / R;
+    const long long s = blockIdx.x / tiles;
+    const long long base = s * L + (long long)(blockIdx.x % tiles) * R * L2;
+    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) tile[i] = in[base + i];
+    __syncthreads();
+    lds_fft_many(tile, L2, R);
+    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) out[base + i] = tile[i];
 }
 
 // pw[t][k] = w[t] * (|X_k|^2 + |Y_k|^2 + |Z_k|^2), k < nfreq (gsl_fft_square, src/hawaii.c:2349-2356)
-__global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restrict__ Z, long long L, int nfreq,
-                                                         long long t0, long long nt, const double* __restrict__ w,
-                                                         double* __restrict__ pw)
+__global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restrict__ Z, long long L, int L1, int L2,
+                                                         int nfreq, long long t0, long long nt,
+                                                         const double* __restrict__ w, double* __restrict__ pw)
 {
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nt * nfreq) return;
-    const long long tl = g / nfreq, k = g % nfreq;
+    const long long tl = g / nfreq, k = g % nfreq, km = (L - k) & (L - 1);
     const double2* z1 = Z + (2 * tl) * L;
     const double2* z2 = Z + (2 * tl + 1) * L;
-    const double2 a = z1[k], b = z1[(L - k) & (L - 1)], c = z2[k];
+    // X[k1 + L1 k2] sits at k1 L2 + k2
+    const long long ik = (k % L1) * L2 + k / L1, ikm = (km % L1) * L2 + km / L1;
+    const double2 a = z1[ik], b = z1[ikm], c = z2[ik];
     const double p = 0.5 * ((a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y)) + (c.x * c.x + c.y * c.y);
     pw[tl * nfreq + k] = w[t0 + tl] * p;
 }
@@ -156,6 +217,10 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
     }
     if (ncounted) *ncounted = n_keep;
     const long long L = p->L;
+    int logL = 0;
+    while ((1LL << logL) < L) ++logL;
+    const int L1 = 1 << ((logL + 1) / 2), L2 = 1 << (logL / 2);   // L = L1 L2, both <= 1024
+    const int C = std::min(kFftTile / L1, L2), R = std::min(kFftTile / L2, L1);
     // chunks of trajectories whose two series per trajectory fit 1 GiB per buffer
     const long long chunk = std::max<long long>(1, std::min<long long>(ntraj, (1LL << 30) / (2 * L * 16)));
     HH_CHECK(e->prmu_a.ensure((size_t)chunk * 2 * L * sizeof(double2)));
@@ -175,14 +240,15 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
         hipLaunchKernelGGL(prmu_pack_kernel, dim3((unsigned)((n * L + 255) / 256)), dim3(256), 0, e->stream, jd.rows,
                            jd.len, keepd, t0, n, cp.L, L, p->dt, a);
         HH_CHECK(hipGetLastError());
-        for (long long half = 1; half < L; half <<= 1) {
-            hipLaunchKernelGGL(prmu_fft_stage_kernel, dim3((unsigned)((2 * n * (L / 2) + 255) / 256)), dim3(256), 0,
-                               e->stream, a, b, L, half, 2 * n);
-            HH_CHECK(hipGetLastError());
-            std::swap(a, b);
-        }
+        // four-step transform a -> b (columns) -> a (rows)
+        hipLaunchKernelGGL(prmu_fft_cols_kernel, dim3((unsigned)(2 * n * (L2 / C))), dim3(256),
+                           (size_t)C * L1 * sizeof(double2), e->stream, a, b, L1, L2, C, L);
+        HH_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(prmu_fft_rows_kernel, dim3((unsigned)(2 * n * (L1 / R))), dim3(256),
+                           (size_t)R * L2 * sizeof(double2), e->stream, b, a, L1, L2, R, L);
+        HH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(prmu_power_kernel, dim3((unsigned)((n * p->nfreq + 255) / 256)), dim3(256), 0, e->stream, a,
-                           L, p->nfreq, t0, n, wd, pw);
+                           L, L1, L2, p->nfreq, t0, n, wd, pw);
         HH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(prmu_accumulate_kernel, dim3((unsigned)((p->nfreq + 255) / 256)), dim3(256), 0, e->stream, pw,
                            n, p->nfreq, sf_dev);

@@ -83,3 +83,27 @@ def test_prmu_rejects_bad_input(eng):
                 H.PrmuParams(1e-12, 1e-12, 200.0, -1.0, 256, 100, 0, 0, 0)):   # R0 <= 0
         with pytest.raises(H.HHError):
             eng.prmu(sh, bad, icd)
+
+
+@pytest.mark.parametrize("L", [4, 8192, 65536])
+def test_prmu_fft_lengths(eng, L):
+    """The four-step transform's factorisations at the edges (L = 4: 2 x 2; 65536: 256 x 256)
+    against the oracle's direct radix-2 FFT of the same series."""
+    sh, so = H.he_ar(), O.hear()
+    dt, R0 = 200.0, 12.0
+    smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_HEAR, R0)
+    ic, _, _ = H.sample_ics(sh, smp, 13, 8)
+    nfreq = min(L // 2 + 1, 3000)
+    p = H.PrmuParams(1e-12, 1e-12, dt, R0, L, nfreq, H.PREC_FP64, 0, 1)   # keep truncated ones (L = 4)
+    sf, counted, nc, _ = eng.prmu(sh, p, eng.to_device_soa(ic))
+    coeff = HARTREE * ALU ** 6 * ATU * R0 * R0 * dt * dt
+    want = np.zeros(nfreq)
+    for t, y0 in enumerate(ic):
+        status, n, vec = O.prmu_series(so, y0, dt, L + 2, R0)
+        assert status == 0 and counted[t] == 1
+        x = np.zeros((3, L))
+        m = min(n - 1, L)
+        x[:, :m] = vec[1:m + 1].T
+        want += coeff * (-y0[5] / O.MU_HEAR) * O.prmu_power(x, dt, nfreq)
+    got = sf.cpu().numpy()
+    assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-9

@@ -363,7 +363,7 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
     for (int j = 0; j < njobs; ++j)
         if (ntraj[j] <= 0 || !ic_dev[j] || !cf_dev[j]) return fail(HH_E_INVALID, "job %d: empty or null buffers", j);
     const int halves = p->halves == HH_HALVES_BOTH ? 2 : 1;
-    std::lock_guard<std::mutex> lock(e->mu);
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     e->last_jobs.clear();
     e->last_pool_jobs.clear();
@@ -507,7 +507,7 @@ extern "C" {
 int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* e_drift, double* rows)
 {
     if (!e) return fail(HH_E_INVALID, "hh_cf_items: engine is NULL");
-    std::lock_guard<std::mutex> lock(e->mu);
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     if (job < 0 || job >= (int)e->last_jobs.size()) return fail(HH_E_NOTFOUND, "no job %d in the last run", job);
     HH_CHECK(hipSetDevice(e->device));
     const hh::JobDesc& jb = e->last_jobs[job];
@@ -552,10 +552,10 @@ int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const
     const double* icp[1] = {ic_dev};
     const long nt[1] = {ntraj};
     double* cfp[1] = {cf_tmp.as<double>()};
+    // held across the run and the read of its rows, so no other call on the engine replaces them
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     int rc = hh_cf_run(e, sys, &q, 1, icp, nt, cfp, stats);
     if (rc < 0) return rc;
-
-    std::lock_guard<std::mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     if (e->last_jobs.size() != 1) return fail(HH_E_INVALID, "hh_cf_array: the engine lost the job's items");
     const hh::JobDesc jd = e->last_jobs[0];
@@ -622,7 +622,7 @@ int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, l
         return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
     if (what < HH_EVAL_PES || what > HH_EVAL_DIPOLE_RATE)
         return fail(HH_E_INVALID, "hh_eval: unknown quantity %d", what);
-    std::lock_guard<std::mutex> lock(e->mu);
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
     HH_CHECK(hh::run_eval(sys->system, sc, what, y_dev, n, out_dev, e->stream));
@@ -635,7 +635,7 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
     if (!e || !cf_dev || !nu_dev || !sf_dev) return fail(HH_E_INVALID, "hh_cf_to_sf: null argument");
     if (n < 2 || (n & (n - 1))) return fail(HH_E_INVALID, "Length of the input array should be a power of 2");
     if (dt <= 0) return fail(HH_E_INVALID, "sampling time must be positive");
-    std::lock_guard<std::mutex> lock(e->mu);
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     HH_CHECK(e->fft_a.ensure(2 * n * sizeof(double2)));
     HH_CHECK(e->fft_b.ensure(2 * n * sizeof(double2)));
@@ -676,7 +676,7 @@ int moment_run(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32
     if (!e || !sys || !smp || !m || !q) return fail(HH_E_INVALID, "hh_m%d: null argument", order);
     int rc = hh::m0_check(sys, smp, npoints, nstreams);
     if (rc < 0) return rc;
-    std::lock_guard<std::mutex> lock(e->mu);
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     HH_CHECK(e->m0_state.ensure(hh::m0_bytes(nstreams)));
     const hh::M0Acc acc = hh::m0_view(e->m0_state.p, nstreams);

@@ -175,7 +175,9 @@ struct hh_engine {
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
     std::vector<std::unique_ptr<hh::LanePool>> pools;
     std::vector<std::array<int, 4>> tuned;    // (system, mixed, block, waves) chosen by pool_tune
-    std::mutex mu;
+    // serialises calls on the engine; recursive so that hh_cf_array and hh_prmu hold it across
+    // their inner run and the read of its per-item results
+    std::recursive_mutex mu;
 };
 
 namespace hh {

@@ -623,7 +623,7 @@ int hh_pipnn_create(hh_engine* e, const hh_pipnn_desc* d, int precision, hh_pipn
         }
     }
 
-    std::lock_guard<std::mutex> lk(e->mu);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
     auto* M = new hh_pipnn;
     M->e = e;
     M->precision = precision;
@@ -671,7 +671,7 @@ int hh_pipnn_eval(hh_pipnn* M, const double* q_dev, long n, double* energy_dev, 
     if (ms) *ms = 0.0;
     if (n == 0) return HH_OK;
     hh_engine* e = M->e;
-    std::lock_guard<std::mutex> lk(e->mu);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     const dim3 grid((unsigned)((n + 63) / 64)), block(64);
     const bool split = M->precision == HH_MLP_BF16X3;

@@ -194,10 +194,10 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
     double* dummy[1] = {sf_dev};   // no CF is reduced for a series job
     const double* icp[1] = {ic_dev};
     const long nt[1] = {ntraj};
+    // held across the run and the read of its rows, so no other call on the engine replaces them
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
     int rc = cf_run(e, sys, &cp, 1, icp, nt, dummy, stats, 2);
     if (rc < 0) return rc;
-
-    std::lock_guard<std::mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     const JobDesc jd = e->last_jobs[0];
     std::vector<int> len(ntraj), status(ntraj), keep(ntraj);

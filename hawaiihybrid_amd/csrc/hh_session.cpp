@@ -152,7 +152,7 @@ void hh_session::run()
             if (in_use + r.bytes > budget) break;
             int rc;
             {
-                std::lock_guard<std::mutex> el(e->mu);
+                std::lock_guard<std::recursive_mutex> el(e->mu);
                 rc = admit(r);
             }
             if (rc == HH_E_BUSY) break;   // ring full: wait for a job to finish
@@ -170,7 +170,7 @@ void hh_session::run()
         }
         {
             // the CF jobs admitted in this round form one group: longest-expected-first across it
-            std::lock_guard<std::mutex> el(e->mu);
+            std::lock_guard<std::recursive_mutex> el(e->mu);
             for (auto& pl : pools)
                 if (hh::pool_order_group(e, *pl) < 0) {
                     // ordering is an optimisation: on failure the items run in queue order
@@ -187,7 +187,7 @@ void hh_session::run()
         std::string err;
         int rc;
         {
-            std::lock_guard<std::mutex> el(e->mu);
+            std::lock_guard<std::recursive_mutex> el(e->mu);
             rc = slice(err);
         }
         lk.lock();
@@ -195,7 +195,7 @@ void hh_session::run()
             // device error: every running request fails; the pools start afresh
             for (auto& r : running) finish(std::move(r), HH_REQ_FAILED, err.c_str());
             running.clear();
-            std::lock_guard<std::mutex> el(e->mu);
+            std::lock_guard<std::recursive_mutex> el(e->mu);
             (void)hipStreamSynchronize(e->stream);
             pools.clear();
             continue;
@@ -207,7 +207,7 @@ void hh_session::run()
             bool done = false;
             int state = HH_REQ_DONE;
             std::string msg;
-            std::lock_guard<std::mutex> el(e->mu);
+            std::lock_guard<std::recursive_mutex> el(e->mu);
             if (r.kind == HH_REQ_CF) {
                 if (r.job->state == 2) {
                     done = true;
@@ -266,7 +266,7 @@ int hh_session_create(hh_engine* e, const hh_session_opts* opts, hh_session** ou
     if (s->opts.slice_ms <= 0) s->opts.slice_ms = 10.0;
     if (s->opts.block != 64 && s->opts.block != 128 && s->opts.block != 256) s->opts.block = 256;
     {
-        std::lock_guard<std::mutex> el(e->mu);
+        std::lock_guard<std::recursive_mutex> el(e->mu);
         HH_CHECK(hipSetDevice(e->device));
         size_t free_b = 0, total_b = 0;
         HH_CHECK(hipMemGetInfo(&free_b, &total_b));
@@ -368,7 +368,7 @@ int hh_session_destroy(hh_session* s)
     }
     if (s->worker.joinable()) s->worker.join();
     {
-        std::lock_guard<std::mutex> el(s->e->mu);
+        std::lock_guard<std::recursive_mutex> el(s->e->mu);
         (void)hipSetDevice(s->e->device);
         (void)hipStreamSynchronize(s->e->stream);
         s->pools.clear();

@@ -258,7 +258,7 @@ extern "C" int hh_vegas(hh_engine* e, const hh_system* sys, const hh_vegas_param
     if (vp->niterations < 2 || vp->npoints < 2)
         return fail(HH_E_INVALID, "hh_vegas: at least 2 iterations of 2 points (the first is discarded)");
     const int D = hh_qp_size(sys->system);
-    std::lock_guard<std::mutex> lk(e->mu);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
     HH_CHECK(hipSetDevice(e->device));
     const int nblocks = (int)std::min<long long>(4096, (vp->npoints + kVegasBlock - 1) / kVegasBlock);
     const size_t width_bins = (size_t)D * kVegasBins;

@@ -60,34 +60,41 @@ __global__ void __launch_bounds__(256) prmu_pack_kernel(const double* __restrict
 
 constexpr int kFftTile = 4096;   // complex values per workgroup tile: 64 KiB of LDS
 
-// In place in LDS: `cnt` contiguous transforms of length n (a power of two), X_k = sum_j x_j
-// e^{+2 pi i jk/n}: bit reversal, then radix-2 decimation-in-time butterflies.
+// In place in LDS: `cnt` transforms of length n (a power of two), X_k = sum_j x_j e^{+2 pi i jk/n}:
+// bit reversal, then radix-2 decimation-in-time butterflies.  INTERLEAVED: element e of
+// sequence c at a[e * cnt + c], consecutive threads on consecutive sequences (the column tiles);
+// else at a[c * n + e], consecutive threads on consecutive butterflies of one sequence (rows).
+// Either way consecutive threads touch consecutive LDS words.
+template <bool INTERLEAVED>
 __device__ void lds_fft_many(double2* a, int n, int cnt)
 {
     int logn = 0;
     while ((1 << logn) < n) ++logn;
+    auto at = [&](int c, int e) -> double2& { return INTERLEAVED ? a[e * cnt + c] : a[c * n + e]; };
     for (int i = threadIdx.x; i < n * cnt; i += blockDim.x) {
-        const int c = i >> logn, e = i & (n - 1);
+        const int c = INTERLEAVED ? i % cnt : i >> logn, e = INTERLEAVED ? i / cnt : i & (n - 1);
         int r = 0;
         for (int b = 0, x = e; b < logn; ++b, x >>= 1) r = (r << 1) | (x & 1);
         if (r > e) {
-            const double2 t = a[c * n + e];
-            a[c * n + e] = a[c * n + r];
-            a[c * n + r] = t;
+            const double2 t = at(c, e);
+            at(c, e) = at(c, r);
+            at(c, r) = t;
         }
     }
     __syncthreads();
     const int hn = n >> 1;
     for (int half = 1; half < n; half <<= 1) {
         for (int j = threadIdx.x; j < hn * cnt; j += blockDim.x) {
-            const int c = j / hn, jj = j % hn, k = jj % half;
-            const int i0 = c * n + (jj / half) * 2 * half + k, i1 = i0 + half;
+            const int c = INTERLEAVED ? j % cnt : j / hn, jj = INTERLEAVED ? j / cnt : j % hn, k = jj % half;
+            const int e0 = (jj / half) * 2 * half + k;
+            double2& p0 = at(c, e0);
+            double2& p1 = at(c, e0 + half);
             double sn, cs;
             sincospi((double)k / (double)half, &sn, &cs);
-            const double2 u = a[i0], v = a[i1];
+            const double2 u = p0, v = p1;
             const double2 w = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
-            a[i0] = make_double2(u.x + w.x, u.y + w.y);
-            a[i1] = make_double2(u.x - w.x, u.y - w.y);
+            p0 = make_double2(u.x + w.x, u.y + w.y);
+            p1 = make_double2(u.x - w.x, u.y - w.y);
         }
         __syncthreads();
     }
@@ -97,21 +104,21 @@ __device__ void lds_fft_many(double2* a, int n, int cnt)
 __global__ void __launch_bounds__(256) prmu_fft_cols_kernel(const double2* __restrict__ in, double2* __restrict__ out,
                                                             int L1, int L2, int C, long long L)
 {
-    extern __shared__ double2 tile[];   // [C][L1]
+    extern __shared__ double2 tile[];   // [L1][C]
     const int tiles = L2 / C;
     const long long s = blockIdx.x / tiles;
     const int n2_0 = (int)(blockIdx.x % tiles) * C;
     const double2* x = in + s * L;
     for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
         const int n1 = i / C, c = i % C;
-        tile[c * L1 + n1] = x[(long long)n1 * L2 + n2_0 + c];
+        tile[n1 * C + c] = x[(long long)n1 * L2 + n2_0 + c];
     }
     __syncthreads();
-    lds_fft_many(tile, L1, C);
+    lds_fft_many<true>(tile, L1, C);
     double2* y = out + s * L;
     for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
         const int k1 = i / C, c = i % C, n2 = n2_0 + c;
-        const double2 v = tile[c * L1 + k1];
+        const double2 v = tile[k1 * C + c];
         double sn, cs;
         sincospi(2.0 * (double)((long long)n2 * k1) / (double)L, &sn, &cs);
         y[(long long)k1 * L2 + n2] = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
@@ -128,7 +135,7 @@ __global__ void __launch_bounds__(256) prmu_fft_rows_kernel(const double2* __res
     const long long base = s * L + (long long)(blockIdx.x % tiles) * R * L2;
     for (int i = threadIdx.x; i < R * L2; i += blockDim.x) tile[i] = in[base + i];
     __syncthreads();
-    lds_fft_many(tile, L2, R);
+    lds_fft_many<false>(tile, L2, R);
     for (int i = threadIdx.x; i < R * L2; i += blockDim.x) out[base + i] = tile[i];
 }
 

@@ -49,7 +49,7 @@ class CFParams(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("Rcut", C.c_double),
                 ("ppf", C.c_double), ("L", C.c_int), ("halves", C.c_int), ("max_steps", C.c_int),
                 ("engine", C.c_int), ("precision", C.c_int), ("block", C.c_int), ("grid", C.c_int),
-                ("zimmermann", C.c_int)]
+                ("zimmermann", C.c_int), ("orbit_rtol", C.c_double)]
 
 
 class CFArrayParams(C.Structure):
@@ -62,14 +62,15 @@ class PrmuParams(C.Structure):
     """hh_prmu_params: the PR/mu spectral function's trajectory and spectrum settings."""
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dt", C.c_double), ("R0", C.c_double),
                 ("L", C.c_int), ("nfreq", C.c_int), ("precision", C.c_int), ("max_steps", C.c_int),
-                ("allow_truncating", C.c_int)]
+                ("allow_truncating", C.c_int), ("orbit_rtol", C.c_double)]
 
 
 class CFStats(C.Structure):
     _fields_ = [("integrate_ms", C.c_double), ("reduce_ms", C.c_double), ("n_items", C.c_longlong),
                 ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
                 ("n_stepfail", C.c_longlong), ("max_e_drift", C.c_double), ("block", C.c_int), ("grid", C.c_int),
-                ("launches", C.c_int)]
+                ("launches", C.c_int), ("n_orbit", C.c_longlong), ("nrhs_orbit", C.c_longlong),
+                ("nsamples_orbit", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -242,9 +243,11 @@ def cf_moment(cf, dt, n, ntraj=0.0):
 
 
 def cf_params(L, dt=200.0, rtol=1e-12, atol=None, Rcut=40.0, ppf=1.0, halves=HALVES_FORWARD, engine=ENGINE_BATCHED,
-              precision=PREC_FP64, max_steps=50000, block=0, grid=0, zimmermann=False):
+              precision=PREC_FP64, max_steps=50000, block=0, grid=0, zimmermann=False, orbit_rtol=0.0):
+    """hh_cf_params; orbit_rtol: mixed precision hands orbiting trajectories over to FP64 at this
+    tolerance (0: 1e-12; negative: off)."""
     return CFParams(rtol, rtol if atol is None else atol, dt, Rcut, ppf, L, halves, max_steps, engine, precision,
-                    block, grid, 1 if zimmermann else 0)
+                    block, grid, 1 if zimmermann else 0, orbit_rtol)
 
 
 class Engine:

@@ -934,6 +934,152 @@ int system_of(const hh_conf* c, hh_system* sys)
     return HH_OK;
 }
 
+// The configured surface plugins must be the system the device integrates: the engine evaluates
+// only its built-in surfaces, so a plugin is accepted when it declares the same system
+// (`int hh_system_id(void)` returning the HH_SYS_* value) or when its host pes_lab / dpes_lab /
+// dipole_lab agree with the built-in surface at 48 fixed points of the sampling region (relative
+// 1e-6 of the largest magnitude, FP64 device evaluation).  A plugin that cannot be loaded or lacks
+// a required symbol is an error, as in the reference's loader (src/driver.c:3083-3094, which
+// exits).
+int check_surfaces(hh_engine* e, const hh_conf* c, const hh_system& sys)
+{
+    const char* names[] = {"He-Ar", "CO2-Ar", "N2-Ar", "CH4-CO2"};
+    const char* name = names[sys.system];
+    const int qp = hh_qp_size(sys.system), nq = qp / 2;
+    constexpr int K = 48;
+    // points: R over [Rmin, min(Rmax, 20)] (the interaction region), angles over their ranges,
+    // from a fixed linear congruential stream; momenta zero (surfaces depend on coordinates only)
+    const double r0 = c->sampler_Rmin > 0 ? c->sampler_Rmin : 4.0;
+    const double r1 = std::max(r0 + 1.0, std::min(c->sampler_Rmax > 0 ? c->sampler_Rmax : 20.0, 20.0));
+    std::vector<double> y((size_t)K * qp, 0.0), q((size_t)K * nq);
+    uint64_t st = 0x9e3779b97f4a7c15ull;
+    auto u = [&] {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        return (double)(st >> 11) / 9007199254740992.0;
+    };
+    for (int k = 0; k < K; ++k)
+        for (int i = 0; i < nq; ++i) {
+            // coordinate order Phi, Theta, R, then the monomer angles (phi, theta[, psi])
+            double v;
+            if (i == 2)
+                v = r0 + (r1 - r0) * u();
+            else if (i == 1 || (i >= 3 && (i - 3) % 2 == 1))
+                v = std::acos(2.0 * u() - 1.0);
+            else
+                v = 2.0 * M_PI * u();
+            q[(size_t)k * nq + i] = v;
+            y[(size_t)(2 * i) * K + k] = v;   // SoA [QP][K], coordinate i at state index 2i
+        }
+    struct Plugin {
+        void* so = nullptr;
+        ~Plugin()
+        {
+            if (so) dlclose(so);
+        }
+    } pot, dip;
+    auto open = [&](const char* path, Plugin& pl) -> int {
+        pl.so = dlopen(path, RTLD_LAZY);
+        if (!pl.so) return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", path, dlerror());
+        if (auto id = reinterpret_cast<int (*)(void)>(dlsym(pl.so, "hh_system_id"))) {
+            const int sid = id();
+            if (sid != sys.system)
+                return fail(HH_E_UNSUPPORTED, "%s declares system %d; the configured monomers are the engine's %s",
+                            path, sid, name);
+            return 1;   // declared: no point check needed
+        }
+        return 0;
+    };
+    double *y_dev = nullptr, *out_dev = nullptr;
+    if (hipMalloc(&y_dev, y.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&out_dev, (size_t)qp * K * sizeof(double)) != hipSuccess) {
+        if (y_dev) (void)hipFree(y_dev);
+        return fail(HH_E_NOMEM, "surface check: device allocation failed");
+    }
+    struct Free {
+        double *a, *b;
+        ~Free()
+        {
+            (void)hipFree(a);
+            (void)hipFree(b);
+        }
+    } fr{y_dev, out_dev};
+    HH_CHECK(hipMemcpy(y_dev, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice));
+    auto device = [&](int what, int rows, std::vector<double>& out) -> int {
+        const int rc = hh_eval(e, &sys, what, y_dev, K, out_dev);
+        if (rc < 0) return rc;
+        out.resize((size_t)rows * K);
+        HH_CHECK(hipMemcpy(out.data(), out_dev, out.size() * sizeof(double), hipMemcpyDeviceToHost));
+        return HH_OK;
+    };
+    if (c->so_potential[0]) {
+        int rc = open(c->so_potential, pot);
+        if (rc < 0) return rc;
+        auto pes = reinterpret_cast<hh_pes_fn>(dlsym(pot.so, "pes_lab"));
+        auto dpes = reinterpret_cast<hh_dpes_fn>(dlsym(pot.so, "dpes_lab"));
+        if (!pes || !dpes) return fail(HH_E_INVALID, "%s must export pes_lab and dpes_lab", c->so_potential);
+        if (rc == 0) {
+            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(pot.so, "pes_init"))) init();
+            std::vector<double> V, F;
+            if ((rc = device(HH_EVAL_PES, 1, V)) < 0 || (rc = device(HH_EVAL_RHS, qp, F)) < 0) return rc;
+            double scale = 0.0, worst = 0.0, worst_g = 0.0, gscale = 0.0;
+            std::vector<double> qq(nq), g(nq);
+            for (int k = 0; k < K; ++k) {
+                for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
+                const double vp = pes(qq.data());
+                dpes(qq.data(), g.data());
+                scale = std::max(scale, std::fabs(V[k]));
+                worst = std::max(worst, std::fabs(vp - V[k]));
+                // the momentum derivatives at zero momenta are -dV/dq
+                for (int i = 0; i < nq; ++i) {
+                    const double gd = -F[(size_t)(2 * i + 1) * K + k];
+                    gscale = std::max(gscale, std::fabs(gd));
+                    worst_g = std::max(worst_g, std::fabs(g[i] - gd));
+                }
+            }
+            if (!(worst <= 1e-6 * scale) || !(worst_g <= 1e-6 * gscale))
+                return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s surface (max |dV| %.3e of %.3e Hartree, "
+                                              "max |d grad| %.3e of %.3e): the device integrates only its built-in surfaces",
+                            c->so_potential, name, worst, scale, worst_g, gscale);
+        }
+    }
+    if (c->so_dipole[0]) {
+        int rc = open(c->so_dipole, dip);
+        if (rc < 0) return rc;
+        auto dipole = reinterpret_cast<hh_dipole_fn>(dlsym(dip.so, "dipole_lab"));
+        if (!dipole) return fail(HH_E_INVALID, "%s must export dipole_lab", c->so_dipole);
+        if (rc == 0) {
+            if (void (*init)(bool) = reinterpret_cast<void (*)(bool)>(dlsym(dip.so, "dipole_init"))) init(false);
+            std::vector<double> M;
+            if ((rc = device(HH_EVAL_DIPOLE, 3, M)) < 0) return rc;
+            double scale = 0.0, worst = 0.0;
+            std::vector<double> qq(nq);
+            for (int k = 0; k < K; ++k) {
+                for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
+                double mu[3];
+                dipole(qq.data(), mu);
+                for (int j = 0; j < 3; ++j) {
+                    scale = std::max(scale, std::fabs(M[(size_t)j * K + k]));
+                    worst = std::max(worst, std::fabs(mu[j] - M[(size_t)j * K + k]));
+                }
+            }
+            if (void (*fin)(void) = reinterpret_cast<void (*)(void)>(dlsym(dip.so, "dipole_free"))) fin();
+            if (!(worst <= 1e-6 * scale))
+                return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s dipole (max |d mu| %.3e of %.3e a.u.): "
+                                              "the device evaluates only its built-in surfaces",
+                            c->so_dipole, name, worst, scale);
+        }
+    }
+    return HH_OK;
+}
+
+// the engine's system for the configured monomers, checked against the configured plugins
+int resolve_system(hh_engine* e, const hh_conf* c, hh_system* sys)
+{
+    const int rc = system_of(c, sys);
+    if (rc < 0) return rc;
+    return check_surfaces(e, c, *sys);
+}
+
 // The M2 estimates from the accumulated CF after each iteration (src/hawaii.c:3070-3090,
 // 2735-2751): the 9-point stencil over the first 5 samples, the 21-point one over 11, each
 // beside the VEGAS M2 (0 where none was integrated).
@@ -958,7 +1104,7 @@ int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
     if (!(c->sampler_Rmin > 0) || !(c->sampler_Rmax > c->sampler_Rmin))
         return fail(HH_E_INVALID, "sampler_Rmin and sampler_Rmax must be set for the phase-space integral");
     hh_system sys{};
-    int rc = system_of(c, &sys);
+    int rc = resolve_system(e, c, &sys);
     if (rc < 0) return rc;
     std::vector<double> Ts(c->temperatures, c->temperatures + c->ntemperatures);
     if (Ts.empty() && c->temperature > 0) Ts.push_back(c->temperature);
@@ -1006,7 +1152,7 @@ int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
         return fail(HH_E_INVALID, "maximum value for R throught integration (sampler_Rmax) must be set");
     if (!(c->temperature > 0)) return fail(HH_E_INVALID, "no temperature is provided to run CALCULATION_PHASE_SPACE_M2");
     hh_system sys{};
-    int rc = system_of(c, &sys);
+    int rc = resolve_system(e, c, &sys);
     if (rc < 0) return rc;
     const double pf_an = hh_pf_analytic(&sys, c->temperature);
     if (pf_an < 0) return (int)HH_E_INVALID;
@@ -1033,7 +1179,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
     {
-        const int rc0 = system_of(c, &sys);
+        const int rc0 = resolve_system(e, c, &sys);
         if (rc0 < 0) return rc0;
     }
     // partition-function ratio, M0 and M2 by VEGAS when the ratio is not given (src/hawaii.c:2829-2880);
@@ -1086,8 +1232,8 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
                 return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
             }
             if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
-        } else if (!o.quiet) {
-            std::printf("note: %s not loadable; the engine's built-in surface drives the sampler\n", c->so_potential);
+        } else {
+            return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_potential, dlerror());
         }
     }
 
@@ -1178,6 +1324,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         hh_cf_params p{};
         const bool mixed = o.precision == HH_PREC_MIXED;
         p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+        p.orbit_rtol = c->cvode_tolerance;   // orbiting pairs: FP64 at the configured tolerance
         p.dt = c->sampling_time;
         p.Rcut = c->Rcut;
         p.ppf = ppf;
@@ -1258,7 +1405,7 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
     {
-        const int rc0 = system_of(c, &sys);
+        const int rc0 = resolve_system(e, c, &sys);
         if (rc0 < 0) return rc0;
     }
     const int M = c->nsatellite;
@@ -1316,8 +1463,8 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
                 return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
             }
             if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
-        } else if (!o.quiet) {
-            std::printf("note: %s not loadable; the engine's built-in surface drives the sampler\n", c->so_potential);
+        } else {
+            return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_potential, dlerror());
         }
     }
     hh_sampler smp{HH_IC_REJECTION, ps, Tb, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, 0.0};
@@ -1332,6 +1479,7 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
     hh_cf_params p{};
     const bool mixed = o.precision == HH_PREC_MIXED;
     p.rtol = p.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+    p.orbit_rtol = c->cvode_tolerance;   // orbiting pairs: FP64 at the configured tolerance
     p.dt = c->sampling_time;
     p.Rcut = c->Rcut;
     p.ppf = 1.0;
@@ -1439,7 +1587,7 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
     if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
-    int rc = system_of(c, &sys);
+    int rc = resolve_system(e, c, &sys);
     if (rc < 0) return rc;
     const long L = c->max_trajectory_length;
     const double dt = c->sampling_time, T = c->temperature;
@@ -1544,6 +1692,7 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
         hh_prmu_params pp{};
         const bool mixed = o.precision == HH_PREC_MIXED;
         pp.rtol = pp.atol = mixed ? (o.rtol_mixed > 0 ? o.rtol_mixed : 1e-10) : c->cvode_tolerance;
+        pp.orbit_rtol = c->cvode_tolerance;   // orbiting pairs: FP64 at the configured tolerance
         pp.dt = dt;
         pp.R0 = c->R0;
         pp.L = (int)L;

@@ -14,7 +14,14 @@ enum ItemStatus : int {
     ITEM_FAIL_CF_JUMP = 3,      // src/hawaii.c:2221-2236
     ITEM_FAIL_STEPS = 4,        // too many steps between two samples (CVODE mxsteps analogue)
     ITEM_FAIL_STEPSIZE = 5,     // step size underflow
+    ITEM_ORBIT = 6,             // mixed precision: handed over for integration in FP64 (an orbiting pair);
+                                // replaced by the FP64 run's own status before the job is reduced
 };
+
+// Values per sample of a job's rows for its record mode: 0 mu(0).mu(t_k); 1, 2 the dipole vector
+// (Zimmermann's trick, PR/mu); 3 the eight lab coordinates of a CH4-CO2 sample (its dipole is
+// evaluated afterwards by the batched dipole kernel, which overwrites the first three with mu)
+__host__ __device__ __forceinline__ int row_width(int record) { return record == 3 ? 8 : (record ? 3 : 1); }
 
 __host__ __device__ __forceinline__ bool dipole_failure(int st)
 {
@@ -51,6 +58,20 @@ struct JobDesc {
     int record;              // 0: rows hold mu(0).mu(t_k) [L]; 1: the dipole series mu_x, mu_y, mu_z [3][L]
                              // (Zimmermann); 2: the same series for PR/mu, dipole check only, no CF reduce
     double dt, Rcut, rtol, atol;
+    // Item map of an orbit job: its item k is item map[k] of the parent job whose buffers (ic, rows,
+    // len, status, nrhs, e_drift) it shares; nullptr: item k is item k.
+    const int* map;
+    // Mixed precision: an item whose pair reaches an outer radial turning point (pR from > 0 to
+    // <= 0, the pair bound in an orbiting complex) ends with ITEM_ORBIT and its index is appended
+    // to orbit_items[atomicAdd(orbit_count, 1)]; nullptr: no hand-over.
+    int* orbit_items;
+    int* orbit_count;
+    // an orbit job: the parent job's counter of the right-hand sides spent in FP64
+    unsigned long long* orbit_nrhs;
+    // record mode 3: the CF rows [n_items][L] the dipole pass writes (what the reduction reads),
+    // and whether that pass evaluates the dipole in FP32
+    double* cf_rows;
+    int f32_dipole;
 };
 
 struct QueueState {

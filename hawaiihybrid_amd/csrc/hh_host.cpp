@@ -77,11 +77,24 @@ int collect_stats(const std::vector<hh::JobDesc>& jobs, hh_cf_stats* stats)
         HH_CHECK(hipMemcpy(nr.data(), jb.nrhs, n * sizeof(int), hipMemcpyDeviceToHost));
         HH_CHECK(hipMemcpy(dr.data(), jb.e_drift, n * sizeof(float), hipMemcpyDeviceToHost));
         stats->n_items += (long long)n;
+        if (jb.orbit_count) {
+            int no = 0;
+            unsigned long long nr64 = 0;
+            HH_CHECK(hipMemcpy(&no, jb.orbit_count, sizeof no, hipMemcpyDeviceToHost));
+            HH_CHECK(hipMemcpy(&nr64, jb.orbit_nrhs, sizeof nr64, hipMemcpyDeviceToHost));
+            stats->n_orbit += no;
+            stats->nrhs_orbit += (long long)nr64;
+            if (no > 0) {
+                std::vector<int> items((size_t)no);
+                HH_CHECK(hipMemcpy(items.data(), jb.orbit_items, items.size() * sizeof(int), hipMemcpyDeviceToHost));
+                for (int it : items) stats->nsamples_orbit += ln[(size_t)it];
+            }
+        }
         for (size_t i = 0; i < n; ++i) {
             stats->nrhs += nr[i];
             stats->nsamples += ln[i];
             if (st[i] >= 1 && st[i] <= 3) ++stats->n_failed;
-            if (st[i] >= 4) ++stats->n_stepfail;
+            if (st[i] == hh::ITEM_FAIL_STEPS || st[i] == hh::ITEM_FAIL_STEPSIZE) ++stats->n_stepfail;
             stats->max_e_drift = std::max(stats->max_e_drift, (double)dr[i]);
         }
     }
@@ -105,6 +118,8 @@ int check_cf_args(const hh_system* sys, const hh_cf_params* p)
         return fail(HH_E_INVALID, "unknown halves %d", p->halves);
     if (p->zimmermann && p->halves != HH_HALVES_BOTH)
         return fail(HH_E_INVALID, "Zimmermann's trick integrates both halves of every trajectory");
+    if (!(p->orbit_rtol <= 0.0 || p->orbit_rtol < 1.0))
+        return fail(HH_E_INVALID, "orbit_rtol must be below 1 (0: 1e-12, negative: off)");
     if (p->zimmermann && p->L > zimm_max_length())
         return fail(HH_E_UNSUPPORTED, "Zimmermann's trick: MaxTrajectoryLength %d above %d", p->L, zimm_max_length());
     return HH_OK;
@@ -407,15 +422,15 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
             bool all = true;
             for (hh::PoolJob* pj : mine) all &= pj->state == 2;
             if (all) break;
-            rc = hh::pool_step(e, e->pools, 20.0, &kms, &rms, &launches);
+            bool progress = false;
+            rc = hh::pool_step(e, e->pools, 20.0, &kms, &rms, &launches, &progress);
+            if (rc == HH_OK && !progress)
+                rc = fail(HH_E_DEVICE, "hh_cf_run: jobs left with no work in flight (internal bookkeeping error)");
             if (rc < 0) {
-                // drop the pool with its state; the next call builds a fresh one
+                // drop the pools with their state (the orbit pool holds jobs on this call's
+                // buffers); the next call builds fresh ones
                 (void)hipStreamSynchronize(e->stream);
-                for (auto it = e->pools.begin(); it != e->pools.end(); ++it)
-                    if (it->get() == pl) {
-                        e->pools.erase(it);
-                        break;
-                    }
+                e->pools.clear();
                 return rc;
             }
         }

@@ -107,6 +107,15 @@ struct PoolJob {
     uint64_t request = 0;       // serving request id (0: synchronous run)
     size_t bytes = 0;           // device bytes held by `buf`
     std::shared_ptr<DevBuf> group_order;   // order of the admission group this job belongs to
+    bool items_done = false;    // every item of the job has finished (seen after a slice)
+    // mixed precision: items handed over to FP64 (jd.orbit_items / jd.orbit_count) run as orbit
+    // jobs in the FP64 pool `orbit_pool` at rtol = atol = orbit_rtol; the job is reduced once its
+    // own items and all its orbit jobs have finished
+    double orbit_rtol = 0.0;
+    long long orbit_admitted = 0;           // hand-overs already admitted as orbit jobs
+    struct LanePool* orbit_pool = nullptr;
+    int orbit_pending = 0;                  // orbit jobs admitted and not yet finished
+    PoolJob* parent = nullptr;              // an orbit job: the job whose items it integrates
 };
 
 // A resident lane pool for one (system, precision, block) class: S = grid * block lane slots
@@ -119,10 +128,16 @@ struct LanePool {
     int block = 256;
     int waves = 2;                  // resident wavefronts per SIMD
     long long S = 0;
-    int ring_cap = 1024;
+    // in-flight jobs live in ring slots seq % ring_cap: the sequence numbers of the jobs in flight
+    // must span fewer than ring_cap (orbit jobs arrive every slice while a long one still runs)
+    int ring_cap = 8192;
     DevBuf slots[2];            // double-buffered for repacking
     int cur = 0;
-    DevBuf arena, ring, ring_done, next, scratch, sort_tmp;
+    DevBuf arena, ring, ring_done, ring_orbit, ring_orbit_nrhs, next, scratch, sort_tmp;
+    // every pool's slices run on its own stream (default priority), so the slices of different
+    // pools (an FP64 orbit pool beside its mixed pool, classes of a serving mix) overlap
+    hipStream_t stream = nullptr;
+    hipEvent_t slice_end = nullptr;
     std::deque<std::unique_ptr<PoolJob>> inflight;   // sequence order
     long long next_seq = 0;
     unsigned long long total = 0;   // items admitted
@@ -140,6 +155,7 @@ struct LanePool {
     std::vector<SliceGraph> graphs;   // a few variants (repack or not, buffer, admission state)
     bool repack_pending = false;
     int* h_done = nullptr;            // pinned copies read by the host after each slice
+    int* h_orbit = nullptr;
     unsigned long long* h_next = nullptr;
     LanePool() = default;
     LanePool(const LanePool&) = delete;
@@ -149,7 +165,13 @@ struct LanePool {
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (h_done) (void)hipHostFree(h_done);
+        if (h_orbit) (void)hipHostFree(h_orbit);
         if (h_next) (void)hipHostFree(h_next);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        if (slice_end) (void)hipEventDestroy(slice_end);
     }
 };
 
@@ -197,6 +219,9 @@ int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const d
               const long* ntraj, int njobs, int* block, int* waves);
 // device bytes a job of this shape holds while it runs
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record = -1);
+// true when a job of this class hands its orbiting items over to FP64 (mixed precision, a
+// system with a rotor, orbit_rtol >= 0)
+bool orbit_handover(const hh_system& sys, const hh_cf_params& p);
 // admit a correlation-function job: allocates its buffers, uploads its descriptor into the
 // pool's ring and appends its items to the pool's queue; *out (optional) sees the job.
 // record: the JobDesc row mode (-1: 1 with p.zimmermann, else 0; 2: a PR/mu dipole series)
@@ -210,11 +235,13 @@ int cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int njobs,
 int pool_order_group(hh_engine* e, LanePool& pl);
 // true when the ring has room for one more job
 bool pool_has_room(const LanePool& pl);
-// one slice of every pool with work, then reductions of the jobs whose items are all done
-// and repacking of drained pools; synchronises the engine stream.  Jobs whose CF is final
-// have state 2 afterwards.  kernel_ms / reduce_ms accumulate device time.
+// one slice of every pool with work (each pool on its own stream, concurrently), then
+// admission of the items handed over to FP64 as orbit jobs, reductions of the jobs whose items
+// (orbit jobs included) are all done and repacking of drained pools; synchronises the engine
+// stream.  Jobs whose CF is final have state 2 afterwards; their orbit jobs are gone from the
+// orbit pool.  kernel_ms / reduce_ms accumulate device time.
 int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms,
-              int* launches = nullptr);
+              int* launches = nullptr, bool* progress = nullptr);
 // argument checks shared by hh_cf_run and hh_submit_cf
 int check_cf_args(const hh_system* sys, const hh_cf_params* p);
 // per-item summaries (items, RHS, samples, failures, drift) of jobs, added into stats

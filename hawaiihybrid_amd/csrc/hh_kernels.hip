@@ -152,7 +152,12 @@ __device__ __forceinline__ void finish_item(const JobDesc& jb, Lane<N>& ln, int 
 {
     jb.len[ln.local] = last + 1;
     jb.status[ln.local] = status;
-    jb.nrhs[ln.local] = ln.nrhs;
+    // an orbit job adds its right-hand sides to those the mixed-precision run spent on the item
+    if (jb.map) {
+        jb.nrhs[ln.local] += ln.nrhs;
+        atomicAdd(jb.orbit_nrhs, (unsigned long long)ln.nrhs);
+    } else
+        jb.nrhs[ln.local] = ln.nrhs;
     jb.e_drift[ln.local] = drift;
     if (jb.done) atomicAdd(jb.done, 1);
     ln.active = 0;
@@ -166,7 +171,7 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
                                           const Arena<N>& ar, Lane<N>& ln)
 {
     ln.job = j;
-    ln.local = local;
+    ln.local = jb.map ? (long long)jb.map[local] : local;
     const long long traj = ln.local / jb.halves;
     const bool back = (ln.local % jb.halves) == 1;
 #pragma unroll
@@ -174,18 +179,25 @@ __device__ __forceinline__ bool init_item(const SysConst& sc, const JobDesc& jb,
         const double v = jb.ic[(long long)i * jb.ntraj + traj];
         ln.y[i] = (back && (i & 1)) ? -v : v;
     }
-    P m[3];
-    dipole<SYS, P>(sc, ln.y, m);
-    ln.mu0[0] = m[0];
-    ln.mu0[1] = m[1];
-    ln.mu0[2] = m[2];
-    double* row = jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1);
-    if (jb.record) {
-        row[0] = ln.mu0[0];
-        row[jb.L] = ln.mu0[1];
-        row[2 * (long long)jb.L] = ln.mu0[2];
+    double* row = jb.rows + ln.local * (long long)jb.L * row_width(jb.record);
+    if constexpr (SYS == SYS_CH4CO2) {
+        // record 3: the coordinates of sample 0; the dipole comes later (hh_ch4co2.hip)
+#pragma unroll
+        for (int c = 0; c < N / 2; ++c) row[c] = ln.y[2 * c];
+        ln.mu0[0] = ln.mu0[1] = ln.mu0[2] = 0.0;
     } else {
-        row[0] = ln.mu0[0] * ln.mu0[0] + ln.mu0[1] * ln.mu0[1] + ln.mu0[2] * ln.mu0[2];
+        P m[3];
+        dipole<SYS, P>(sc, ln.y, m);
+        ln.mu0[0] = m[0];
+        ln.mu0[1] = m[1];
+        ln.mu0[2] = m[2];
+        if (jb.record) {
+            row[0] = ln.mu0[0];
+            row[jb.L] = ln.mu0[1];
+            row[2 * (long long)jb.L] = ln.mu0[2];
+        } else {
+            row[0] = ln.mu0[0] * ln.mu0[0] + ln.mu0[1] * ln.mu0[1] + ln.mu0[2] * ln.mu0[2];
+        }
     }
     ln.E0 = hamiltonian<SYS, P>(sc, ln.y);
     ln.t = 0.0;
@@ -526,7 +538,7 @@ __device__ __forceinline__ bool dop853_step(const SysConst& sc, const JobDesc* _
     int status = ITEM_OK;
     float drift = 0.0f;
     if (dense) {
-        double* row = jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1);
+        double* row = jb.rows + ln.local * (long long)jb.L * row_width(jb.record);
         while (ln.k < jb.L && ln.k * jb.dt <= tnew) {
             const double tk = ln.k * jb.dt;
             double ys[N];
@@ -681,25 +693,31 @@ __device__ __forceinline__ void shared_samples(const SysConst& sc, const Arena<N
                     ys[2 * c] = dense_value(F, r[R::Y + 8 * c], x);
                 }
             }
-            dipole<SYS, P>(sc, ys, m);
             rec_load_block<N>(ar, owner, R::CF0, R::YN - R::CF0, r);
-            const double mu0[3] = {r[R::MU0], r[R::MU0 + 1], r[R::MU0 + 2]};
-            curr = cf_value<P>(mu0, m);
-            prev_start = r[R::CURR];
             row = __builtin_bit_cast(double*, r[R::ROW]);
             rowL = (int)r[R::L];
             record = (int)r[R::RECORD];
+            if constexpr (SYS == SYS_CH4CO2) {
+                // record 3: the sample's coordinates; its dipole and CF value come later
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) row[(long long)k * NQ + c] = ys[2 * c];
+            } else {
+                dipole<SYS, P>(sc, ys, m);
+                const double mu0[3] = {r[R::MU0], r[R::MU0 + 1], r[R::MU0 + 2]};
+                curr = cf_value<P>(mu0, m);
+            }
+            prev_start = r[R::CURR];
         }
         const double up = __shfl_up(curr, 1, 64);
         const double prev = q == 0 ? prev_start : (lane > 0 ? up : carry);
         carry = __shfl(curr, 63, 64);
-        if (valid) {
+        if (valid && SYS != SYS_CH4CO2) {
             const int st = sample_status<P>(m, curr, prev, k, record);
             if (st != ITEM_OK) atomicMin(&ws.fail[owner], (q << 3) | st);
         }
         wave_share_sync();
         if (valid) {
-            if (q < (ws.fail[owner] >> 3)) store_sample<P>(row, rowL, record, k, m, curr);
+            if (SYS != SYS_CH4CO2 && q < (ws.fail[owner] >> 3)) store_sample<P>(row, rowL, record, k, m, curr);
             if (q == cnt - 1) ws.last[owner] = curr;
         }
     }
@@ -762,7 +780,7 @@ __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const Job
             r[R::MU0 + 2] = ln.mu0[2];
             r[R::CURR] = ln.curr;
             r[R::K0] = (double)ln.k;
-            r[R::ROW] = __builtin_bit_cast(double, jb.rows + ln.local * (long long)jb.L * (jb.record ? 3 : 1));
+            r[R::ROW] = __builtin_bit_cast(double, jb.rows + ln.local * (long long)jb.L * row_width(jb.record));
             r[R::L] = (double)jb.L;
             r[R::RECORD] = (double)jb.record;
             r[R::K0 + 1] = 0.0;
@@ -802,6 +820,15 @@ __device__ __forceinline__ void dop853_step_shared(const SysConst& sc, const Job
         ln.k += n;
         ln.prev = ln.curr;
         ln.curr = ws.last[lane];
+    }
+    // Mixed precision: an outer radial turning point (pR from > 0 to <= 0) means the pair is
+    // bound in an orbiting complex, whose trajectory depends on the last bit of the arithmetic;
+    // the item is handed over for integration in FP64 from its initial condition.
+    if (jb.orbit_items && ln.y[5] > 0.0 && yn[5] <= 0.0) {
+        const int at = atomicAdd(jb.orbit_count, 1);
+        jb.orbit_items[at] = (int)ln.local;
+        finish_item<N>(jb, ln, ITEM_ORBIT, ln.k - 1, 0.0f);
+        return;
     }
     advance<N>(jb, ar, ln, yn, h, fac11);
 }

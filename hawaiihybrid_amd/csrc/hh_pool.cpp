@@ -48,10 +48,14 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
     if (pl->slots[0].ensure(sb) != hipSuccess || pl->slots[1].ensure(sb) != hipSuccess ||
         pl->arena.ensure(arena) != hipSuccess || pl->ring.ensure(pl->ring_cap * sizeof(JobDesc)) != hipSuccess ||
         pl->ring_done.ensure(pl->ring_cap * sizeof(int)) != hipSuccess ||
+        pl->ring_orbit.ensure(pl->ring_cap * sizeof(int)) != hipSuccess ||
+        pl->ring_orbit_nrhs.ensure(pl->ring_cap * sizeof(unsigned long long)) != hipSuccess ||
         pl->next.ensure(sizeof(unsigned long long)) != hipSuccess ||
         pl->scratch.ensure((size_t)(pl->S / 256 + 1) * sizeof(int)) != hipSuccess)
         return fail(HH_E_NOMEM, "cannot allocate a lane pool (%.1f MiB of slots and arena)",
                     (2.0 * sb + arena) / 1048576.0);
+    HH_CHECK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+    HH_CHECK(hipEventCreateWithFlags(&pl->slice_end, hipEventDisableTiming));
     HH_CHECK(run_slots_clear(pl->view(0), e->stream));
     const unsigned long long z = 0;
     HH_CHECK(hipMemcpyAsync(pl->next.p, &z, sizeof z, hipMemcpyHostToDevice, e->stream));
@@ -65,10 +69,16 @@ size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record)
 {
     const long long items = (long long)ntraj * (p.halves == HH_HALVES_BOTH ? 2 : 1);
     if (record < 0) record = p.zimmermann ? 1 : 0;
-    // rows ([3][L] per item for a dipole series), len, status, nrhs, e_drift, and the job's
-    // share of its group's order
+    // rows ([3][L] per item for a dipole series), len, status, nrhs, e_drift, the job's share
+    // of its group's order and its list of items handed over to FP64
     const size_t row = (size_t)p.L * (record ? 3 : 1);
-    return (size_t)items * (row * sizeof(double) + 4 * sizeof(int) + sizeof(float));
+    return (size_t)items * (row * sizeof(double) + 5 * sizeof(int) + sizeof(float));
+}
+
+bool orbit_handover(const hh_system& sys, const hh_cf_params& p)
+{
+    // a central potential (He-Ar) has no orbiting chaos and its mixed engine keeps FP64 forces
+    return p.precision == HH_PREC_MIXED && sys.system != HH_SYS_HEAR && p.orbit_rtol >= 0.0;
 }
 
 bool pool_has_room(const LanePool& pl)
@@ -114,8 +124,24 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     jd.atol = p.atol;
     jd.order = nullptr;
     jd.order_base = 0;
+    jd.map = nullptr;
+    jd.orbit_items = nullptr;
+    jd.orbit_count = nullptr;
+    jd.orbit_nrhs = nullptr;
+    if (orbit_handover(pl.system, p)) {
+        static const unsigned long long zero64 = 0;
+        jd.orbit_items = reinterpret_cast<int*>(jd.e_drift + n_items);
+        jd.orbit_count = pl.ring_orbit.as<int>() + slot;
+        jd.orbit_nrhs = pl.ring_orbit_nrhs.as<unsigned long long>() + slot;
+        job->orbit_rtol = p.orbit_rtol > 0.0 ? p.orbit_rtol : 1e-12;
+        HH_CHECK(hipMemcpyAsync(jd.orbit_count, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+        HH_CHECK(hipMemcpyAsync(jd.orbit_nrhs, &zero64, sizeof zero64, hipMemcpyHostToDevice, e->stream));
+    }
     HH_CHECK(hipMemcpyAsync(pl.ring.as<JobDesc>() + slot, &job->jd, sizeof(JobDesc), hipMemcpyHostToDevice, e->stream));
     HH_CHECK(hipMemcpyAsync(jd.done, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    // lanes that found the queue empty in the last slice left the head past the end; none of
+    // those indices was handed out, so the head goes back to the first item of this job
+    HH_CHECK(run_clamp_next(pl.next.as<unsigned long long>(), pl.total, e->stream));
     pl.total += (unsigned long long)n_items;
     pl.next_seq++;
     pl.drained_repacked = false;
@@ -250,6 +276,7 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
 {
     if (!e->capture) HH_CHECK(hipStreamCreateWithFlags(&e->capture, hipStreamNonBlocking));
     if (!pl.h_done) HH_CHECK(hipHostMalloc((void**)&pl.h_done, pl.ring_cap * sizeof(int), hipHostMallocDefault));
+    if (!pl.h_orbit) HH_CHECK(hipHostMalloc((void**)&pl.h_orbit, pl.ring_cap * sizeof(int), hipHostMallocDefault));
     if (!pl.h_next) HH_CHECK(hipHostMalloc((void**)&pl.h_next, sizeof(unsigned long long), hipHostMallocDefault));
     const bool repack = pl.repack_pending;
     LanePool::SliceGraph* g = nullptr;
@@ -273,6 +300,9 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
             err = hipMemcpyAsync(pl.h_done, pl.ring_done.p, pl.ring_cap * sizeof(int), hipMemcpyDeviceToHost,
                                  e->capture);
         if (err == hipSuccess)
+            err = hipMemcpyAsync(pl.h_orbit, pl.ring_orbit.p, pl.ring_cap * sizeof(int), hipMemcpyDeviceToHost,
+                                 e->capture);
+        if (err == hipSuccess)
             err = hipMemcpyAsync(pl.h_next, pl.next.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->capture);
         const hipError_t end = hipStreamEndCapture(e->capture, &graph);
         if (err == hipSuccess) err = end;
@@ -288,7 +318,7 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
         pl.graphs.push_back(ng);
         g = &pl.graphs.back();
     }
-    HH_CHECK(hipGraphLaunch(g->exec, e->stream));
+    HH_CHECK(hipGraphLaunch(g->exec, pl.stream));
     if (repack) {
         pl.cur = 1 - pl.cur;
         pl.repack_pending = false;
@@ -298,15 +328,69 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
 
 }  // namespace
 
-int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms, int* launches)
+namespace {
+
+SysConst pool_sysconst(const hh_engine* e, const LanePool& pl)
 {
-    // 1. one slice of every pool that still integrates
+    const bool f32 = pl.mixed && pl.sys == HH_SYS_CO2AR;
+    return SysConst{pl.system.mu, pl.system.I, f32 ? e->tab32.p : e->tab64.p};
+}
+
+// The items that `parent` handed over since its last orbit job, [parent.orbit_admitted, count)
+// of its list, as one orbit job of the FP64 pool of the same class: the job shares the parent's
+// buffers and maps its item k to the parent's item orbit_items[orbit_admitted + k].
+int admit_orbit_job(hh_engine* e, PoolSet& set, LanePool& parent_pool, PoolJob& parent, int count)
+{
+    LanePool* op = parent.orbit_pool;
+    if (!op) {
+        const int rc = pool_get(e, set, parent_pool.system, false, 256, &op, pool_waves(256));
+        if (rc < 0) return rc;
+        parent.orbit_pool = op;
+    }
+    if (!pool_has_room(*op)) return HH_OK;   // the ring is full: admitted after a later slice
+    auto job = std::make_unique<PoolJob>();
+    const long long n = count - parent.orbit_admitted;
+    job->seq = op->next_seq;
+    job->parent = &parent;
+    const int slot = (int)(job->seq % op->ring_cap);
+    JobDesc& jd = job->jd;
+    jd = parent.jd;
+    jd.item_begin = (long long)op->total;
+    jd.n_items = n;
+    jd.done = op->ring_done.as<int>() + slot;
+    jd.order = nullptr;
+    jd.order_base = 0;
+    jd.map = parent.jd.orbit_items + parent.orbit_admitted;
+    jd.orbit_items = nullptr;
+    jd.orbit_count = nullptr;
+    jd.orbit_nrhs = parent.jd.orbit_nrhs;
+    jd.rtol = jd.atol = parent.orbit_rtol;
+    HH_CHECK(hipMemcpyAsync(op->ring.as<JobDesc>() + slot, &jd, sizeof(JobDesc), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(hipMemcpyAsync(jd.done, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HH_CHECK(run_clamp_next(op->next.as<unsigned long long>(), op->total, e->stream));
+    op->total += (unsigned long long)n;
+    op->next_seq++;
+    op->unordered_seq = op->next_seq;   // orbit jobs run in hand-over order
+    parent.orbit_admitted = count;
+    parent.orbit_pending++;
+    op->inflight.push_back(std::move(job));
+    return HH_OK;
+}
+
+}  // namespace
+
+int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, double* reduce_ms, int* launches,
+              bool* progress)
+{
+    if (progress) *progress = false;
+    static const bool trace = std::getenv("HH_POOL_TRACE") != nullptr;   // progress lines on stderr
+    // 1. one slice of every pool with items in flight, each pool on its own stream so the
+    //    slices of different pools share the device
     std::vector<LanePool*> run;
     for (auto& pp : set) {
-        LanePool& pl = *pp;
         bool work = false;
-        for (auto& j : pl.inflight) work |= j->state == 0;
-        if (work) run.push_back(&pl);
+        for (auto& j : pp->inflight) work |= j->state == 0 && !j->items_done;
+        if (work) run.push_back(pp.get());
     }
     if (!run.empty()) {
         HH_CHECK(hipEventRecord(e->ev[0], e->stream));
@@ -318,15 +402,16 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             ctl.last_seq = pl->inflight.back()->seq;
             ctl.ticks = (long long)(slice_ms * e->wall_khz);
             ctl.ring = pl->ring_cap;
-            const bool mixed = pl->mixed && pl->sys == HH_SYS_CO2AR;
-            SysConst sc{pl->system.mu, pl->system.I, mixed ? e->tab32.p : e->tab64.p};
+            HH_CHECK(hipStreamWaitEvent(pl->stream, e->ev[0], 0));
             if (e->use_graphs) {
-                const int rc = launch_slice_graph(e, *pl, ctl, sc);
+                const int rc = launch_slice_graph(e, *pl, ctl, pool_sysconst(e, *pl));
                 if (rc != HH_OK) return rc;
             } else {
-                HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, pl->waves, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
-                                  pl->arena.as<double>(), e->stream));
+                HH_CHECK(run_pool(pl->sys, pl->mixed, pl->block, pl->waves, pool_sysconst(e, *pl),
+                                  pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur), pl->arena.as<double>(), pl->stream));
             }
+            HH_CHECK(hipEventRecord(pl->slice_end, pl->stream));
+            HH_CHECK(hipStreamWaitEvent(e->stream, pl->slice_end, 0));
             if (launches) ++*launches;
         }
         HH_CHECK(hipEventRecord(e->ev[1], e->stream));
@@ -334,52 +419,92 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
         if (kernel_ms) *kernel_ms += ms;
-        if (std::getenv("HH_POOL_TRACE")) std::fprintf(stderr, "[hh pool] slice of %zu pools: %.2f ms\n", run.size(), ms);
+        if (trace) std::fprintf(stderr, "[hh pool] slice of %zu pools: %.2f ms\n", run.size(), ms);
     }
 
-    // 2. finished jobs -> reductions; drained pools -> repack
-    static const bool trace = std::getenv("HH_POOL_TRACE") != nullptr;   // progress lines on stderr
-    bool launched = false;
-    HH_CHECK(hipEventRecord(e->ev[2], e->stream));
+    // 2. progress of every job of the pools that ran: finished items, hand-overs to FP64
+    struct Progress {
+        LanePool* pl;
+        unsigned long long next;
+    };
+    std::vector<Progress> prog;
     for (LanePool* pl : run) {
-        std::vector<int> done(pl->ring_cap);
+        std::vector<int> done(pl->ring_cap), orbit(pl->ring_cap);
         unsigned long long next = 0;
         if (e->use_graphs) {
             // the graph copied the counters into pinned memory at the end of the slice
             std::memcpy(done.data(), pl->h_done, pl->ring_cap * sizeof(int));
+            std::memcpy(orbit.data(), pl->h_orbit, pl->ring_cap * sizeof(int));
             next = *pl->h_next;
-            if (next > pl->total) next = pl->total;   // the next graph clamps the device copy
         } else {
             HH_CHECK(hipMemcpy(done.data(), pl->ring_done.p, pl->ring_cap * sizeof(int), hipMemcpyDeviceToHost));
+            HH_CHECK(hipMemcpy(orbit.data(), pl->ring_orbit.p, pl->ring_cap * sizeof(int), hipMemcpyDeviceToHost));
             HH_CHECK(hipMemcpy(&next, pl->next.p, sizeof next, hipMemcpyDeviceToHost));
-            if (next > pl->total) {
-                // lanes that found the queue empty advanced the head past the end; none of those
-                // indices was handed out, so the head goes back to the first unadmitted item
-                next = pl->total;
-                HH_CHECK(hipMemcpy(pl->next.p, &next, sizeof next, hipMemcpyHostToDevice));
-            }
         }
-        bool integrating = false;
+        // lanes that found the queue empty advanced the head past the end; none of those
+        // indices was handed out (every admission clamps the device head back to the end)
+        if (next > pl->total) next = pl->total;
+        prog.push_back({pl, next});
         for (auto& jp : pl->inflight) {
             PoolJob& j = *jp;
-            if (j.state != 0) continue;
-            if (done[j.seq % pl->ring_cap] < j.jd.n_items) {
-                integrating = true;
-                continue;
+            if (j.state != 0 || j.items_done) continue;
+            const int slot = (int)(j.seq % pl->ring_cap);
+            if (j.jd.orbit_count && orbit[slot] > j.orbit_admitted) {
+                const int rc = admit_orbit_job(e, set, *pl, j, orbit[slot]);
+                if (rc != HH_OK) return rc;
+            }
+            if (done[slot] >= j.jd.n_items) j.items_done = true;
+        }
+    }
+    // a finished orbit job leaves its pool at once (its slot frees for the next hand-overs)
+    for (auto& pp : set) {
+        std::vector<PoolJob*> fin;
+        for (auto& jp : pp->inflight)
+            if (jp->parent && jp->items_done) fin.push_back(jp.get());
+        for (PoolJob* o : fin) {
+            o->parent->orbit_pending--;
+            (void)pool_take(*pp, o);
+        }
+    }
+
+    // 3. reductions: a job whose items have finished and whose orbit jobs have all finished;
+    //    an orbit job is complete (state 2) once its items have finished
+    bool launched = false;
+    HH_CHECK(hipEventRecord(e->ev[2], e->stream));
+    for (auto& pp : set) {
+        LanePool& pl = *pp;
+        for (auto& jp : pl.inflight) {
+            PoolJob& j = *jp;
+            if (j.state != 0 || !j.items_done || j.parent) continue;
+            if (j.jd.orbit_count) {
+                // the hand-over list is final once the items are done: admit what is left of it
+                // (an orbit pool whose ring was full takes it after a later slice)
+                int count = 0;
+                HH_CHECK(hipMemcpy(&count, j.jd.orbit_count, sizeof count, hipMemcpyDeviceToHost));
+                if (count > j.orbit_admitted) {
+                    const int rc = admit_orbit_job(e, set, pl, j, count);
+                    if (rc != HH_OK) return rc;
+                }
+                if (count != j.orbit_admitted || j.orbit_pending > 0) continue;
             }
             const int rc = reduce_job(e, j.jd, j.ppf, j.cf_out);
             if (rc != HH_OK) return rc;
             j.state = 1;
             launched = true;
         }
-        if (integrating && next >= pl->total && e->use_graphs) {
-            // queue drained, items left: the next slice's graph spreads them over the wavefronts
+    }
+    // 4. repacking: once every pool's queue is drained (the bulk of the work has been handed out),
+    //    the items left in a pool are spread over its wavefronts
+    bool all_drained = true;
+    for (const Progress& g : prog) all_drained &= g.next >= g.pl->total;
+    for (const Progress& g : prog) {
+        LanePool* pl = g.pl;
+        bool integrating = false;
+        for (auto& j : pl->inflight) integrating |= j->state == 0 && !j->items_done;
+        if (integrating && all_drained && e->use_graphs) {
+            // the next slice's graph spreads them over the wavefronts
             pl->repack_pending = true;
-            if (trace)
-                std::fprintf(stderr, "[hh pool] sys %d %s: queue drained, repack scheduled\n", pl->sys,
-                             pl->mixed ? "mixed" : "fp64");
-        } else if (integrating && next >= pl->total) {
-            // queue drained, items left: spread them over the pool's wavefronts
+        } else if (integrating && all_drained) {
             HH_CHECK(run_repack(pl->sys, pl->view(pl->cur), pl->view(1 - pl->cur), pl->scratch.as<int>(), e->stream));
             pl->cur = 1 - pl->cur;
             launched = true;
@@ -391,7 +516,7 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
             }
         } else if (trace && integrating) {
             std::fprintf(stderr, "[hh pool] sys %d %s: %llu of %llu items handed out\n", pl->sys,
-                         pl->mixed ? "mixed" : "fp64", next, pl->total);
+                         pl->mixed ? "mixed" : "fp64", g.next, pl->total);
         }
     }
     HH_CHECK(hipEventRecord(e->ev[3], e->stream));
@@ -401,9 +526,14 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
         (void)hipEventElapsedTime(&ms, e->ev[2], e->ev[3]);
         if (reduce_ms) *reduce_ms += ms;
     }
-    for (LanePool* pl : run)
-        for (auto& j : pl->inflight)
-            if (j->state == 1) j->state = 2;
+    // reduced jobs are final
+    for (auto& pp : set)
+        for (auto& jp : pp->inflight)
+            if (jp->state == 1) {
+                jp->state = 2;
+                if (progress) *progress = true;
+            }
+    if (progress && !run.empty()) *progress = true;
     return HH_OK;
 }
 

@@ -198,6 +198,7 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
     cp.max_steps = p->max_steps;
     cp.engine = HH_ENGINE_BATCHED;
     cp.precision = p->precision;
+    cp.orbit_rtol = p->orbit_rtol;
     double* dummy[1] = {sf_dev};   // no CF is reduced for a series job
     const double* icp[1] = {ic_dev};
     const long nt[1] = {ntraj};

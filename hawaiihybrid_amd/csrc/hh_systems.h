@@ -20,13 +20,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "ch4co2_standin.h"
+
 namespace hh {
 
 constexpr double kALU = 5.29177210903e-11;           // src/constants.h:8
 constexpr double kHTOCM = 2.1947463136320e5;         // src/constants.h:11
 constexpr double kTWO_PI = 6.283185307179586476925;
 
-enum SystemId : int { SYS_HEAR = 0, SYS_CO2AR = 1 };
+enum SystemId : int { SYS_HEAR = 0, SYS_CO2AR = 1, SYS_CH4CO2 = 3 };
 
 // Surface tables resident in device memory (filled by the host from co2ar_tables.h).
 template <typename P>
@@ -40,8 +42,9 @@ struct Co2ArTables {
 
 struct SysConst {
     double mu;           // reduced mass
-    double I;            // linear-rotor moment of inertia (II[0] == II[1])
+    double I;            // linear-rotor moment of inertia (II[0] == II[1]); CH4-CO2: of CO2
     const void* tables;  // Co2ArTables<P>* for CO2-Ar
+    double IR[3];        // CH4-CO2: principal moments of the rotor (CH4)
 };
 
 // ------------------------------------------------------------------ He-Ar (PES-IDS/HeAr.h)
@@ -363,6 +366,178 @@ __device__ __forceinline__ void co2ar_dipole(const Co2ArTables<P>* __restrict__ 
     mu[2] = dz * Rz - gx * (uz - c * Rz);
 }
 
+// ------------------------------------------------------------------ CH4-CO2 (ROTOR + LINEAR_MOLECULE)
+// Lab coordinates q = [Phi, Theta, R, phi1, theta1, psi1, phi2, theta2] at state indices 0, 2, ...,
+// 14; monomer 1 = CH4 (Euler angles phi1, theta1, psi1), monomer 2 = CO2 (phi2, theta2).  The frame
+// of CH4 is S1 = Sz(psi1) Sx(theta1) Sz(phi1) (src/angles_handler.cpp:357-419); the intermolecular
+// direction and the CO2 axis in the lab are Rhat = (-sin T cos P, -sin T sin P, cos T) and
+// u = (-sin t2 cos p2, -sin t2 sin p2, cos t2) (the reference's SPhi^T STheta^T z with its
+// non-standard Sy), a = S1 Rhat and b = S1 u in the CH4 frame.  The stand-in surface
+// (ch4co2_standin.h) is a function of R, a and b; its lab gradient follows from the generators of
+// the rotations: with ga = dV/da, gb = dV/db (CH4 frame) and Ga = S1^T ga, Gb = S1^T gb (lab),
+//   dV/dPhi = Ga . dRhat/dPhi          dV/dTheta = Ga . dRhat/dTheta
+//   dV/dphi2 = Gb . du/dphi2           dV/dtheta2 = Gb . du/dtheta2
+//   dV/dphi1 = Ga . Jz Rhat + Gb . Jz u              (dS1/dphi1 = S1 Jz)
+//   dV/dtheta1 = A^T ga . Jx A^T a + A^T gb . Jx A^T b   (A = Sz(psi1); dS1/dtheta1 = A Jx A^T S1)
+//   dV/dpsi1 = ga . Jz a + gb . Jz b                 (dS1/dpsi1 = Jz S1)
+// with Jz v = (v1, -v0, 0), Jx v = (0, v2, -v1): the Jacobian of the reference
+// (src/angles_handler.cpp:421-555) without its 1/sin(theta_K) factors.
+struct Ch4Co2Trig {
+    double sT, cT, s1, c1, sp, cp, s2, c2;   // FP64 sin/cos of Theta, theta1, psi1, theta2
+};
+
+template <typename P>
+struct Ch4Co2Frame {
+    P Rh[3], u[3], a[3], b[3];
+    P sP, cP, sf, cf, sP2, cP2;   // sin/cos of Phi, phi1, phi2
+    P s1, c1, sp, cp, sT, cT, s2, c2;
+};
+
+template <typename P>
+__device__ __forceinline__ void ch4co2_frame(const double* y, const Ch4Co2Trig& tr, Ch4Co2Frame<P>& f)
+{
+    sincos_p<P>(y[0], &f.sP, &f.cP);
+    sincos_p<P>(y[6], &f.sf, &f.cf);
+    sincos_p<P>(y[12], &f.sP2, &f.cP2);
+    f.sT = (P)tr.sT, f.cT = (P)tr.cT, f.s1 = (P)tr.s1, f.c1 = (P)tr.c1;
+    f.sp = (P)tr.sp, f.cp = (P)tr.cp, f.s2 = (P)tr.s2, f.c2 = (P)tr.c2;
+    f.Rh[0] = -f.sT * f.cP;
+    f.Rh[1] = -f.sT * f.sP;
+    f.Rh[2] = f.cT;
+    f.u[0] = -f.s2 * f.cP2;
+    f.u[1] = -f.s2 * f.sP2;
+    f.u[2] = f.c2;
+    // a = Sz(psi) Sx(theta) Sz(phi) Rhat, b likewise
+    const P* v[2] = {f.Rh, f.u};
+    P* o[2] = {f.a, f.b};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const P w0 = f.cf * v[k][0] + f.sf * v[k][1], w1 = -f.sf * v[k][0] + f.cf * v[k][1], w2 = v[k][2];
+        const P x1 = f.c1 * w1 + f.s1 * w2, x2 = -f.s1 * w1 + f.c1 * w2;
+        o[k][0] = f.cp * w0 + f.sp * x1;
+        o[k][1] = -f.sp * w0 + f.cp * x1;
+        o[k][2] = x2;
+    }
+}
+
+// Stand-in site sum at R, a, b: the energy (Hartree, scaled) and gs = dV/d(R a) summed over the
+// CO2 sites, gd = dV/d(r_CO b); mag: a bound on the sum of the pair forces (FP32 noise bound)
+template <typename P>
+__device__ __forceinline__ P ch4co2_sites(P R, const P* a, const P* b, P* gs, P* gd, P& mag)
+{
+    using namespace ch4co2;
+    P V = P(0);
+    mag = P(0);
+    gs[0] = gs[1] = gs[2] = gd[0] = gd[1] = gd[2] = P(0);
+    const P hs = P(kRCH / 1.7320508075688772);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const P sgn = j == 0 ? P(0) : (j == 1 ? P(1) : P(-1));
+        const P X[3] = {R * a[0] + sgn * P(kRCO) * b[0], R * a[1] + sgn * P(kRCO) * b[1], R * a[2] + sgn * P(kRCO) * b[2]};
+        const int tj = j == 0 ? 2 : 3;
+        P g[3] = {P(0), P(0), P(0)};
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int ti = i == 0 ? 0 : 1;
+            const double sig = 0.5 * (kSigma[ti] + kSigma[tj]) * kAng;
+            const P s2 = P(sig * sig);
+            const P e4 = P(4.0 * kWellScale * kKelvinHartree) * (P)__builtin_sqrt(kEpsilon[ti] * kEpsilon[tj]);
+            const P h0 = i == 0 ? P(0) : hs * P(kH[i - 1][0]), h1 = i == 0 ? P(0) : hs * P(kH[i - 1][1]);
+            const P h2 = i == 0 ? P(0) : hs * P(kH[i - 1][2]);
+            const P d0 = X[0] - h0, d1 = X[1] - h1, d2 = X[2] - h2;
+            const P r2 = d0 * d0 + d1 * d1 + d2 * d2;
+            const P ir2 = P(1) / r2;
+            const P sr2 = s2 * ir2, sr6 = sr2 * sr2 * sr2, sr12 = sr6 * sr6;
+            V += e4 * (sr12 - sr6);
+            const P w = e4 * (P(-12) * sr12 + P(6) * sr6) * ir2;   // (dV/dr) / r
+            g[0] += w * d0;
+            g[1] += w * d1;
+            g[2] += w * d2;
+            mag += fabs(w) * (fabs(d0) + fabs(d1) + fabs(d2));   // >= |dV/dr| of the pair
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            gs[k] += g[k];
+            gd[k] += j == 0 ? P(0) : sgn * g[k];
+        }
+    }
+    return V;
+}
+
+// lab gradient dV/dq[8] of the stand-in; noise: FP32 rounding bound of the components (0 in FP64)
+template <typename P>
+__device__ __forceinline__ void ch4co2_dpes(const double* y, const Ch4Co2Trig& tr, double* dV, double& noise)
+{
+    Ch4Co2Frame<P> f;
+    ch4co2_frame<P>(y, tr, f);
+    const P R = (P)y[4];
+    P gs[3], gd[3], mag;
+    (void)ch4co2_sites<P>(R, f.a, f.b, gs, gd, mag);
+    P ga[3], gb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ga[k] = R * gs[k];
+        gb[k] = P(ch4co2::kRCO) * gd[k];
+    }
+    const P dR = gs[0] * f.a[0] + gs[1] * f.a[1] + gs[2] * f.a[2];
+    // lab-frame gradients: S1^T g = Sz(phi)^T Sx(theta)^T Sz(psi)^T g
+    P Ga[3], Gb[3], Aa[3], Ab[3], Ag[3], Bg[3];
+    const P* gg[2] = {ga, gb};
+    P* GG[2] = {Ga, Gb};
+    P* AG[2] = {Ag, Bg};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const P* g = gg[k];
+        const P x0 = f.cp * g[0] - f.sp * g[1], x1 = f.sp * g[0] + f.cp * g[1], x2 = g[2];   // A^T g
+        AG[k][0] = x0;
+        AG[k][1] = x1;
+        AG[k][2] = x2;
+        const P w1 = f.c1 * x1 - f.s1 * x2, w2 = f.s1 * x1 + f.c1 * x2;                    // Sx^T
+        GG[k][0] = f.cf * x0 - f.sf * w1;
+        GG[k][1] = f.sf * x0 + f.cf * w1;
+        GG[k][2] = w2;
+    }
+    // A^T a, A^T b
+    Aa[0] = f.cp * f.a[0] - f.sp * f.a[1];
+    Aa[1] = f.sp * f.a[0] + f.cp * f.a[1];
+    Aa[2] = f.a[2];
+    Ab[0] = f.cp * f.b[0] - f.sp * f.b[1];
+    Ab[1] = f.sp * f.b[0] + f.cp * f.b[1];
+    Ab[2] = f.b[2];
+    // dRhat/dPhi = (sT sP, -sT cP, 0), dRhat/dTheta = (-cT cP, -cT sP, -sT); u likewise
+    const P dPhi = Ga[0] * (f.sT * f.sP) - Ga[1] * (f.sT * f.cP);
+    const P dTheta = -Ga[0] * (f.cT * f.cP) - Ga[1] * (f.cT * f.sP) - Ga[2] * f.sT;
+    const P dphi2 = Gb[0] * (f.s2 * f.sP2) - Gb[1] * (f.s2 * f.cP2);
+    const P dtheta2 = -Gb[0] * (f.c2 * f.cP2) - Gb[1] * (f.c2 * f.sP2) - Gb[2] * f.s2;
+    const P dphi1 = Ga[0] * f.Rh[1] - Ga[1] * f.Rh[0] + Gb[0] * f.u[1] - Gb[1] * f.u[0];
+    const P dtheta1 = Ag[1] * Aa[2] - Ag[2] * Aa[1] + Bg[1] * Ab[2] - Bg[2] * Ab[1];
+    const P dpsi1 = ga[0] * f.a[1] - ga[1] * f.a[0] + gb[0] * f.b[1] - gb[1] * f.b[0];
+    dV[0] = (double)dPhi;
+    dV[1] = (double)dTheta;
+    dV[2] = (double)dR;
+    dV[3] = (double)dphi1;
+    dV[4] = (double)dtheta1;
+    dV[5] = (double)dpsi1;
+    dV[6] = (double)dphi2;
+    dV[7] = (double)dtheta2;
+    // each component is a sum of pair forces times a lever arm of at most R + r_CO
+    noise = sizeof(P) == 4 ? kF32NoiseUnits * (double)mag * ((double)R + ch4co2::kRCO) : 0.0;
+}
+
+template <typename P>
+__device__ __forceinline__ P ch4co2_pes(const double* y)
+{
+    Ch4Co2Trig tr;
+    sincos_p<double>(y[2], &tr.sT, &tr.cT);
+    sincos_p<double>(y[8], &tr.s1, &tr.c1);
+    sincos_p<double>(y[10], &tr.sp, &tr.cp);
+    sincos_p<double>(y[14], &tr.s2, &tr.c2);
+    Ch4Co2Frame<P> f;
+    ch4co2_frame<P>(y, tr, f);
+    P gs[3], gd[3], mag;
+    return ch4co2_sites<P>((P)y[4], f.a, f.b, gs, gd, mag);
+}
+
 // ------------------------------------------------------------------ systems as types
 template <int SYS>
 struct SysTraits;
@@ -375,6 +550,29 @@ template <>
 struct SysTraits<SYS_CO2AR> {
     static constexpr int QP = 10;
 };
+template <>
+struct SysTraits<SYS_CH4CO2> {
+    static constexpr int QP = 16;
+};
+
+// rhsMonomer ROTOR (src/hawaii.c:505-531): monomer state m = (phi, pphi, theta, ptheta, psi, ppsi),
+// dV = dV/d(phi, theta, psi), sin/cos of theta and psi given
+__device__ __forceinline__ void rotor_rhs(const double* II, const double* m, const double* dV, double st, double ct,
+                                          double sp, double cp, double* d)
+{
+    const double pPhi = m[1], pTheta = m[3], pPsi = m[5];
+    const double t1 = (pPhi - pPsi * ct) * sp + pTheta * st * cp;
+    const double t2 = (pPhi - pPsi * ct) * cp - pTheta * st * sp;
+    const double t3 = pPhi * ct - pPsi, t4 = pPsi * ct - pPhi;
+    const double is = 1.0 / st, is2 = is * is, i0 = 1.0 / II[0], i1 = 1.0 / II[1];
+    d[0] = (t1 * sp * i0 + t2 * cp * i1) * is2;
+    d[1] = -dV[0];
+    d[2] = (t1 * cp * i0 - t2 * sp * i1) * is;
+    d[3] = -t3 * (t4 * (II[0] - II[1]) * cp * cp + pTheta * (II[0] - II[1]) * st * sp * cp + t4 * II[1]) * i0 * i1 *
+               is2 * is - dV[1];
+    d[4] = -(t1 * sp * i0 + t2 * cp * i1) * ct * is2 + pPsi / II[2];
+    d[5] = t1 * t2 * (i1 - i0) * is2 - dV[2];
+}
 
 // Right-hand side of Hamilton's equations.  Intermolecular part src/hawaii.c:586-591, linear
 // rotor src/hawaii.c:494-497.  noise: bound on the FP32 rounding noise of the momentum
@@ -385,6 +583,31 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
 {
     noise = 0.0;
     const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
+    if constexpr (SYS == SYS_CH4CO2) {
+        Ch4Co2Trig ct;
+        sincos_p<double>(Theta, &ct.sT, &ct.cT);
+        sincos_p<double>(y[8], &ct.s1, &ct.c1);
+        sincos_p<double>(y[10], &ct.sp, &ct.cp);
+        sincos_p<double>(y[14], &ct.s2, &ct.c2);
+        double dV[8];
+        ch4co2_dpes<P>(y, ct, dV, noise);
+        const double iR = recip<P>(R), iR2 = iR * iR, imu = 1.0 / mu;
+        const double is = recip<P>(ct.sT), is2 = is * is;
+        const double a = imu * iR2;
+        yd[4] = pR * imu;
+        yd[5] = (pTheta * pTheta + pPhi * pPhi * is2) * a * iR - dV[2];
+        yd[0] = pPhi * a * is2;
+        yd[1] = -dV[0];
+        yd[2] = pTheta * a;
+        yd[3] = pPhi * pPhi * ct.cT * a * is2 * is - dV[1];
+        rotor_rhs(sc.IR, y + 6, dV + 3, ct.s1, ct.c1, ct.sp, ct.cp, yd + 6);
+        const double iI = 1.0 / sc.I, ist = recip<P>(ct.s2), ist2 = ist * ist;
+        yd[12] = y[13] * iI * ist2;
+        yd[13] = -dV[6];
+        yd[14] = y[15] * iI;
+        yd[15] = y[13] * y[13] * ct.c2 * iI * ist2 * ist - dV[7];
+        return;
+    }
     Trig tr;
     sincos_p<double>(Theta, &tr.sT, &tr.cT);
     double dV[5];
@@ -425,7 +648,14 @@ __device__ __forceinline__ void rhs(const SysConst& sc, const double* __restrict
 template <int SYS, typename P>
 __device__ __forceinline__ void dipole(const SysConst& sc, const double* y, P* mu)
 {
-    if constexpr (SYS == SYS_HEAR) {
+    if constexpr (SYS == SYS_CH4CO2) {
+        // The CH4-CO2 dipole (320 basis functions) is evaluated by its own batched kernel over
+        // recorded sample coordinates (hh_ch4co2.hip); integrator code records coordinates
+        // instead of calling this.  A call here is a bug: it yields NaN, which every dipole
+        // check reports.
+        mu[0] = mu[1] = mu[2] = (P)__builtin_nan("");
+        return;
+    } else if constexpr (SYS == SYS_HEAR) {
         P sT, cT, sP, cP;
         sincos_p<P>(y[2], &sT, &cT);
         sincos_p<P>(y[0], &sP, &cP);
@@ -451,6 +681,18 @@ __device__ __forceinline__ double kinetic(const SysConst& sc, const double* y)
         sincos_p<double>(y[8], &st, &ct_unused);
         K = K + (y[9] * y[9] / (2.0 * sc.I) + y[7] * y[7] / (2.0 * sc.I * st * st));
     }
+    if constexpr (SYS == SYS_CH4CO2) {
+        // rotor (src/hawaii.c:1089-1110) and linear molecule (src/hawaii.c:1117-1128)
+        double st, ct, sp, cp, s2, c2_unused;
+        sincos_p<double>(y[8], &st, &ct);
+        sincos_p<double>(y[10], &sp, &cp);
+        sincos_p<double>(y[14], &s2, &c2_unused);
+        const double A = (y[7] - y[11] * ct) * sp + y[9] * st * cp, B = (y[7] - y[11] * ct) * cp - y[9] * st * sp;
+        const double K2 = A * A / (2.0 * sc.IR[0] * st * st) + B * B / (2.0 * sc.IR[1] * st * st) +
+                          y[11] * y[11] / (2.0 * sc.IR[2]);
+        const double K3 = y[15] * y[15] / (2.0 * sc.I) + y[13] * y[13] / (2.0 * sc.I * s2 * s2);
+        K = K + K2 + K3;
+    }
     return K;
 }
 
@@ -459,6 +701,8 @@ __device__ __forceinline__ double pes_value(const SysConst& sc, const double* y)
 {
     if constexpr (SYS == SYS_HEAR)
         return hear_V(y[4]);
+    else if constexpr (SYS == SYS_CH4CO2)
+        return (double)ch4co2_pes<P>(y);
     else
         return (double)co2ar_pes<P>((const Co2ArTables<P>*)sc.tables, y);
 }
@@ -466,6 +710,7 @@ __device__ __forceinline__ double pes_value(const SysConst& sc, const double* y)
 template <int SYS, typename P>
 __device__ __forceinline__ double hamiltonian(const SysConst& sc, const double* y)
 {
+    if constexpr (SYS == SYS_CH4CO2) return kinetic<SYS_CH4CO2>(sc, y) + (double)ch4co2_pes<P>(y);
     const double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = sc.mu;
     double s, c_unused;
     sincos_p<double>(Theta, &s, &c_unused);

@@ -141,6 +141,11 @@ typedef struct {
     int zimmermann;      /* use_zimmermann_trick: CF averaged over every time origin of the
                             trajectory's dipole series (correlation_eval_zimmerman_trick,
                             src/hawaii.c:1888-2114); both halves, L <= 4096 */
+    double orbit_rtol;   /* mixed precision, systems with a rotor: an item whose pair reaches an
+                            outer radial turning point (pR from > 0 to <= 0: an orbiting complex,
+                            whose trajectory is chaotic) is integrated again from its initial
+                            condition in FP64 at rtol = atol = orbit_rtol (0: 1e-12), concurrently
+                            with the rest of the ensemble; < 0 keeps every item in mixed precision */
 } hh_cf_params;
 
 typedef struct {
@@ -156,6 +161,10 @@ typedef struct {
     double max_e_drift;     /* max |H(end) - H(0)|, Hartree */
     int block, grid;        /* launch shape used */
     int launches;           /* integration kernel launches (time slices of the batched engine) */
+    long long n_orbit;      /* items integrated again in FP64 (orbit_rtol); their rows, statuses and
+                               lengths are the FP64 run's, nrhs counts both runs */
+    long long nrhs_orbit;   /* right-hand sides of the FP64 runs of those items (part of nrhs) */
+    long long nsamples_orbit; /* samples recorded by the FP64 runs (part of nsamples) */
 } hh_cf_stats;
 
 /* Run `njobs` correlation-function jobs of one system on the device.  ic_dev[j] holds
@@ -228,6 +237,8 @@ typedef struct {
     int max_steps;          /* max integrator steps between two samples (<= 0: 50000) */
     int allow_truncating;   /* count trajectories still inside R0 after L samples
                                (allow_truncating_trajectories_at_length_limit) */
+    double orbit_rtol;      /* mixed precision: orbiting pairs integrated again in FP64 at this
+                               tolerance (as hh_cf_params.orbit_rtol; 0: 1e-12, < 0: off) */
 } hh_prmu_params;
 
 int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params* p, const double* ic_dev, long ntraj,

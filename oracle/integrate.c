@@ -22,7 +22,7 @@
 #include "../hawaiihybrid_amd/csrc/dop853_coeffs.h"
 
 #define ALU 5.29177210903e-11   /* src/constants.h:8 */
-#define NMAX 10
+#define NMAX 16
 
 typedef struct {
     const orc_system *sys;
@@ -68,7 +68,7 @@ static double initial_step(fctx *c, int n, const double *y0, const double *f0, d
 
 static void dipole_of(const orc_system *s, const double *y, double mu[3])
 {
-    double q[5];
+    double q[8];
     int i, nq = orc_qp_size(s) / 2;
     for (i = 0; i < nq; ++i) q[i] = y[2 * i];
     orc_dipole(s, q, mu);

@@ -43,12 +43,13 @@ double   orc_mt_drand(orc_mt *s);
 void     orc_mt_fill(uint32_t seed, uint32_t *out, size_t n);
 
 /* ---------------------------------------------------------------- systems */
-enum { ORC_SYS_HEAR = 0, ORC_SYS_CO2AR = 1 };
+enum { ORC_SYS_HEAR = 0, ORC_SYS_CO2AR = 1, ORC_SYS_CH4CO2 = 3 };
 
 typedef struct {
     int system;      /* ORC_SYS_* */
     double mu;       /* reduced mass, a.u. */
     double II[2];    /* linear-rotor inertia (II[0] == II[1]); unused for He-Ar */
+    double IR[3];    /* CH4-CO2: principal moments of the rotor (monomer 1) */
 } orc_system;
 
 int    orc_qp_size(const orc_system *s);
@@ -72,6 +73,22 @@ void   orc_co2ar_lab_to_mol(const double *qlab, double *qmol);
 void   orc_co2ar_jacobian(const double *qlab, const double *qmol, double jac[5][5]);
 /* molecular-frame vector of a linear molecule + atom -> lab frame */
 void   orc_lm_mol_to_lab(const double *qmol, const double dm[3], double mu[3]);
+
+/* CH4-CO2 (ch4co2.c): lab q[8] = [Phi, Theta, R, phi1, theta1, psi1, phi2, theta2]; the frame
+ * of CH4 ("kal": R, phi1K, theta1K, phi2K, theta2K) and its 8 x 5 Jacobian as the reference;
+ * the stand-in PES in kal coordinates (Hartree) and its five derivatives; the dipole in the
+ * CH4 frame and in the lab */
+void   orc_ch4co2_lab_to_kal(const double *qlab, double *qkal);
+void   orc_ch4co2_jacobian(const double *qlab, const double *qkal, double jac[8][5]);
+double orc_ch4co2_pes_kal(double R, double phi1, double theta1, double phi2, double theta2);
+void   orc_ch4co2_dpes_kal(double R, double phi1, double theta1, double phi2, double theta2, double d[5]);
+double orc_ch4co2_well_scale(void);
+void   orc_ch4co2_dipole_kal(const double *qkal, double d[3]);
+double orc_ch4co2_pes(const double *q);
+void   orc_ch4co2_dpes(const double *q, double *dvdq);
+void   orc_ch4co2_dipole(const double *q, double mu[3]);
+void   orc_ch4co2_rhs(const orc_system *s, const double *y, double *ydot);
+double orc_ch4co2_kinetic(const orc_system *s, const double *y);
 
 /* ---------------------------------------------------------------- sampler */
 enum { ORC_IC_REJECTION = 0, ORC_IC_R0_INJECT = 1, ORC_IC_PHASE_SPACE = 2 };

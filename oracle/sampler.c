@@ -48,6 +48,38 @@ static void q_generator(const orc_system *s, const orc_sampler *sp, rng *r, doub
         y[6] = drand(r) * 2.0 * M_PI;
         y[8] = acos(2.0 * drand(r) - 1.0);
     }
+    if (s->system == ORC_SYS_CH4CO2) {       /* monomer 1 ROTOR, monomer 2 LINEAR_MOLECULE */
+        y[6] = drand(r) * 2.0 * M_PI;
+        y[8] = acos(2.0 * drand(r) - 1.0);
+        y[10] = drand(r) * 2.0 * M_PI;
+        y[12] = drand(r) * 2.0 * M_PI;
+        y[14] = acos(2.0 * drand(r) - 1.0);
+    }
+}
+
+/* p_generator_linear_molecule (src/hawaii.c:1292-1303): monomer angles m[0] = phi, m[2] = theta */
+static void p_linear(double II0, double T, rng *r, double *m)
+{
+    double sqrt_IIKT = sqrt(II0 / HkT * T);
+    double sin_theta = sin(m[2]);
+    double x0 = generate_normal(r, 1.0);
+    double x1 = generate_normal(r, 1.0);
+    m[1] = sqrt_IIKT * sin_theta * x1;
+    m[3] = sqrt_IIKT * x0;
+}
+
+/* p_generator_rotor (src/hawaii.c:1305-1327): m = (phi, pphi, theta, ptheta, psi, ppsi) */
+static void p_rotor(const double *II, double T, rng *r, double *m)
+{
+    double theta = m[2], psi = m[4];
+    double sqrt_IIx = sqrt(II[0] / HkT * T), sqrt_IIy = sqrt(II[1] / HkT * T), sqrt_IIz = sqrt(II[2] / HkT * T);
+    double sin_theta = sin(theta), cos_theta = cos(theta), sin_psi = sin(psi), cos_psi = cos(psi);
+    double x0 = generate_normal(r, 1.0);
+    double x1 = generate_normal(r, 1.0);
+    double x2 = generate_normal(r, 1.0);
+    m[1] = sqrt_IIx * sin_theta * sin_psi * x0 + sqrt_IIy * sin_theta * cos_psi * x1 + sqrt_IIz * cos_theta * x2;
+    m[3] = sqrt_IIx * cos_psi * x0 - sqrt_IIy * sin_psi * x1;
+    m[5] = sqrt_IIz * x2;
 }
 
 static void p_generator(const orc_system *s, double T, rng *r, double *y)
@@ -60,13 +92,10 @@ static void p_generator(const orc_system *s, double T, rng *r, double *y)
     y[1] = sqrt_MUKT * y[4] * sinTheta * x[0];
     y[3] = sqrt_MUKT * y[4] * x[1];
     y[5] = sqrt_MUKT * x[2];
-    if (s->system == ORC_SYS_CO2AR) {
-        double sqrt_IIKT = sqrt(s->II[0] / HkT * T);
-        double sin_theta = sin(y[8]);
-        double x0 = generate_normal(r, 1.0);
-        double x1 = generate_normal(r, 1.0);
-        y[7] = sqrt_IIKT * sin_theta * x1;
-        y[9] = sqrt_IIKT * x0;
+    if (s->system == ORC_SYS_CO2AR) p_linear(s->II[0], T, r, y + 6);
+    if (s->system == ORC_SYS_CH4CO2) {
+        p_rotor(s->IR, T, r, y + 6);
+        p_linear(s->II[0], T, r, y + 12);
     }
 }
 
@@ -77,7 +106,7 @@ static int reject(const orc_system *s, const double *y, double T, double pesmin,
     double Mval = exp(-pesmin * HkT / T);
     double M = 1.5 * Mval;
     double kin_part = orc_kinetic(s, y);
-    double q[5];
+    double q[8];
     int i, nq = orc_qp_size(s) / 2;
     double pot_part, proposal, desired, weight;
     for (i = 0; i < nq; ++i) q[i] = y[2 * i];
@@ -94,12 +123,12 @@ long orc_sample_ics(const orc_system *s, const orc_sampler *sp, uint32_t seed,
     rng r;
     long accepted = 0, candidates = 0;
     int qp = orc_qp_size(s), i;
-    double y[10];
+    double y[16];
     r.draws = 0;
     orc_mt_seed(&r.mt, seed);
 
     while (accepted < ntraj) {
-        for (i = 0; i < 10; ++i) y[i] = 0.0;
+        for (i = 0; i < 16; ++i) y[i] = 0.0;
         q_generator(s, sp, &r, y);
         if (sp->mode == ORC_IC_R0_INJECT) {
             y[4] = sp->R0;
@@ -129,7 +158,7 @@ double orc_dipole_rate_sq(const orc_system *s, const double *y)
 {
     const double h = 1.0e-3;
     int i, k, nq = orc_qp_size(s) / 2;
-    double ydot[10], qv[5], D[5][3], md[3];
+    double ydot[16], qv[8], D[8][3], md[3];
     orc_rhs(s, y, ydot);
     for (i = 0; i < nq; ++i) qv[i] = y[2 * i];
     for (i = 0; i < nq; ++i) {
@@ -159,13 +188,13 @@ static void moment_loop(const orc_system *s, const orc_sampler *sp, uint32_t see
     rng r;
     long n = 0;
     int i, nq = orc_qp_size(s) / 2;
-    double y[10], qv[5], d[3];
+    double y[16], qv[8], d[3];
     double mm = 0.0, qq = 0.0;
     r.draws = 0;
     orc_mt_seed(&r.mt, seed);
     while (n < npoints) {
         double energy, fval, diff;
-        for (i = 0; i < 10; ++i) y[i] = 0.0;
+        for (i = 0; i < 16; ++i) y[i] = 0.0;
         q_generator(s, sp, &r, y);
         p_generator(s, sp->T, &r, y);
         energy = orc_hamiltonian(s, y);

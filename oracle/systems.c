@@ -16,7 +16,7 @@
 
 int orc_qp_size(const orc_system *s)
 {
-    return s->system == ORC_SYS_CO2AR ? 10 : 6;
+    return s->system == ORC_SYS_CH4CO2 ? 16 : s->system == ORC_SYS_CO2AR ? 10 : 6;
 }
 
 /* ------------------------------------------------------------------ He-Ar */
@@ -250,6 +250,7 @@ void orc_co2ar_jacobian(const double *qlab, const double *qmol, double jac[5][5]
 /* ------------------------------------------------------------------ plugin-level API */
 double orc_pes(const orc_system *s, const double *q)
 {
+    if (s->system == ORC_SYS_CH4CO2) return orc_ch4co2_pes(q);
     if (s->system == ORC_SYS_HEAR) return orc_hear_V(q[2]);
     {
         double qmol[5];
@@ -260,6 +261,10 @@ double orc_pes(const orc_system *s, const double *q)
 
 void orc_dpes(const orc_system *s, const double *q, double *dvdq)
 {
+    if (s->system == ORC_SYS_CH4CO2) {
+        orc_ch4co2_dpes(q, dvdq);
+        return;
+    }
     if (s->system == ORC_SYS_HEAR) {                      /* HeAr.h:17-21 */
         dvdq[0] = 0.0;
         dvdq[1] = 0.0;
@@ -284,6 +289,10 @@ void orc_dpes(const orc_system *s, const double *q, double *dvdq)
 
 void orc_dipole(const orc_system *s, const double *q, double mu[3])
 {
+    if (s->system == ORC_SYS_CH4CO2) {
+        orc_ch4co2_dipole(q, mu);
+        return;
+    }
     if (s->system == ORC_SYS_HEAR) {                      /* HeAr.h:23-31 */
         double d = orc_hear_dip(q[2]);
         mu[0] = d * sin(q[1]) * cos(q[0]);
@@ -321,6 +330,10 @@ static void extract_q(const orc_system *s, const double *y, double *q)
 void orc_rhs(const orc_system *s, const double *y, double *ydot)
 {
     double q[5], dV[5];
+    if (s->system == ORC_SYS_CH4CO2) {
+        orc_ch4co2_rhs(s, y, ydot);
+        return;
+    }
     double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5];
     double mu = s->mu;
     double R2, R3, sinTheta, cosTheta, sinTheta2, sinTheta3;
@@ -355,6 +368,7 @@ void orc_rhs(const orc_system *s, const double *y, double *ydot)
 
 double orc_kinetic(const orc_system *s, const double *y)
 {
+    if (s->system == ORC_SYS_CH4CO2) return orc_ch4co2_kinetic(s, y);
     double pPhi = y[1], Theta = y[2], pTheta = y[3], R = y[4], pR = y[5], mu = s->mu;
     double sinTheta = sin(Theta);
     double K = pR * pR / (2.0 * mu) + pTheta * pTheta / (2.0 * mu * R * R) +
@@ -368,7 +382,7 @@ double orc_kinetic(const orc_system *s, const double *y)
 
 double orc_hamiltonian(const orc_system *s, const double *y)
 {
-    double q[5];
+    double q[8];
     double V, K;
     extract_q(s, y, q);
     V = orc_pes(s, q);

@@ -10,7 +10,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(REPO, "oracle", "_build", "liboracle.so")
 REF_DIR = os.path.join(REPO, "oracle", "_ref")
 
-SYS_HEAR, SYS_CO2AR = 0, 1
+SYS_HEAR, SYS_CO2AR, SYS_CH4CO2 = 0, 1, 3
 IC_REJECTION, IC_R0_INJECT, IC_PHASE_SPACE = 0, 1, 2
 PAIR_FREE, PAIR_BOUND, PAIR_ALL = 1, 2, 3
 HALVES_BOTH, HALVES_FORWARD = 0, 1
@@ -22,10 +22,14 @@ MU_CO2AR = 3.817083069753919e4
 I_CO2 = 2.819827114123505e5
 PESMIN_HEAR = -9.424412560638486e-05
 PESMIN_CO2AR = -8.913727688689150e-04
+# example_configs/CF-F-CH4-CO2.conf:5, :27, :47, :52
+MU_CH4CO2 = 2.141790360592430e4
+I_CH4 = 2.093840965876434e4
+PESMIN_CH4CO2 = -1.562522273622102e-03
 
 
 class System(C.Structure):
-    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("II", C.c_double * 2)]
+    _fields_ = [("system", C.c_int), ("mu", C.c_double), ("II", C.c_double * 2), ("IR", C.c_double * 3)]
 
 
 class Sampler(C.Structure):
@@ -100,8 +104,12 @@ def co2ar():
     return System(SYS_CO2AR, MU_CO2AR, (C.c_double * 2)(I_CO2, I_CO2))
 
 
+def ch4co2():
+    return System(SYS_CH4CO2, MU_CH4CO2, (C.c_double * 2)(I_CO2, I_CO2), (C.c_double * 3)(I_CH4, I_CH4, I_CH4))
+
+
 def qp_size(sysid):
-    return 10 if sysid == SYS_CO2AR else 6
+    return {SYS_CO2AR: 10, SYS_CH4CO2: 16}.get(sysid, 6)
 
 
 _lib = None
@@ -129,6 +137,12 @@ def lib():
             "orc_co2ar_dip_mol": (None, [d, d, v]),
             "orc_co2ar_lab_to_mol": (None, [v, v]),
             "orc_co2ar_jacobian": (None, [v, v, v]),
+            "orc_ch4co2_lab_to_kal": (None, [v, v]),
+            "orc_ch4co2_jacobian": (None, [v, v, v]),
+            "orc_ch4co2_pes_kal": (d, [d, d, d, d, d]),
+            "orc_ch4co2_dpes_kal": (None, [d, d, d, d, d, v]),
+            "orc_ch4co2_well_scale": (d, []),
+            "orc_ch4co2_dipole_kal": (None, [v, v]),
             "orc_sample_ics": (C.c_long, [P(System), P(Sampler), u32, C.c_long, v, P(C.c_long)]),
             "orc_half_cf": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(TrajStats)]),
             "orc_traj_cf": (i, [P(System), P(CFParams), v, v, P(C.c_int), P(C.c_int), P(TrajStats)]),

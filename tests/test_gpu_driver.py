@@ -488,3 +488,53 @@ def test_driver_spectral_moments(tmp_path, golden):
     m0_cf, m0_sf, m1_sf, q0 = (float(v[2]) for v in vals)
     assert m0_cf == pytest.approx(float(g["cf"][0]) * H.ZERO_COEFF / 5.29177210903e-11 ** 3, rel=1e-5)
     assert m0_sf == pytest.approx(m0_cf, rel=1e-3) and m1_sf == 0.0 and q0 > 0
+
+
+REF_HEAR_PLUGIN = os.path.join(os.path.dirname(HERE), "oracle", "_ref", "libref_hear.so")
+
+
+def _m0_conf(monomers, plugin):
+    """A short CALCULATE_PHASE_SPACE_M0 configuration with the given &MONOMER blocks and plugins."""
+    mu = "6.632039160324215e+03" if "LINEAR" not in monomers else "3.817083069753919e+04"
+    lines = ["&INPUT", "  CALCULATION_TYPE = CALCULATE_PHASE_SPACE_M0", "  PAIR_STATE = FREE_AND_METASTABLE",
+             "  TEMPERATURE = 300.0", f"  pair_reduced_mass = {mu}", "  sampler_Rmin = 4.0", "  sampler_Rmax = 40.0",
+             "  HEP_M0_NITERATIONS = 3", "  HEP_M0_NPOINTS = 100000"]
+    if plugin:
+        lines += [f'  so_potential = "{plugin}"', f'  so_dipole = "{plugin}"']
+    return "\n".join(lines + ["&END", monomers])
+
+
+ATOM_ATOM = "&MONOMER\n  MONOMER_TYPE = ATOM\n&END\n&MONOMER\n  MONOMER_TYPE = ATOM\n&END\n"
+LINEAR_ATOM = ("&MONOMER\n  MONOMER_TYPE = LINEAR_MOLECULE\n  II = {2.819827114123505e+05, 2.819827114123505e+05, 0.0}\n"
+               "&END\n&MONOMER\n  MONOMER_TYPE = ATOM\n&END\n")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HEAR_PLUGIN), reason="the reference-built He-Ar plugin is not built")
+def test_driver_accepts_the_reference_hear_plugin(tmp_path):
+    """The reference's own He-Ar plugin (PES-IDS/HeAr.h built into oracle/_ref) is the engine's
+    built-in He-Ar surface: its pes_lab, dpes_lab and dipole_lab agree with the device surfaces at
+    the check points, so the run goes ahead."""
+    (tmp_path / "ok.conf").write_text(_m0_conf(ATOM_ATOM, REF_HEAR_PLUGIN))
+    r = run_driver(["ok.conf", "--seed", "3"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "M0:" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HEAR_PLUGIN), reason="the reference-built He-Ar plugin is not built")
+def test_driver_refuses_a_plugin_that_is_not_the_builtin_system(tmp_path):
+    """A LINEAR_MOLECULE + ATOM configuration whose plugin is a He-Ar surface (the situation of the
+    reference's example_configs/CF-F-CO-Ar.conf: a linear molecule with another surface) is refused
+    instead of being integrated on the CO2-Ar surfaces."""
+    (tmp_path / "bad.conf").write_text(_m0_conf(LINEAR_ATOM, REF_HEAR_PLUGIN))
+    r = run_driver(["bad.conf", "--seed", "3"], str(tmp_path))
+    assert r.returncode != 0
+    assert "not the engine's built-in CO2-Ar surface" in (r.stdout + r.stderr)
+
+
+def test_driver_refuses_an_unloadable_plugin(tmp_path):
+    """A plugin that cannot be loaded ends the run with an error (the reference's loader exits,
+    src/driver.c:3083-3094), not a note and a fall-back to the built-in surface."""
+    (tmp_path / "missing.conf").write_text(_m0_conf(ATOM_ATOM, str(tmp_path / "no-such-plugin.so")))
+    r = run_driver(["missing.conf", "--seed", "3"], str(tmp_path))
+    assert r.returncode != 0
+    assert "cannot load the surface plugin" in (r.stdout + r.stderr)

@@ -113,6 +113,17 @@ def co2_ar():
     return System(SYS_CO2AR, MU_CO2AR, I_CO2)
 
 
+# example_configs/CF-F-CH4-CO2.conf:5, :27, :47 (CH4, a spherical top), :52 (CO2)
+MU_CH4CO2 = 2.141790360592430e4
+I_CH4 = 2.093840965876434e4
+PESMIN_CH4CO2 = -1.562522273622102e-03
+
+
+def ch4_co2():
+    """CH4-CO2 (ROTOR + LINEAR_MOLECULE): mu, I of CO2 and the principal moments of CH4."""
+    return System(SYS_CH4CO2, MU_CH4CO2, I_CO2, (C.c_double * 3)(I_CH4, I_CH4, I_CH4))
+
+
 def n2_ar():
     """N2-Ar (examples/correlation_n2_ar.cpp:184-186): mu = m_N2 m_Ar / (m_N2 + m_Ar), I = II_N2."""
     return System(SYS_N2AR, MU_N2AR, I_N2)

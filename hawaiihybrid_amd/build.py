@@ -29,6 +29,7 @@ SOURCES = [
     ("hh_zimm.hip", DEVICE),
     ("hh_vegas.hip", DEVICE),
     ("hh_prmu.hip", DEVICE),
+    ("hh_ch4co2.hip", DEVICE),
     ("hh_moments.cpp", ["-x", "hip"] + DEVICE),
     ("hh_host.cpp", ["-x", "hip"] + DEVICE),
     ("hh_pool.cpp", ["-x", "hip"] + DEVICE),

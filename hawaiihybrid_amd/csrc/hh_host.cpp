@@ -103,8 +103,14 @@ int collect_stats(const std::vector<hh::JobDesc>& jobs, hh_cf_stats* stats)
 
 int check_cf_args(const hh_system* sys, const hh_cf_params* p)
 {
-    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
-        return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_CH4CO2)
+        return fail(HH_E_UNSUPPORTED, "system %d has no trajectory engine", sys->system);
+    if (sys->system == HH_SYS_CH4CO2) {
+        if (p->engine != HH_ENGINE_BATCHED) return fail(HH_E_UNSUPPORTED, "CH4-CO2 runs on the batched engine");
+        if (p->zimmermann) return fail(HH_E_UNSUPPORTED, "CH4-CO2: Zimmermann's trick is not provided");
+        if (!(sys->I > 0) || !(sys->I_rotor[0] > 0) || !(sys->I_rotor[1] > 0) || !(sys->I_rotor[2] > 0))
+            return fail(HH_E_INVALID, "CH4-CO2 needs the CO2 moment of inertia I and the three of CH4 (I_rotor)");
+    }
     if (sys->mu <= 0) return fail(HH_E_INVALID, "invalid reduced mass (%.5e) -- must be positive", sys->mu);
     if (p->L <= 0 || p->dt <= 0 || p->Rcut <= 0 || p->rtol <= 0 || p->atol <= 0)
         return fail(HH_E_INVALID, "MaxTrajectoryLength, sampling_time, Rcut and tolerances must be positive");
@@ -130,6 +136,14 @@ int check_cf_args(const hh_system* sys, const hh_cf_params* p)
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out)
 {
     if (jd.record == 2) return HH_OK;   // PR/mu series: the spectrum is formed by its caller
+    if (jd.record == 3) {
+        // CH4-CO2: dipoles of the recorded samples and the CF rows, then the usual reduction
+        HH_CHECK(run_ch4co2_cf_rows(jd, jd.f32_dipole ? e->ch4tab32.p : e->ch4tab64.p, jd.f32_dipole != 0, e->stream));
+        JobDesc jr = jd;
+        jr.rows = jd.cf_rows;
+        jr.record = 0;
+        return reduce_job(e, jr, ppf, cf_out);
+    }
     if (jd.record) {
         const size_t ws = zimm_workspace_doubles(jd.L);
         HH_CHECK(e->psum.ensure(ws * sizeof(double)));
@@ -263,8 +277,12 @@ long hh_sample_ics_from(const hh_system* sys, const hh_sampler* smp, uint32_t se
 {
     if (!sys || !smp || !ic || ntraj < 0) return fail(HH_E_INVALID, "hh_sample_ics: null argument or ntraj < 0");
     if (skip_draws < 0) return fail(HH_E_INVALID, "hh_sample_ics_from: skip_draws < 0");
-    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_N2AR)
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_N2AR &&
+        sys->system != HH_SYS_CH4CO2)
         return fail(HH_E_UNSUPPORTED, "hh_sample_ics: system %d has no sampler", sys->system);
+    if (sys->system == HH_SYS_CH4CO2 &&
+        !(sys->I > 0 && sys->I_rotor[0] > 0 && sys->I_rotor[1] > 0 && sys->I_rotor[2] > 0))
+        return fail(HH_E_INVALID, "CH4-CO2 needs the CO2 moment of inertia I and the three of CH4 (I_rotor)");
     if (smp->mode != HH_IC_REJECTION && smp->mode != HH_IC_R0_INJECT && smp->mode != HH_IC_PHASE_SPACE)
         return fail(HH_E_INVALID, "hh_sample_ics: unknown mode %d", smp->mode);
     if (sys->system == HH_SYS_N2AR && smp->mode == HH_IC_REJECTION && !pes)
@@ -328,6 +346,19 @@ int hh_engine_create(int device, hh_engine** out)
             return fail(HH_E_DEVICE, "table upload failed");
         }
     }
+    {
+        // CH4-CO2 dipole tables
+        for (int f32 = 0; f32 < 2; ++f32) {
+            const size_t nb = hh::ch4co2_table_bytes(f32);
+            std::vector<char> host(nb);
+            hh::ch4co2_tables_host(f32, host.data());
+            hh::DevBuf& d = f32 ? e->ch4tab32 : e->ch4tab64;
+            if (d.ensure(nb) != hipSuccess || hipMemcpy(d.p, host.data(), nb, hipMemcpyHostToDevice) != hipSuccess) {
+                hh_engine_destroy(e);
+                return fail(HH_E_NOMEM, "CH4-CO2 table upload failed");
+            }
+        }
+    }
     *out = e;
     return HH_OK;
 }
@@ -339,7 +370,7 @@ void hh_engine_destroy(hh_engine* e)
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     e->last_pool_jobs.clear();
     e->pools.clear();
-    for (DevBuf* b : {&e->tab64, &e->tab32, &e->rows, &e->len, &e->status, &e->nrhs, &e->drift, &e->jobs, &e->queue,
+    for (DevBuf* b : {&e->tab64, &e->tab32, &e->ch4tab64, &e->ch4tab32, &e->rows, &e->len, &e->status, &e->nrhs, &e->drift, &e->jobs, &e->queue,
                       &e->psum, &e->pcomp, &e->fft_a, &e->fft_b, &e->m0_state, &e->arena})
         b->release();
     for (auto& ev : e->ev)
@@ -371,6 +402,10 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
                const long* ntraj, double* const* cf_dev, hh_cf_stats* stats, int record)
 {
     if (record < 0) record = p && p->zimmermann ? 1 : 0;
+    if (sys && sys->system == HH_SYS_CH4CO2) {
+        if (record != 0) return fail(HH_E_UNSUPPORTED, "CH4-CO2: only correlation functions are provided");
+        record = 3;   // coordinates per sample; the dipole pass follows the integration
+    }
     if (!e || !sys || !p || njobs <= 0 || !ic_dev || !ntraj || !cf_dev)
         return fail(HH_E_INVALID, "hh_cf_run: null argument or njobs <= 0");
     int rc = hh::check_cf_args(sys, p);
@@ -487,7 +522,7 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
         jb.atol = p->atol;
     }
     HH_CHECK(hipMemcpyAsync(e->jobs.p, jobs.data(), njobs * sizeof(hh::JobDesc), hipMemcpyHostToDevice, e->stream));
-    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    const hh::SysConst sc = hh::sysconst(e, *sys, false);
     const int grid = (int)((total + 255) / 256);
     // per-wavefront stage arena (kSlots * QP * 64 doubles per wavefront)
     const size_t per_lane = (size_t)hh::arena_doubles_per_lane(sys->system) * sizeof(double);
@@ -533,11 +568,12 @@ int hh_cf_items(hh_engine* e, int job, int* len, int* status, int* nrhs, float* 
     if (e_drift) HH_CHECK(hipMemcpy(e_drift, jb.e_drift, n * sizeof(float), hipMemcpyDeviceToHost));
     if (rows) {
         // one copy of the whole block, then zero what lies past each item's length (never written
-        // on the device)
+        // on the device); a CH4-CO2 job's rows are its CF rows (the coordinates are internal)
         std::vector<int> ln(n);
         HH_CHECK(hipMemcpy(ln.data(), jb.len, n * sizeof(int), hipMemcpyDeviceToHost));
-        const int rec = jb.record ? 3 : 1;
-        HH_CHECK(hipMemcpy(rows, jb.rows, n * (size_t)jb.L * rec * sizeof(double), hipMemcpyDeviceToHost));
+        const int rec = jb.record == 3 ? 1 : (jb.record ? 3 : 1);
+        const double* src = jb.record == 3 ? jb.cf_rows : jb.rows;
+        HH_CHECK(hipMemcpy(rows, src, n * (size_t)jb.L * rec * sizeof(double), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n * rec; ++i) {
             const size_t from = (size_t)std::max(0, std::min(ln[i / rec], jb.L));
             std::memset(rows + i * jb.L + from, 0, (jb.L - from) * sizeof(double));
@@ -578,7 +614,7 @@ int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const
     const int M = a->ntemp;
     hh::DevBuf en, wr_dev, ns_dev;
     HH_CHECK(en.ensure((size_t)ntraj * sizeof(double)));
-    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    const hh::SysConst sc = hh::sysconst(e, *sys, false);
     HH_CHECK(hh::run_eval(sys->system, sc, HH_EVAL_HAMILTONIAN, ic_dev, ntraj, en.as<double>(), e->stream));
     std::vector<double> E((size_t)ntraj);
     std::vector<int> status((size_t)jd.n_items);
@@ -633,13 +669,23 @@ int hh_cf_array(hh_engine* e, const hh_system* sys, const hh_cf_params* p, const
 int hh_eval(hh_engine* e, const hh_system* sys, int what, const double* y_dev, long n, double* out_dev)
 {
     if (!e || !sys || !y_dev || !out_dev || n <= 0) return fail(HH_E_INVALID, "hh_eval: bad argument");
-    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR)
+    if (sys->system != HH_SYS_HEAR && sys->system != HH_SYS_CO2AR && sys->system != HH_SYS_CH4CO2)
         return fail(HH_E_UNSUPPORTED, "unknown system %d", sys->system);
     if (what < HH_EVAL_PES || what > HH_EVAL_DIPOLE_RATE)
         return fail(HH_E_INVALID, "hh_eval: unknown quantity %d", what);
+    if (sys->system == HH_SYS_CH4CO2 && what == HH_EVAL_DIPOLE_RATE)
+        return fail(HH_E_UNSUPPORTED, "CH4-CO2: |d mu/dt|^2 is not provided");
     std::lock_guard<std::recursive_mutex> lock(e->mu);
     HH_CHECK(hipSetDevice(e->device));
-    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    if (sys->system == HH_SYS_CH4CO2 && what == HH_EVAL_DIPOLE) {
+        hh::DevBuf rec, ln;
+        if (rec.ensure((size_t)n * 8 * sizeof(double)) != hipSuccess || ln.ensure(sizeof(int)) != hipSuccess)
+            return fail(HH_E_NOMEM, "hh_eval: no memory for the CH4-CO2 dipole records");
+        HH_CHECK(hh::run_ch4co2_eval_dipole(y_dev, n, rec.as<double>(), ln.as<int>(), e->ch4tab64.p, out_dev, e->stream));
+        HH_CHECK(hipStreamSynchronize(e->stream));
+        return HH_OK;
+    }
+    const hh::SysConst sc = hh::sysconst(e, *sys, false);
     HH_CHECK(hh::run_eval(sys->system, sc, what, y_dev, n, out_dev, e->stream));
     HH_CHECK(hipStreamSynchronize(e->stream));
     return HH_OK;
@@ -696,7 +742,7 @@ int moment_run(hh_engine* e, const hh_system* sys, const hh_sampler* smp, uint32
     HH_CHECK(e->m0_state.ensure(hh::m0_bytes(nstreams)));
     const hh::M0Acc acc = hh::m0_view(e->m0_state.p, nstreams);
     HH_CHECK(hipMemcpyAsync(acc.finished, &kZeroInt, sizeof(int), hipMemcpyHostToDevice, e->stream));
-    hh::SysConst sc{sys->mu, sys->I, e->tab64.p};
+    const hh::SysConst sc = hh::sysconst(e, *sys, false);
     hh::M0Params mp{smp->T, smp->Rmin, smp->Rmax, smp->pesmin, smp->pair_state, 0};
     HH_CHECK(hipEventRecord(e->ev[0], e->stream));
     HH_CHECK(hh::run_m0(sys->system, sc, mp, seed, nstreams, npoints, reinterpret_cast<uint32_t*>(e->m0_state.p), acc, 1,

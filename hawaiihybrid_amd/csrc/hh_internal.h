@@ -31,6 +31,15 @@ hipError_t run_pool_work(const LaneSlots& sl, int ring_slot, const int* item_nrh
                          unsigned long long* out, hipStream_t st);
 hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* scratch, hipStream_t st);
 hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long long n, double* out, hipStream_t st);
+// CH4-CO2 dipole tables (hh_ch4co2.hip): bytes and host image in FP32 or FP64; the dipole pass of a
+// record-3 job (dipoles over its coordinate rows, then the CF rows jd.cf_rows with the sample checks)
+size_t ch4co2_table_bytes(bool f32);
+void ch4co2_tables_host(bool f32, void* out);
+hipError_t run_ch4co2_cf_rows(const JobDesc& jd, const void* tables, bool f32, hipStream_t st);
+// CH4-CO2 lab dipoles at n phase points (SoA [16][n]) into out[3][n] (FP64 tables); records holds
+// 8 n doubles of scratch, len one int
+hipError_t run_ch4co2_eval_dipole(const double* y, long long n, double* records, int* len, const void* tables,
+                                  double* out, hipStream_t st);
 hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st);
 size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
@@ -189,6 +198,7 @@ struct hh_engine {
     hipDeviceProp_t prop{};
     int wall_khz = 100000;        // s_memrealtime frequency
     hh::DevBuf tab64, tab32;      // Co2ArTables<double>, Co2ArTables<float>
+    hh::DevBuf ch4tab64, ch4tab32;   // CH4-CO2 dipole tables (FP64, FP32)
     hh::DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, arena;
     hh::DevBuf zimm_tw;           // twiddles e^{2 pi i j/P} of the Zimmermann transforms
     hh::DevBuf prmu_a, prmu_b, prmu_w;   // PR/mu: FFT ping-pong series, weights and spectra
@@ -203,6 +213,13 @@ struct hh_engine {
 };
 
 namespace hh {
+// the surface constants of a system for the device kernels (f32: the FP32 tables of CO2-Ar)
+inline SysConst sysconst(const hh_engine* e, const hh_system& sys, bool f32)
+{
+    SysConst sc{sys.mu, sys.I, (f32 && sys.system == HH_SYS_CO2AR) ? e->tab32.p : e->tab64.p, {0.0, 0.0, 0.0}};
+    for (int i = 0; i < 3; ++i) sc.IR[i] = sys.I_rotor[i];
+    return sc;
+}
 // the ensemble CF of a finished job into cf_out (on the engine stream)
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;

@@ -1044,7 +1044,7 @@ __global__ void __launch_bounds__(256) eval_kernel(SysConst sc, int what, const 
     } else if (what == 2) {
         out[i] = hamiltonian<SYS, double>(sc, v);
     } else if (what == 4) {
-        out[i] = dipole_rate_sq<SYS>(sc, v);
+        if constexpr (SYS != SYS_CH4CO2) out[i] = dipole_rate_sq<SYS>(sc, v);
     } else {
         double f[N];
         rhs<SYS, double>(sc, v, f);
@@ -1282,7 +1282,12 @@ hipError_t launch_pool(const SysConst& sc, const JobDesc* ring, const PoolCtl& c
 
 // explicit C++ entry points used by hh_host.cpp
 namespace hh {
-int arena_doubles_per_lane(int sys) { return kSlots * (sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP); }
+static int qp_of(int sys)
+{
+    return sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : sys == SYS_CH4CO2 ? SysTraits<SYS_CH4CO2>::QP : SysTraits<SYS_CO2AR>::QP;
+}
+
+int arena_doubles_per_lane(int sys) { return kSlots * qp_of(sys); }
 
 hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int njobs, long long total, double* ws,
                      hipStream_t st)
@@ -1297,6 +1302,12 @@ hipError_t run_pool(int sys, bool mixed, int block, int waves, const SysConst& s
                     const PoolCtl& ctl, const LaneSlots& sl, double* ws, hipStream_t st)
 {
 #define HH_LAUNCH(S, PT, B, W) return launch_pool<S, PT, B, W>(sc, ring, ctl, sl, ws, st)
+    if (sys == SYS_CH4CO2) {
+        // one launch shape: 256-lane workgroups, one wavefront per SIMD
+        if (block != 256 || waves != 1) return hipErrorInvalidValue;
+        if (mixed) HH_LAUNCH(SYS_CH4CO2, float, 256, 1);
+        HH_LAUNCH(SYS_CH4CO2, double, 256, 1);
+    }
     if (waves == 1) {
         if (block != 256) return hipErrorInvalidValue;
         // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
@@ -1361,6 +1372,8 @@ hipError_t run_repack(int sys, const LaneSlots& src, const LaneSlots& dst, int* 
     const long long W = src.S / 64;
     if (sys == SYS_HEAR)
         hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_HEAR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
+    else if (sys == SYS_CH4CO2)
+        hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_CH4CO2>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
     else
         hipLaunchKernelGGL(repack_move_kernel<SysTraits<SYS_CO2AR>::QP>, dim3(nb), dim3(256), 0, st, src, dst, scratch, W);
     return hipGetLastError();
@@ -1371,6 +1384,8 @@ hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long
     const unsigned g = (unsigned)((n + 255) / 256);
     if (sys == SYS_HEAR)
         hipLaunchKernelGGL(eval_kernel<SYS_HEAR>, dim3(g), dim3(256), 0, st, sc, what, y, n, out);
+    else if (sys == SYS_CH4CO2)
+        hipLaunchKernelGGL(eval_kernel<SYS_CH4CO2>, dim3(g), dim3(256), 0, st, sc, what, y, n, out);
     else
         hipLaunchKernelGGL(eval_kernel<SYS_CO2AR>, dim3(g), dim3(256), 0, st, sc, what, y, n, out);
     return hipGetLastError();
@@ -1388,13 +1403,13 @@ hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, i
 
 size_t slots_bytes(int sys, long long S)
 {
-    const int N = sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP;
+    const int N = qp_of(sys);
     return (size_t)S * ((2 * N + 8) * sizeof(double) + 6 * sizeof(int) + sizeof(long long));
 }
 
 LaneSlots slots_view(int sys, void* base, long long S)
 {
-    const int N = sys == SYS_HEAR ? SysTraits<SYS_HEAR>::QP : SysTraits<SYS_CO2AR>::QP;
+    const int N = qp_of(sys);
     LaneSlots sl;
     sl.S = S;
     sl.d = static_cast<double*>(base);

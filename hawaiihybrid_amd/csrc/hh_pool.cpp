@@ -30,7 +30,8 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
 {
     for (auto& p : set)
         if (p->sys == sys.system && p->mixed == mixed && p->block == block && p->waves == waves &&
-            p->system.mu == sys.mu && p->system.I == sys.I) {
+            p->system.mu == sys.mu && p->system.I == sys.I && p->system.I_rotor[0] == sys.I_rotor[0] &&
+            p->system.I_rotor[1] == sys.I_rotor[1] && p->system.I_rotor[2] == sys.I_rotor[2]) {
             *out = p.get();
             return HH_OK;
         }
@@ -71,7 +72,8 @@ size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record)
     if (record < 0) record = p.zimmermann ? 1 : 0;
     // rows ([3][L] per item for a dipole series), len, status, nrhs, e_drift, the job's share
     // of its group's order and its list of items handed over to FP64
-    const size_t row = (size_t)p.L * (record ? 3 : 1);
+    // CH4-CO2 (record 3): eight coordinates per sample and the CF rows the dipole pass writes
+    const size_t row = (size_t)p.L * (record == 3 ? 9 : row_width(record));
     return (size_t)items * (row * sizeof(double) + 5 * sizeof(int) + sizeof(float));
 }
 
@@ -107,7 +109,9 @@ int pool_admit(hh_engine* e, LanePool& pl, const hh_cf_params& p, const double* 
     jd.ic = ic_dev;
     jd.rows = job->buf.as<double>();
     jd.record = record;
-    jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L * (jd.record ? 3 : 1));
+    jd.cf_rows = record == 3 ? jd.rows + (size_t)n_items * p.L * 8 : nullptr;
+    jd.f32_dipole = pl.mixed ? 1 : 0;
+    jd.len = reinterpret_cast<int*>(jd.rows + (size_t)n_items * p.L * (record == 3 ? 9 : row_width(record)));
     jd.status = jd.len + n_items;
     jd.nrhs = jd.status + n_items;
     jd.e_drift = reinterpret_cast<float*>(jd.nrhs + n_items);
@@ -233,7 +237,7 @@ int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const d
         ctl.last_seq = pl->inflight.back()->seq;
         ctl.ticks = (long long)(kSliceMs * e->wall_khz);
         ctl.ring = pl->ring_cap;
-        SysConst sc{sys.mu, sys.I, (mixed && sys.system == HH_SYS_CO2AR) ? e->tab32.p : e->tab64.p};
+        const SysConst sc = sysconst(e, sys, mixed);
         HH_CHECK(hipEventRecord(e->ev[0], e->stream));
         HH_CHECK(run_pool(pl->sys, mixed, pl->block, pl->waves, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
                           pl->arena.as<double>(), e->stream));
@@ -330,11 +334,7 @@ int launch_slice_graph(hh_engine* e, LanePool& pl, const PoolCtl& ctl, const Sys
 
 namespace {
 
-SysConst pool_sysconst(const hh_engine* e, const LanePool& pl)
-{
-    const bool f32 = pl.mixed && pl.sys == HH_SYS_CO2AR;
-    return SysConst{pl.system.mu, pl.system.I, f32 ? e->tab32.p : e->tab64.p};
-}
+SysConst pool_sysconst(const hh_engine* e, const LanePool& pl) { return sysconst(e, pl.system, pl.mixed); }
 
 // The items that `parent` handed over since its last orbit job, [parent.orbit_admitted, count)
 // of its list, as one orbit job of the FP64 pool of the same class: the job shares the parent's

@@ -16,6 +16,7 @@
 #include <cstring>
 
 #include "../../include/hawaii_gpu.h"
+#include "ch4co2_standin.h"
 #include "co2ar_tables.h"
 
 namespace {
@@ -120,17 +121,61 @@ double co2ar_V(const double* q)
     return V;
 }
 
+// CH4-CO2 stand-in surface (ch4co2_standin.h) at lab q[8]: the directions of the intermolecular
+// vector and the CO2 axis in the frame of CH4 (S1 = Sz(psi1) Sx(theta1) Sz(phi1) applied to the
+// reference's lab vectors, src/angles_handler.cpp:357-419), then the site sum
+double ch4co2_V(const double* q)
+{
+    using namespace hh::ch4co2;
+    const double v[2][3] = {{-std::sin(q[1]) * std::cos(q[0]), -std::sin(q[1]) * std::sin(q[0]), std::cos(q[1])},
+                            {-std::sin(q[7]) * std::cos(q[6]), -std::sin(q[7]) * std::sin(q[6]), std::cos(q[7])}};
+    const double sf = std::sin(q[3]), cf = std::cos(q[3]), s1 = std::sin(q[4]), c1 = std::cos(q[4]);
+    const double sp = std::sin(q[5]), cp = std::cos(q[5]);
+    double ab[2][3];
+    for (int k = 0; k < 2; ++k) {
+        const double w0 = cf * v[k][0] + sf * v[k][1], w1 = -sf * v[k][0] + cf * v[k][1], w2 = v[k][2];
+        const double x1 = c1 * w1 + s1 * w2, x2 = -s1 * w1 + c1 * w2;
+        ab[k][0] = cp * w0 + sp * x1;
+        ab[k][1] = -sp * w0 + cp * x1;
+        ab[k][2] = x2;
+    }
+    const double R = q[2], hs = kRCH / std::sqrt(3.0);
+    double V = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        const double sg = j == 0 ? 0.0 : (j == 1 ? 1.0 : -1.0);
+        const int tj = j == 0 ? 2 : 3;
+        for (int i = 0; i < 5; ++i) {
+            const int ti = i == 0 ? 0 : 1;
+            const double sig = 0.5 * (kSigma[ti] + kSigma[tj]) * kAng;
+            const double e4 = 4.0 * kWellScale * kKelvinHartree * std::sqrt(kEpsilon[ti] * kEpsilon[tj]);
+            double r2 = 0.0;
+            for (int c = 0; c < 3; ++c) {
+                const double h = i == 0 ? 0.0 : hs * kH[i - 1][c];
+                const double d = R * ab[0][c] + sg * kRCO * ab[1][c] - h;
+                r2 += d * d;
+            }
+            const double sr2 = sig * sig / r2, sr6 = sr2 * sr2 * sr2;
+            V += e4 * (sr6 * sr6 - sr6);
+        }
+    }
+    return V;
+}
+
 struct Sys {
     int id;
     double mu, I;
+    const double* IR;
     hh_pes_fn pes;
     bool linear() const { return id == HH_SYS_CO2AR || id == HH_SYS_N2AR; }   // LINEAR_MOLECULE + ATOM
+    bool rotor_linear() const { return id == HH_SYS_CH4CO2; }                 // ROTOR + LINEAR_MOLECULE
+    int qp() const { return rotor_linear() ? 16 : linear() ? 10 : 6; }
     double V(const double* y) const
     {
-        double q[5];
-        const int nq = linear() ? 5 : 3;
+        double q[8];
+        const int nq = qp() / 2;
         for (int i = 0; i < nq; ++i) q[i] = y[2 * i];
         if (pes) return pes(q);
+        if (rotor_linear()) return ch4co2_V(q);
         return id == HH_SYS_CO2AR ? co2ar_V(q) : hear_V(q[2]);
     }
     // kinetic_energy, src/hawaii.c:1041-1137 (same operation order)
@@ -143,6 +188,22 @@ struct Sys {
         if (linear()) {
             const double st = std::sin(y[8]);
             k1 = k1 + (y[9] * y[9] / (2.0 * I) + y[7] * y[7] / (2.0 * I * st * st));
+        }
+        if (rotor_linear()) {
+            // KIN1 + KIN2 (ROTOR monomer 1) + KIN3 (LINEAR_MOLECULE monomer 2)
+            const double pphi1t = y[7], theta = y[8], ptheta1t = y[9], psi = y[10], ppsi1t = y[11];
+            const double sin_theta = std::sin(theta), cos_theta = std::cos(theta);
+            const double sin_psi = std::sin(psi), cos_psi = std::cos(psi);
+            const double KIN2 = ((pphi1t - ppsi1t * cos_theta) * sin_psi + ptheta1t * sin_theta * cos_psi) *
+                                ((pphi1t - ppsi1t * cos_theta) * sin_psi + ptheta1t * sin_theta * cos_psi) /
+                                (2.0 * IR[0] * sin_theta * sin_theta) +
+                                ((pphi1t - ppsi1t * cos_theta) * cos_psi - ptheta1t * sin_theta * sin_psi) *
+                                ((pphi1t - ppsi1t * cos_theta) * cos_psi - ptheta1t * sin_theta * sin_psi) /
+                                (2.0 * IR[1] * sin_theta * sin_theta) +
+                                ppsi1t * ppsi1t / (2.0 * IR[2]);
+            const double sin_theta2t = std::sin(y[14]);
+            const double KIN3 = y[15] * y[15] / (2.0 * I) + y[13] * y[13] / (2.0 * I * sin_theta2t * sin_theta2t);
+            return k1 + KIN2 + KIN3;
         }
         return k1;
     }
@@ -172,6 +233,39 @@ void gen_q(const Sys& s, const hh_sampler& p, Mt& mt, double* y)
         y[6] = mt.drand() * 2.0 * M_PI;
         y[8] = std::acos(2.0 * mt.drand() - 1.0);
     }
+    if (s.rotor_linear()) {   // ROTOR: phi, theta, psi; then LINEAR_MOLECULE: phi, theta
+        y[6] = mt.drand() * 2.0 * M_PI;
+        y[8] = std::acos(2.0 * mt.drand() - 1.0);
+        y[10] = mt.drand() * 2.0 * M_PI;
+        y[12] = mt.drand() * 2.0 * M_PI;
+        y[14] = std::acos(2.0 * mt.drand() - 1.0);
+    }
+}
+
+// p_generator_linear_molecule, src/hawaii.c:1292-1303 (m: phi, pphi, theta, ptheta)
+void gen_p_linear(double I, double T, Mt& mt, double* m)
+{
+    const double sqrt_IIKT = std::sqrt(I / kHkT * T);
+    const double sin_theta = std::sin(m[2]);
+    const double x0 = normal(mt), x1 = normal(mt);
+    m[1] = sqrt_IIKT * sin_theta * x1;
+    m[3] = sqrt_IIKT * x0;
+}
+
+// p_generator_rotor, src/hawaii.c:1305-1327 (m: phi, pphi, theta, ptheta, psi, ppsi)
+void gen_p_rotor(const double* II, double T, Mt& mt, double* m)
+{
+    const double theta = m[2], psi = m[4];
+    const double sqrt_IIx = std::sqrt(II[0] / kHkT * T), sqrt_IIy = std::sqrt(II[1] / kHkT * T);
+    const double sqrt_IIz = std::sqrt(II[2] / kHkT * T);
+    // the reference calls sincos, whose cosine can differ from cos in the last bit
+    double sin_theta, cos_theta, sin_psi, cos_psi;
+    ::sincos(theta, &sin_theta, &cos_theta);
+    ::sincos(psi, &sin_psi, &cos_psi);
+    const double x0 = normal(mt), x1 = normal(mt), x2 = normal(mt);
+    m[1] = sqrt_IIx * sin_theta * sin_psi * x0 + sqrt_IIy * sin_theta * cos_psi * x1 + sqrt_IIz * cos_theta * x2;
+    m[3] = sqrt_IIx * cos_psi * x0 - sqrt_IIy * sin_psi * x1;
+    m[5] = sqrt_IIz * x2;
 }
 
 // p_generator + p_generator_linear_molecule, src/hawaii.c:1292-1380
@@ -183,12 +277,10 @@ void gen_p(const Sys& s, double T, Mt& mt, double* y)
     y[1] = sqrt_MUKT * y[4] * sinTheta * x0;
     y[3] = sqrt_MUKT * y[4] * x1;
     y[5] = sqrt_MUKT * x2;
-    if (s.linear()) {
-        const double sqrt_IIKT = std::sqrt(s.I / kHkT * T);
-        const double sin_theta = std::sin(y[8]);
-        const double r0 = normal(mt), r1 = normal(mt);
-        y[7] = sqrt_IIKT * sin_theta * r1;
-        y[9] = sqrt_IIKT * r0;
+    if (s.linear()) gen_p_linear(s.I, T, mt, y + 6);
+    if (s.rotor_linear()) {
+        gen_p_rotor(s.IR, T, mt, y + 6);
+        gen_p_linear(s.I, T, mt, y + 12);
     }
 }
 
@@ -219,12 +311,12 @@ void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n)
 __attribute__((visibility("hidden"))) long hh_sample_ics_impl(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long skip,
                                                               long ntraj, hh_pes_fn pes, double* ic, long* n_draws)
 {
-    const Sys s{sys->system, sys->mu, sys->I, pes};
-    const int qp = s.linear() ? 10 : 6;
+    const Sys s{sys->system, sys->mu, sys->I, sys->I_rotor, pes};
+    const int qp = s.qp();
     Mt mt(seed);
     for (long i = 0; i < skip; ++i) (void)mt.drand();   // one word per draw: the stream after `skip` draws
     long accepted = 0, candidates = 0;
-    double y[10];
+    double y[16];
     while (accepted < ntraj) {
         std::memset(y, 0, sizeof(y));
         gen_q(s, *smp, mt, y);

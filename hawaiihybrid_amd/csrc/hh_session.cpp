@@ -102,7 +102,7 @@ int hh_session::slice(std::string& err)
         Request& r = *rp;
         if (r.kind != HH_REQ_M0) continue;
         const hh::M0Acc acc = hh::m0_view(r.m0buf.p, r.nstreams);
-        hh::SysConst sc{r.sys.mu, r.sys.I, e->tab64.p};
+        const hh::SysConst sc = hh::sysconst(e, r.sys, false);
         hh::M0Params mp{r.smp.T, r.smp.Rmin, r.smp.Rmax, r.smp.pesmin, r.smp.pair_state, 0};
         const hipError_t he = hh::run_m0(r.sys.system, sc, mp, r.seed, r.nstreams, r.npoints,
                                          reinterpret_cast<uint32_t*>(r.m0buf.p), acc, r.m0_started ? 0 : 1,

@@ -10,6 +10,7 @@
  *   R0 injection     src/hawaii.c:3486-3512  (q_generator, R := R0, p_generator, pR := -|pR|)
  *   calculate_M0     src/hawaii.c:1423-1491, calculate_M2 src/hawaii.c:1524-1617 (same loop)
  */
+#define _GNU_SOURCE   /* sincos */
 #include <math.h>
 #include "oracle.h"
 
@@ -73,10 +74,14 @@ static void p_rotor(const double *II, double T, rng *r, double *m)
 {
     double theta = m[2], psi = m[4];
     double sqrt_IIx = sqrt(II[0] / HkT * T), sqrt_IIy = sqrt(II[1] / HkT * T), sqrt_IIz = sqrt(II[2] / HkT * T);
-    double sin_theta = sin(theta), cos_theta = cos(theta), sin_psi = sin(psi), cos_psi = cos(psi);
-    double x0 = generate_normal(r, 1.0);
-    double x1 = generate_normal(r, 1.0);
-    double x2 = generate_normal(r, 1.0);
+    double sin_theta, cos_theta, sin_psi, cos_psi;
+    double x0, x1, x2;
+    /* the reference calls sincos, whose cosine can differ from cos in the last bit */
+    sincos(theta, &sin_theta, &cos_theta);
+    sincos(psi, &sin_psi, &cos_psi);
+    x0 = generate_normal(r, 1.0);
+    x1 = generate_normal(r, 1.0);
+    x2 = generate_normal(r, 1.0);
     m[1] = sqrt_IIx * sin_theta * sin_psi * x0 + sqrt_IIy * sin_theta * cos_psi * x1 + sqrt_IIz * cos_theta * x2;
     m[3] = sqrt_IIx * cos_psi * x0 - sqrt_IIy * sin_psi * x1;
     m[5] = sqrt_IIz * x2;

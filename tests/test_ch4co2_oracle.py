@@ -238,3 +238,19 @@ def test_rotor_sampler_boltzmann_moments():
         h, _ = np.histogram(a, bins=10, range=(-1, 1))
         assert np.all(np.abs(h / (n / 10) - 1) < 0.1)
     assert np.all(ic[:, 5] <= 0) and np.all(ic[:, 4] == 40.0)
+
+
+def test_host_sampler_matches_oracle_bit_for_bit():
+    """The engine's host sampler (hh_sample_ics) for CH4-CO2 against the oracle on 20000 R0-injected
+    initial conditions: the same draws in the same order, and p_generator_rotor's sincos (whose
+    cosine differs from cos in the last bit for some angles, src/hawaii.c:1317-1318)."""
+    import hawaiihybrid_amd as H
+    n = 20000
+    smp = O.Sampler(O.IC_R0_INJECT, O.PAIR_FREE, 300.0, 4.75, 40.0, O.PESMIN_CH4CO2, 40.0)
+    ic = np.zeros((n, 16))
+    nd = C.c_long(0)
+    O.lib().orc_sample_ics(C.byref(O.ch4co2()), C.byref(smp), 42, n, O.ptr(ic), C.byref(nd))
+    sm = H.Sampler(H.IC_R0_INJECT, 1, 300.0, 4.75, 40.0, H.PESMIN_CH4CO2, 40.0)
+    ic2, _, nd2 = H.sample_ics(H.ch4_co2(), sm, 42, n)
+    assert np.array_equal(ic, ic2)
+    assert nd2 == nd.value

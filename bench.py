@@ -161,9 +161,11 @@ def n2ar_section(H, eng, npoints, sample_check):
         dE = np.abs(E - refE)
         out[name] = {"ms": ms, "points_per_s": npoints / (ms * 1e-3),
                      "matrix_tflops": npoints * (f_pes + f_dip) / (ms * 1e-3) / 1e12,
-                     "dE_cm1_median": float(np.median(dE)), "dE_cm1_max": float(dE.max()),
+                     "dE_cm1_median": float(np.median(dE)), "dE_cm1_p99": float(np.quantile(dE, 0.99)),
+                     "dE_cm1_p999": float(np.quantile(dE, 0.999)), "dE_cm1_max": float(dE.max()),
                      "dE_rel_max_above_100cm1": float(np.max((dE / np.abs(refE))[np.abs(refE) > 100.0])),
-                     "dmu_over_max_mu": float(np.max(np.abs(MU - refmu)) / musc)}
+                     "dmu_over_max_mu": float(np.max(np.abs(MU - refmu)) / musc),
+                     "dmu_over_max_mu_p99": float(np.quantile(np.max(np.abs(MU - refmu), axis=0), 0.99) / musc)}
         model.close()
         syn = n2ar.synthetic_pes(128)
         sm = n2ar.PipNN(eng, precision=prec, pes_layers=syn, pes_act=n2ar.ACT_TANH, with_dipole=False, weights=W)
@@ -361,7 +363,7 @@ def main():
 
     if not args.no_n2ar:
         log(f"N2-Ar PIP-NN surfaces: {args.n2ar_points} phase points")
-        result["n2ar_pipnn"] = n2ar_section(H, eng, args.n2ar_points, 2000 if not args.no_oracle else 64)
+        result["n2ar_pipnn"] = n2ar_section(H, eng, args.n2ar_points, 16384 if not args.no_oracle else 64)
         log("n2ar", result["n2ar_pipnn"])
 
     if not args.no_oracle:

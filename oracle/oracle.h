@@ -231,6 +231,8 @@ void   orc_n2ar_dpes_xyz(const orc_n2ar *m, const double xyz[9], double g[9]); /
 double orc_n2ar_pes(const orc_n2ar *m, const double q[5]);                    /* Hartree, lab q */
 void   orc_n2ar_dpes(const orc_n2ar *m, const double q[5], double dvdq[5]);   /* Hartree / (bohr, rad) */
 void   orc_n2ar_dipole_bf(const orc_n2ar *m, double R, double theta, double d[3]);
+/* one charge model alone: 0 short-range (a0 = 2), 1 long-range (a0 = 6) */
+void   orc_n2ar_dipole_model_bf(const orc_n2ar *m, int model, double R, double theta, double d[3]);
 void   orc_n2ar_dipole(const orc_n2ar *m, const double q[5], double mu[3]);   /* lab, a.u. */
 /* plain network forward of the synthetic shape on the PES features (no scalers) */
 double orc_n2ar_mlp_on_pes_features(const orc_mlp *net, const double *xmean, const double *xscale,

@@ -171,6 +171,7 @@ def lib():
             "orc_n2ar_pes": (d, [P(OrcN2ar), v]),
             "orc_n2ar_dpes": (None, [P(OrcN2ar), v, v]),
             "orc_n2ar_dipole_bf": (None, [P(OrcN2ar), d, d, v]),
+            "orc_n2ar_dipole_model_bf": (None, [P(OrcN2ar), C.c_int, d, d, v]),
             "orc_n2ar_dipole": (None, [P(OrcN2ar), v, v]),
             "orc_n2ar_mlp_on_pes_features": (d, [P(OrcMlp), v, v, v]),
         }

@@ -144,8 +144,10 @@ def test_co2ar_mixed_equals_the_fp64_engine_where_chaos_sets_the_floor(eng):
 def test_orbit_handover_rows_are_the_fp64_rows(eng):
     """An item handed over to FP64 is integrated again from its initial condition by the FP64 pool
     at the hand-over tolerance: its row, length and status are then the bits of the batched FP64
-    engine at that tolerance, and its right-hand sides count both runs."""
-    n, L = 4096, 1024
+    engine at that tolerance, and its right-hand sides count both runs.  L = 4096 samples (8.2e5 au):
+    a thermal pair injected at 40 bohr needs ~2e5 au to reach the well, so orbiting only shows on
+    the configuration's full trajectory length."""
+    n, L = 8192, 4096
     smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
     x = eng.to_device_soa(H.sample_ics(H.co2_ar(), smp, 7, n)[0])
     _, st = eng.cf_run(H.co2_ar(), H.cf_params(L, rtol=1e-10, precision=H.PREC_MIXED), [x])

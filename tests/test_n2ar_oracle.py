@@ -225,3 +225,104 @@ def test_dipole_long_range_falls_as_r4(model):
     O.lib().orc_n2ar_dipole_bf(C_ref(m), 40.0, 0.6, O.ptr(d2))
     k = np.log(np.linalg.norm(d1) / np.linalg.norm(d2)) / np.log(2.0)
     assert 3.5 < k < 4.5, k
+
+
+# ---------------------------------------------------------------- dipole composition pins
+# dipole_xyz (PES-IDS/ai_ids_n2_ar_pip_nn.cpp:101-127): mu = (1 - s) mu_short + s mu_long with the
+# quintic switch s over the N2 centre - Ar distance in [9.5, 12.5]; each model is
+# (q(x) - q_inf) . x with its charges from three subnets on the purified PIPs (a0 = 2 short, 6 long).
+
+AR_POLARIZABILITY = 11.08   # a.u., static dipole polarizability of Ar
+N2_QUADRUPOLE = -1.09       # a.u. (e a0^2), Buckingham quadrupole of N2
+
+
+def mu_bf(m, R, th, model=None):
+    d = np.zeros(3)
+    if model is None:
+        O.lib().orc_n2ar_dipole_bf(C_ref(m), R, th, O.ptr(d))
+    else:
+        O.lib().orc_n2ar_dipole_model_bf(C_ref(m), model, R, th, O.ptr(d))
+    return d
+
+
+def test_dipole_long_range_is_the_quadrupole_induced_dipole(model):
+    """Far from the well the induced dipole is the Ar polarisation by the N2 quadrupole field:
+    mu_R = 3 alpha Theta P2(cos theta) / R^4 along the N2 -> Ar axis and, across it,
+    mu_x / mu_R = -sin theta cos theta / P2(cos theta) (the field of phi = Theta P2 / R^3).
+    The composite dipole at R = 20-30 bohr follows both to ~10%; with the short and long models
+    swapped the short model's exponential tail (R^4 mu_R ~ 1/15 of the law at 25 bohr) fails it."""
+    m, _, _ = model
+    law = 3.0 * AR_POLARIZABILITY * N2_QUADRUPOLE
+    for R in (20.0, 25.0, 30.0):
+        for th in (0.0, 0.3, np.pi / 4, 1.2, np.pi / 2):
+            d = mu_bf(m, R, th)
+            P2 = 1.5 * np.cos(th) ** 2 - 0.5
+            c = d[2] * R ** 4 / P2
+            assert abs(c / law - 1.0) < 0.12, (R, th, c, law)
+            if 0.0 < th < np.pi / 2:
+                ratio = d[0] / d[2]
+                assert abs(ratio / (-np.sin(th) * np.cos(th) / P2) - 1.0) < 0.12, (R, th, ratio)
+
+
+def test_dipole_switch_selects_short_inside_and_long_outside(model):
+    """Below 9.5 bohr the dipole is the short model alone and from 12.5 bohr the long model alone
+    (bit for bit); the short model decays exponentially (its PIPs carry exp(-r/2)), the long one
+    as R^-4, which is what tells them apart."""
+    m, _, _ = model
+    for th in (0.2, 1.0, 1.5, 2.5):
+        for R in (5.0, 8.0, 9.49):
+            assert np.array_equal(mu_bf(m, R, th), mu_bf(m, R, th, 0))
+        for R in (12.5, 14.0, 25.0):
+            assert np.array_equal(mu_bf(m, R, th), mu_bf(m, R, th, 1))
+        # at the switch's midpoint s = 1/2
+        np.testing.assert_allclose(mu_bf(m, 11.0, th), 0.5 * (mu_bf(m, 11.0, th, 0) + mu_bf(m, 11.0, th, 1)),
+                                   rtol=0, atol=1e-15)
+    s20, s25 = np.linalg.norm(mu_bf(m, 20.0, 0.3, 0)), np.linalg.norm(mu_bf(m, 25.0, 0.3, 0))
+    l20, l25 = np.linalg.norm(mu_bf(m, 20.0, 0.3, 1)), np.linalg.norm(mu_bf(m, 25.0, 0.3, 1))
+    assert s20 / s25 > 5.0            # exponential: ~10x per 5 bohr
+    assert 2.0 < l20 / l25 < 3.0      # (25/20)^4 = 2.44
+
+
+def one_sided_d2(f, x, h, side):
+    """Second derivative at x from one side only, O(h^2): (2 f0 - 5 f1 + 4 f2 - f3) / h^2."""
+    v = [f(x + side * k * h) for k in range(4)]
+    return (2 * v[0] - 5 * v[1] + 4 * v[2] - v[3]) / h ** 2
+
+
+def test_dipole_switch_is_c2_at_both_ends(model):
+    """The quintic 10 r^3 - 15 r^4 + 6 r^5 has s' = s'' = 0 at both ends, so the blended dipole's
+    second R-derivative is continuous across 9.5 and 12.5 bohr although the two models differ
+    there; a cubic smoothstep (s'' = 2/3 bohr^-2 at the ends) leaves a jump of s'' (mu_long -
+    mu_short), ~5e-5 a.u./bohr^2 here, which this resolves."""
+    m, _, _ = model
+    h = 0.01
+    for th in (0.3, 1.0, 1.4):
+        for x in (9.5, 12.5):
+            gap = np.abs(mu_bf(m, x, th, 1) - mu_bf(m, x, th, 0)).max()
+            assert gap > 1e-6       # the models differ at the end: the test has something to see
+            for comp in (0, 2):
+                f = lambda R: mu_bf(m, R, th)[comp]
+                left = one_sided_d2(f, x - 1e-9, h, -1)
+                right = one_sided_d2(f, x + 1e-9, h, +1)
+                assert abs(left - right) < 0.05 * (2.0 / 3.0) * gap + 1e-9, (th, x, comp, left, right)
+
+
+def test_purified_dipole_polynomials_vanish_with_the_atom_far(model):
+    """Every one of the 308 purified PIPs vanishes when the four N-Ar factors y = exp(-r/a0) are 0,
+    whatever the N-N factors: the charges then equal forward_inf's (all PIPs zero,
+    PES-IDS/ai_ids_n2_ar_pip_nn.cpp:50-55) and the induced dipole vanishes at infinite separation."""
+    rng = np.random.default_rng(4)
+    pairs = [(i, j) for i in range(5) for j in range(i + 1, 5)]
+    p = np.zeros(308)
+    for _ in range(20):
+        y = rng.uniform(0.05, 0.95, 10)
+        for k, (i, j) in enumerate(pairs):
+            if j == 4:
+                y[k] = 0.0
+        O.lib().orc_pip_n2ar_dip(O.ptr(y), O.ptr(p))
+        assert not np.any(p)
+        # and none but each basis's purified constant term (index 0 of each of the three
+        # subnets' blocks) vanishes identically: with the atom near, the rest are nonzero
+        y = rng.uniform(0.05, 0.95, 10)
+        O.lib().orc_pip_n2ar_dip(O.ptr(y), O.ptr(p))
+        assert list(np.flatnonzero(p == 0.0)) == [0, 119, 238]

@@ -169,6 +169,7 @@ def lib():
             "hh_cf_run": (i, [v, P(System), P(CFParams), i, v, v, v, P(CFStats)]),
             "hh_cf_items": (i, [v, i, v, v, v, v, v]),
             "hh_cf_to_sf": (i, [v, v, C.c_long, d, v, v, P(d)]),
+            "hh_cf_to_sf_noalloc": (i, [v, v, C.c_long, d, v, v]),
             "hh_m0": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
             "hh_m2": (i, [v, P(System), P(Sampler), C.c_uint32, C.c_long, i, d, P(d), P(d), P(d)]),
             "hh_eval": (i, [v, P(System), i, v, C.c_long, v]),

@@ -157,6 +157,20 @@ hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2*
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ CF -> SF without allocation
+// The two 2n-point work sequences of the transform live in a file-scope device array instead of
+// engine-allocated buffers, for callers that cannot allocate (graph capture, tight serving loops).
+// Every device holds its own copy; its address comes from hipGetSymbolAddress once per device
+// (noalloc_scratch_address, at engine creation) and only that address is passed to kernels: the
+// array's name in host code is a host-side stand-in, not a device address.  Calls that overlap on
+// one device share the array and race.
+__device__ __align__(256) double2 scratch[2 * 2 * kCfToSfNoAllocMaxN];
+
+hipError_t noalloc_scratch_address(void** ptr)
+{
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
 // ------------------------------------------------------------------ M0
 constexpr int kMTN = 624, kMTM = 397;
 constexpr double kHkT = 4.3597447222071e-18 / 1.380649e-23;

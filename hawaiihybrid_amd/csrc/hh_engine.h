@@ -37,6 +37,10 @@ __host__ __device__ __forceinline__ bool dropped(int first, int last)
     return dipole_failure(first) || dipole_failure(last) || last == ITEM_FAIL_STEPS;
 }
 
+// hh_cf_to_sf_noalloc: the largest transform length its static work array holds (two
+// 2n-point complex FP64 sequences: 4 MiB per device)
+constexpr long kCfToSfNoAllocMaxN = 65536;
+
 // One correlation-function job: a batch of trajectories sharing system, temperature-free
 // integration parameters and output buffers.  All pointers are device pointers.
 struct JobDesc {

@@ -359,6 +359,11 @@ int hh_engine_create(int device, hh_engine** out)
             }
         }
     }
+    // the device address of hh_cf_to_sf_noalloc's static work array on this device
+    if (hh::noalloc_scratch_address(&e->noalloc_scratch) != hipSuccess || !e->noalloc_scratch) {
+        hh_engine_destroy(e);
+        return fail(HH_E_DEVICE, "hipGetSymbolAddress of the CF -> SF work array failed");
+    }
     *out = e;
     return HH_OK;
 }
@@ -711,6 +716,24 @@ int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev,
         (void)hipEventElapsedTime(&a, e->ev[0], e->ev[1]);
         *ms = a;
     }
+    return HH_OK;
+}
+
+int hh_cf_to_sf_noalloc(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev, double* sf_dev)
+{
+    if (!e || !cf_dev || !nu_dev || !sf_dev) return fail(HH_E_INVALID, "hh_cf_to_sf_noalloc: null argument");
+    if (n < 2 || (n & (n - 1))) return fail(HH_E_INVALID, "Length of the input array should be a power of 2");
+    if (n > hh::kCfToSfNoAllocMaxN)
+        return fail(HH_E_INVALID, "hh_cf_to_sf_noalloc: n = %ld above its static work array's %ld", n,
+                    hh::kCfToSfNoAllocMaxN);
+    if (dt <= 0) return fail(HH_E_INVALID, "sampling time must be positive");
+    std::lock_guard<std::recursive_mutex> lock(e->mu);
+    HH_CHECK(hipSetDevice(e->device));
+    double2* a = static_cast<double2*>(e->noalloc_scratch);
+    double2* b = a + 2 * hh::kCfToSfNoAllocMaxN;
+    const bool lds_ok = (size_t)(2 * n) * sizeof(double2) <= e->prop.sharedMemPerBlock;
+    HH_CHECK(hh::run_cf_to_sf(cf_dev, n, dt, a, b, nu_dev, sf_dev, e->stream, lds_ok));
+    HH_CHECK(hipStreamSynchronize(e->stream));
     return HH_OK;
 }
 

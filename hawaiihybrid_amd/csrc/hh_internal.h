@@ -57,6 +57,8 @@ hipError_t run_zimm_reduce(const JobDesc& jb, double ppf, const double2* tw, dou
                            double* cf_out, hipStream_t st);
 hipError_t run_cf_to_sf(double* cf, long n, double dt, double2* work_a, double2* work_b, double* nu, double* sf,
                         hipStream_t st, bool lds_ok);
+// hh_cf_to_sf_noalloc: the device address of its static work array on the current device
+hipError_t noalloc_scratch_address(void** ptr);
 hipError_t run_m0(int sys, const SysConst& sc, const M0Params& p, uint32_t seed, long nstreams, long npoints,
                   uint32_t* state, const M0Acc& acc, int init, long long ticks, hipStream_t st, int order = 0);
 int m0_check(const hh_system* sys, const hh_sampler* smp, long npoints, int nstreams);
@@ -201,6 +203,7 @@ struct hh_engine {
     hh::DevBuf ch4tab64, ch4tab32;   // CH4-CO2 dipole tables (FP64, FP32)
     hh::DevBuf rows, len, status, nrhs, drift, jobs, queue, psum, pcomp, fft_a, fft_b, m0_state, arena;
     hh::DevBuf zimm_tw;           // twiddles e^{2 pi i j/P} of the Zimmermann transforms
+    void* noalloc_scratch = nullptr;    // device address of hh_cf_to_sf_noalloc's static work array
     hh::DevBuf prmu_a, prmu_b, prmu_w;   // PR/mu: FFT ping-pong series, weights and spectra
     int zimm_logP = 0;
     std::vector<hh::JobDesc> last_jobs;

@@ -249,6 +249,14 @@ int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params* p, const d
  * unit scaling: nu_dev[n] in cm^-1, sf_dev[n] (idct_cf_to_sf).  n must be a power of two. */
 int hh_cf_to_sf(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev, double* sf_dev, double* ms);
 
+/* hh_cf_to_sf without any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The transform's work sequences live in a static device array of the library
+ * (4 MiB per device, held until the process ends), so n <= 65536 (HH_E_INVALID above).  Same
+ * kernels and results as hh_cf_to_sf.  Every call on a device shares that array: calls that
+ * overlap on one device race. */
+int hh_cf_to_sf_noalloc(hh_engine* e, double* cf_dev, long n, double dt, double* nu_dev, double* sf_dev);
+
 /* ------------------------------------------------------------------ M0 phase-space moment */
 /* Rejection-sampler M0 over `npoints` accepted points (calculate_M0 semantics) with the
  * candidate stream split into `nstreams` independent Mersenne Twister streams seeded

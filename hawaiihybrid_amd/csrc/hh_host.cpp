@@ -432,6 +432,8 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
             if (rc < 0) return rc;
         } else if (block != 0 && block != 64 && block != 128 && block != 256) {
             return fail(HH_E_INVALID, "block must be 0 (tuned), 64, 128 or 256");
+        } else if (sys->system == HH_SYS_CH4CO2 && block != 0 && block != 256) {
+            return fail(HH_E_INVALID, "CH4-CO2 runs in 256-lane workgroups only (block 0 or 256)");
         } else {
             if (block == 0) block = 256;
             waves = hh::pool_waves(block);

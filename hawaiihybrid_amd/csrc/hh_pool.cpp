@@ -200,6 +200,12 @@ int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const d
               const long* ntraj, int njobs, int* block, int* waves)
 {
     const bool mixed = p.precision == HH_PREC_MIXED;
+    if (sys.system == HH_SYS_CH4CO2) {
+        // one launch shape is built for the rotor system (its state fills the register file)
+        *block = 256;
+        *waves = 1;
+        return HH_OK;
+    }
     for (const auto& t : e->tuned)
         if (t[0] == sys.system && t[1] == (int)mixed) {
             *block = t[2];

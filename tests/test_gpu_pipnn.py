@@ -3,15 +3,18 @@ evaluated as deviations from the value at infinite separation) against the FP64 
 points drawn by q_generator + p_generator at 100, 300 and 500 K (R in [4.51, 40] bohr).
 
 Tolerances, from the errors measured on these (seeded, deterministic) points with a margin of
-about 2: energy |dE| in cm^-1 (max and median over the set), dipole max |dmu| relative to the
-largest |mu| of the set.
-  single BF16 product      max |dE| <= 80,    median <= 1e-2;   dmu <= 0.3
-  split BF16 (3 products)  max |dE| <= 0.25,  median <= 1e-3;   dmu <= 1e-3
-(Measured: single BF16 median 1.7e-3 cm^-1, max 38 cm^-1 on the repulsive wall, dipole 0.15;
-split median 2.4e-4, max 0.11 cm^-1, dipole 4.1e-4.  The largest energy errors sit where the
-network's terms are large and cancel, not where |E| is largest.  The induced dipole is a small
-difference of site charges times positions up to 40 bohr, so its relative error is far larger
-than the charges'; the split mode is the one for dynamics.)
+about 2: energy |dE| in cm^-1 over the set, dipole |dmu| relative to the largest |mu| of the set.
+  split BF16 (3 products, the default and the mode for dynamics)
+                           max |dE| <= 0.25,  median <= 1e-3;   max dmu <= 1e-3
+  single BF16 product      median |dE| <= 1e-2, 99th percentile <= 4, |dE| / |E| <= 12% where
+                           |E| > 100 cm^-1;   99th percentile of dmu <= 0.12
+(Measured here: split median 2.4e-4, max 0.11 cm^-1, dipole 4.1e-4; single median 1.7e-3 cm^-1,
+99th percentile 1.9 cm^-1, relative 6.2% above 100 cm^-1, dipole 99th percentile 0.062.
+The single product's error has a heavy tail: its largest errors sit on the repulsive wall, where
+the network's terms are large and cancel (38 cm^-1 on these 4200 points, 93 cm^-1 on 2000 of 2^20
+points of the benchmark), and its dipole reaches ~10% of the largest |mu|, since the induced
+dipole is a small difference of site charges times positions up to 40 bohr.  Its maximum grows
+with the sample, so it is bounded by quantiles: a throughput mode, not a parity-grade surface.)
 """
 import ctypes as C
 
@@ -45,8 +48,8 @@ def setup():
     return dict(eng=eng, W=W, q=q, qd=qd, ref_E=ref_E, ref_mu=ref_mu, n2ar=n2ar, keep=keep, m=m)
 
 
-@pytest.mark.parametrize("prec,tol_max,tol_med,tol_mu", [(0, 80.0, 1e-2, 0.3), (1, 0.25, 1e-3, 1e-3)])
-def test_reference_surfaces_match_oracle(setup, prec, tol_max, tol_med, tol_mu):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_reference_surfaces_match_oracle(setup, prec):
     s = setup
     print(f"pipnn precision {prec}: {s['q'].shape[1]} points", flush=True)
     model = s["n2ar"].PipNN(s["eng"], precision=prec, weights=s["W"])
@@ -55,12 +58,19 @@ def test_reference_surfaces_match_oracle(setup, prec, tol_max, tol_med, tol_mu):
     mu = mu.cpu().numpy()
     dE = np.abs(E - s["ref_E"]) * HTOCM
     scale = np.max(np.linalg.norm(s["ref_mu"], axis=0))
-    dmu = np.max(np.abs(mu - s["ref_mu"])) / scale
-    print(f"  max |dE| {dE.max():.3e} cm^-1, median {np.median(dE):.3e}; max |dmu|/max|mu| {dmu:.3e}; {ms:.3f} ms",
-          flush=True)
+    dmu_pt = np.max(np.abs(mu - s["ref_mu"]), axis=0) / scale
+    big = np.abs(s["ref_E"]) * HTOCM > 100.0
+    rel_big = np.max(dE[big] / (np.abs(s["ref_E"][big]) * HTOCM))
+    print(f"  |dE| cm^-1: median {np.median(dE):.3e}, p99 {np.quantile(dE, 0.99):.3e}, max {dE.max():.3e}, "
+          f"max rel above 100 cm^-1 {rel_big:.3e}; |dmu|/max|mu| p99 {np.quantile(dmu_pt, 0.99):.3e}, "
+          f"max {dmu_pt.max():.3e}; {ms:.3f} ms", flush=True)
     assert np.all(np.isfinite(E)) and np.all(np.isfinite(mu))
-    assert dE.max() < tol_max and np.median(dE) < tol_med
-    assert dmu < tol_mu
+    if prec == 1:
+        assert dE.max() < 0.25 and np.median(dE) < 1e-3
+        assert dmu_pt.max() < 1e-3
+    else:
+        assert np.median(dE) < 1e-2 and np.quantile(dE, 0.99) < 4.0 and rel_big < 0.12
+        assert np.quantile(dmu_pt, 0.99) < 0.12
     model.close()
 
 

@@ -3,19 +3,30 @@
 The slices of a lane pool run as captured graphs (repack when the queue has drained, the pool
 kernel, copies of the counters into pinned memory), replayed many times per run with the host's
 reductions and bookkeeping between replays.  Every replay must give the same bits as the
-directly launched slices (HH_NO_GRAPHS=1), and the result must not depend on the launch shape
-the autotuner picks (each item's arithmetic is independent of where it runs).
+directly launched slices (an engine created without HH_GRAPHS=1), and the result must not depend
+on the launch shape the autotuner picks (each item's arithmetic is independent of where it runs).
+The second and later runs on one engine reuse its lane pool, whose device queue head the first
+run left past the end (pool_admit clamps it back before the new jobs).
 """
-import os
-
 import numpy as np
 import pytest
 
 import hawaiihybrid_amd as H
 
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="graph-captured slices and the launch-shape tuner are opt-in (HH_GRAPHS, "
-                                      "HH_TUNE): a run with both enabled went silent and the cause is not established")]
+              pytest.mark.skip(reason="the 64- and 128-lane pools (two wavefronts per SIMD) give rows that differ "
+                                      "from run to run in the last bits on 3 x 8192 trajectories; graph replays "
+                                      "and the queue head across runs are correct, the shape dependence is open")]
+
+
+def graph_engine(monkeypatch):
+    """An engine with graph-captured slices and the launch-shape tuner (both read at creation)."""
+    monkeypatch.setenv("HH_GRAPHS", "1")
+    monkeypatch.setenv("HH_TUNE", "1")
+    e = H.Engine(0)
+    monkeypatch.delenv("HH_GRAPHS")
+    monkeypatch.delenv("HH_TUNE")
+    return e
 
 
 def run(eng, ic_dev, block, L=512):
@@ -24,11 +35,11 @@ def run(eng, ic_dev, block, L=512):
     return [o.cpu().numpy() for o in outs], st
 
 
-def test_graph_replays_equal_direct_launches_and_shapes():
+def test_graph_replays_equal_direct_launches_and_shapes(monkeypatch):
     smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
     ic, _, _ = H.sample_ics(H.co2_ar(), smp, 42, 3 * 8192)
     print("graphs: 3 jobs of 8192 CO2-Ar trajectories, L=512", flush=True)
-    g = H.Engine(0)
+    g = graph_engine(monkeypatch)
     xs = [g.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
     ref, st = run(g, xs, 0)
     print(f"  tuned shape: block {st.block}, grid {st.grid}, {st.launches} slices", flush=True)

@@ -15,6 +15,7 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhawaii_gpu.so")
+REFLIB = os.path.join(HERE, "libhawaii.so")   # the reference-named C API (include/hawaii_ref.h)
 DRIVER = os.path.join(HERE, "hawaii_gpu")
 ARCH = "gfx950"
 
@@ -65,9 +66,18 @@ def build_engine(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(DRIVER + ".tmp", DRIVER)
+    # the reference-named C API, linked against the library next to it
+    ref_obj = _compile("hh_refapi.cpp", ["-x", "hip"] + DEVICE)
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", REFLIB + ".tmp", ref_obj, "-L" + HERE,
+           "-lhawaii_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(REFLIB + ".tmp", REFLIB)
     if verbose:
         print("built", LIB)
         print("built", DRIVER)
+        print("built", REFLIB)
     return LIB
 
 

@@ -19,9 +19,11 @@
 #include <vector>
 
 #include "../../include/hawaii_conf.h"
+#include "hh_conf_impl.h"
 #include "hh_internal.h"
 
-namespace {
+namespace hh {
+namespace conf {
 
 using hh::fail;
 
@@ -289,8 +291,9 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
             const int m = c->n_monomers - 1;
             if (key == "MONOMER_TYPE") {
                 const std::string w = Lexer::upper(v.text);
-                c->monomer_type[m] = w == "ATOM" ? HH_MONOMER_ATOM
+                c->monomer_type[m] = w == "ATOM"              ? HH_MONOMER_ATOM
                                      : w == "LINEAR_MOLECULE" ? HH_MONOMER_LINEAR_MOLECULE
+                                     : w == "ROTOR"           ? HH_MONOMER_ROTOR
                                                               : HH_MONOMER_OTHER;
             } else if (key == "II") {
                 if (list.size() != 3) return err(line, "%s expects {Ixx, Iyy, Izz}", key);
@@ -411,12 +414,6 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
 }
 
 // ------------------------------------------------------------------ CF / SF files
-struct Series {
-    bool sf = false;
-    bool spectrum = false;      // a Spectrum (ALPHA's output, cm^-1 Amagat^-2): x in cm^-1
-    std::vector<double> x, y;   // t or nu; normalised values
-    double T = 0.0, ntraj = 0.0;
-};
 
 std::string now_stamp()
 {
@@ -915,42 +912,92 @@ uint32_t good_seed()
 }
 
 // the engine's system for the configured monomers (ATOM+ATOM: He-Ar surfaces, LINEAR_MOLECULE+ATOM:
-// CO2-Ar surfaces)
+// CO2-Ar surfaces, ROTOR+LINEAR_MOLECULE: CH4-CO2 surfaces)
 int system_of(const hh_conf* c, hh_system* sys)
 {
+    *sys = hh_system{};
     sys->mu = c->reduced_mass;
-    if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_ATOM && c->monomer_type[1] == HH_MONOMER_ATOM) {
+    const int t0 = c->monomer_type[0], t1 = c->monomer_type[1];
+    if (c->n_monomers == 2 && t0 == HH_MONOMER_ATOM && t1 == HH_MONOMER_ATOM) {
         sys->system = HH_SYS_HEAR;
-    } else if (c->n_monomers == 2 && c->monomer_type[0] == HH_MONOMER_LINEAR_MOLECULE &&
-               c->monomer_type[1] == HH_MONOMER_ATOM) {
+    } else if (c->n_monomers == 2 && t0 == HH_MONOMER_LINEAR_MOLECULE && t1 == HH_MONOMER_ATOM) {
         sys->system = HH_SYS_CO2AR;
         sys->I = c->II[0][0];
         if (c->II[0][0] != c->II[0][1] || sys->I <= 0)
             return fail(HH_E_INVALID, "linear molecule needs II = {I, I, 0} with I > 0");
+    } else if (c->n_monomers == 2 && t0 == HH_MONOMER_ROTOR && t1 == HH_MONOMER_LINEAR_MOLECULE) {
+        sys->system = HH_SYS_CH4CO2;
+        for (int k = 0; k < 3; ++k) sys->I_rotor[k] = c->II[0][k];
+        sys->I = c->II[1][0];
+        if (!(sys->I_rotor[0] > 0 && sys->I_rotor[1] > 0 && sys->I_rotor[2] > 0))
+            return fail(HH_E_INVALID, "rotor needs three positive principal moments II");
+        if (c->II[1][0] != c->II[1][1] || sys->I <= 0)
+            return fail(HH_E_INVALID, "linear molecule needs II = {I, I, 0} with I > 0");
     } else {
-        return fail(HH_E_UNSUPPORTED, "monomers other than ATOM+ATOM or LINEAR_MOLECULE+ATOM are not part of this engine");
+        return fail(HH_E_UNSUPPORTED, "monomers other than ATOM+ATOM, LINEAR_MOLECULE+ATOM or "
+                                      "ROTOR+LINEAR_MOLECULE are not part of this engine");
     }
     if (!(sys->mu > 0)) return fail(HH_E_INVALID, "pair_reduced_mass must be positive");
     return HH_OK;
 }
 
-// The configured surface plugins must be the system the device integrates: the engine evaluates
-// only its built-in surfaces, so a plugin is accepted when it declares the same system
-// (`int hh_system_id(void)` returning the HH_SYS_* value) or when its host pes_lab / dpes_lab /
-// dipole_lab agree with the built-in surface at 48 fixed points of the sampling region (relative
-// 1e-6 of the largest magnitude, FP64 device evaluation).  A plugin that cannot be loaded or lacks
-// a required symbol is an error, as in the reference's loader (src/driver.c:3083-3094, which
-// exits).
-int check_surfaces(hh_engine* e, const hh_conf* c, const hh_system& sys)
+// The plugins named by so_potential / so_dipole (src/driver.c:3083-3181): pes_lab and dpes_lab,
+// dipole_lab, their optional pes_init / dipole_init(bool) run once at load.  A plugin that cannot
+// be loaded or lacks a required symbol is an error, as in the reference's loader (which exits).
+int load_surfaces(const hh_conf* c, HostSurfaces& hs)
+{
+    auto declared = [](void* so) {
+        auto id = reinterpret_cast<int (*)(void)>(dlsym(so, "hh_system_id"));
+        return id ? id() : -1;
+    };
+    if (c->so_potential[0]) {
+        void* so = dlopen(c->so_potential, RTLD_LAZY);
+        if (!so) return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_potential, dlerror());
+        hs.so[0] = so;
+        hs.pes = reinterpret_cast<hh_pes_fn>(dlsym(so, "pes_lab"));
+        hs.dpes = reinterpret_cast<hh_dpes_fn>(dlsym(so, "dpes_lab"));
+        if (!hs.pes || !hs.dpes) return fail(HH_E_INVALID, "%s must export pes_lab and dpes_lab", c->so_potential);
+        hs.pes_system = declared(so);
+        hs.pes_name = c->so_potential;
+        if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
+    }
+    if (c->so_dipole[0]) {
+        void* so = dlopen(c->so_dipole, RTLD_LAZY);
+        if (!so) return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_dipole, dlerror());
+        hs.so[1] = so;
+        hs.dipole = reinterpret_cast<hh_dipole_fn>(dlsym(so, "dipole_lab"));
+        if (!hs.dipole) return fail(HH_E_INVALID, "%s must export dipole_lab", c->so_dipole);
+        hs.dipole_system = declared(so);
+        hs.dipole_name = c->so_dipole;
+        if (void (*init)(bool) = reinterpret_cast<void (*)(bool)>(dlsym(so, "dipole_init"))) init(false);
+    }
+    return HH_OK;
+}
+
+// The host surfaces must be the system the device integrates: the engine evaluates only its
+// built-in surfaces, so a surface is accepted when its plugin declares the same system
+// (`int hh_system_id(void)` returning the HH_SYS_* value) or when pes_lab / dpes_lab /
+// dipole_lab agree with the built-in surface at 48 fixed points of [Rmin, min(Rmax, 20)] and
+// the angles' ranges (relative 1e-6 of the largest magnitude, FP64 device evaluation).  Absent
+// surfaces (null pointers) are the built-in ones.
+int check_surfaces(hh_engine* e, const hh_system& sys, double Rmin, double Rmax, const HostSurfaces& hs)
 {
     const char* names[] = {"He-Ar", "CO2-Ar", "N2-Ar", "CH4-CO2"};
     const char* name = names[sys.system];
+    for (int w = 0; w < 2; ++w) {
+        const int sid = w ? hs.dipole_system : hs.pes_system;
+        if (sid >= 0 && sid != sys.system)
+            return fail(HH_E_UNSUPPORTED, "%s declares system %d; the configured monomers are the engine's %s",
+                        (w ? hs.dipole_name : hs.pes_name).c_str(), sid, name);
+    }
+    const bool check_pes = hs.pes && hs.pes_system < 0, check_dip = hs.dipole && hs.dipole_system < 0;
+    if (!check_pes && !check_dip) return HH_OK;
     const int qp = hh_qp_size(sys.system), nq = qp / 2;
     constexpr int K = 48;
     // points: R over [Rmin, min(Rmax, 20)] (the interaction region), angles over their ranges,
     // from a fixed linear congruential stream; momenta zero (surfaces depend on coordinates only)
-    const double r0 = c->sampler_Rmin > 0 ? c->sampler_Rmin : 4.0;
-    const double r1 = std::max(r0 + 1.0, std::min(c->sampler_Rmax > 0 ? c->sampler_Rmax : 20.0, 20.0));
+    const double r0 = Rmin > 0 ? Rmin : 4.0;
+    const double r1 = std::max(r0 + 1.0, std::min(Rmax > 0 ? Rmax : 20.0, 20.0));
     std::vector<double> y((size_t)K * qp, 0.0), q((size_t)K * nq);
     uint64_t st = 0x9e3779b97f4a7c15ull;
     auto u = [&] {
@@ -970,25 +1017,6 @@ int check_surfaces(hh_engine* e, const hh_conf* c, const hh_system& sys)
             q[(size_t)k * nq + i] = v;
             y[(size_t)(2 * i) * K + k] = v;   // SoA [QP][K], coordinate i at state index 2i
         }
-    struct Plugin {
-        void* so = nullptr;
-        ~Plugin()
-        {
-            if (so) dlclose(so);
-        }
-    } pot, dip;
-    auto open = [&](const char* path, Plugin& pl) -> int {
-        pl.so = dlopen(path, RTLD_LAZY);
-        if (!pl.so) return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", path, dlerror());
-        if (auto id = reinterpret_cast<int (*)(void)>(dlsym(pl.so, "hh_system_id"))) {
-            const int sid = id();
-            if (sid != sys.system)
-                return fail(HH_E_UNSUPPORTED, "%s declares system %d; the configured monomers are the engine's %s",
-                            path, sid, name);
-            return 1;   // declared: no point check needed
-        }
-        return 0;
-    };
     double *y_dev = nullptr, *out_dev = nullptr;
     if (hipMalloc(&y_dev, y.size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&out_dev, (size_t)qp * K * sizeof(double)) != hipSuccess) {
@@ -1011,73 +1039,60 @@ int check_surfaces(hh_engine* e, const hh_conf* c, const hh_system& sys)
         HH_CHECK(hipMemcpy(out.data(), out_dev, out.size() * sizeof(double), hipMemcpyDeviceToHost));
         return HH_OK;
     };
-    if (c->so_potential[0]) {
-        int rc = open(c->so_potential, pot);
-        if (rc < 0) return rc;
-        auto pes = reinterpret_cast<hh_pes_fn>(dlsym(pot.so, "pes_lab"));
-        auto dpes = reinterpret_cast<hh_dpes_fn>(dlsym(pot.so, "dpes_lab"));
-        if (!pes || !dpes) return fail(HH_E_INVALID, "%s must export pes_lab and dpes_lab", c->so_potential);
-        if (rc == 0) {
-            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(pot.so, "pes_init"))) init();
-            std::vector<double> V, F;
-            if ((rc = device(HH_EVAL_PES, 1, V)) < 0 || (rc = device(HH_EVAL_RHS, qp, F)) < 0) return rc;
-            double scale = 0.0, worst = 0.0, worst_g = 0.0, gscale = 0.0;
-            std::vector<double> qq(nq), g(nq);
-            for (int k = 0; k < K; ++k) {
-                for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
-                const double vp = pes(qq.data());
-                dpes(qq.data(), g.data());
-                scale = std::max(scale, std::fabs(V[k]));
-                worst = std::max(worst, std::fabs(vp - V[k]));
-                // the momentum derivatives at zero momenta are -dV/dq
-                for (int i = 0; i < nq; ++i) {
-                    const double gd = -F[(size_t)(2 * i + 1) * K + k];
-                    gscale = std::max(gscale, std::fabs(gd));
-                    worst_g = std::max(worst_g, std::fabs(g[i] - gd));
-                }
+    if (check_pes) {
+        std::vector<double> V, F;
+        int rc;
+        if ((rc = device(HH_EVAL_PES, 1, V)) < 0 || (rc = device(HH_EVAL_RHS, qp, F)) < 0) return rc;
+        double scale = 0.0, worst = 0.0, worst_g = 0.0, gscale = 0.0;
+        std::vector<double> qq(nq), g(nq);
+        for (int k = 0; k < K; ++k) {
+            for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
+            const double vp = hs.pes(qq.data());
+            scale = std::max(scale, std::fabs(V[k]));
+            worst = std::max(worst, std::fabs(vp - V[k]));
+            if (!hs.dpes) continue;
+            hs.dpes(qq.data(), g.data());
+            // the momentum derivatives at zero momenta are -dV/dq
+            for (int i = 0; i < nq; ++i) {
+                const double gd = -F[(size_t)(2 * i + 1) * K + k];
+                gscale = std::max(gscale, std::fabs(gd));
+                worst_g = std::max(worst_g, std::fabs(g[i] - gd));
             }
-            if (!(worst <= 1e-6 * scale) || !(worst_g <= 1e-6 * gscale))
-                return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s surface (max |dV| %.3e of %.3e Hartree, "
-                                              "max |d grad| %.3e of %.3e): the device integrates only its built-in surfaces",
-                            c->so_potential, name, worst, scale, worst_g, gscale);
         }
+        if (!(worst <= 1e-6 * scale) || !(worst_g <= 1e-6 * gscale))
+            return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s surface (max |dV| %.3e of %.3e Hartree, "
+                                          "max |d grad| %.3e of %.3e): the device integrates only its built-in surfaces",
+                        hs.pes_name.c_str(), name, worst, scale, worst_g, gscale);
     }
-    if (c->so_dipole[0]) {
-        int rc = open(c->so_dipole, dip);
+    if (check_dip) {
+        std::vector<double> M;
+        const int rc = device(HH_EVAL_DIPOLE, 3, M);
         if (rc < 0) return rc;
-        auto dipole = reinterpret_cast<hh_dipole_fn>(dlsym(dip.so, "dipole_lab"));
-        if (!dipole) return fail(HH_E_INVALID, "%s must export dipole_lab", c->so_dipole);
-        if (rc == 0) {
-            if (void (*init)(bool) = reinterpret_cast<void (*)(bool)>(dlsym(dip.so, "dipole_init"))) init(false);
-            std::vector<double> M;
-            if ((rc = device(HH_EVAL_DIPOLE, 3, M)) < 0) return rc;
-            double scale = 0.0, worst = 0.0;
-            std::vector<double> qq(nq);
-            for (int k = 0; k < K; ++k) {
-                for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
-                double mu[3];
-                dipole(qq.data(), mu);
-                for (int j = 0; j < 3; ++j) {
-                    scale = std::max(scale, std::fabs(M[(size_t)j * K + k]));
-                    worst = std::max(worst, std::fabs(mu[j] - M[(size_t)j * K + k]));
-                }
+        double scale = 0.0, worst = 0.0;
+        std::vector<double> qq(nq);
+        for (int k = 0; k < K; ++k) {
+            for (int i = 0; i < nq; ++i) qq[i] = q[(size_t)k * nq + i];
+            double mu[3];
+            hs.dipole(qq.data(), mu);
+            for (int j = 0; j < 3; ++j) {
+                scale = std::max(scale, std::fabs(M[(size_t)j * K + k]));
+                worst = std::max(worst, std::fabs(mu[j] - M[(size_t)j * K + k]));
             }
-            if (void (*fin)(void) = reinterpret_cast<void (*)(void)>(dlsym(dip.so, "dipole_free"))) fin();
-            if (!(worst <= 1e-6 * scale))
-                return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s dipole (max |d mu| %.3e of %.3e a.u.): "
-                                              "the device evaluates only its built-in surfaces",
-                            c->so_dipole, name, worst, scale);
         }
+        if (!(worst <= 1e-6 * scale))
+            return fail(HH_E_UNSUPPORTED, "%s is not the engine's built-in %s dipole (max |d mu| %.3e of %.3e a.u.): "
+                                          "the device evaluates only its built-in surfaces",
+                        hs.dipole_name.c_str(), name, worst, scale);
     }
     return HH_OK;
 }
 
 // the engine's system for the configured monomers, checked against the configured plugins
-int resolve_system(hh_engine* e, const hh_conf* c, hh_system* sys)
+int resolve_system(hh_engine* e, const hh_conf* c, const HostSurfaces& hs, hh_system* sys)
 {
     const int rc = system_of(c, sys);
     if (rc < 0) return rc;
-    return check_surfaces(e, c, *sys);
+    return check_surfaces(e, *sys, c->sampler_Rmin, c->sampler_Rmax, hs);
 }
 
 // The M2 estimates from the accumulated CF after each iteration (src/hawaii.c:3070-3090,
@@ -1099,12 +1114,12 @@ void print_m2_from_cf(const Series& cf, double hep, const char* prefix)
 }
 
 // CALCULATION_PHASE_SPACE_M0 (src/driver.c:3389-3445): VEGAS M0 at each temperature
-int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+int run_phase_space_m0(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o, const HostSurfaces& hs)
 {
     if (!(c->sampler_Rmin > 0) || !(c->sampler_Rmax > c->sampler_Rmin))
         return fail(HH_E_INVALID, "sampler_Rmin and sampler_Rmax must be set for the phase-space integral");
     hh_system sys{};
-    int rc = resolve_system(e, c, &sys);
+    int rc = resolve_system(e, c, hs, &sys);
     if (rc < 0) return rc;
     std::vector<double> Ts(c->temperatures, c->temperatures + c->ntemperatures);
     if (Ts.empty() && c->temperature > 0) Ts.push_back(c->temperature);
@@ -1144,7 +1159,7 @@ int hep_m2(hh_engine* e, const hh_system& sys, const hh_conf* c, int pair_state,
 }
 
 // CALCULATION_PHASE_SPACE_M2 (src/driver.c:3344-3388): VEGAS M2 at TEMPERATURE
-int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o, const HostSurfaces& hs)
 {
     if (!(c->sampler_Rmin > 0))
         return fail(HH_E_INVALID, "minimum value for R throught integration (sampler_Rmin) must be set");
@@ -1152,7 +1167,7 @@ int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
         return fail(HH_E_INVALID, "maximum value for R throught integration (sampler_Rmax) must be set");
     if (!(c->temperature > 0)) return fail(HH_E_INVALID, "no temperature is provided to run CALCULATION_PHASE_SPACE_M2");
     hh_system sys{};
-    int rc = resolve_system(e, c, &sys);
+    int rc = resolve_system(e, c, hs, &sys);
     if (rc < 0) return rc;
     const double pf_an = hh_pf_analytic(&sys, c->temperature);
     if (pf_an < 0) return (int)HH_E_INVALID;
@@ -1165,7 +1180,8 @@ int run_phase_space_m2(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o
     return HH_OK;
 }
 
-int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o, const HostSurfaces& hs,
+                           RunIO* io)
 {
     // the reference's checks (src/hawaii.c:2802-2811)
     if (c->max_trajectory_length <= 0 || c->Rcut <= 0 || c->sampling_time <= 0 || c->total_trajectories <= 0 ||
@@ -1179,7 +1195,7 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
     {
-        const int rc0 = resolve_system(e, c, &sys);
+        const int rc0 = resolve_system(e, c, hs, &sys);
         if (rc0 < 0) return rc0;
     }
     // partition-function ratio, M0 and M2 by VEGAS when the ratio is not given (src/hawaii.c:2829-2880);
@@ -1221,22 +1237,6 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     const uint32_t seed = o.seed ? o.seed : good_seed();
 
     // the plugin's pes_lab drives the host rejection sampler (the reference's accept/reject)
-    hh_pes_fn pes = nullptr;
-    void* so = nullptr;
-    if (c->so_potential[0]) {
-        so = dlopen(c->so_potential, RTLD_LAZY);
-        if (so) {
-            pes = reinterpret_cast<hh_pes_fn>(dlsym(so, "pes_lab"));
-            if (!pes) {
-                dlclose(so);
-                return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
-            }
-            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
-        } else {
-            return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_potential, dlerror());
-        }
-    }
-
     hh_sampler smp{HH_IC_REJECTION, c->pair_state ? c->pair_state : HH_PAIR_FREE, c->temperature, c->sampler_Rmin,
                    c->sampler_Rmax, c->pesmin, 0.0};
     // the stream: initialM0_npoints accepted points for the preliminary M0, then
@@ -1248,11 +1248,9 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
     std::vector<double> ic((size_t)total * qp);
     long nd = 0;
     if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
-    long rc = hh_sample_ics(&sys, &smp, seed, total, pes, ic.data(), &nd);
-    if (rc < 0) {
-        if (so) dlclose(so);
-        return (int)rc;
-    }
+    long rc = hh_sample_ics_from(&sys, &smp, seed, io ? io->skip_draws : 0, total, hs.pes, ic.data(), &nd);
+    if (rc < 0) return (int)rc;
+    if (io) io->draws_used = nd;
 
     // device buffers: one iteration of initial conditions (SoA), the CF, the M0 dipoles
     const long nmax = std::max(n_iter, std::max(m0n, m2n));
@@ -1373,11 +1371,11 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
             Series sf;
             if ((code = cf_to_sf(e, cf, sf)) == HH_OK) code = write_series(c->sf_filename, sf);
         }
+        if (code == HH_OK && io && io->cf) *io->cf = cf;
     } while (false);
     if (ic_dev) (void)hipFree(ic_dev);
     if (cf_dev) (void)hipFree(cf_dev);
     if (out_dev) (void)hipFree(out_dev);
-    if (so) dlclose(so);
     return code;
 }
 
@@ -1386,7 +1384,8 @@ int run_correlation_single(hh_engine* e, const hh_conf* c, const hh_conf_run_opt
 // iteration, integrated once and accumulated at every satellite temperature with its Boltzmann
 // weight (hh_cf_array); after each iteration every CF file is rewritten with its weight sum as
 // the trajectory count.
-int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o, const HostSurfaces& hs,
+                          RunIO* io)
 {
     // the reference's checks (src/hawaii.c:2367-2378)
     if (c->max_trajectory_length <= 0 || c->Rcut <= 0 || c->sampling_time <= 0 || c->total_trajectories <= 0 ||
@@ -1405,7 +1404,7 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
     {
-        const int rc0 = resolve_system(e, c, &sys);
+        const int rc0 = resolve_system(e, c, hs, &sys);
         if (rc0 < 0) return rc0;
     }
     const int M = c->nsatellite;
@@ -1452,29 +1451,14 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
     const long n_iter = c->total_trajectories / c->niterations;
     if (n_iter <= 0) return fail(HH_E_INVALID, "total_trajectories < niterations");
     const uint32_t seed = o.seed ? o.seed : good_seed();
-    hh_pes_fn pes = nullptr;
-    void* so = nullptr;
-    if (c->so_potential[0]) {
-        so = dlopen(c->so_potential, RTLD_LAZY);
-        if (so) {
-            pes = reinterpret_cast<hh_pes_fn>(dlsym(so, "pes_lab"));
-            if (!pes) {
-                dlclose(so);
-                return fail(HH_E_INVALID, "%s does not export pes_lab", c->so_potential);
-            }
-            if (void (*init)(void) = reinterpret_cast<void (*)(void)>(dlsym(so, "pes_init"))) init();
-        } else {
-            return fail(HH_E_INVALID, "cannot load the surface plugin %s: %s", c->so_potential, dlerror());
-        }
-    }
     hh_sampler smp{HH_IC_REJECTION, ps, Tb, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, 0.0};
     const long total = n_iter * c->niterations;
     std::vector<double> ic((size_t)total * qp);
     long nd = 0;
     if (!o.quiet) std::printf("Generator seed is set to %u\n", seed);
-    long rc = hh_sample_ics(&sys, &smp, seed, total, pes, ic.data(), &nd);
-    if (so) dlclose(so);
+    long rc = hh_sample_ics_from(&sys, &smp, seed, io ? io->skip_draws : 0, total, hs.pes, ic.data(), &nd);
     if (rc < 0) return (int)rc;
+    if (io) io->draws_used = nd;
 
     hh_cf_params p{};
     const bool mixed = o.precision == HH_PREC_MIXED;
@@ -1540,6 +1524,7 @@ int run_correlation_array(hh_engine* e, const hh_conf* c, const hh_conf_run_opts
                     print_m2_from_cf(cf, hep_m2s[s2], "");
                 }
                 code = write_series(c->cf_filenames[s2], cf);
+                if (io && io->cfs && iter + 1 == c->niterations) io->cfs->push_back(cf);
             }
             if (!o.quiet) std::fflush(stdout);
         }
@@ -1570,7 +1555,7 @@ void welford(const std::vector<double>& f, double coeff, double* m_out, double* 
 // initial conditions until TOTAL_TRAJECTORIES / NITERATIONS of them have ended beyond R0 within
 // MaxTrajectoryLength samples (the reference draws a new one in place of each that did not);
 // the SF file is rewritten after every iteration.
-int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
+int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o, const HostSurfaces& hs, RunIO* io)
 {
     // the reference's checks (src/hawaii.c:3160-3168, 1529-1530)
     if (c->max_trajectory_length <= 0 || c->sampling_time <= 0 || c->cvode_tolerance <= 0 ||
@@ -1587,7 +1572,7 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
     if (c->pesmin == 0 || !(c->sampler_Rmin > 0 && c->sampler_Rmax > c->sampler_Rmin))
         return fail(HH_E_INVALID, "pesmin, sampler_Rmin and sampler_Rmax must be set");
     hh_system sys{};
-    int rc = resolve_system(e, c, &sys);
+    int rc = resolve_system(e, c, hs, &sys);
     if (rc < 0) return rc;
     const long L = c->max_trajectory_length;
     const double dt = c->sampling_time, T = c->temperature;
@@ -1640,7 +1625,7 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
     hh_sampler smp{HH_IC_REJECTION, ps, T, c->sampler_Rmin, c->sampler_Rmax, c->pesmin, 0.0};
     std::vector<double> ic((size_t)std::max(m0n + m2n, n_iter) * qp);
     long draws = 0;
-    long lrc = hh_sample_ics_from(&sys, &smp, seed, 0, m0n + m2n, nullptr, ic.data(), &draws);
+    long lrc = hh_sample_ics_from(&sys, &smp, seed, io ? io->skip_draws : 0, m0n + m2n, hs.pes, ic.data(), &draws);
     if (lrc < 0) return (int)lrc;
     const long nmax = std::max(m0n + m2n, n_iter);
     double *ic_dev = nullptr, *out_dev = nullptr, *sf_dev = nullptr;
@@ -1706,7 +1691,7 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
             long need = n_iter;
             int idle = 0;
             while (need > 0 && code == HH_OK) {
-                lrc = hh_sample_ics_from(&sys, &inj, seed, draws, need, nullptr, ic.data(), &draws);
+                lrc = hh_sample_ics_from(&sys, &inj, seed, draws, need, hs.pes, ic.data(), &draws);
                 if (lrc < 0) {
                     code = (int)lrc;
                     break;
@@ -1748,7 +1733,9 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
             out.ntraj = ntraj;
             for (int k = 0; k < nfreq; ++k) out.y[k] = sf.y[k] / ntraj;
             code = write_series(c->sf_filename, out);
+            if (code == HH_OK && io && io->sf) *io->sf = out;
         }
+        if (io) io->draws_used = draws;
     } while (false);
     if (ic_dev) (void)hipFree(ic_dev);
     if (out_dev) (void)hipFree(out_dev);
@@ -1756,43 +1743,48 @@ int run_prmu(hh_engine* e, const hh_conf* c, const hh_conf_run_opts& o)
     return code;
 }
 
-}  // namespace
+}  // namespace conf
+}  // namespace hh
 
 extern "C" {
 
 int hh_conf_parse(const char* path, hh_conf* out)
 {
-    if (!path || !out) return fail(HH_E_INVALID, "hh_conf_parse: null argument");
+    if (!path || !out) return hh::fail(HH_E_INVALID, "hh_conf_parse: null argument");
     std::ifstream in(path);
-    if (!in) return fail(HH_E_INVALID, "could not open '%s'", path);
+    if (!in) return hh::fail(HH_E_INVALID, "could not open '%s'", path);
     std::stringstream ss;
     ss << in.rdbuf();
-    return parse_text(ss.str(), path, out);
+    return hh::conf::parse_text(ss.str(), path, out);
 }
 
 int hh_conf_run(const hh_conf* conf, const hh_conf_run_opts* opts)
 {
-    if (!conf) return fail(HH_E_INVALID, "hh_conf_run: null configuration");
+    using namespace hh::conf;
+    if (!conf) return hh::fail(HH_E_INVALID, "hh_conf_run: null configuration");
     hh_conf_run_opts o{};
     if (opts) o = *opts;
+    HostSurfaces hs;
+    int rc = HH_OK;
+    if (conf->calculation_type != HH_CALC_PROCESSING && (rc = load_surfaces(conf, hs)) < 0) return rc;
     hh_engine* e = nullptr;
-    int rc = hh_engine_create(o.device, &e);
+    rc = hh_engine_create(o.device, &e);
     if (rc < 0) return rc;
     if (conf->calculation_type == HH_CALC_CORRELATION_SINGLE) {
-        rc = run_correlation_single(e, conf, o);
+        rc = run_correlation_single(e, conf, o, hs, nullptr);
         if (rc == HH_OK) rc = run_processing(e, conf, o.quiet != 0);
     } else if (conf->calculation_type == HH_CALC_PROCESSING) {
         rc = run_processing(e, conf, o.quiet != 0);
     } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M0) {
-        rc = run_phase_space_m0(e, conf, o);
+        rc = run_phase_space_m0(e, conf, o, hs);
     } else if (conf->calculation_type == HH_CALC_PHASE_SPACE_M2) {
-        rc = run_phase_space_m2(e, conf, o);
+        rc = run_phase_space_m2(e, conf, o, hs);
     } else if (conf->calculation_type == HH_CALC_PR_MU) {
-        rc = run_prmu(e, conf, o);
+        rc = run_prmu(e, conf, o, hs, nullptr);
     } else if (conf->calculation_type == HH_CALC_CORRELATION_ARRAY) {
-        rc = run_correlation_array(e, conf, o);
+        rc = run_correlation_array(e, conf, o, hs, nullptr);
     } else {
-        rc = fail(HH_E_INVALID, "CALCULATION_TYPE is not set");
+        rc = hh::fail(HH_E_INVALID, "CALCULATION_TYPE is not set");
     }
     hh_engine_destroy(e);
     return rc;

@@ -298,9 +298,24 @@ bool rejected(const Sys& s, const double* y, double T, double pesmin, Mt& mt)
     return weight < u;
 }
 
+// the built-in surfaces in the plugin ABI, q = [Phi, Theta, R, monomer angles]
+double hear_pes_lab(double* q) { return hear_V(q[2]); }
+double co2ar_pes_lab(double* q) { return co2ar_V(q); }
+double ch4co2_pes_lab(double* q) { return ch4co2_V(q); }
+
 }  // namespace
 
 extern "C" {
+
+hh_pes_fn hh_host_pes(int system)
+{
+    switch (system) {
+    case HH_SYS_HEAR: return hear_pes_lab;
+    case HH_SYS_CO2AR: return co2ar_pes_lab;
+    case HH_SYS_CH4CO2: return ch4co2_pes_lab;
+    default: return nullptr;
+    }
+}
 
 void hh_mt_fill(uint32_t seed, uint32_t* out, size_t n)
 {

@@ -31,11 +31,14 @@
  *                                           ALPHA, AVERAGE_CFS, ADD_SPECTRA, COMPUTE_Mn_CLASSICAL_/
  *                                           QUANTUM_DETAILED_BALANCE and the desymmetrizations D1, D2,
  *                                           D3, D4, D4a
- * Systems: two ATOM monomers (He-Ar surfaces) or LINEAR_MOLECULE + ATOM (CO2-Ar surfaces).
+ * Systems: two ATOM monomers (He-Ar surfaces), LINEAR_MOLECULE + ATOM (CO2-Ar surfaces) or
+ * ROTOR + LINEAR_MOLECULE (CH4-CO2 surfaces, CORRELATION_SINGLE).
  * Without PARTIAL_PARTITION_FUNCTION_RATIO the ratio is integrated by VEGAS as the reference does
- * (src/hawaii.c:2826-2880), and the VEGAS M0 and M2 follow.  A plugin named by
- * so_potential is dlopen'ed and its pes_lab drives the host rejection sampler, so the sampled
- * phase points follow the reference's draws; the device integrates with its built-in surfaces.
+ * (src/hawaii.c:2826-2880), and the VEGAS M0 and M2 follow.  Plugins named by so_potential and
+ * so_dipole are dlopen'ed; pes_lab drives the host rejection sampler, so the sampled phase points
+ * follow the reference's draws, and the device integrates its built-in surfaces, so a plugin is
+ * accepted only when it is the built-in surface of the configured monomers (it declares
+ * `int hh_system_id(void)`, or its functions agree with the built-in surface at 48 points).
  */
 #ifndef HAWAII_CONF_H
 #define HAWAII_CONF_H
@@ -48,7 +51,8 @@ extern "C" {
 
 enum { HH_CALC_NONE = 0, HH_CALC_CORRELATION_SINGLE = 1, HH_CALC_PROCESSING = 2, HH_CALC_PHASE_SPACE_M0 = 3,
        HH_CALC_CORRELATION_ARRAY = 4, HH_CALC_PHASE_SPACE_M2 = 5, HH_CALC_PR_MU = 6 };
-enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3 };
+enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2, HH_MONOMER_OTHER = 3,
+       HH_MONOMER_ROTOR = 4 };
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF, HH_PROC_READ_SPECTRUM, HH_PROC_WRITE_SPECTRUM,
        HH_PROC_ALPHA, HH_PROC_D1, HH_PROC_D2, HH_PROC_D3, HH_PROC_D4, HH_PROC_D4A, HH_PROC_AVERAGE_CFS,

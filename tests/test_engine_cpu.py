@@ -17,7 +17,8 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     names = set()
     for fn in os.listdir(os.path.join(REPO, "include")):
-        if not fn.endswith(".h"):
+        # hawaii_ref.h is libhawaii.so's (tests/test_ref_api.py)
+        if not fn.endswith(".h") or fn == "hawaii_ref.h":
             continue
         text = open(os.path.join(REPO, "include", fn)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)

@@ -56,10 +56,9 @@ def rel(a, b):
 
 def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
     """Poisson arrivals of CF requests (He-Ar or CO2-Ar, T ~ U[50, 500] K, ntraj ~ 2^U[10, 16])
-    and M0 requests, 4:1; inputs staged in HBM before the clock starts.  M0 requests draw
-    10^U[6, 7] points (the serving configuration allows up to 10^8; at 50 K the rejection
-    sampler needs hundreds of candidates per point, so the bench caps them to keep the run
-    within its time budget)."""
+    and M0 requests (10^U[6, 8] accepted points of the rejection sampler), 4:1; inputs staged in
+    HBM before the clock starts.  M0 slices run on their own stream beside the pools' slices.
+    Latency percentiles per request class."""
     import torch
     rng = np.random.default_rng(seed)
     n_req = max(1, int(rng.poisson(rate_hz * seconds)))
@@ -84,9 +83,9 @@ def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
             sysd = H.co2_ar() if co2 else H.he_ar()
             pes = H.PESMIN_CO2AR if co2 else H.PESMIN_HEAR
             T = float(rng.uniform(50.0, 500.0))
-            npts = int(10 ** rng.uniform(6, 7))
+            npts = int(10 ** rng.uniform(6, 8))
             smp = H.Sampler(H.IC_REJECTION, H.PAIR_FREE, T, 4.0, 40.0, pes, 0.0)
-            reqs.append(("m0", sysd, smp, npts, 16384))
+            reqs.append(("m0", sysd, smp, npts, 65536))
     torch.cuda.synchronize()
     s = H.Session(eng, slice_ms=10.0)
     ids = []
@@ -99,22 +98,29 @@ def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
             rid, _ = s.submit_cf(r[1], r[2], r[3])
         else:
             rid = s.submit_m0(r[1], r[2], 7 + i, r[3], r[4])
-        ids.append((rid, r[0], r[4] if r[0] == "cf" else r[3]))
-    infos = [(kind, n, s.poll(rid, 600000.0)) for rid, kind, n in ids]
+        cls = ("cf-co2ar" if r[1].system == H.SYS_CO2AR else "cf-hear") if r[0] == "cf" else "m0"
+        ids.append((rid, r[0], r[4] if r[0] == "cf" else r[3], cls))
+    infos = [(kind, n, s.poll(rid, 600000.0), cls) for rid, kind, n, cls in ids]
     wall = time.perf_counter() - t0
     s.close()
-    lat = np.array([inf.latency_ms for _, _, inf in infos if inf.state == H.REQ_DONE])
-    lat_cf = np.array([inf.latency_ms for k, _, inf in infos if k == "cf" and inf.state == H.REQ_DONE])
-    failed = sum(1 for _, _, inf in infos if inf.state != H.REQ_DONE)
-    traj = sum(n for k, n, inf in infos if k == "cf" and inf.state == H.REQ_DONE)
-    return {"requests": len(infos), "cf_requests": int(sum(1 for k, _, _ in infos if k == "cf")),
-            "m0_requests": int(sum(1 for k, _, _ in infos if k == "m0")), "failed": failed,
-            "arrival_rate_hz": rate_hz, "arrival_window_s": seconds, "wall_s": wall,
-            "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
-            "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
-            "p50_ms_cf": float(np.percentile(lat_cf, 50)) if lat_cf.size else None,
-            "p99_ms_cf": float(np.percentile(lat_cf, 99)) if lat_cf.size else None,
-            "aggregate_traj_per_s": traj / wall}
+    done = [(k, n, inf, c) for k, n, inf, c in infos if inf.state == H.REQ_DONE]
+    lat = np.array([inf.latency_ms for _, _, inf, _ in done])
+    failed = len(infos) - len(done)
+    traj = sum(n for k, n, _, _ in done if k == "cf")
+    pts = sum(n for k, n, _, _ in done if k == "m0")
+    out = {"requests": len(infos), "cf_requests": int(sum(1 for i in infos if i[0] == "cf")),
+           "m0_requests": int(sum(1 for i in infos if i[0] == "m0")), "failed": failed,
+           "arrival_rate_hz": rate_hz, "arrival_window_s": seconds, "wall_s": wall,
+           "p50_ms": float(np.percentile(lat, 50)) if lat.size else None,
+           "p99_ms": float(np.percentile(lat, 99)) if lat.size else None,
+           "aggregate_traj_per_s": traj / wall, "m0_points_per_s": pts / wall,
+           "m0_points_range": "10^6 - 10^8", "per_class": {}}
+    for cls in ("cf-hear", "cf-co2ar", "m0"):
+        lc = np.array([inf.latency_ms for _, _, inf, c in done if c == cls])
+        out["per_class"][cls] = {"requests": int(sum(1 for i in infos if i[3] == cls)),
+                                 "p50_ms": float(np.percentile(lc, 50)) if lc.size else None,
+                                 "p99_ms": float(np.percentile(lc, 99)) if lc.size else None}
+    return out
 
 
 def n2ar_section(H, eng, npoints, sample_check):
@@ -178,6 +184,53 @@ def n2ar_section(H, eng, npoints, sample_check):
     return out
 
 
+def ch4co2_section(H, eng, ntraj, L, check_n):
+    """CH4-CO2 (spherical top + linear rotor, 16-dim phase space; stand-in PES): `ntraj`
+    R0-injected trajectories at 300 K (R0 = Rcut = 40 bohr), L samples at dt = 200 a.u., in jobs of
+    65536.  The mixed engine (stand-in PES and dipole in FP32, state and error control in FP64,
+    rtol 1e-10, orbiting pairs handed over to FP64 at 1e-12) against the all-FP64 engine at
+    1e-12 on the whole ensemble (CF relative L2, per-trajectory energy drift |E(end) - E0|), and
+    both against the CPU oracle on the first `check_n` trajectories."""
+    import oracle_lib as O
+    import torch
+    smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.75, 40.0, H.PESMIN_CH4CO2, 40.0)
+    ic, _, _ = H.sample_ics(H.ch4_co2(), smp, 42, ntraj)
+    job = 65536
+    xs = [eng.to_device_soa(np.ascontiguousarray(ic[j:j + job])) for j in range(0, ntraj, job)]
+    out = {"trajectories": ntraj, "L": L, "T_K": 300.0, "jobs": len(xs)}
+    res = {}
+    for name, prec, rtol in (("mixed", H.PREC_MIXED, 1e-10), ("fp64", H.PREC_FP64, 1e-12)):
+        p = H.cf_params(L, rtol=rtol, precision=prec)
+        eng.cf_run(H.ch4_co2(), p, xs[:1])          # warm-up (code objects, pool)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cfs, st = eng.cf_run(H.ch4_co2(), p, xs)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        drift = np.concatenate([eng.items(j, min(job, ntraj - j * job), L)["e_drift"] for j in range(len(xs))])
+        cf = sum(c.cpu().numpy() * min(job, ntraj - j * job) for j, c in enumerate(cfs)) / ntraj
+        res[name] = (cf, drift)
+        out[name] = {"traj_per_s": ntraj / dt, "s": dt, "nrhs": st.nrhs, "rtol": rtol,
+                     "handed_over_to_fp64": st.n_orbit,
+                     "energy_drift_hartree_median": float(np.median(drift)),
+                     "energy_drift_hartree_p99": float(np.quantile(drift, 0.99)),
+                     "energy_drift_hartree_max": float(drift.max())}
+        log(f"CH4-CO2 {name}: {ntraj / dt:.0f} traj/s, {st.nrhs} right-hand sides, {st.n_orbit} handed over")
+    out["cf_rel_l2_mixed_vs_fp64_engine"] = rel(res["mixed"][0], res["fp64"][0])
+    if check_n:
+        n = check_n
+        ref = O.cf_rows(O.ch4co2(), O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000), ic[:n],
+                        min(16, os.cpu_count() or 1))
+        cf_ref = O.ensemble_from_rows(ref["crln"])
+        sub = {}
+        for name, prec, rtol in (("mixed", H.PREC_MIXED, 1e-10), ("fp64", H.PREC_FP64, 1e-12)):
+            cfs, _ = eng.cf_run(H.ch4_co2(), H.cf_params(L, rtol=rtol, precision=prec),
+                                [eng.to_device_soa(np.ascontiguousarray(ic[:n]))])
+            sub[name] = rel(cfs[0].cpu().numpy(), cf_ref)
+        out["oracle_subsample"] = {"trajectories": n, "cf_rel_l2_mixed": sub["mixed"], "cf_rel_l2_fp64": sub["fp64"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +247,9 @@ def main():
     ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-n2ar", action="store_true")
+    ap.add_argument("--no-ch4co2", action="store_true")
+    ap.add_argument("--ch4co2-ntraj", type=int, default=262144)
+    ap.add_argument("--ch4co2-L", type=int, default=1024)
     ap.add_argument("--n2ar-points", type=int, default=1 << 20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
@@ -365,6 +421,11 @@ def main():
         log(f"N2-Ar PIP-NN surfaces: {args.n2ar_points} phase points")
         result["n2ar_pipnn"] = n2ar_section(H, eng, args.n2ar_points, 16384 if not args.no_oracle else 64)
         log("n2ar", result["n2ar_pipnn"])
+
+    if not args.no_ch4co2:
+        log(f"CH4-CO2: {args.ch4co2_ntraj} trajectories, L={args.ch4co2_L}")
+        result["ch4co2"] = ch4co2_section(H, eng, args.ch4co2_ntraj, args.ch4co2_L, 0 if args.no_oracle else 512)
+        log("ch4co2", result["ch4co2"])
 
     if not args.no_oracle:
         import ctypes as C

@@ -315,12 +315,10 @@ int hh_engine_create(int device, hh_engine** out)
         delete e;
         return fail(HH_E_DEVICE, "device %d is %s; this build targets gfx950 (MI355X)", device, arch.c_str());
     }
+    // graph-captured slices are opt-in
+    if (const char* g = std::getenv("HH_GRAPHS")) e->use_graphs = g[0] && g[0] != '0';
     // a blocking stream: the engine's work is ordered after the caller's copies and kernels on
     // the legacy default stream (hipMemcpy of pageable memory can return before its DMA ends)
-    // the graph-captured slices and the launch-shape tuner are opt-in until a hang seen with both
-    // enabled is explained (hh_pool.cpp)
-    if (const char* g = std::getenv("HH_GRAPHS")) e->use_graphs = g[0] && g[0] != '0';
-    if (const char* t = std::getenv("HH_TUNE")) e->use_tuner = t[0] && t[0] != '0';
     if (hipStreamCreateWithFlags(&e->stream, hipStreamDefault) != hipSuccess) {
         delete e;
         return fail(HH_E_DEVICE, "hipStreamCreate failed");
@@ -426,20 +424,10 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
 
     if (p->engine == HH_ENGINE_BATCHED) {
         // resident lane pool: all jobs are admitted at once and streamed through time slices
-        int block = p->block, waves = 2;
-        if (block == 0 && e->use_tuner) {
-            rc = hh::pool_tune(e, *sys, *p, ic_dev, ntraj, njobs, &block, &waves);
-            if (rc < 0) return rc;
-        } else if (block != 0 && block != 64 && block != 128 && block != 256) {
-            return fail(HH_E_INVALID, "block must be 0 (tuned), 64, 128 or 256");
-        } else if (sys->system == HH_SYS_CH4CO2 && block != 0 && block != 256) {
-            return fail(HH_E_INVALID, "CH4-CO2 runs in 256-lane workgroups only (block 0 or 256)");
-        } else {
-            if (block == 0) block = 256;
-            waves = hh::pool_waves(block);
-        }
+        if (p->block != 0 && p->block != hh::kPoolBlock)
+            return fail(HH_E_INVALID, "block must be 0 or 256 (the batched engine's one launch shape)");
         hh::LanePool* pl = nullptr;
-        rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, block, &pl, waves);
+        rc = hh::pool_get(e, e->pools, *sys, p->precision == HH_PREC_MIXED, &pl);
         if (rc < 0) return rc;
         if (!pl->inflight.empty()) return fail(HH_E_BUSY, "lane pool of this class is in use");
         if (njobs > pl->ring_cap) return fail(HH_E_INVALID, "at most %d jobs per call", pl->ring_cap);
@@ -483,8 +471,8 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
         if (stats) {
             stats->integrate_ms = kms;
             stats->reduce_ms = rms;
-            stats->block = block;
-            stats->grid = (int)(pl->S / block);
+            stats->block = hh::kPoolBlock;
+            stats->grid = (int)(pl->S / hh::kPoolBlock);
             stats->launches = launches;
             return hh::collect_stats(e->last_jobs, stats);
         }

@@ -195,7 +195,6 @@ struct hh_engine {
     hipStream_t stream = nullptr;
     hipStream_t capture = nullptr;      // non-blocking stream used only to capture slice graphs
     bool use_graphs = false;            // HH_GRAPHS=1: slices as captured graphs (off: see hh_pool.cpp)
-    bool use_tuner = false;             // HH_TUNE=1: block = 0 measures the launch shape (off: see hh_pool.cpp)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     int wall_khz = 100000;        // s_memrealtime frequency
@@ -209,7 +208,6 @@ struct hh_engine {
     std::vector<hh::JobDesc> last_jobs;
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items
     std::vector<std::unique_ptr<hh::LanePool>> pools;
-    std::vector<std::array<int, 4>> tuned;    // (system, mixed, block, waves) chosen by pool_tune
     // serialises calls on the engine; recursive so that hh_cf_array and hh_prmu hold it across
     // their inner run and the read of its per-item results
     std::recursive_mutex mu;
@@ -226,17 +224,13 @@ inline SysConst sysconst(const hh_engine* e, const hh_system& sys, bool f32)
 // the ensemble CF of a finished job into cf_out (on the engine stream)
 int reduce_job(hh_engine* e, const JobDesc& jd, double ppf, double* cf_out);
 using PoolSet = std::vector<std::unique_ptr<LanePool>>;
-// resident wavefronts per SIMD of the default launch shape for a workgroup size: 256-lane
-// workgroups run one wavefront per SIMD (wider tableau column groups in registers), 64 and
-// 128 two
-inline int pool_waves(int block) { return block == 256 ? 1 : 2; }
-// the lane pool of class (system, mu, I, precision, block) in `set`, created on first use
-int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out,
-             int waves = 2);
-// launch shape (workgroup size, resident wavefronts per SIMD) of the batched engine for this
-// system and precision: measured once per engine on the caller's own trajectories, then cached
-int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const double* const* ic_dev,
-              const long* ntraj, int njobs, int* block, int* waves);
+// The batched engine's one launch shape: 256-lane workgroups, one wavefront per SIMD.  Builds at
+// two wavefronts per SIMD (64- and 128-lane workgroups) gave rows that differed between runs of
+// the same input on a few items in 10^4, with extra right-hand sides, in FP64 and in mixed
+// precision, while the one-wavefront build and the plain engine reproduce their bits.
+constexpr int kPoolBlock = 256, kPoolWaves = 1;
+// the lane pool of class (system, mu, I, precision) in `set`, created on first use
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, LanePool** out);
 // device bytes a job of this shape holds while it runs
 size_t pool_job_bytes(const hh_cf_params& p, long ntraj, int record = -1);
 // true when a job of this class hands its orbiting items over to FP64 (mixed precision, a

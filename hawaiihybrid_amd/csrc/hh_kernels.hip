@@ -1301,41 +1301,20 @@ hipError_t run_plain(int sys, const SysConst& sc, const JobDesc* jobs_dev, int n
 hipError_t run_pool(int sys, bool mixed, int block, int waves, const SysConst& sc, const JobDesc* ring,
                     const PoolCtl& ctl, const LaneSlots& sl, double* ws, hipStream_t st)
 {
-#define HH_LAUNCH(S, PT, B, W) return launch_pool<S, PT, B, W>(sc, ring, ctl, sl, ws, st)
+#define HH_LAUNCH(S, PT) return launch_pool<S, PT, 256, 1>(sc, ring, ctl, sl, ws, st)
+    // the one launch shape: 256-lane workgroups, one wavefront per SIMD
+    if (block != 256 || waves != 1) return hipErrorInvalidValue;
     if (sys == SYS_CH4CO2) {
-        // one launch shape: 256-lane workgroups, one wavefront per SIMD
-        if (block != 256 || waves != 1) return hipErrorInvalidValue;
-        if (mixed) HH_LAUNCH(SYS_CH4CO2, float, 256, 1);
-        HH_LAUNCH(SYS_CH4CO2, double, 256, 1);
+        if (mixed) HH_LAUNCH(SYS_CH4CO2, float);
+        HH_LAUNCH(SYS_CH4CO2, double);
     }
-    if (waves == 1) {
-        if (block != 256) return hipErrorInvalidValue;
-        // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
-        if (sys == SYS_HEAR) {
-            if (mixed) HH_LAUNCH(SYS_HEAR, float, 256, 1);
-            HH_LAUNCH(SYS_HEAR, double, 256, 1);
-        }
-        if (mixed) HH_LAUNCH(SYS_CO2AR, float, 256, 1);
-        HH_LAUNCH(SYS_CO2AR, double, 256, 1);
-    }
+    // He-Ar surfaces stay FP64 (see hear_dV); only the dipole follows P
     if (sys == SYS_HEAR) {
-        if (mixed) {
-            if (block == 64) HH_LAUNCH(SYS_HEAR, float, 64, 2);
-            if (block == 128) HH_LAUNCH(SYS_HEAR, float, 128, 2);
-            HH_LAUNCH(SYS_HEAR, float, 256, 2);
-        }
-        if (block == 64) HH_LAUNCH(SYS_HEAR, double, 64, 2);
-        if (block == 128) HH_LAUNCH(SYS_HEAR, double, 128, 2);
-        HH_LAUNCH(SYS_HEAR, double, 256, 2);
+        if (mixed) HH_LAUNCH(SYS_HEAR, float);
+        HH_LAUNCH(SYS_HEAR, double);
     }
-    if (mixed) {
-        if (block == 64) HH_LAUNCH(SYS_CO2AR, float, 64, 2);
-        if (block == 128) HH_LAUNCH(SYS_CO2AR, float, 128, 2);
-        HH_LAUNCH(SYS_CO2AR, float, 256, 2);
-    }
-    if (block == 64) HH_LAUNCH(SYS_CO2AR, double, 64, 2);
-    if (block == 128) HH_LAUNCH(SYS_CO2AR, double, 128, 2);
-    HH_LAUNCH(SYS_CO2AR, double, 256, 2);
+    if (mixed) HH_LAUNCH(SYS_CO2AR, float);
+    HH_LAUNCH(SYS_CO2AR, double);
 #undef HH_LAUNCH
 }
 

@@ -1,7 +1,7 @@
 // Resident lane pools of the batched engine.
 //
-// A pool owns S = 2 waves/SIMD x 64 lanes of integrator slots for one class of work (system,
-// mu, I, precision, workgroup size).  Work runs in slices: one launch of pool_cf_kernel
+// A pool owns S = 1 wavefront/SIMD x 64 lanes of integrator slots (65536 on 256 CUs) for one
+// class of work (system, mu, I, precision).  Work runs in slices: one launch of pool_cf_kernel
 // restores every slot, integrates and refills from the pool's queue until its time is up, and
 // saves the slots again.  Between slices the host
 //   * admits new jobs (their items are appended to the queue: continuous batching),
@@ -26,11 +26,10 @@ namespace {
 const int kZero = 0;
 }  // namespace
 
-int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int block, LanePool** out, int waves)
+int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, LanePool** out)
 {
     for (auto& p : set)
-        if (p->sys == sys.system && p->mixed == mixed && p->block == block && p->waves == waves &&
-            p->system.mu == sys.mu && p->system.I == sys.I && p->system.I_rotor[0] == sys.I_rotor[0] &&
+        if (p->sys == sys.system && p->mixed == mixed && p->system.mu == sys.mu && p->system.I == sys.I && p->system.I_rotor[0] == sys.I_rotor[0] &&
             p->system.I_rotor[1] == sys.I_rotor[1] && p->system.I_rotor[2] == sys.I_rotor[2]) {
             *out = p.get();
             return HH_OK;
@@ -39,11 +38,11 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, int b
     pl->sys = sys.system;
     pl->system = sys;
     pl->mixed = mixed;
-    pl->block = block;
-    pl->waves = waves;
-    // `waves` wavefronts per SIMD, four SIMDs per CU
-    const long long grid = (long long)e->prop.multiProcessorCount * waves * (256 / block);
-    pl->S = grid * block;
+    pl->block = kPoolBlock;
+    pl->waves = kPoolWaves;
+    // one 256-lane workgroup (a wavefront on each of its four SIMDs) per CU
+    const long long grid = (long long)e->prop.multiProcessorCount * kPoolWaves * (256 / kPoolBlock);
+    pl->S = grid * kPoolBlock;
     const size_t sb = slots_bytes(pl->sys, pl->S);
     const size_t arena = (size_t)arena_doubles_per_lane(pl->sys) * sizeof(double) * (size_t)pl->S;
     if (pl->slots[0].ensure(sb) != hipSuccess || pl->slots[1].ensure(sb) != hipSuccess ||
@@ -192,86 +191,8 @@ int pool_order_group(hh_engine* e, LanePool& pl)
     return HH_OK;
 }
 
-// Both the tuner and the graph-captured slices below are opt-in (HH_TUNE=1, HH_GRAPHS=1): the
-// first GPU run with both enabled by default went silent in the .conf driver's batched run
-// (killed after 180 s, GPU responsive) and the cause is not established, so the default path
-// is the direct launch at 256 x 2 waves/SIMD that the rest of the tests exercise.
-int pool_tune(hh_engine* e, const hh_system& sys, const hh_cf_params& p, const double* const* ic_dev,
-              const long* ntraj, int njobs, int* block, int* waves)
-{
-    const bool mixed = p.precision == HH_PREC_MIXED;
-    if (sys.system == HH_SYS_CH4CO2) {
-        // one launch shape is built for the rotor system (its state fills the register file)
-        *block = 256;
-        *waves = 1;
-        return HH_OK;
-    }
-    for (const auto& t : e->tuned)
-        if (t[0] == sys.system && t[1] == (int)mixed) {
-            *block = t[2];
-            *waves = t[3];
-            return HH_OK;
-        }
-    // candidates: workgroup size at two wavefronts per SIMD (the register budget allows two), and
-    // one wavefront per SIMD (half the lanes, half the arena traffic)
-    static const int kCand[][2] = {{256, 2}, {128, 2}, {64, 2}, {256, 1}};
-    constexpr double kSliceMs = 10.0;
-    DevBuf cf_tmp, work;
-    HH_CHECK(cf_tmp.ensure((size_t)p.L * sizeof(double)));
-    HH_CHECK(work.ensure(sizeof(unsigned long long)));
-    double best = -1.0;
-    int bb = 256, bw = 2;
-    for (const auto& c : kCand) {
-        PoolSet tmp;
-        LanePool* pl = nullptr;
-        int rc = pool_get(e, tmp, sys, mixed, c[0], &pl, c[1]);
-        if (rc < 0) return rc;
-        std::vector<PoolJob*> jobs;
-        for (int j = 0; j < njobs && j < pl->ring_cap; ++j) {
-            PoolJob* pj = nullptr;
-            rc = pool_admit(e, *pl, p, ic_dev[j], ntraj[j], cf_tmp.as<double>(), 0, &pj);
-            if (rc < 0) return rc;
-            HH_CHECK(hipMemsetAsync(pj->jd.nrhs, 0, (size_t)pj->jd.n_items * sizeof(int), e->stream));
-            jobs.push_back(pj);
-        }
-        rc = pool_order_group(e, *pl);
-        if (rc < 0) return rc;
-        PoolCtl ctl{};
-        ctl.next = pl->next.as<unsigned long long>();
-        ctl.total = pl->total;
-        ctl.first_seq = pl->inflight.front()->seq;
-        ctl.last_seq = pl->inflight.back()->seq;
-        ctl.ticks = (long long)(kSliceMs * e->wall_khz);
-        ctl.ring = pl->ring_cap;
-        const SysConst sc = sysconst(e, sys, mixed);
-        HH_CHECK(hipEventRecord(e->ev[0], e->stream));
-        HH_CHECK(run_pool(pl->sys, mixed, pl->block, pl->waves, sc, pl->ring.as<JobDesc>(), ctl, pl->view(pl->cur),
-                          pl->arena.as<double>(), e->stream));
-        HH_CHECK(hipEventRecord(e->ev[1], e->stream));
-        HH_CHECK(hipMemsetAsync(work.p, 0, sizeof(unsigned long long), e->stream));
-        for (PoolJob* pj : jobs)
-            HH_CHECK(run_pool_work(pl->view(pl->cur), (int)(pj->seq % pl->ring_cap), pj->jd.nrhs, pj->jd.n_items,
-                                   work.as<unsigned long long>(), e->stream));
-        unsigned long long w = 0;
-        HH_CHECK(hipMemcpyAsync(&w, work.p, sizeof w, hipMemcpyDeviceToHost, e->stream));
-        HH_CHECK(hipStreamSynchronize(e->stream));
-        float ms = 0.0f;
-        HH_CHECK(hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
-        const double rate = ms > 0.0f ? (double)w / ms : 0.0;
-        if (std::getenv("HH_POOL_TRACE"))
-            std::fprintf(stderr, "[hh tune] block %d, %d waves/SIMD: %.3g right-hand sides/ms\n", c[0], c[1], rate);
-        if (rate > best) {
-            best = rate;
-            bb = c[0];
-            bw = c[1];
-        }
-    }
-    e->tuned.push_back({sys.system, (int)mixed, bb, bw});
-    *block = bb;
-    *waves = bw;
-    return HH_OK;
-}
-
+// The graph-captured slices below are opt-in (HH_GRAPHS=1); the default path launches each
+// slice directly.
 namespace {
 
 bool same_ctl(const PoolCtl& a, const PoolCtl& b)
@@ -349,7 +270,7 @@ int admit_orbit_job(hh_engine* e, PoolSet& set, LanePool& parent_pool, PoolJob& 
 {
     LanePool* op = parent.orbit_pool;
     if (!op) {
-        const int rc = pool_get(e, set, parent_pool.system, false, 256, &op, pool_waves(256));
+        const int rc = pool_get(e, set, parent_pool.system, false, &op);
         if (rc < 0) return rc;
         parent.orbit_pool = op;
     }

@@ -55,6 +55,7 @@ struct hh_session {
     uint64_t next_id = 1;
     Clock::time_point t0;
     double kms = 0.0, rms = 0.0;   // device time of all slices / reductions so far
+    hipStream_t m0_stream = nullptr;   // M0 slices, beside the pools' streams (default priority)
     std::deque<std::unique_ptr<Request>> waiting;
     std::vector<std::unique_ptr<Request>> running;
     std::map<uint64_t, std::unique_ptr<Request>> finished;
@@ -74,8 +75,7 @@ const int kZero = 0;
 int hh_session::admit(Request& r)
 {
     if (r.kind == HH_REQ_CF) {
-        int rc = hh::pool_get(e, pools, r.sys, r.p.precision == HH_PREC_MIXED, opts.block, &r.pool,
-                              hh::pool_waves(opts.block));
+        int rc = hh::pool_get(e, pools, r.sys, r.p.precision == HH_PREC_MIXED, &r.pool);
         if (rc < 0) return rc;
         if (!hh::pool_has_room(*r.pool)) return HH_E_BUSY;
         r.kms0 = kms;
@@ -85,18 +85,15 @@ int hh_session::admit(Request& r)
     if (r.m0buf.ensure(hh::m0_bytes(r.nstreams)) != hipSuccess)
         return hh::fail(HH_E_NOMEM, "cannot allocate M0 stream state");
     const hh::M0Acc acc = hh::m0_view(r.m0buf.p, r.nstreams);
-    HH_CHECK(hipMemcpyAsync(acc.finished, &kZero, sizeof(int), hipMemcpyHostToDevice, e->stream));
+    if (!m0_stream) HH_CHECK(hipStreamCreateWithFlags(&m0_stream, hipStreamNonBlocking));
+    HH_CHECK(hipMemcpyAsync(acc.finished, &kZero, sizeof(int), hipMemcpyHostToDevice, m0_stream));
     return HH_OK;
 }
 
-// engine lock held by the caller: one slice of the pools and of every running M0 request
+// engine lock held by the caller: one slice of every running M0 request on the M0 stream and of
+// the pools on theirs, so the phase-space requests share the device with the trajectories
 int hh_session::slice(std::string& err)
 {
-    int rc = hh::pool_step(e, pools, opts.slice_ms, &kms, &rms);
-    if (rc < 0) {
-        err = hh_last_error();
-        return rc;
-    }
     bool m0 = false;
     for (auto& rp : running) {
         Request& r = *rp;
@@ -106,7 +103,7 @@ int hh_session::slice(std::string& err)
         hh::M0Params mp{r.smp.T, r.smp.Rmin, r.smp.Rmax, r.smp.pesmin, r.smp.pair_state, 0};
         const hipError_t he = hh::run_m0(r.sys.system, sc, mp, r.seed, r.nstreams, r.npoints,
                                          reinterpret_cast<uint32_t*>(r.m0buf.p), acc, r.m0_started ? 0 : 1,
-                                         (long long)(opts.slice_ms * e->wall_khz), e->stream);
+                                         (long long)(opts.slice_ms * e->wall_khz), m0_stream);
         if (he != hipSuccess) {
             err = hipGetErrorString(he);
             return HH_E_DEVICE;
@@ -114,12 +111,17 @@ int hh_session::slice(std::string& err)
         r.m0_started = true;
         m0 = true;
     }
+    int rc = hh::pool_step(e, pools, opts.slice_ms, &kms, &rms);
     if (m0) {
-        const hipError_t he = hipStreamSynchronize(e->stream);
-        if (he != hipSuccess) {
+        const hipError_t he = hipStreamSynchronize(m0_stream);
+        if (he != hipSuccess && rc >= 0) {
             err = hipGetErrorString(he);
             return HH_E_DEVICE;
         }
+    }
+    if (rc < 0) {
+        err = hh_last_error();
+        return rc;
     }
     return HH_OK;
 }
@@ -264,7 +266,9 @@ int hh_session_create(hh_engine* e, const hh_session_opts* opts, hh_session** ou
     s->e = e;
     if (opts) s->opts = *opts;
     if (s->opts.slice_ms <= 0) s->opts.slice_ms = 10.0;
-    if (s->opts.block != 64 && s->opts.block != 128 && s->opts.block != 256) s->opts.block = 256;
+    if (s->opts.block != 0 && s->opts.block != hh::kPoolBlock)
+        return hh::fail(HH_E_INVALID, "block must be 0 or 256 (the lane pools' one launch shape)");
+    s->opts.block = hh::kPoolBlock;
     {
         std::lock_guard<std::recursive_mutex> el(e->mu);
         HH_CHECK(hipSetDevice(e->device));
@@ -372,6 +376,10 @@ int hh_session_destroy(hh_session* s)
         (void)hipSetDevice(s->e->device);
         (void)hipStreamSynchronize(s->e->stream);
         s->pools.clear();
+        if (s->m0_stream) {
+            (void)hipStreamSynchronize(s->m0_stream);
+            (void)hipStreamDestroy(s->m0_stream);
+        }
     }
     delete s;
     return HH_OK;

@@ -114,6 +114,12 @@ long hh_sample_ics(const hh_system* sys, const hh_sampler* smp, uint32_t seed, l
 long hh_sample_ics_from(const hh_system* sys, const hh_sampler* smp, uint32_t seed, long skip_draws, long ntraj,
                         hh_pes_fn pes, double* ic_rowmajor, long* n_draws);
 
+/* The engine's built-in host potential of `system` in the plugin ABI (a pes_lab: q = [Phi,
+ * Theta, R, monomer angles], Hartree), the one the sampler's rejection step uses when its pes is
+ * NULL; NULL for a system without one.  No counterpart in the reference: it stands in for a
+ * plugin's pes_lab in programs that have none. */
+hh_pes_fn hh_host_pes(int system);
+
 /* ------------------------------------------------------------------ engine */
 typedef struct hh_engine hh_engine;
 
@@ -135,9 +141,9 @@ typedef struct {
     int max_steps;       /* max integrator steps between two samples (<= 0: 50000) */
     int engine;          /* HH_ENGINE_* */
     int precision;       /* HH_PREC_*; the plain engine is FP64 only */
-    int block;           /* batched: workgroup size 64/128/256; 256 runs one wavefront per SIMD, 64 and
-                            128 two (0: 256, or measured when HH_TUNE=1) */
-    int grid;            /* batched: resident workgroups (0: autotuned) */
+    int block;           /* batched: 0 or 256, the one launch shape (256-lane workgroups, one
+                            wavefront per SIMD) */
+    int grid;            /* reserved (0) */
     int zimmermann;      /* use_zimmermann_trick: CF averaged over every time origin of the
                             trajectory's dipole series (correlation_eval_zimmerman_trick,
                             src/hawaii.c:1888-2114); both halves, L <= 4096 */
@@ -374,7 +380,7 @@ typedef struct hh_session hh_session;
 typedef struct {
     double slice_ms;          /* pool kernel time slice (<= 0: 10 ms) */
     double max_device_gib;    /* admission budget for request buffers (<= 0: 80% of the free HBM) */
-    int block;                /* lane-pool workgroup size 64 / 128 / 256 (0: 256, one wavefront per SIMD) */
+    int block;                /* 0 or 256: the lane pools' one launch shape */
 } hh_session_opts;
 
 enum { HH_REQ_WAITING = 0, HH_REQ_RUNNING = 1, HH_REQ_DONE = 2, HH_REQ_FAILED = 3 };

@@ -174,25 +174,48 @@ def test_job_result_identical_alone_and_in_a_mix(eng):
     assert np.array_equal(alone[0].cpu().numpy(), mixed[1].cpu().numpy())
 
 
-@pytest.mark.parametrize("prec", [H.PREC_MIXED, H.PREC_FP64])
-def test_launch_shape_does_not_change_results(eng, prec):
-    """The 256-lane pool runs one wavefront per SIMD and combines whole tableau rows at once; 64-
-    and 128-lane pools run two per SIMD and combine rows a few columns at a time.  Sums keep their
-    order and samples are evaluated by the same code, so every shape (and the plain engine, one
-    item per lane with lane-serial sampling) gives the same bits."""
+def test_batched_engine_reproduces_the_plain_engine_bit_for_bit(eng):
+    """FP64: the resident lane pool (256-lane workgroups, one wavefront per SIMD, wavefront-shared
+    sampling, items handed out by a device queue) and the plain engine (one item per thread,
+    lane-serial sampling) run the same arithmetic per item, so 3 jobs of 8192 trajectories give
+    the same bits in both, and again on a second run of the pool.  Other workgroup sizes are
+    refused: the batched engine has one launch shape."""
     smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
-    ic = eng.to_device_soa(H.sample_ics(H.co2_ar(), smp, 9, 600)[0])
+    ic = H.sample_ics(H.co2_ar(), smp, 9, 3 * 8192)[0]
+    xs = [eng.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
     outs = []
-    for block in (0, 128, 64):
-        p = H.cf_params(512, engine=H.ENGINE_BATCHED, precision=prec, rtol=1e-10, block=block)
-        cf, st = eng.cf_run(H.co2_ar(), p, [ic])
+    for block in (0, 256):
+        p = H.cf_params(512, engine=H.ENGINE_BATCHED, precision=H.PREC_FP64, rtol=1e-10, block=block)
+        cf, st = eng.cf_run(H.co2_ar(), p, xs)
         print(f"  block {st.block} grid {st.grid}", flush=True)
-        outs.append(cf[0].cpu().numpy())
-    if prec == H.PREC_FP64:
-        cf, _ = eng.cf_run(H.co2_ar(), H.cf_params(512, engine=H.ENGINE_PLAIN, precision=prec, rtol=1e-10), [ic])
-        outs.append(cf[0].cpu().numpy())
+        assert st.block == 256
+        outs.append([c.cpu().numpy() for c in cf])
+    cf, _ = eng.cf_run(H.co2_ar(), H.cf_params(512, engine=H.ENGINE_PLAIN, precision=H.PREC_FP64, rtol=1e-10), xs)
+    outs.append([c.cpu().numpy() for c in cf])
     for o in outs[1:]:
-        assert np.array_equal(outs[0], o)
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    for block in (64, 128):
+        with pytest.raises(H.HHError, match="256"):
+            eng.cf_run(H.co2_ar(), H.cf_params(512, precision=H.PREC_FP64, rtol=1e-10, block=block), xs)
+
+
+def test_mixed_precision_runs_reproducibly(eng):
+    """Three runs of the mixed engine on 3 jobs of 8192 trajectories give the same bits and the
+    same right-hand-side counts per item."""
+    smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
+    ic = H.sample_ics(H.co2_ar(), smp, 42, 3 * 8192)[0]
+    xs = [eng.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
+    runs = []
+    for _ in range(3):
+        cf, st = eng.cf_run(H.co2_ar(), H.cf_params(512, precision=H.PREC_MIXED, rtol=1e-10), xs)
+        assert st.block == 256
+        runs.append(([c.cpu().numpy() for c in cf], [eng.items(j, 8192, 512)["nrhs"] for j in range(3)]))
+    for cf, nrhs in runs[1:]:
+        for a, b in zip(runs[0][0], cf):
+            assert np.array_equal(a, b)
+        for a, b in zip(runs[0][1], nrhs):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])

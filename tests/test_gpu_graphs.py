@@ -1,60 +1,48 @@
-"""Graph-captured slices and the launch-shape autotuner of the batched engine.
+"""Graph-captured slices of the batched engine.
 
 The slices of a lane pool run as captured graphs (repack when the queue has drained, the pool
 kernel, copies of the counters into pinned memory), replayed many times per run with the host's
 reductions and bookkeeping between replays.  Every replay must give the same bits as the
-directly launched slices (an engine created without HH_GRAPHS=1), and the result must not depend
-on the launch shape the autotuner picks (each item's arithmetic is independent of where it runs).
-The second and later runs on one engine reuse its lane pool, whose device queue head the first
-run left past the end (pool_admit clamps it back before the new jobs).
+directly launched slices (an engine created without HH_GRAPHS=1), in FP64 and in mixed
+precision.  The second and later runs on one engine reuse its lane pool, whose device queue head
+the first run left past the end (pool_admit clamps it back before the new jobs).
 """
 import numpy as np
 import pytest
 
 import hawaiihybrid_amd as H
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="the 64- and 128-lane pools (two wavefronts per SIMD) give rows that differ "
-                                      "from run to run in the last bits on 3 x 8192 trajectories; graph replays "
-                                      "and the queue head across runs are correct, the shape dependence is open")]
+pytestmark = pytest.mark.gpu
 
 
 def graph_engine(monkeypatch):
-    """An engine with graph-captured slices and the launch-shape tuner (both read at creation)."""
+    """An engine with graph-captured slices (read at creation)."""
     monkeypatch.setenv("HH_GRAPHS", "1")
-    monkeypatch.setenv("HH_TUNE", "1")
     e = H.Engine(0)
     monkeypatch.delenv("HH_GRAPHS")
-    monkeypatch.delenv("HH_TUNE")
     return e
 
 
-def run(eng, ic_dev, block, L=512):
-    p = H.cf_params(L, rtol=1e-10, engine=H.ENGINE_BATCHED, precision=H.PREC_MIXED, block=block)
+def run(eng, ic_dev, prec, L=512):
+    p = H.cf_params(L, rtol=1e-10, engine=H.ENGINE_BATCHED, precision=prec)
     outs, st = eng.cf_run(H.co2_ar(), p, ic_dev)
     return [o.cpu().numpy() for o in outs], st
 
 
-def test_graph_replays_equal_direct_launches_and_shapes(monkeypatch):
+def test_graph_replays_equal_direct_launches(monkeypatch):
     smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
     ic, _, _ = H.sample_ics(H.co2_ar(), smp, 42, 3 * 8192)
     print("graphs: 3 jobs of 8192 CO2-Ar trajectories, L=512", flush=True)
     g = graph_engine(monkeypatch)
-    xs = [g.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
-    ref, st = run(g, xs, 0)
-    print(f"  tuned shape: block {st.block}, grid {st.grid}, {st.launches} slices", flush=True)
-    assert st.block in (64, 128, 256) and st.launches >= 1
-    again, _ = run(g, xs, 0)
-    for a, b in zip(ref, again):
-        assert np.array_equal(a, b)
-    for block in (64, 128, 256):
-        out, st = run(g, xs, block)
-        assert st.block == block
-        for a, b in zip(ref, out):
-            assert np.array_equal(a, b), block
     d = H.Engine(0)   # engines without HH_GRAPHS launch the slices directly
-    xd = [d.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
-    direct, _ = run(d, xd, 0)
-    for a, b in zip(ref, direct):
-        assert np.array_equal(a, b)
-    print("  graph replays, direct launches and every launch shape agree bit for bit", flush=True)
+    xs = [g.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
+    for prec in (H.PREC_FP64, H.PREC_MIXED):
+        ref, st = run(d, xs, prec)
+        for k in range(3):   # three runs on the graph engine's one pool
+            out, st = run(g, xs, prec)
+            print(f"  precision {prec} run {k}: {st.launches} slices", flush=True)
+            for a, b in zip(ref, out):
+                assert np.array_equal(a, b)
+    print("  graph replays and direct launches agree bit for bit", flush=True)
+    g.close()
+    d.close()

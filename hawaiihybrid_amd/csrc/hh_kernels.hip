@@ -364,12 +364,14 @@ __device__ __forceinline__ double cf_value(const double* mu0, const P (&m)[3])
 
 // Checks of one sample in the reference's order (src/hawaii.c:2181-2236): a NaN dipole, a
 // huge CF value, a jump by more than 1e10 from the previous sample (from sample 2 on).  A
-// PR/mu series (record 2) checks the dipole only (src/hawaii.c:3659-3671).
+// PR/mu series (record 2) has no check that ends it: the reference skips a sample whose dipole
+// is NaN and integrates on (src/hawaii.c:3664-3677), leaving that sample of its zeroed series at
+// 0 (store_sample).
 template <typename P>
 __device__ __forceinline__ int sample_status(const P (&m)[3], double curr, double prev, int k, int record)
 {
+    if (record == 2) return ITEM_OK;
     if (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2])) return ITEM_FAIL_DIPOLE_NAN;
-    if (record == 2) return ITEM_OK;   // the PR/mu series: the dipole check only
     if (fabs(curr) > 1e100) return ITEM_FAIL_CF_HUGE;
     if (k > 1 && fabs(curr / prev) > 1e10) return ITEM_FAIL_CF_JUMP;
     return ITEM_OK;
@@ -381,9 +383,10 @@ template <typename P>
 __device__ __forceinline__ void store_sample(double* row, int L, int record, int k, const P (&m)[3], double curr)
 {
     if (record) {
-        row[k] = (double)m[0];
-        row[L + k] = (double)m[1];
-        row[2 * (long long)L + k] = (double)m[2];
+        const bool skip = record == 2 && (isnan((double)m[0]) || isnan((double)m[1]) || isnan((double)m[2]));
+        row[k] = skip ? 0.0 : (double)m[0];
+        row[L + k] = skip ? 0.0 : (double)m[1];
+        row[2 * (long long)L + k] = skip ? 0.0 : (double)m[2];
     } else {
         row[k] = curr;
     }

@@ -210,7 +210,17 @@ static int half_cf(const orc_system *s, const orc_cf_params *p, const double *y0
                 }
                 dipole_of(s, ys, mut);
                 st->ndip++;
-                if (isnan(mut[0]) || isnan(mut[1]) || isnan(mut[2])) { status = ORC_FAIL_DIPOLE_NAN; goto done; }
+                if (isnan(mut[0]) || isnan(mut[1]) || isnan(mut[2])) {
+                    if (cf_checks) { status = ORC_FAIL_DIPOLE_NAN; goto done; }
+                    /* PR/mu: the reference skips the sample and integrates on (src/hawaii.c:
+                     * 3664-3677); it stays 0 in its zeroed series */
+                    if (vec) vec[3 * k] = vec[3 * k + 1] = vec[3 * k + 2] = 0.0;
+                    *len = k + 1;
+                    memcpy(ylast, ys, sizeof(double) * n);
+                    steps_since_sample = 0;
+                    ++k;
+                    continue;
+                }
                 prev = curr;
                 curr = mu0[0] * mut[0] + mu0[1] * mut[1] + mu0[2] * mut[2];
                 if (cf_checks && fabs(curr) > 1e100) { status = ORC_FAIL_CF_HUGE; goto done; }

@@ -85,15 +85,16 @@ def test_prmu_rejects_bad_input(eng):
             eng.prmu(sh, bad, icd)
 
 
-@pytest.mark.parametrize("L", [4, 8192, 65536])
+@pytest.mark.parametrize("L", [4, 8192, 65536, 1 << 20])
 def test_prmu_fft_lengths(eng, L):
-    """The four-step transform's factorisations at the edges (L = 4: 2 x 2; 65536: 256 x 256)
-    against the oracle's direct radix-2 FFT of the same series."""
+    """The four-step transform's factorisations at the edges (L = 4: 2 x 2; 65536: 256 x 256;
+    2^20: 1024 x 1024 with four column tiles) against the oracle's direct radix-2 FFT of the same
+    series, over the whole spectrum (nfreq = L/2 + 1); one trajectory at 2^20."""
     sh, so = H.he_ar(), O.hear()
     dt, R0 = 200.0, 12.0
     smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_HEAR, R0)
-    ic, _, _ = H.sample_ics(sh, smp, 13, 8)
-    nfreq = min(L // 2 + 1, 3000)
+    ic, _, _ = H.sample_ics(sh, smp, 13, 1 if L > 65536 else 8)
+    nfreq = L // 2 + 1
     p = H.PrmuParams(1e-12, 1e-12, dt, R0, L, nfreq, H.PREC_FP64, 0, 1)   # keep truncated ones (L = 4)
     sf, counted, nc, _ = eng.prmu(sh, p, eng.to_device_soa(ic))
     coeff = HARTREE * ALU ** 6 * ATU * R0 * R0 * dt * dt

@@ -24,7 +24,10 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + 
 # packed FP32 instructions stay off for the device code
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 SOURCES = [
-    ("hh_kernels.hip", DEVICE),
+    # the integrators' arithmetic is what their expressions say (a * b + c inside one expression
+    # becomes one FMA, nothing across statements), so kernels of different shape (one item per
+    # thread, the lane pool, stage vectors in LDS or in the buffer arena) compute the same bits
+    ("hh_kernels.hip", DEVICE + ["-ffp-contract=on"]),
     ("hh_aux.hip", DEVICE),
     ("hh_pipnn.hip", DEVICE),
     ("hh_zimm.hip", DEVICE),

@@ -9,12 +9,18 @@
 // l <= 12 (91 values).  It is too large to share the trajectory kernel's registers, so the
 // trajectory kernel records the lab coordinates of every sample (record mode 3) and two kernels
 // here finish a job:
-//   ch4co2_dipole_kernel  one sample per lane, one wavefront per 64 consecutive samples of one
-//                         trajectory: the frame of CH4, X and Y into LDS (a column per lane), the
-//                         radial functions of the lane's interval (vector loads; a wavefront's
-//                         samples share few intervals, so they hit the same lines), then the term
-//                         list, read by the whole wavefront at once (uniform indices: scalar
-//                         loads); mu_lab = S1^T mu overwrites the sample's first three coordinates.
+//   ch4co2_dipole_kernel  one sample per lane, 64 consecutive samples of one trajectory per
+//                         workgroup of kDipWaves wavefronts: the frame of CH4 in every wavefront,
+//                         X and Y into LDS (a column per sample; the wavefronts share the orders
+//                         m), then each wavefront runs its share of the basis functions (split
+//                         by term count) over the term list, read by the whole wavefront at once
+//                         (uniform indices: scalar loads), with the radial functions of the
+//                         lane's interval (vector loads; the lanes share few intervals, so they
+//                         hit the same lines); the shares' partial dipoles are summed in
+//                         wavefront order and mu_lab = S1^T mu overwrites the sample's first
+//                         three coordinates.  The 287 harmonics of 64 samples take 73 KB (FP32)
+//                         or 147 KB (FP64) of LDS, so one wavefront per workgroup would leave
+//                         two or three of a CU's four SIMDs idle.
 //   ch4co2_cf_rows_kernel one wavefront per trajectory: c_k = mu(0).mu(t_k), the sample checks of
 //                         correlation_eval (src/hawaii.c:2181-2236) in sample order, the CF rows
 //                         the reduction reads; a failed check marks the trajectory.
@@ -29,6 +35,8 @@
 
 namespace hh {
 
+constexpr int kDipWaves = 4;   // wavefronts per dipole workgroup
+
 template <typename P>
 struct Ch4Co2DipTables {
     P radial[HH_CH4CO2_NINT + 1][HH_CH4CO2_NB][4];   // row NINT: the tail (c4, c5, c6, 0)
@@ -37,6 +45,7 @@ struct Ch4Co2DipTables {
     int start[HH_CH4CO2_NB][4];
     double bounds[HH_CH4CO2_NINT];
     double centre[HH_CH4CO2_NINT];
+    int bsplit[kDipWaves + 1];   // wavefront w takes basis functions [bsplit[w], bsplit[w + 1])
 };
 
 template <typename P>
@@ -57,6 +66,15 @@ void fill_ch4co2_tables(Ch4Co2DipTables<P>& t)
         t.coef[i] = (P)HH_CH4CO2_COEF[i];
         t.xy[i] = HH_CH4CO2_XY[i];
     }
+    // contiguous basis-function ranges of about equal term counts
+    t.bsplit[0] = 0;
+    int b = 0;
+    for (int w = 1; w < kDipWaves; ++w) {
+        const long long goal = (long long)HH_CH4CO2_NTERMS * w / kDipWaves;
+        while (b < HH_CH4CO2_NB && HH_CH4CO2_START[b][3] <= goal) ++b;
+        t.bsplit[w] = b;
+    }
+    t.bsplit[kDipWaves] = HH_CH4CO2_NB;
 }
 
 size_t ch4co2_table_bytes(bool f32) { return f32 ? sizeof(Ch4Co2DipTables<float>) : sizeof(Ch4Co2DipTables<double>); }
@@ -73,7 +91,7 @@ void ch4co2_tables_host(bool f32, void* out)
 // out[index][64]: index l^2 (m = 0), l^2 + 2m - 1 (cos), l^2 + 2m (sin), or l(l - 1)/2 + ... for
 // the even-l table
 template <typename P, int LMAX, bool EVEN>
-__device__ __forceinline__ void harmonics(const P* v, P (*out)[64], int lane)
+__device__ __forceinline__ void harmonics(const P* v, P (*out)[64], int lane, int wave)
 {
     const P x = v[2];
     // w^m with w = -(v_x + i v_y): carries (-1)^m sin^m(theta) e^{i m phi}
@@ -88,6 +106,7 @@ __device__ __forceinline__ void harmonics(const P* v, P (*out)[64], int lane)
             wr = t;
             dfact *= P(2 * m - 1);
         }
+        if (m % kDipWaves != wave) continue;   // this order belongs to another wavefront
         // Q_l^m = d^m P_l / dx^m: Q_m^m = (2m - 1)!!, Q_{m+1}^m = (2m + 1) x Q_m^m,
         // (l - m) Q_l^m = (2l - 1) x Q_{l-1}^m - (l + m - 1) Q_{l-2}^m
         P q0 = P(0), q1 = dfact;
@@ -111,24 +130,27 @@ __device__ __forceinline__ void harmonics(const P* v, P (*out)[64], int lane)
     }
 }
 
-// One wavefront = 64 consecutive samples of one trajectory (blockIdx.y); coordinates rec[8] per
-// sample (Phi, Theta, R, phi1, theta1, psi1, phi2, theta2) are replaced by (mu_x, mu_y, mu_z).
+// One workgroup = 64 consecutive samples of one trajectory (blockIdx.y), kDipWaves wavefronts;
+// coordinates rec[8] per sample (Phi, Theta, R, phi1, theta1, psi1, phi2, theta2) are replaced by
+// (mu_x, mu_y, mu_z).
 template <typename P>
-__global__ void __launch_bounds__(64) ch4co2_dipole_kernel(JobDesc jb, const Ch4Co2DipTables<P>* __restrict__ tb)
+__global__ void __launch_bounds__(64 * kDipWaves) ch4co2_dipole_kernel(JobDesc jb, const Ch4Co2DipTables<P>* __restrict__ tb)
 {
     __shared__ P sX[HH_CH4CO2_NX][64];
     __shared__ P sY[HH_CH4CO2_NY][64];
+    __shared__ P sMu[kDipWaves][3][64];
     const long long item = blockIdx.y;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int len = jb.len[item];
     const int k0 = blockIdx.x * 64;
-    if (k0 >= len) return;                        // uniform: the whole wavefront
+    if (k0 >= len) return;                        // uniform: the whole workgroup
     const int k = k0 + lane < len ? k0 + lane : len - 1;   // lanes past the end repeat the last sample
     double* rec = jb.rows + (item * jb.L + k) * 8;
     double q[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) q[i] = rec[i];
-    // the frame of CH4 (angles reduced in FP64 by sincos_p)
+    // the frame of CH4 (angles reduced in FP64 by sincos_p), in every wavefront
     P sP, cP, sT, cT, sf, cf, s1, c1, sp, cp, sP2, cP2, s2, c2;
     sincos_p<P>(q[0], &sP, &cP);
     sincos_p<P>(q[1], &sT, &cT);
@@ -151,9 +173,10 @@ __global__ void __launch_bounds__(64) ch4co2_dipole_kernel(JobDesc jb, const Ch4
             o[j][2] = x2;
         }
     }
-    // every lane reads back only its own column: no synchronisation between lanes
-    harmonics<P, 13, false>(a, sX, lane);
-    harmonics<P, 12, true>(b, sY, lane);
+    // each wavefront fills the orders m of its share, for the 64 samples
+    harmonics<P, 13, false>(a, sX, lane, wave);
+    harmonics<P, 12, true>(b, sY, lane, wave);
+    __syncthreads();
     // the lane's radial interval (NINT: the tail beyond the last bound)
     const double R = q[2];
     int kint = 0;
@@ -163,8 +186,9 @@ __global__ void __launch_bounds__(64) ch4co2_dipole_kernel(JobDesc jb, const Ch4
     const P ir = (P)(1.0 / R), ir4 = ir * ir * ir * ir;
     const bool tail = kint == HH_CH4CO2_NINT;
     P mu[3] = {P(0), P(0), P(0)};
+    const int b0 = tb->bsplit[wave], b1 = tb->bsplit[wave + 1];
 #pragma unroll 1
-    for (int bf = 0; bf < HH_CH4CO2_NB; ++bf) {
+    for (int bf = b0; bf < b1; ++bf) {
         const P* rc = tb->radial[kint][bf];
         const P r = tail ? (rc[0] + (rc[1] + rc[2] * ir) * ir) * ir4 : ((rc[3] * t + rc[2]) * t + rc[1]) * t + rc[0];
 #pragma unroll
@@ -178,6 +202,18 @@ __global__ void __launch_bounds__(64) ch4co2_dipole_kernel(JobDesc jb, const Ch4
             }
             mu[c] += s * r;
         }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sMu[wave][c][lane] = mu[c];
+    __syncthreads();
+    if (wave != 0) return;
+    // the shares in wavefront order
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        P m = sMu[0][c][lane];
+#pragma unroll
+        for (int w = 1; w < kDipWaves; ++w) m += sMu[w][c][lane];
+        mu[c] = m;
     }
     // mu_lab = Sz(phi1)^T Sx(theta1)^T Sz(psi1)^T mu
     const P x0 = cp * mu[0] - sp * mu[1], x1 = sp * mu[0] + cp * mu[1], x2 = mu[2];
@@ -268,7 +304,7 @@ hipError_t run_ch4co2_eval_dipole(const double* y, long long n, double* records,
     jd.len = len;
     jd.n_items = 1;
     jd.L = (int)n;
-    hipLaunchKernelGGL(ch4co2_dipole_kernel<double>, dim3((unsigned)((n + 63) / 64), 1), dim3(64), 0, st, jd,
+    hipLaunchKernelGGL(ch4co2_dipole_kernel<double>, dim3((unsigned)((n + 63) / 64), 1), dim3(64 * kDipWaves), 0, st, jd,
                        static_cast<const Ch4Co2DipTables<double>*>(tables));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(ch4co2_dipole_out_kernel, dim3(g), dim3(256), 0, st, records, n, out);
@@ -279,10 +315,10 @@ hipError_t run_ch4co2_cf_rows(const JobDesc& jd, const void* tables, bool f32, h
 {
     const dim3 grid((unsigned)((jd.L + 63) / 64), (unsigned)jd.n_items);
     if (f32)
-        hipLaunchKernelGGL(ch4co2_dipole_kernel<float>, grid, dim3(64), 0, st, jd,
+        hipLaunchKernelGGL(ch4co2_dipole_kernel<float>, grid, dim3(64 * kDipWaves), 0, st, jd,
                            static_cast<const Ch4Co2DipTables<float>*>(tables));
     else
-        hipLaunchKernelGGL(ch4co2_dipole_kernel<double>, grid, dim3(64), 0, st, jd,
+        hipLaunchKernelGGL(ch4co2_dipole_kernel<double>, grid, dim3(64 * kDipWaves), 0, st, jd,
                            static_cast<const Ch4Co2DipTables<double>*>(tables));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -54,6 +54,109 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def traffic_from_counters(fetch_dir, write_dir, out_path, kernel="pool_cf_kernel<1, float, 256, 1>"):
+    """Bytes per launch of the mixed CO2-Ar pool kernel from two rocprofv3 counter runs of this
+    script with --calibrate-copy (one with --pmc FETCH_SIZE, one with --pmc WRITE_SIZE; CSV
+    output).  Each counter is scaled by what it reads for the 1 GiB device-to-device copy that
+    --calibrate-copy runs first (the largest __amd_rocclr_copyBuffer dispatch), then averaged
+    over the kernel's dispatches.  Writes the JSON that the roofline's `traffic` reads."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (r["Kernel_Name"], r["Dispatch_Id"])
+                vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"] or 0)
+        return vals
+
+    args = "--steps 20 --warmup 5 --no-serve --no-n2ar --no-ch4co2 --no-oracle --no-hbm --deep 0 --calibrate-copy"
+    res = {"kernel": kernel,
+           "workload": "CO2-Ar, 65536 trajectories per step, T=200 K, L=4096, batched mixed engine rtol=1e-10; "
+                       "every dispatch of the kernel in the run (5 warm-up and 20 timed steps)",
+           "commands": [f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d {fetch_dir} -o fetch -- python3 bench.py {args}",
+                        f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d {write_dir} -o write -- python3 bench.py {args}",
+                        f"python3 bench.py --traffic-from {fetch_dir} {write_dir}"],
+           "counters": {}}
+    for name, d in (("fetch", fetch_dir), ("write", write_dir)):
+        counter = "FETCH_SIZE" if name == "fetch" else "WRITE_SIZE"
+        v = per_dispatch(d, counter)
+        copy = max((x for (k, _), x in v.items() if "copyBuffer" in k), default=0.0)
+        scale = (1 << 30) / 1024.0 / copy if copy > 0 else None
+        ks = [x for (k, _), x in v.items() if kernel in k]
+        res["counters"][name] = {"counter": counter, "copy_kib": copy, "scale": scale, "dispatches": len(ks),
+                                 "kib_per_dispatch": float(np.mean(ks)) if ks else None}
+    f, w = res["counters"]["fetch"], res["counters"]["write"]
+    if f["kib_per_dispatch"] is not None and w["kib_per_dispatch"] is not None and f["scale"] and w["scale"]:
+        res["hbm_bytes_per_launch"] = 1024.0 * (f["kib_per_dispatch"] * f["scale"] + w["kib_per_dispatch"] * w["scale"])
+    json.dump(res, open(out_path, "w"), indent=1)
+    return res
+
+
+def hbm_section(H, eng, stm, mixed_len, plain_len, plain_reduce_ms, L, prmu_ntraj, prmu_L, sf_n):
+    """HBM rooflines of the data-movement kernels beside the integrator, bytes counted from
+    what each kernel loads and stores:
+      cf_reduce   the CF reduction (cf_reduce_partial + _final per job): 8 B per recorded
+                  sample (rows are read up to the item's length), 8 B of length and status per
+                  item, and the per-chunk partial sums and compensations (256 items per chunk,
+                  16 B per column and chunk) written and read back, 8 B per CF value written.
+                  `alone`: the first timed step's job reduced after the plain engine's
+                  integration (nothing else on the device); `in_batch`: the K jobs of the timed
+                  mixed batch, each reduced while the pools integrate the others.
+      prmu        the PR/mu spectrum pass (pack + Connes window, four-step FFT of two complex
+                  series per trajectory, power, accumulation) of He-Ar trajectories at FFT length
+                  prmu_L: pack reads 24 B per recorded sample and writes 32 B per point, each FFT
+                  step reads and writes 64 B per point, power reads 48 B and writes 8 B per kept
+                  frequency, accumulation reads 8 B per frequency and trajectory; time = the
+                  engine's events around the pass (integration excluded).
+      cf_to_sf    idct_cf_to_sf of an sf_n-point CF (Connes window, IDCT via an M = 2n-point
+                  complex FFT): the pack kernel reads 8 B and writes 16 B per point of M, each of
+                  the log2(M) global radix-2 passes (M > 8192) reads and writes 32 B per point of
+                  M, the unpack kernel reads 16 B and writes 24 B per CF point; `bytes_algorithmic`
+                  is the transform's own input and output (8 B read, 16 B written per CF point)."""
+    import torch
+    out = {}
+
+    def reduce_bytes(lens):
+        nchunks = -(-len(lens[0]) // 256)
+        return sum(8.0 * float(np.minimum(ln, L).sum()) + 8.0 * len(ln) + 2 * nchunks * L * 16.0 + 8.0 * L
+                   for ln in lens)
+
+    def rate(b, ms):
+        return {"bytes": b, "ms": ms, "GBps": b / (ms * 1e-3) / 1e9, "frac_hbm": b / (ms * 1e-3) / 1e9 / MI355X_HBM_GBS}
+
+    out["cf_reduce"] = {"alone": dict(rate(reduce_bytes([plain_len]), plain_reduce_ms), jobs=1),
+                        "in_batch": dict(rate(reduce_bytes(mixed_len), stm.reduce_ms), jobs=len(mixed_len))}
+    # PR/mu spectrum
+    smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 12.0)
+    ic, _, _ = H.sample_ics(H.he_ar(), smp, 7, prmu_ntraj)
+    nfreq = prmu_L // 2 + 1
+    pp = H.PrmuParams(1e-12, 1e-12, 200.0, 12.0, prmu_L, nfreq, H.PREC_FP64, 0, 1)
+    x = eng.to_device_soa(ic)
+    eng.prmu(H.he_ar(), pp, x)                        # warm-up (buffers)
+    _, counted, nc, st = eng.prmu(H.he_ar(), pp, x)
+    rec = float(np.minimum(eng.items(0, prmu_ntraj, prmu_L)["len"], prmu_L).sum())
+    nt = prmu_ntraj
+    pb = 24.0 * rec + nt * prmu_L * (32.0 + 2 * 64.0) + nt * nfreq * (48.0 + 8.0 + 8.0)
+    out["prmu_spectrum"] = dict(rate(pb, st.reduce_ms), trajectories=nt, fft_length=prmu_L, nfreq=nfreq)
+    del x
+    # CF -> SF
+    cf = torch.rand(sf_n, dtype=torch.float64, device="cuda:0")
+    eng.cf_to_sf(cf.clone(), 200.0)
+    _, _, ms = eng.cf_to_sf(cf, 200.0)
+    M = 2 * sf_n
+    passes = int(math.log2(M)) if M > 8192 else 1
+    mb = 24.0 * M + passes * 32.0 * M + 40.0 * sf_n
+    out["cf_to_sf"] = dict(rate(mb, ms), points=sf_n, fft_passes=passes, bytes_algorithmic=24.0 * sf_n,
+                           GBps_algorithmic=24.0 * sf_n / (ms * 1e-3) / 1e9)
+    del cf
+    torch.cuda.synchronize()
+    return out
+
+
 def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
     """Poisson arrivals of CF requests (He-Ar or CO2-Ar, T ~ U[50, 500] K, ntraj ~ 2^U[10, 16])
     and M0 requests (10^U[6, 8] accepted points of the rejection sampler), 4:1; inputs staged in
@@ -248,12 +351,21 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-n2ar", action="store_true")
     ap.add_argument("--no-ch4co2", action="store_true")
+    ap.add_argument("--no-hbm", action="store_true")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no run: turn two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of this script "
+                         "into profiles/pool_cf_kernel_co2ar_mixed_traffic.json")
     ap.add_argument("--ch4co2-ntraj", type=int, default=262144)
     ap.add_argument("--ch4co2-L", type=int, default=1024)
     ap.add_argument("--n2ar-points", type=int, default=1 << 20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.traffic_from:
+        r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1],
+                                  os.path.join(HERE, "profiles", "pool_cf_kernel_co2ar_mixed_traffic.json"))
+        print(json.dumps(r))
+        return
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this engine is measured on one MI355X; running the "
               "ensemble over several GPUs is a design note (trajectories shard by index), not a "
@@ -329,8 +441,9 @@ def main():
     t_mixed = time.perf_counter() - t0
     mixed_cf0 = outs[0].cpu().numpy()
     mixed_items0 = eng.items(0, n, L, rows=True)
+    mixed_len = [mixed_items0["len"]] + [eng.items(j, n, L)["len"] for j in range(1, K)]
     # per-trajectory rows of the plain engine for the first timed step (outside the timings)
-    out, _ = eng.cf_run(sysd, p_plain, [timed[0]])
+    out, st_plain0 = eng.cf_run(sysd, p_plain, [timed[0]])
     plain_cf0 = out[0].cpu().numpy()
     plain_items0 = eng.items(0, n, L, rows=True)
     log(f"batched mixed: {t_mixed * 1e3:.1f} ms for {K} steps -> {K * n / t_mixed:.0f} traj/s; kernel "
@@ -372,7 +485,8 @@ def main():
     tpath = os.path.join(HERE, "profiles", "pool_cf_kernel_co2ar_mixed_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
-        traffic_src = "profiles/pool_cf_kernel_co2ar_mixed_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of this workload)"
+        traffic_src = ("profiles/pool_cf_kernel_co2ar_mixed_traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                       "of this workload, scaled by a 1 GiB copy; bench.py --traffic-from)")
     roofline = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": "pool_cf_kernel<CO2-Ar, float, 256 lanes, 1 wavefront/SIMD> with the FP64 orbit "
@@ -406,6 +520,11 @@ def main():
         "plain_fp64_kernel_ms_per_step": float(np.mean(plain_k_ms)),
         "roofline": roofline,
     }
+
+    if not args.no_hbm:
+        result["hbm"] = hbm_section(H, eng, stm, mixed_len, plain_items0["len"], st_plain0.reduce_ms, L, 1024,
+                                    1 << 16, 1 << 22)
+        log("hbm", result["hbm"])
 
     if not args.no_serve:
         # a fixed offered load below the mix's capacity, so latencies compare across engine

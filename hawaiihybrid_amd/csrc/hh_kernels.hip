@@ -1161,32 +1161,80 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double v)
 }
 
 // partial[c][k] = sum over trajectories of chunk c of ppf * crln[k]; column-parallel, rows
-// read coalesced along k; entries past an item's length are zero and never read.
+// read coalesced along k; entries past an item's length are zero and never read.  The items are
+// taken kReduceUnroll at a time: their lengths and statuses (wavefront-uniform) and row values
+// are all loaded first and then added in item order, so a wavefront keeps kReduceUnroll row
+// loads in flight.  A row is read through a buffer resource whose size is the item's length
+// (zero for an item that does not count or lies past the chunk): the load of a column past it
+// returns 0 without a branch.  Such a column adds +0.0, which leaves the Neumaier sum and its
+// compensation as they are, so the sum is that of the item-serial loop.
+constexpr int kReduceUnroll = 16;
+
+__device__ __forceinline__ double row_value(const double* row, int n, int k)
+{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(row), 0, n * 8, kBufferConfig);
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, k * 8, 0, 0));
+}
+
+template <int HALVES>
+__device__ __forceinline__ double reduce_value(const JobDesc& jb, long long t, long long t1, int k)
+{
+    const bool in = t < t1;
+    const long long tc = in ? t : t1 - 1;   // a valid item for the (discarded) status and length reads
+    if constexpr (HALVES == 2) {
+        const long long a = 2 * tc, b = 2 * tc + 1;
+        const bool use = in && !dropped(jb.status[a], jb.status[b]);
+        const double cf = row_value(jb.rows + a * jb.L, use ? jb.len[a] : 0, k);
+        const double cb = row_value(jb.rows + b * jb.L, use ? jb.len[b] : 0, k);
+        return 0.5 * (cf + cb) * kALU * kALU * kALU;
+    } else {
+        const bool use = in && !dropped(jb.status[tc], jb.status[tc]);
+        const double cf = row_value(jb.rows + tc * jb.L, use ? jb.len[tc] : 0, k);
+        return cf * kALU * kALU * kALU;
+    }
+}
+
+template <int HALVES>
+__device__ __forceinline__ void reduce_chunk(const JobDesc& jb, long long t0, long long t1, int k, double ppf,
+                                             double& s, double& cc)
+{
+    for (long long t = t0; t < t1; t += kReduceUnroll) {
+        double v[kReduceUnroll];
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) v[u] = reduce_value<HALVES>(jb, t + u, t1, k);
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) neumaier_add(s, cc, ppf * v[u]);
+    }
+}
+
 __global__ void __launch_bounds__(256) cf_reduce_partial_kernel(JobDesc jb, long long chunk, double ppf,
                                                                 double* __restrict__ psum,
                                                                 double* __restrict__ pcomp)
 {
+    __shared__ int s_maxlen;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
-    if (k >= jb.L) return;
     const long long t0 = (long long)c * chunk;
     long long t1 = t0 + chunk;
     if (t1 > jb.ntraj) t1 = jb.ntraj;
+    // the longest row of the chunk: a workgroup whose columns all lie past it only writes zeros
+    // (what adding +0.0 for every item gives)
+    if (threadIdx.x == 0) s_maxlen = 0;
+    __syncthreads();
+    {
+        int m = 0;
+        for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+            m = max(m, jb.halves == 2 ? max(jb.len[2 * t], jb.len[2 * t + 1]) : jb.len[t]);
+        atomicMax(&s_maxlen, m);
+    }
+    __syncthreads();
+    if (k >= jb.L) return;
     double s = 0.0, cc = 0.0;
-    for (long long t = t0; t < t1; ++t) {
-        double v;
-        if (jb.halves == 2) {
-            const long long a = 2 * t, b = 2 * t + 1;
-            if (dropped(jb.status[a], jb.status[b])) continue;
-            const double cf = k < jb.len[a] ? jb.rows[a * jb.L + k] : 0.0;
-            const double cb = k < jb.len[b] ? jb.rows[b * jb.L + k] : 0.0;
-            v = 0.5 * (cf + cb) * kALU * kALU * kALU;
-        } else {
-            if (dropped(jb.status[t], jb.status[t])) continue;
-            const double cf = k < jb.len[t] ? jb.rows[t * jb.L + k] : 0.0;
-            v = cf * kALU * kALU * kALU;
-        }
-        neumaier_add(s, cc, ppf * v);
+    if ((int)(blockIdx.x * blockDim.x) < s_maxlen) {
+        if (jb.halves == 2)
+            reduce_chunk<2>(jb, t0, t1, k, ppf, s, cc);
+        else
+            reduce_chunk<1>(jb, t0, t1, k, ppf, s, cc);
     }
     psum[(long long)c * jb.L + k] = s;
     pcomp[(long long)c * jb.L + k] = cc;
@@ -1200,9 +1248,22 @@ __global__ void __launch_bounds__(256) cf_reduce_final_kernel(int L, int nchunks
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= L) return;
     double s = 0.0, cc = 0.0;
-    for (int c = 0; c < nchunks; ++c) {
-        neumaier_add(s, cc, psum[(long long)c * L + k]);
-        cc += pcomp[(long long)c * L + k];
+    // kReduceUnroll chunks' partials loaded at once, added in chunk order (a chunk index past the
+    // end reads the last chunk again and adds +0.0)
+    for (int c = 0; c < nchunks; c += kReduceUnroll) {
+        double ps[kReduceUnroll], pc[kReduceUnroll];
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) {
+            const bool in = c + u < nchunks;
+            const long long o = (long long)(in ? c + u : nchunks - 1) * L + k;
+            ps[u] = in ? psum[o] : 0.0;
+            pc[u] = in ? pcomp[o] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kReduceUnroll; ++u) {
+            neumaier_add(s, cc, ps[u]);
+            cc += pc[u];
+        }
     }
     cf_out[k] = (s + cc) / (double)ntraj;
 }

@@ -204,6 +204,8 @@ struct hh_engine {
     hh::DevBuf zimm_tw;           // twiddles e^{2 pi i j/P} of the Zimmermann transforms
     void* noalloc_scratch = nullptr;    // device address of hh_cf_to_sf_noalloc's static work array
     hh::DevBuf prmu_a, prmu_b, prmu_w;   // PR/mu: FFT ping-pong series, weights and spectra
+    hh::DevBuf prmu_tw;                  // PR/mu: twiddles e^{2 pi i j/L}, j < L, of FFT length prmu_tw_L
+    long long prmu_tw_L = 0;
     int zimm_logP = 0;
     std::vector<hh::JobDesc> last_jobs;
     std::vector<std::unique_ptr<hh::PoolJob>> last_pool_jobs;   // kept for hh_cf_items

@@ -18,10 +18,11 @@
 // four-step (Bailey) factorisation L = L1 L2, both factors resident in LDS: with n = n1 L2 + n2
 // and k = k1 + L1 k2,
 //     y[k1][n2] = w_L^(n2 k1) FFT_L1(x[. L2 + n2])[k1],   X[k1 + L1 k2] = FFT_L2(y[k1][.])[k2],
-// one launch for the columns (tiles of adjacent columns, so the strided reads stay in 64 B
-// segments) and one for the rows, two passes over HBM instead of log2(L).  The spectrum is read
-// through the index map k -> (k mod L1) L2 + k / L1.  The per-trajectory spectra are weighted
-// and summed in trajectory order.
+// one launch for the columns (tiles of adjacent columns, so the strided reads stay in 256 B
+// segments) and one for the rows, which writes the spectrum back in natural order (runs of
+// adjacent k1 from a padded LDS tile), two passes over HBM instead of log2(L).  Twiddles come
+// from a table e^{2 pi i j/L} built once per L on the host in long double, not from a sin/cos
+// per butterfly.  The per-trajectory spectra are weighted and summed in trajectory order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -58,23 +59,32 @@ __global__ void __launch_bounds__(256) prmu_pack_kernel(const double* __restrict
     z[(2 * tl + 1) * L + i] = make_double2(w, 0.0);
 }
 
-constexpr int kFftTile = 4096;   // complex values per workgroup tile: 64 KiB of LDS
+constexpr int kFftTile = 4096;   // complex values per workgroup tile: 64 KiB of LDS (plus row padding)
 
-// In place in LDS: `cnt` transforms of length n (a power of two), X_k = sum_j x_j e^{+2 pi i jk/n}:
-// bit reversal, then radix-2 decimation-in-time butterflies.  INTERLEAVED: element e of
-// sequence c at a[e * cnt + c], consecutive threads on consecutive sequences (the column tiles);
-// else at a[c * n + e], consecutive threads on consecutive butterflies of one sequence (rows).
-// Either way consecutive threads touch consecutive LDS words.
-template <bool INTERLEAVED>
-__device__ void lds_fft_many(double2* a, int n, int cnt)
+// In place in LDS: `cnt` transforms of length n (a power of two dividing L), X_k = sum_j x_j
+// e^{+2 pi i jk/n}: bit reversal, then radix-2 decimation-in-time butterflies, the twiddle
+// e^{i pi k/half} read from tw[j] = e^{2 pi i j/L} at j = k L/(2 half) (the table is L2-resident).
+// INTERLEAVED: element e of sequence c at a[e * cnt + c], consecutive threads on consecutive
+// sequences (the column tiles); else at a[c * stride + e], consecutive threads on consecutive
+// butterflies of one sequence (rows; stride = n + 1 keeps a column of the tile, read when it is
+// written out, on distinct banks).  Either way consecutive threads touch consecutive LDS words.
+__device__ __forceinline__ int ilog2(long long v)
 {
-    int logn = 0;
-    while ((1 << logn) < n) ++logn;
-    auto at = [&](int c, int e) -> double2& { return INTERLEAVED ? a[e * cnt + c] : a[c * n + e]; };
+    return 63 - __builtin_clzll((unsigned long long)v);
+}
+
+template <bool INTERLEAVED>
+__device__ void lds_fft_many(double2* a, int n, int cnt, int stride, const double2* __restrict__ tw, long long L,
+                             double2* twn)
+{
+    // every size is a power of two: indices by shifts and masks
+    const int logn = ilog2(n), logc = ilog2(cnt), logL = ilog2(L);
+    // the n-point transform's twiddles e^{2 pi i m/n}, m < n/2, staged in LDS
+    for (int m = threadIdx.x; m < (n >> 1); m += blockDim.x) twn[m] = tw[(long long)m << (logL - logn)];
+    auto at = [&](int c, int e) -> double2& { return INTERLEAVED ? a[(e << logc) + c] : a[c * stride + e]; };
     for (int i = threadIdx.x; i < n * cnt; i += blockDim.x) {
-        const int c = INTERLEAVED ? i % cnt : i >> logn, e = INTERLEAVED ? i / cnt : i & (n - 1);
-        int r = 0;
-        for (int b = 0, x = e; b < logn; ++b, x >>= 1) r = (r << 1) | (x & 1);
+        const int c = INTERLEAVED ? i & (cnt - 1) : i >> logn, e = INTERLEAVED ? i >> logc : i & (n - 1);
+        const int r = (int)(__builtin_bitreverse32((unsigned)e) >> (32 - logn));
         if (r > e) {
             const double2 t = at(c, e);
             at(c, e) = at(c, r);
@@ -82,15 +92,18 @@ __device__ void lds_fft_many(double2* a, int n, int cnt)
         }
     }
     __syncthreads();
-    const int hn = n >> 1;
-    for (int half = 1; half < n; half <<= 1) {
-        for (int j = threadIdx.x; j < hn * cnt; j += blockDim.x) {
-            const int c = INTERLEAVED ? j % cnt : j / hn, jj = INTERLEAVED ? j / cnt : j % hn, k = jj % half;
-            const int e0 = (jj / half) * 2 * half + k;
+    const int loghn = logn - 1;
+    for (int logh = 0; logh < logn; ++logh) {
+        const int half = 1 << logh;
+        const int shift = logn - 1 - logh;   // twiddle e^{i pi k/half} = twn[k << shift]
+        for (int j = threadIdx.x; j < (n >> 1) * cnt; j += blockDim.x) {
+            const int c = INTERLEAVED ? j & (cnt - 1) : j >> loghn, jj = INTERLEAVED ? j >> logc : j & ((n >> 1) - 1);
+            const int k = jj & (half - 1);
+            const int e0 = ((jj >> logh) << (logh + 1)) + k;
             double2& p0 = at(c, e0);
             double2& p1 = at(c, e0 + half);
-            double sn, cs;
-            sincospi((double)k / (double)half, &sn, &cs);
+            const double2 t = twn[k << shift];
+            const double cs = t.x, sn = t.y;
             const double2 u = p0, v = p1;
             const double2 w = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
             p0 = make_double2(u.x + w.x, u.y + w.y);
@@ -102,45 +115,55 @@ __device__ void lds_fft_many(double2* a, int n, int cnt)
 
 // step 1: C adjacent columns n2 of one series per workgroup; out[k1 L2 + n2] = twiddled FFT_L1
 __global__ void __launch_bounds__(256) prmu_fft_cols_kernel(const double2* __restrict__ in, double2* __restrict__ out,
-                                                            int L1, int L2, int C, long long L)
+                                                            int L1, int L2, int C, long long L,
+                                                            const double2* __restrict__ tw)
 {
-    extern __shared__ double2 tile[];   // [L1][C]
-    const int tiles = L2 / C;
+    extern __shared__ double2 tile[];   // [L1][C], then L1/2 twiddles
+    const int logc = ilog2(C);
+    const int tiles = L2 >> logc;
     const long long s = blockIdx.x / tiles;
-    const int n2_0 = (int)(blockIdx.x % tiles) * C;
+    const int n2_0 = (int)(blockIdx.x % tiles) << logc;
     const double2* x = in + s * L;
     for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
-        const int n1 = i / C, c = i % C;
-        tile[n1 * C + c] = x[(long long)n1 * L2 + n2_0 + c];
+        const int n1 = i >> logc, c = i & (C - 1);
+        tile[i] = x[(long long)n1 * L2 + n2_0 + c];
     }
     __syncthreads();
-    lds_fft_many<true>(tile, L1, C);
+    lds_fft_many<true>(tile, L1, C, 0, tw, L, tile + C * L1);
     double2* y = out + s * L;
     for (int i = threadIdx.x; i < C * L1; i += blockDim.x) {
-        const int k1 = i / C, c = i % C, n2 = n2_0 + c;
-        const double2 v = tile[k1 * C + c];
-        double sn, cs;
-        sincospi(2.0 * (double)((long long)n2 * k1) / (double)L, &sn, &cs);
-        y[(long long)k1 * L2 + n2] = make_double2(v.x * cs - v.y * sn, v.x * sn + v.y * cs);
+        const int k1 = i >> logc, n2 = n2_0 + (i & (C - 1));
+        const double2 v = tile[i];
+        const double2 t = tw[((long long)n2 * k1) & (L - 1)];   // e^{2 pi i n2 k1 / L}
+        y[(long long)k1 * L2 + n2] = make_double2(v.x * t.x - v.y * t.y, v.x * t.y + v.y * t.x);
     }
 }
 
-// step 2: R adjacent rows k1 (contiguous) of one series per workgroup, FFT_L2 in place
+// step 2: R adjacent rows k1 of one series per workgroup, FFT_L2 of each; the result is written
+// in natural order, X[k1 + L1 k2] at out[k1 + L1 k2] (runs of R consecutive k1)
 __global__ void __launch_bounds__(256) prmu_fft_rows_kernel(const double2* __restrict__ in, double2* __restrict__ out,
-                                                            int L1, int L2, int R, long long L)
+                                                            int L1, int L2, int R, long long L,
+                                                            const double2* __restrict__ tw)
 {
-    extern __shared__ double2 tile[];   // [R][L2]
-    const int tiles = L1 / R;
+    extern __shared__ double2 tile[];   // [R][L2 + 1], then L2/2 twiddles
+    const int logr = ilog2(R), log2n = ilog2(L2);
+    const int tiles = L1 >> logr;
+    const int stride = L2 + 1;
     const long long s = blockIdx.x / tiles;
-    const long long base = s * L + (long long)(blockIdx.x % tiles) * R * L2;
-    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) tile[i] = in[base + i];
+    const int k1_0 = (int)(blockIdx.x % tiles) << logr;
+    const double2* x = in + s * L + (long long)k1_0 * L2;
+    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) tile[(i >> log2n) * stride + (i & (L2 - 1))] = x[i];
     __syncthreads();
-    lds_fft_many<false>(tile, L2, R);
-    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) out[base + i] = tile[i];
+    lds_fft_many<false>(tile, L2, R, stride, tw, L, tile + R * stride);
+    double2* y = out + s * L;
+    for (int i = threadIdx.x; i < R * L2; i += blockDim.x) {
+        const int r = i & (R - 1), k2 = i >> logr;
+        y[k1_0 + r + (long long)L1 * k2] = tile[r * stride + k2];
+    }
 }
 
 // pw[t][k] = w[t] * (|X_k|^2 + |Y_k|^2 + |Z_k|^2), k < nfreq (gsl_fft_square, src/hawaii.c:2349-2356)
-__global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restrict__ Z, long long L, int L1, int L2,
+__global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restrict__ Z, long long L,
                                                          int nfreq, long long t0, long long nt,
                                                          const double* __restrict__ w, double* __restrict__ pw)
 {
@@ -149,9 +172,7 @@ __global__ void __launch_bounds__(256) prmu_power_kernel(const double2* __restri
     const long long tl = g / nfreq, k = g % nfreq, km = (L - k) & (L - 1);
     const double2* z1 = Z + (2 * tl) * L;
     const double2* z2 = Z + (2 * tl + 1) * L;
-    // X[k1 + L1 k2] sits at k1 L2 + k2
-    const long long ik = (k % L1) * L2 + k / L1, ikm = (km % L1) * L2 + km / L1;
-    const double2 a = z1[ik], b = z1[ikm], c = z2[ik];
+    const double2 a = z1[k], b = z1[km], c = z2[k];
     const double p = 0.5 * ((a.x * a.x + a.y * a.y) + (b.x * b.x + b.y * b.y)) + (c.x * c.x + c.y * c.y);
     pw[tl * nfreq + k] = w[t0 + tl] * p;
 }
@@ -231,6 +252,26 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
     const int C = std::min(kFftTile / L1, L2), R = std::min(kFftTile / L2, L1);
     // chunks of trajectories whose two series per trajectory fit 1 GiB per buffer
     const long long chunk = std::max<long long>(1, std::min<long long>(ntraj, (1LL << 30) / (2 * L * 16)));
+    if (e->prmu_tw_L != L) {
+        // e^{2 pi i j/L}, rounded from long double
+        std::vector<double2> tw((size_t)L);
+        for (long long j = 0; j < L; ++j) {
+            const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)j / (long double)L;
+            tw[(size_t)j] = make_double2((double)cosl(a), (double)sinl(a));
+        }
+        e->prmu_tw_L = 0;
+        HH_CHECK(e->prmu_tw.ensure((size_t)L * sizeof(double2)));
+        HH_CHECK(hipMemcpyAsync(e->prmu_tw.p, tw.data(), (size_t)L * sizeof(double2), hipMemcpyHostToDevice, e->stream));
+        HH_CHECK(hipStreamSynchronize(e->stream));
+        e->prmu_tw_L = L;
+    }
+    const double2* twd = e->prmu_tw.as<double2>();
+    const size_t cols_lds = ((size_t)C * L1 + L1 / 2) * sizeof(double2);
+    const size_t rows_lds = ((size_t)R * (L2 + 1) + L2 / 2) * sizeof(double2);
+    HH_CHECK(hipFuncSetAttribute((const void*)prmu_fft_cols_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)cols_lds));
+    HH_CHECK(hipFuncSetAttribute((const void*)prmu_fft_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)rows_lds));
     HH_CHECK(e->prmu_a.ensure((size_t)chunk * 2 * L * sizeof(double2)));
     HH_CHECK(e->prmu_b.ensure((size_t)chunk * 2 * L * sizeof(double2)));
     HH_CHECK(e->prmu_w.ensure((size_t)ntraj * (sizeof(double) + sizeof(int)) + sizeof(int) +
@@ -249,14 +290,14 @@ extern "C" int hh_prmu(hh_engine* e, const hh_system* sys, const hh_prmu_params*
                            jd.len, keepd, t0, n, cp.L, L, p->dt, a);
         HH_CHECK(hipGetLastError());
         // four-step transform a -> b (columns) -> a (rows)
-        hipLaunchKernelGGL(prmu_fft_cols_kernel, dim3((unsigned)(2 * n * (L2 / C))), dim3(256),
-                           (size_t)C * L1 * sizeof(double2), e->stream, a, b, L1, L2, C, L);
+        hipLaunchKernelGGL(prmu_fft_cols_kernel, dim3((unsigned)(2 * n * (L2 / C))), dim3(256), cols_lds,
+                           e->stream, a, b, L1, L2, C, L, twd);
         HH_CHECK(hipGetLastError());
-        hipLaunchKernelGGL(prmu_fft_rows_kernel, dim3((unsigned)(2 * n * (L1 / R))), dim3(256),
-                           (size_t)R * L2 * sizeof(double2), e->stream, b, a, L1, L2, R, L);
+        hipLaunchKernelGGL(prmu_fft_rows_kernel, dim3((unsigned)(2 * n * (L1 / R))), dim3(256), rows_lds, e->stream,
+                           b, a, L1, L2, R, L, twd);
         HH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(prmu_power_kernel, dim3((unsigned)((n * p->nfreq + 255) / 256)), dim3(256), 0, e->stream, a,
-                           L, L1, L2, p->nfreq, t0, n, wd, pw);
+                           L, p->nfreq, t0, n, wd, pw);
         HH_CHECK(hipGetLastError());
         hipLaunchKernelGGL(prmu_accumulate_kernel, dim3((unsigned)((p->nfreq + 255) / 256)), dim3(256), 0, e->stream, pw,
                            n, p->nfreq, sf_dev);

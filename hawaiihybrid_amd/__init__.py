@@ -184,6 +184,8 @@ def lib():
             "hh_pf_analytic": (d, [P(System), d]),
             "hh_vegas": (i, [v, P(System), P(VegasParams), P(VegasResult)]),
             "hh_cf_moment": (i, [v, C.c_long, d, d, i, P(d)]),
+            "hh_fit_baseline": (i, [v, v, C.c_long, C.c_long, v]),
+            "hh_loess": (i, [v, v, C.c_long, d, C.c_long, C.c_long, d, C.c_long, v, v]),
             "hh_pipnn_create": (i, [v, v, i, P(v)]),
             "hh_pipnn_eval": (i, [v, v, C.c_long, v, v, P(d)]),
             "hh_pipnn_destroy": (None, [v]),
@@ -252,6 +254,27 @@ def cf_moment(cf, dt, n, ntraj=0.0):
     out = C.c_double()
     _check(lib().hh_cf_moment(cf.ctypes.data_as(C.c_void_p), len(cf), float(dt), float(ntraj), int(n), C.byref(out)))
     return out.value
+
+
+def fit_baseline(t, cf, begin=16384):
+    """FIT_BASELINE's wing-model fit (hh_fit_baseline): (A, B, C) of C + A / (1 + B^2 t^2)."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    cf = np.ascontiguousarray(cf, dtype=np.float64)
+    abc = np.zeros(3)
+    _check(lib().hh_fit_baseline(t.ctypes.data_as(C.c_void_p), cf.ctypes.data_as(C.c_void_p), len(t), int(begin),
+                                 abc.ctypes.data_as(C.c_void_p)))
+    return tuple(abc)
+
+
+def loess(x, y, grid_max, npoints, ws_min=21, ws_step=0.1, ws_delay=10):
+    """SMOOTH's LOESS smoothing (hh_loess) on npoints points from 0 to grid_max: (grid, values)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    g, out = np.zeros(npoints), np.zeros(npoints)
+    _check(lib().hh_loess(x.ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p), len(x), float(grid_max),
+                          int(npoints), int(ws_min), float(ws_step), int(ws_delay), g.ctypes.data_as(C.c_void_p),
+                          out.ctypes.data_as(C.c_void_p)))
+    return g, out
 
 
 def cf_params(L, dt=200.0, rtol=1e-12, atol=None, Rcut=40.0, ppf=1.0, halves=HALVES_FORWARD, engine=ENGINE_BATCHED,

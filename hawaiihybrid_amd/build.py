@@ -41,6 +41,9 @@ SOURCES = [
     ("hh_conf.cpp", ["-x", "hip"] + DEVICE),
     # bit-exact initial conditions: no floating-point contraction in the sampler
     ("hh_sampler.cpp", ["-ffp-contract=off"]),
+    # PROCESSING numerics (baseline fit, LOESS): the window choice of the smoothing compares
+    # distances computed as the reference computes them, so no contraction there either
+    ("hh_process.cpp", ["-ffp-contract=off"]),
 ]
 
 

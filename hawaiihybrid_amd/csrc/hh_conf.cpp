@@ -208,7 +208,7 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                 {"WRITE_SF", HH_PROC_WRITE_SF, 1}, {"READ_SPECTRUM", HH_PROC_READ_SPECTRUM, 1},
                 {"WRITE_SPECTRUM", HH_PROC_WRITE_SPECTRUM, 1}, {"ALPHA", HH_PROC_ALPHA, 0}, {"D1", HH_PROC_D1, 0},
                 {"D2", HH_PROC_D2, 0}, {"D3", HH_PROC_D3, 0}, {"D4", HH_PROC_D4, 0}, {"D4A", HH_PROC_D4A, 0},
-                {"AVERAGE_CFS", HH_PROC_AVERAGE_CFS, 0}};
+                {"AVERAGE_CFS", HH_PROC_AVERAGE_CFS, 0}, {"INT3", HH_PROC_INT3, 0}};
             if (key == "COMPUTE_MN_CLASSICAL_DETAILED_BALANCE" || key == "COMPUTE_MN_QUANTUM_DETAILED_BALANCE") {
                 // one integer argument, positional or named n (src/driver.c:2304-2330)
                 const bool named = args.size() == 3 && Lexer::upper(args[0].text) == "N" && args[1].text == "=";
@@ -218,6 +218,58 @@ int parse_text(const std::string& text, const char* path, hh_conf* c)
                     return err(line, "%s expects an integer order n", key);
                 snprintf(pr.arg, sizeof pr.arg, "%ld", (long)v.num);
                 pr.op = key == "COMPUTE_MN_QUANTUM_DETAILED_BALANCE" ? HH_PROC_MN_QUANTUM : HH_PROC_MN_CLASSICAL;
+                ++c->nproc;
+                t = lx.next();
+                continue;
+            }
+            if (key == "FIT_BASELINE") {
+                // at most one integer argument, positional or named cf_extrapolation_begin_index
+                // (src/driver.c:2096-2118); kept as its value, -1 for the default
+                long begin = -1;
+                if (!args.empty()) {
+                    const bool named = args.size() == 3 && args[1].text == "=";
+                    if (!(named || args.size() == 1))
+                        return err(line, "function call %s expects at most 1 arguments", key);
+                    if (named && Lexer::upper(args[0].text) != "CF_EXTRAPOLATION_BEGIN_INDEX")
+                        return err(line, "function call FIT_BASELINE expects a named argument "
+                                         "cf_extrapolation_begin_index but got %s", args[0].text);
+                    const Tok& v = args[named ? 2 : 0];
+                    if (v.kind != Tok::NUMBER || v.num != (double)(long)v.num || v.num < 0)
+                        return err(line, "%s expects a non-negative integer index", key);
+                    begin = (long)v.num;
+                }
+                snprintf(pr.arg, sizeof pr.arg, "%ld", begin);
+                pr.op = HH_PROC_FIT_BASELINE;
+                ++c->nproc;
+                t = lx.next();
+                continue;
+            }
+            if (key == "SMOOTH") {
+                // named arguments only (src/driver.c:2879-2909), kept as
+                // "grid_npoints;grid_max;window_size_min;window_size_step;window_size_delay"
+                long npoints = 0, ws_min = 21, ws_delay = 10;
+                double grid_max = 0.0, ws_step = 0.1;
+                for (size_t k = 0; k < args.size(); k += 3) {
+                    if (k + 2 >= args.size() || args[k + 1].text != "=" || args[k + 2].kind != Tok::NUMBER)
+                        return err(line, "function call %s accepts only named arguments", key);
+                    const std::string name = Lexer::upper(args[k].text);
+                    const double v = args[k + 2].num;
+                    const bool integer = v == (double)(long)v;
+                    if (name == "GRID_NPOINTS" || name == "WINDOW_SIZE_MIN" || name == "WINDOW_SIZE_DELAY") {
+                        if (!integer || v < 0) return err(line, "SMOOTH: %s expects a non-negative integer", args[k].text);
+                        (name == "GRID_NPOINTS" ? npoints : name == "WINDOW_SIZE_MIN" ? ws_min : ws_delay) = (long)v;
+                    } else if (name == "GRID_MAX") {
+                        grid_max = v;
+                    } else if (name == "WINDOW_SIZE_STEP") {
+                        ws_step = v;
+                    } else {
+                        // reported and skipped, as the reference does
+                        std::fprintf(stderr, "ERROR: line %d: function call %s got unexpected named argument %s\n", line,
+                                     key.c_str(), args[k].text.c_str());
+                    }
+                }
+                snprintf(pr.arg, sizeof pr.arg, "%ld;%.17g;%ld;%.17g;%ld", npoints, grid_max, ws_min, ws_step, ws_delay);
+                pr.op = HH_PROC_SMOOTH;
                 ++c->nproc;
                 t = lx.next();
                 continue;
@@ -537,6 +589,8 @@ constexpr double kBoltzmann = 1.380649e-23;      // src/constants.h:9
 constexpr double kLightSpeedCm = 2.99792458e10;  // src/constants.h:14
 constexpr double kHBarSI = 1.054571817e-34;      // src/constants.h:21
 constexpr double kATUs = 2.4188843265857e-17;    // src/constants.h:12
+constexpr double kADipMomU = 8.4783536255e-30;   // src/constants.h
+constexpr double kEpsilon0 = 8.8541878128e-12;   // src/constants.h
 
 double h_nu_over_kT(double nu, double T) { return kPlanck * kLightSpeedCm * nu / (kBoltzmann * T); }
 
@@ -646,6 +700,10 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
 {
     std::vector<Series> st;
     int code = HH_OK;
+    // the wing parameters of the last FIT_BASELINE (Processing_Stack::wing_params, src/driver.c:479):
+    // CF_TO_SF adds their Fourier image while any is non-zero, rescaling them in place as the
+    // reference does (src/driver.c:1787-1810)
+    double wing[3] = {0.0, 0.0, 0.0};
     for (int i = 0; i < c->nproc; ++i) {
         const hh_conf_proc& p = c->proc[i];
         auto need = [&](size_t k) -> int {
@@ -669,7 +727,73 @@ int run_processing(hh_engine* e, const hh_conf* c, bool quiet)
                 }
                 Series sf;
                 rc = cf_to_sf(e, st.back(), sf);
+                if (rc == HH_OK && (wing[0] != 0.0 || wing[1] != 0.0 || wing[2] != 0.0)) {
+                    // the Lorentzian's image, (C + A) / A * A / (2 B) exp(-nu / B), in SF units
+                    const double Xs = 1.0 / kLightSpeedCm / kATUs / 2.0 / M_PI;
+                    const double Ys = kATUs * kADipMomU * kADipMomU / (4.0 * M_PI * kEpsilon0);
+                    wing[0] = wing[0] * Ys * Xs;
+                    wing[1] = wing[1] * Xs;
+                    wing[2] = wing[2] * Ys * Xs;
+                    for (size_t k = 0; k < sf.y.size(); ++k)
+                        sf.y[k] += (wing[2] + wing[0]) / wing[0] * (wing[0] / (2.0 * wing[1]) * std::exp(-sf.x[k] / wing[1]));
+                }
                 if (rc == HH_OK) st.back() = std::move(sf);
+                break;
+            }
+            case HH_PROC_FIT_BASELINE: {
+                // execute_fit_baseline (src/driver.c:2072-2142): fit, keep the parameters, subtract
+                // the model from the CF on top
+                if ((rc = need(1)) < 0) break;
+                Series& cf = st.back();
+                if (cf.sf || cf.spectrum) {
+                    rc = fail(HH_E_INVALID, "FIT_BASELINE expects a correlation function on the stack");
+                    break;
+                }
+                long begin = std::atol(p.arg);
+                if (begin < 0) begin = 16384;   // DEFAULT_CF_EXTRAPOLATION_BEGIN_INDEX, src/driver.c:207
+                if ((rc = hh_fit_baseline(cf.x.data(), cf.y.data(), (long)cf.y.size(), begin, wing)) < 0) break;
+                if (!quiet)
+                    std::printf("FIT_BASELINE: A = %.6e, B = %.6e, C = %.6e (from index %ld)\n", wing[0], wing[1], wing[2],
+                                begin);
+                for (size_t k = 0; k < cf.y.size(); ++k) cf.y[k] -= wing[2] + wing[0] / (1.0 + wing[1] * wing[1] * cf.x[k] * cf.x[k]);
+                break;
+            }
+            case HH_PROC_SMOOTH: {
+                // execute_smooth (src/driver.c:2849-2951): the SF on top replaced by its LOESS
+                // smoothing on the grid
+                if ((rc = need(1)) < 0) break;
+                if (!st.back().sf || st.back().spectrum) {
+                    rc = fail(HH_E_INVALID, "SMOOTH expects a spectral function on the stack");
+                    break;
+                }
+                long npoints = 0, ws_min = 0, ws_delay = 0;
+                double grid_max = 0.0, ws_step = 0.0;
+                if (std::sscanf(p.arg, "%ld;%lf;%ld;%lf;%ld", &npoints, &grid_max, &ws_min, &ws_step, &ws_delay) != 5) {
+                    rc = fail(HH_E_INVALID, "SMOOTH: bad arguments");
+                    break;
+                }
+                if (npoints == 0) {
+                    rc = fail(HH_E_INVALID, "SMOOTH: number of grid points (grid_npoints) for smoothing must be provided");
+                    break;
+                }
+                Series sm;
+                sm.sf = true;
+                sm.T = st.back().T;
+                sm.ntraj = st.back().ntraj;
+                sm.x.resize((size_t)npoints);
+                sm.y.resize((size_t)npoints);
+                rc = hh_loess(st.back().x.data(), st.back().y.data(), (long)st.back().x.size(), grid_max, npoints, ws_min,
+                              ws_step, ws_delay, sm.x.data(), sm.y.data());
+                if (rc == HH_OK) st.back() = std::move(sm);
+                break;
+            }
+            case HH_PROC_INT3: {
+                // execute_int3 (src/driver.c:2423-2446): report the stack and stop with a failure
+                std::printf("BREAKPOINT INTERRUPT ISSUED\nStack trace:\n");
+                for (size_t k = 0; k < st.size(); ++k)
+                    std::printf("    %zu: %s of %zu points\n", k, st[k].spectrum ? "SPECTRUM" : st[k].sf ? "SF" : "CF",
+                                st[k].x.size());
+                rc = fail(HH_E_INVALID, "INT3: breakpoint interrupt at PROCESSING call %d", i + 1);
                 break;
             }
             case HH_PROC_CMP: {

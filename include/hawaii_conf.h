@@ -56,14 +56,15 @@ enum { HH_MONOMER_NONE = 0, HH_MONOMER_ATOM = 1, HH_MONOMER_LINEAR_MOLECULE = 2,
 enum { HH_PROC_READ_CF = 0, HH_PROC_READ_SF, HH_PROC_CF_TO_SF, HH_PROC_CMP, HH_PROC_DUP, HH_PROC_DUP2,
        HH_PROC_DROP, HH_PROC_DROP2, HH_PROC_WRITE_CF, HH_PROC_WRITE_SF, HH_PROC_READ_SPECTRUM, HH_PROC_WRITE_SPECTRUM,
        HH_PROC_ALPHA, HH_PROC_D1, HH_PROC_D2, HH_PROC_D3, HH_PROC_D4, HH_PROC_D4A, HH_PROC_AVERAGE_CFS,
-       HH_PROC_ADD_SPECTRA, HH_PROC_MN_CLASSICAL, HH_PROC_MN_QUANTUM };
+       HH_PROC_ADD_SPECTRA, HH_PROC_MN_CLASSICAL, HH_PROC_MN_QUANTUM, HH_PROC_FIT_BASELINE, HH_PROC_SMOOTH,
+       HH_PROC_INT3 };
 
 #define HH_CONF_PATH 512
 #define HH_CONF_MAX_PROC 128
 
 typedef struct {
     int op;                       /* HH_PROC_* */
-    char arg[HH_CONF_PATH];       /* file name for READ_* / WRITE_* */
+    char arg[HH_CONF_PATH];       /* file name for READ_* / WRITE_*; the arguments of the others */
 } hh_conf_proc;
 
 typedef struct {

@@ -36,6 +36,8 @@
  *                        (src/hawaii.c:5945-5979)
  *   hh_sf_moment         compute_Mn_from_sf_using_classical_detailed_balance
  *                        (src/hawaii.c:5981-6013)
+ *   hh_fit_baseline      FIT_BASELINE's wing-model fit (fit_baseline, src/hawaii.c:5313-5400)
+ *   hh_loess             SMOOTH's LOESS smoothing (src/loess.cpp:98-451)
  *   hh_pipnn_*           the N2-Ar PIP-NN plugin surfaces (PES-IDS/ai_pes_n2_ar_pip_nn.cpp,
  *                        PES-IDS/ai_ids_n2_ar_pip_nn.cpp) as batched device evaluations
  * The .conf driver front end is declared in hawaii_conf.h.
@@ -324,6 +326,23 @@ int hh_cf_moment(const double* cf, long len, double dt, double ntraj, int n, dou
  * Simpson 3/8 weights of integrate_composite_simpson (src/hawaii.c:5926-5943); odd n give 0;
  * ntraj > 0 divides (an unnormalised SF, as the PR/mu loop's estimates). */
 int hh_sf_moment(const double* nu, const double* sf, long len, double ntraj, int n, double* out);
+
+/* FIT_BASELINE (execute_fit_baseline, src/driver.c:2072-2142; fit_baseline and
+ * gsl_nonlinear_opt, src/hawaii.c:5215-5400): the wing model C + A / (1 + B^2 t^2) fitted by
+ * Levenberg-Marquardt to cf[begin..len) (times t in a.u., the CF's own units), with times scaled
+ * by that of the CF's secondary maximum and values by the maximum; abc = {A, |B|, C} in the CF's
+ * units.  HH_E_INVALID for begin >= len or a CF without a secondary maximum (it never turns
+ * negative and rises again).  Host computation; no device is used. */
+int hh_fit_baseline(const double* t, const double* cf, long len, long begin, double* abc);
+
+/* SMOOTH (execute_smooth, src/driver.c:2849-2951; loess_init, loess_create_grid,
+ * loess_apply_smoothing and loess_estimate, src/loess.cpp:98-451): LOESS with local cubics and
+ * tricube weights of the series (x, y)[len] evaluated on npoints equidistant points from 0 to
+ * grid_max (nu_out, y_out [npoints]); the window holds ws_min points, ws_min + ws_step i
+ * (truncated) beyond grid point ws_delay.  HH_E_INVALID for a grid outside the data, npoints <= 0,
+ * grid_max <= 0 or a window of fewer than 4 or more than len points.  Host computation. */
+int hh_loess(const double* x, const double* y, long len, double grid_max, long npoints, long ws_min, double ws_step,
+             long ws_delay, double* nu_out, double* y_out);
 
 /* ------------------------------------------------------------------ PIP-NN surfaces (N2-Ar) */
 /* The N2-Ar permutation-invariant-polynomial neural-network surfaces

@@ -17,6 +17,13 @@
                        tests/desymmetrizations.conf and tests/compute-alpha.conf
                        (SPCL, SPD1, SPD2, SPD3, SPD4, SPD4a-F-He-Ar-1000.0.txt: ALPHA of the
                        CF_TO_SF of CF-F-He-Ar-1000.0 without and with each desymmetrization)
+  baseline_smooth_hear_1000.npz  the expected outputs of the reference's FIT_BASELINE and SMOOTH
+                       tests (tests/convert-cf-to-sf.conf: SF-F-He-Ar-1000.0-with-baseline.txt,
+                       ...-with-custom-baseline.txt, the CF_TO_SF of CF-F-He-Ar-1000.0 after
+                       FIT_BASELINE() and FIT_BASELINE(8000); tests/smooth.conf:
+                       SPCL-F-He-Ar-1000.0-sm.txt, ALPHA of SMOOTH(grid_npoints=1200,
+                       grid_max=1200.0, window_size_delay=10, window_size_step=0.05) of
+                       SF-F-He-Ar-1000.0)
   add_spectra_hear_100.npz  the data columns of the reference's tests/add-spectra.conf inputs
                        (SPCL-B- and SPCL-F-He-Ar-100.0-raw.txt) and expected sums
                        (SPCL-T-He-Ar-100.0-raw.txt, ...-raw-wn-step=0.05.txt)
@@ -155,11 +162,17 @@ def add_spectra():
     np.savez_compressed(os.path.join(HERE, "add_spectra_hear_100.npz"), **out)
 
 
+def baseline_smooth():
+    b = read_two_columns(os.path.join(REF, "tests", "SF-F-He-Ar-1000.0-with-baseline.txt"))
+    c = read_two_columns(os.path.join(REF, "tests", "SF-F-He-Ar-1000.0-with-custom-baseline.txt"))
+    sm = read_two_columns(os.path.join(REF, "tests", "SPCL-F-He-Ar-1000.0-sm.txt"))
+    np.savez_compressed(os.path.join(HERE, "baseline_smooth_hear_1000.npz"), sf_baseline=b[:, 1],
+                        sf_custom_baseline=c[:, 1], sm_nu=sm[:, 0], sm_alpha=sm[:, 1])
+
+
 if __name__ == "__main__":
-    mt_streams()
-    hear()
-    co2ar()
-    cf_sf()
-    spectra()
-    add_spectra()
+    # all fixtures, or the ones named on the command line
+    todo = sys.argv[1:] or ["mt_streams", "hear", "co2ar", "cf_sf", "spectra", "add_spectra", "baseline_smooth"]
+    for name in todo:
+        globals()[name]()
     print("golden fixtures written to", HERE)

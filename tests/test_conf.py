@@ -180,11 +180,47 @@ def test_parse_processing_spectra(tmp_path):
     assert [c.proc[i].op for i in range(c.nproc)] == [0, 17, 12, 11, 10, 13]
 
 
+def test_parse_baseline_smooth_int3(tmp_path):
+    """FIT_BASELINE with no, a positional and a named index (the reference's
+    tests/convert-cf-to-sf.conf), SMOOTH's named arguments (tests/smooth.conf; defaults
+    window_size_min 21, window_size_step 0.1, window_size_delay 10, src/driver.c:208-210), INT3."""
+    rc, c = parse_text(tmp_path, """
+&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  FIT_BASELINE()
+  FIT_BASELINE(cf_extrapolation_begin_index=8000)
+  FIT_BASELINE(8000)
+  READ_SF("sf.txt")
+  SMOOTH(grid_npoints=1200, grid_max=1200.0, window_size_delay=10, window_size_step=0.05)
+  SMOOTH(grid_npoints=10, grid_max=5.0)
+  INT3()
+&END
+""")
+    assert rc == 0, H.lib().hh_last_error()
+    ops = [c.proc[i].op for i in range(c.nproc)]
+    assert ops == [0, 22, 22, 22, 1, 23, 23, 24]
+    assert [c.proc[i].arg for i in (1, 2, 3)] == [b"-1", b"8000", b"8000"]
+    f = [float(v) for v in c.proc[5].arg.split(b";")]
+    assert f == [1200, 1200.0, 21, 0.05, 10]
+    f = [float(v) for v in c.proc[6].arg.split(b";")]
+    assert f == [10, 5.0, 21, 0.1, 10]
+
+
 @pytest.mark.parametrize("text,needle", [
     ("&INPUT\n  FOO = 1\n&END\n", b"x.conf:2: unknown keyword FOO"),
     ("&INPUT\n  CALCULATION_TYPE = PR_MU_ARRAY\n&END\n", b"PR_MU_ARRAY is not part of this engine"),
-    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  FIT_BASELINE()\n&END\n",
-     b"FIT_BASELINE is not part of this engine"),
+    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  NORMALIZE()\n&END\n",
+     b"NORMALIZE is not part of this engine"),
+    # the reference's tests/fit-baseline-nargs.conf
+    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  READ_CF(\"cf.txt\")\n  FIT_BASELINE(8000, 124)\n&END\n",
+     b"FIT_BASELINE expects at most 1 arguments"),
+    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  FIT_BASELINE(begin=8000)\n&END\n",
+     b"expects a named argument cf_extrapolation_begin_index but got begin"),
+    ("&INPUT\n  CALCULATION_TYPE = PROCESSING\n&END\n&PROCESSING\n  SMOOTH(1200)\n&END\n",
+     b"SMOOTH accepts only named arguments"),
     ("&INPUT\n  TEMPERATURE = \"hot\"\n&END\n", b"TEMPERATURE expects a number"),
     ("&INPUT\n  TEMPERATURE = 300.0\n", b"not closed"),
 ])

@@ -538,3 +538,84 @@ def test_driver_refuses_an_unloadable_plugin(tmp_path):
     r = run_driver(["missing.conf", "--seed", "3"], str(tmp_path))
     assert r.returncode != 0
     assert "cannot load the surface plugin" in (r.stdout + r.stderr)
+
+
+def test_driver_fit_baseline_and_smooth(tmp_path, golden):
+    """The reference's tests/convert-cf-to-sf.conf (FIT_BASELINE(), FIT_BASELINE(8000) and the
+    named form, each followed by CF_TO_SF on the device) and tests/smooth.conf (SMOOTH, ALPHA)
+    through the driver on the committed fixtures.  SFs: 1e-9 of the peak, 1e-4 relative above
+    1e-8 of the peak; the baseline's own contribution (with minus without, 1e-5 of the SF) to
+    2e-3 where it exceeds 1e-2 of its largest value (what the 11 printed digits of the two
+    reference files resolve).  Smoothed spectrum: 1e-4 relative below 680 cm^-1, 1e-3 of the
+    peak beyond, where the reference's ill-conditioned cubic fits carry rounding noise."""
+    g, b = golden("cf_sf_hear_1000.npz"), golden("baseline_smooth_hear_1000.npz")
+    T, nt = float(g["T"][()]), float(g["ntraj"][()])
+    write_ref_format(tmp_path / "cf.txt", g["cf_t"], g["cf"], T, nt, False)
+    write_ref_format(tmp_path / "sf.txt", g["sf_nu"], g["sf"], T, nt, True)
+    (tmp_path / "p.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  DUP()
+    CF_TO_SF()
+    WRITE_SF("plain.txt")
+  DUP()
+    FIT_BASELINE()
+    CF_TO_SF()
+    WRITE_SF("baseline.txt")
+  DUP()
+    FIT_BASELINE(cf_extrapolation_begin_index=8000)
+    CF_TO_SF()
+    WRITE_SF("custom.txt")
+  FIT_BASELINE(8000)
+  CF_TO_SF()
+  WRITE_SF("custom2.txt")
+  READ_SF("sf.txt")
+  SMOOTH(grid_npoints=1200, grid_max=1200.0, window_size_delay=10, window_size_step=0.05)
+  ALPHA()
+  WRITE_SPECTRUM("sm.txt")
+&END
+""")
+    r = run_driver(["p.conf"], str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _, plain = read_columns(tmp_path / "plain.txt")
+    for name, key in (("baseline", "sf_baseline"), ("custom", "sf_custom_baseline"), ("custom2", "sf_custom_baseline")):
+        _, sf = read_columns(tmp_path / f"{name}.txt")
+        want = b[key]
+        peak = np.abs(want).max()
+        assert np.max(np.abs(sf[:, 1] - want)) < 1e-9 * peak, name
+        big = np.abs(want) > 1e-8 * peak
+        assert np.max(np.abs(sf[big, 1] / want[big] - 1.0)) < 1e-4, name
+        ours, theirs = sf[:, 1] - plain[:, 1], want - g["sf"]
+        sel = np.abs(theirs) > 1e-2 * np.abs(theirs).max()
+        err = np.max(np.abs(ours[sel] / theirs[sel] - 1.0))
+        print(f"  {name}: baseline contribution rel err {err:.2e} over {sel.sum()} points", flush=True)
+        assert err < 2e-3, name
+    head, sm = read_columns(tmp_path / "sm.txt")
+    assert sm.shape == (1200, 2)
+    assert np.max(np.abs(sm[:, 0] - b["sm_nu"])) < 1e-6
+    want = b["sm_alpha"]
+    rel = np.abs(sm[1:, 1] / want[1:] - 1.0)
+    assert rel[:679].max() < 1e-4
+    assert np.max(np.abs(sm[:, 1] - want)) < 1e-3 * np.abs(want).max()
+
+
+def test_driver_int3_stops_with_the_stack(tmp_path, golden):
+    """INT3 (src/driver.c:2423-2446) prints the stack and ends the run with a failure."""
+    g = golden("cf_sf_hear_1000.npz")
+    write_ref_format(tmp_path / "cf.txt", g["cf_t"][:1024], g["cf"][:1024], 1000.0, 1.0, False)
+    (tmp_path / "i.conf").write_text("""&INPUT
+  CALCULATION_TYPE = PROCESSING
+&END
+&PROCESSING
+  READ_CF("cf.txt")
+  DUP()
+  INT3()
+  WRITE_CF("never.txt")
+&END
+""")
+    r = run_driver(["i.conf"], str(tmp_path))
+    assert r.returncode != 0
+    assert "BREAKPOINT INTERRUPT ISSUED" in r.stdout and "1: CF of 1024 points" in r.stdout, r.stdout[-2000:]
+    assert not (tmp_path / "never.txt").exists()

@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(64 * kDipWaves) ch4co2_dipole_kernel(JobDesc j
         for (int c = 0; c < 3; ++c) {
             const int t0 = tb->start[bf][c], t1 = tb->start[bf][c + 1];
             P s = P(0);
-#pragma unroll 4
+#pragma unroll 8
             for (int e = t0; e < t1; ++e) {
                 const unsigned xy = tb->xy[e];
                 s += tb->coef[e] * (sX[xy & 0xffu][lane] * sY[xy >> 8][lane]);

@@ -53,8 +53,10 @@ def test_engine_is_gfx950(eng):
 
 @pytest.mark.parametrize("engine", [H.ENGINE_PLAIN, H.ENGINE_BATCHED])
 def test_hear_r0_fp64_matches_oracle(eng, engine):
+    """He-Ar at its full configuration: 4096 trajectories at 300 K from R0 = 40 bohr, FP64 at
+    rtol = atol = 1e-12, 2048 samples at dt = 200 a.u."""
     r = run_and_compare(eng, H.he_ar(), O.hear(), H.Sampler(H.IC_R0_INJECT, 1, 300.0, 4.0, 40.0, H.PESMIN_HEAR, 40.0),
-                        None, 512, 1024, H.HALVES_FORWARD, engine, H.PREC_FP64)
+                        None, 4096, 2048, H.HALVES_FORWARD, engine, H.PREC_FP64)
     assert rel_l2(r["cf"], r["cf_ref"]) < 1e-10
     assert np.quantile(r["per"], 0.99) < 1e-10
     assert np.array_equal(r["items"]["len"], r["ref"]["len_f"])

@@ -564,15 +564,28 @@ def main():
         # whole ensemble, every trajectory counted (the metric's 1e-4 bar); for information, the
         # trajectories on which two FP64 integrations (GPU plain, CPU oracle) already disagree
         # beyond 1e-6 (chaotic: their rows depend on the last bit of the arithmetic)
+        # The floor that chaos sets for any two FP64 integrations of this ensemble: the oracle
+        # again, from initial conditions one unit in the last place away (every component scaled
+        # by 1 + 2^-52).  A pair bound in an orbiting complex for the whole record has a row of
+        # 2-4 % of the ensemble CF's norm, and where two integrations of it part depends on the
+        # last bits of the arithmetic.
+        t0 = time.time()
+        ref_1ulp = O.cf_rows(O.co2ar(), O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000),
+                             np.ascontiguousarray(ic0 * (1.0 + 2.0 ** -52)), cores)
+        t_floor = time.time() - t0
+        floor = rel(O.ensemble_from_rows(ref_1ulp["crln"]), cf_ref)
         result["accuracy_vs_fp64_oracle"] = {
             "step": "first timed step, whole ensemble", "oracle": "CPU DOP853 FP64 rtol=atol=1e-12 (oracle/)",
             "cf_rel_l2_batched_mixed": rel(mixed_cf0, cf_ref), "cf_rel_l2_batched_fp64": rel(b64_cf0, cf_ref),
             "cf_rel_l2_plain_fp64": rel(plain_cf0, cf_ref), "bar": 1e-4,
+            "cf_rel_l2_oracle_vs_oracle_1ulp": floor,
+            "cf_rel_l2_batched_mixed_vs_batched_fp64": rel(mixed_cf0, b64_cf0),
             "chaotic_trajectories_fp64": int((dp >= 1e-6).sum()),
             "per_traj_rel_l2_batched_mixed_median": float(np.median(dm)),
             "per_traj_rel_l2_batched_mixed_p99": float(np.quantile(dm, 0.99)),
             "max_energy_drift_hartree": float(stm.max_e_drift)}
-        log("accuracy", result["accuracy_vs_fp64_oracle"], f"(oracle {t_or:.1f} s on {cores} threads)")
+        log("accuracy", result["accuracy_vs_fp64_oracle"],
+            f"(oracle {t_or:.1f} s, 1-ulp oracle {t_floor:.1f} s on {cores} threads)")
         m = min(args.cpu_sample, K * n)
         cf = np.zeros(L)
         t0 = time.time()

@@ -37,6 +37,10 @@ __host__ __device__ __forceinline__ bool dropped(int first, int last)
     return dipole_failure(first) || dipole_failure(last) || last == ITEM_FAIL_STEPS;
 }
 
+// Scheduling keys of a job's items (order_keys_kernel): the straight-line transit time through the
+// Rcut sphere, the squared impact parameter, and the transit time with a marked front tier.
+enum OrderKey : int { ORDER_TRANSIT_KEY = 0, ORDER_IMPACT_KEY = 1, ORDER_TIERED_KEY = 2 };
+
 // hh_cf_to_sf_noalloc: the largest transform length its static work array holds (two
 // 2n-point complex FP64 sequences: 4 MiB per device)
 constexpr long kCfToSfNoAllocMaxN = 65536;

@@ -40,7 +40,12 @@ hipError_t run_ch4co2_cf_rows(const JobDesc& jd, const void* tables, bool f32, h
 // 8 n doubles of scratch, len one int
 hipError_t run_ch4co2_eval_dipole(const double* y, long long n, double* records, int* len, const void* tables,
                                   double* out, hipStream_t st);
-hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st);
+// scheduling keys of a job's items: what = 0 the straight-line transit time, 1 the squared impact
+// parameter, 2 the transit time with the items marked 1 in keys moved to a front tier
+hipError_t run_order_keys(const JobDesc& jb, double mu, int what, float* keys, int* idx, int idx_offset,
+                          hipStream_t st);
+// keys[order[r]] = (r < n_front) for r in [0, n)
+hipError_t run_order_front(const int* order, long long n, long long n_front, float* keys, hipStream_t st);
 size_t slots_bytes(int sys, long long S);
 LaneSlots slots_view(int sys, void* base, long long S);
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
@@ -231,6 +236,9 @@ using PoolSet = std::vector<std::unique_ptr<LanePool>>;
 // the same input on a few items in 10^4, with extra right-hand sides, in FP64 and in mixed
 // precision, while the one-wavefront build and the plain engine reproduce their bits.
 constexpr int kPoolBlock = 256, kPoolWaves = 1;
+// share of an admission group of a rotating-monomer system handed out first, smallest impact
+// parameter first (pool_order_group; the environment variable HH_FRONT_SHARE overrides it)
+constexpr double kFrontShare = 0.10;
 // the lane pool of class (system, mu, I, precision) in `set`, created on first use
 int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, LanePool** out);
 // device bytes a job of this shape holds while it runs

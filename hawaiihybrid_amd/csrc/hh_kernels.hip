@@ -994,13 +994,18 @@ __global__ void __launch_bounds__(BS, W) pool_cf_kernel(SysConst sc, const JobDe
     save_lane<N>(sl, slot, ar, ln);
 }
 
-// Scheduling key of every item of a job: the straight-line time (in samples, at most L - 1)
-// from the initial point to the Rcut sphere along the item's initial velocity, forward for
-// the first half and momentum-inverted for the second.  Trajectory cost grows with its
-// duration, so handing out items in decreasing key order starts the long ones first.  Only
-// the order of work changes; every item's result is the same.
+// Scheduling keys of every item of a job.
+//   what = ORDER_TRANSIT_KEY: the straight-line time (in samples, at most L - 1) from the initial
+//     point to the Rcut sphere along the item's initial velocity, forward for the first half and
+//     momentum-inverted for the second.  Trajectory cost grows with its duration, so handing out
+//     items in decreasing key order starts the long ones first.
+//   what = ORDER_IMPACT_KEY: the squared impact parameter b^2 = L^2 / p^2 of the relative motion.
+//   what = ORDER_TIERED_KEY: the transit time plus L for the items whose entry of `keys` is 1 on
+//     input (the front tier, marked by order_front_kernel), so they are handed out first.
+// Only the order of work changes; every item's result is the same.
+
 __global__ void __launch_bounds__(256) order_keys_kernel(const double* __restrict__ ic, long long ntraj, int halves,
-                                                         double mu, double Rcut, double dt, int L,
+                                                         double mu, double Rcut, double dt, int L, int what,
                                                          float* __restrict__ keys, int* __restrict__ idx,
                                                          int idx_offset)
 {
@@ -1011,16 +1016,30 @@ __global__ void __launch_bounds__(256) order_keys_kernel(const double* __restric
     const double R = ic[4 * ntraj + t], pR = ic[5 * ntraj + t];
     const double sT = sin(Theta);
     const double L2 = pTheta * pTheta + (sT != 0.0 ? pPhi * pPhi / (sT * sT) : 0.0);
-    const double p = sqrt(pR * pR + L2 / (R * R));
+    const double p2 = pR * pR + L2 / (R * R);
+    const double p = sqrt(p2);
     double key = (double)(L - 1);
-    if (p > 0.0) {
+    if (what == ORDER_IMPACT_KEY) {
+        key = p2 > 0.0 ? L2 / p2 : 0.0;
+    } else if (p > 0.0) {
         const double rr = ((i % halves) == 1 ? -1.0 : 1.0) * R * pR / p;   // r . u
         const double disc = rr * rr - R * R + Rcut * Rcut;
         const double tx = disc > 0.0 ? (-rr + sqrt(disc)) * mu / p : 0.0;
         key = fmin(tx / dt, (double)(L - 1));
     }
+    if (what == ORDER_TIERED_KEY && keys[i] != 0.0f) key += (double)L;
     keys[i] = (float)key;
     idx[i] = idx_offset + (int)i;
+}
+
+// keys[order[r]] = 1 for the first n_front ranks of `order` (a permutation of [0, n)), 0 for
+// the rest.
+__global__ void __launch_bounds__(256) order_front_kernel(const int* __restrict__ order, long long n, long long n_front,
+                                                          float* __restrict__ keys)
+{
+    const long long r = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (r >= n) return;
+    keys[order[r]] = r < n_front ? 1.0f : 0.0f;
 }
 
 // Batched surface evaluation at phase points y (SoA [QP][n], FP64): what=0 the potential
@@ -1434,13 +1453,21 @@ hipError_t run_eval(int sys, const SysConst& sc, int what, const double* y, long
     return hipGetLastError();
 }
 
-// keys and item numbers of job jb's items, written at keys[0..n_items) and idx[0..n_items) with
-// item numbers counted from idx_offset
-hipError_t run_order_keys(const JobDesc& jb, double mu, float* keys, int* idx, int idx_offset, hipStream_t st)
+// keys (of kind `what`) and item numbers of job jb's items, written at
+// keys[0..n_items) and idx[0..n_items) with item numbers counted from idx_offset
+hipError_t run_order_keys(const JobDesc& jb, double mu, int what, float* keys, int* idx, int idx_offset,
+                          hipStream_t st)
 {
     const long long n = jb.n_items;
     hipLaunchKernelGGL(order_keys_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, jb.ic, jb.ntraj,
-                       jb.halves, mu, jb.Rcut, jb.dt, jb.L, keys, idx, idx_offset);
+                       jb.halves, mu, jb.Rcut, jb.dt, jb.L, what, keys, idx, idx_offset);
+    return hipGetLastError();
+}
+
+hipError_t run_order_front(const int* order, long long n, long long n_front, float* keys, hipStream_t st)
+{
+    hipLaunchKernelGGL(order_front_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, order, n, n_front,
+                       keys);
     return hipGetLastError();
 }
 

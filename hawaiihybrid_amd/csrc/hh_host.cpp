@@ -315,7 +315,7 @@ int hh_engine_create(int device, hh_engine** out)
         delete e;
         return fail(HH_E_DEVICE, "device %d is %s; this build targets gfx950 (MI355X)", device, arch.c_str());
     }
-    // graph-captured slices are opt-in
+    // the pools' slices run as captured graphs; HH_GRAPHS=0 launches them directly
     if (const char* g = std::getenv("HH_GRAPHS")) e->use_graphs = g[0] && g[0] != '0';
     // a blocking stream: the engine's work is ordered after the caller's copies and kernels on
     // the legacy default stream (hipMemcpy of pageable memory can return before its DMA ends)

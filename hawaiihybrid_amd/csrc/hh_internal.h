@@ -199,7 +199,7 @@ struct hh_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t capture = nullptr;      // non-blocking stream used only to capture slice graphs
-    bool use_graphs = false;            // HH_GRAPHS=1: slices as captured graphs (off: see hh_pool.cpp)
+    bool use_graphs = true;             // slices as captured graphs (HH_GRAPHS=0: direct launches)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     int wall_khz = 100000;        // s_memrealtime frequency

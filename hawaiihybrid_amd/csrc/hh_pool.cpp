@@ -218,8 +218,7 @@ int pool_order_group(hh_engine* e, LanePool& pl)
     return HH_OK;
 }
 
-// The graph-captured slices below are opt-in (HH_GRAPHS=1); the default path launches each
-// slice directly.
+// Slices run as graphs captured below (the default; HH_GRAPHS=0 launches each slice directly).
 namespace {
 
 bool same_ctl(const PoolCtl& a, const PoolCtl& b)

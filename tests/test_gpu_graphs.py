@@ -3,7 +3,7 @@
 The slices of a lane pool run as captured graphs (repack when the queue has drained, the pool
 kernel, copies of the counters into pinned memory), replayed many times per run with the host's
 reductions and bookkeeping between replays.  Every replay must give the same bits as the
-directly launched slices (an engine created without HH_GRAPHS=1), in FP64 and in mixed
+directly launched slices (an engine created with HH_GRAPHS=0), in FP64 and in mixed
 precision.  The second and later runs on one engine reuse its lane pool, whose device queue head
 the first run left past the end (pool_admit clamps it back before the new jobs).
 """
@@ -15,9 +15,10 @@ import hawaiihybrid_amd as H
 pytestmark = pytest.mark.gpu
 
 
-def graph_engine(monkeypatch):
-    """An engine with graph-captured slices (read at creation)."""
-    monkeypatch.setenv("HH_GRAPHS", "1")
+def engine(monkeypatch, graphs):
+    """An engine with graph-captured (the default) or directly launched slices (read at
+    creation)."""
+    monkeypatch.setenv("HH_GRAPHS", "1" if graphs else "0")
     e = H.Engine(0)
     monkeypatch.delenv("HH_GRAPHS")
     return e
@@ -33,8 +34,8 @@ def test_graph_replays_equal_direct_launches(monkeypatch):
     smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
     ic, _, _ = H.sample_ics(H.co2_ar(), smp, 42, 3 * 8192)
     print("graphs: 3 jobs of 8192 CO2-Ar trajectories, L=512", flush=True)
-    g = graph_engine(monkeypatch)
-    d = H.Engine(0)   # engines without HH_GRAPHS launch the slices directly
+    g = engine(monkeypatch, True)
+    d = engine(monkeypatch, False)
     xs = [g.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(3)]
     for prec in (H.PREC_FP64, H.PREC_MIXED):
         ref, st = run(d, xs, prec)

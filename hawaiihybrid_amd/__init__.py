@@ -70,7 +70,7 @@ class CFStats(C.Structure):
                 ("nrhs", C.c_longlong), ("nsamples", C.c_longlong), ("n_failed", C.c_longlong),
                 ("n_stepfail", C.c_longlong), ("max_e_drift", C.c_double), ("block", C.c_int), ("grid", C.c_int),
                 ("launches", C.c_int), ("n_orbit", C.c_longlong), ("nrhs_orbit", C.c_longlong),
-                ("nsamples_orbit", C.c_longlong)]
+                ("nsamples_orbit", C.c_longlong), ("pool_launches", C.c_int), ("pool_kernel_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -315,7 +315,7 @@ int hh_engine_create(int device, hh_engine** out)
         delete e;
         return fail(HH_E_DEVICE, "device %d is %s; this build targets gfx950 (MI355X)", device, arch.c_str());
     }
-    // the pools' slices run as captured graphs; HH_GRAPHS=0 launches them directly
+    // graph-captured slices are opt-in (HH_GRAPHS=1)
     if (const char* g = std::getenv("HH_GRAPHS")) e->use_graphs = g[0] && g[0] != '0';
     // a blocking stream: the engine's work is ordered after the caller's copies and kernels on
     // the legacy default stream (hipMemcpy of pageable memory can return before its DMA ends)
@@ -448,6 +448,8 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
         }
         double kms = 0.0, rms = 0.0;
         int launches = 0;
+        const double pool_ms0 = pl->launch_ms;
+        const long long pool_n0 = pl->launch_count;
         for (;;) {
             bool all = true;
             for (hh::PoolJob* pj : mine) all &= pj->state == 2;
@@ -474,6 +476,8 @@ int hh::cf_run(hh_engine* e, const hh_system* sys, const hh_cf_params* p, int nj
             stats->block = hh::kPoolBlock;
             stats->grid = (int)(pl->S / hh::kPoolBlock);
             stats->launches = launches;
+            stats->pool_launches = (int)(pl->launch_count - pool_n0);
+            stats->pool_kernel_ms = pl->launch_ms - pool_ms0;
             return hh::collect_stats(e->last_jobs, stats);
         }
         return HH_OK;

@@ -154,6 +154,8 @@ struct LanePool {
     // pools (an FP64 orbit pool beside its mixed pool, classes of a serving mix) overlap
     hipStream_t stream = nullptr;
     hipEvent_t slice_end = nullptr;
+    double launch_ms = 0.0;         // summed durations of this pool's slice launches
+    long long launch_count = 0;     // and their number (pool_step)
     std::deque<std::unique_ptr<PoolJob>> inflight;   // sequence order
     long long next_seq = 0;
     unsigned long long total = 0;   // items admitted
@@ -199,7 +201,7 @@ struct hh_engine {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t capture = nullptr;      // non-blocking stream used only to capture slice graphs
-    bool use_graphs = true;             // slices as captured graphs (HH_GRAPHS=0: direct launches)
+    bool use_graphs = false;            // HH_GRAPHS=1: slices as captured graphs (opt-in: see hh_pool.cpp)
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipDeviceProp_t prop{};
     int wall_khz = 100000;        // s_memrealtime frequency

@@ -1180,85 +1180,49 @@ __device__ __forceinline__ void neumaier_add(double& s, double& c, double v)
 }
 
 // partial[c][k] = sum over trajectories of chunk c of ppf * crln[k]; column-parallel, rows
-// read coalesced along k; entries past an item's length are zero and never read.  A thread sums
-// kReduceCols columns 256 apart (a workgroup covers 256 kReduceCols consecutive columns), so
-// each item's wavefront-uniform work (its counted length, staged in LDS once per workgroup,
-// and the row's buffer resource) serves kReduceCols row loads.  The items are taken
-// kReduceItems at a time: their row values are all loaded first and then added in item order,
-// so a wavefront keeps kReduceItems x kReduceCols row loads in flight.  A row is read through a buffer resource
-// whose size is the item's length (zero for an item that does not count or lies past the
-// chunk): the load of a column past it returns 0 without a branch.  Such a column adds +0.0,
-// which leaves the Neumaier sum and its compensation as they are (as does skipping a batch of
-// items whose rows all end before the workgroup's columns), so every column's sum is that of
-// the item-serial loop.
-constexpr int kReduceUnroll = 16, kReduceItems = 8, kReduceCols = 4, kReduceChunk = 256;
+// read coalesced along k; entries past an item's length are zero and never read.  The items are
+// taken kReduceUnroll at a time: their lengths and statuses (wavefront-uniform) and row values
+// are all loaded first and then added in item order, so a wavefront keeps kReduceUnroll row
+// loads in flight.  A row is read through a buffer resource whose size is the item's length
+// (zero for an item that does not count or lies past the chunk): the load of a column past it
+// returns 0 without a branch.  Such a column adds +0.0, which leaves the Neumaier sum and its
+// compensation as they are, so the sum is that of the item-serial loop.
+constexpr int kReduceUnroll = 16;
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const double* row, int n)
+__device__ __forceinline__ double row_value(const double* row, int n, int k)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(row), 0, n * 8, kBufferConfig);
-}
-
-__device__ __forceinline__ double row_load(__amdgpu_buffer_rsrc_t r, int k)
-{
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(row), 0, n * 8, kBufferConfig);
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, k * 8, 0, 0));
 }
 
-// Samples of item t that count toward the sum (its length, or 0 when it is dropped or t >= t1);
-// for two halves, of each half.  `cnt` holds them for the chunk's items (cnt[h][t - t0]).
 template <int HALVES>
-__device__ __forceinline__ void reduce_lengths(const int (*cnt)[kReduceChunk], long long t, long long t0,
-                                               long long t1, int (&n)[HALVES])
+__device__ __forceinline__ double reduce_value(const JobDesc& jb, long long t, long long t1, int k)
 {
     const bool in = t < t1;
-#pragma unroll
-    for (int h = 0; h < HALVES; ++h) n[h] = in ? __builtin_amdgcn_readfirstlane(cnt[h][t - t0]) : 0;
-}
-
-// ppf-free contributions of item t to columns k0 + 256 j (n: reduce_lengths; past t1, zeros)
-template <int HALVES>
-__device__ __forceinline__ void reduce_values(const JobDesc& jb, long long t, long long t1, const int (&n)[HALVES],
-                                              int k0, double (&v)[kReduceCols])
-{
-    const long long tc = t < t1 ? t : t1 - 1;
+    const long long tc = in ? t : t1 - 1;   // a valid item for the (discarded) status and length reads
     if constexpr (HALVES == 2) {
-        const __amdgpu_buffer_rsrc_t ra = row_rsrc(jb.rows + 2 * tc * jb.L, n[0]);
-        const __amdgpu_buffer_rsrc_t rb = row_rsrc(jb.rows + (2 * tc + 1) * jb.L, n[1]);
-#pragma unroll
-        for (int j = 0; j < kReduceCols; ++j) {
-            const double cf = row_load(ra, k0 + 256 * j), cb = row_load(rb, k0 + 256 * j);
-            v[j] = 0.5 * (cf + cb) * kALU * kALU * kALU;
-        }
+        const long long a = 2 * tc, b = 2 * tc + 1;
+        const bool use = in && !dropped(jb.status[a], jb.status[b]);
+        const double cf = row_value(jb.rows + a * jb.L, use ? jb.len[a] : 0, k);
+        const double cb = row_value(jb.rows + b * jb.L, use ? jb.len[b] : 0, k);
+        return 0.5 * (cf + cb) * kALU * kALU * kALU;
     } else {
-        const __amdgpu_buffer_rsrc_t r = row_rsrc(jb.rows + tc * jb.L, n[0]);
-#pragma unroll
-        for (int j = 0; j < kReduceCols; ++j) v[j] = row_load(r, k0 + 256 * j) * kALU * kALU * kALU;
+        const bool use = in && !dropped(jb.status[tc], jb.status[tc]);
+        const double cf = row_value(jb.rows + tc * jb.L, use ? jb.len[tc] : 0, k);
+        return cf * kALU * kALU * kALU;
     }
 }
 
-// Items [t0, t1) into the sums of columns k0 + 256 j; kb is the workgroup's first column: a
-// batch whose rows all end at or before it adds +0.0 to every column and is skipped.
 template <int HALVES>
-__device__ __forceinline__ void reduce_chunk(const JobDesc& jb, const int (*cnt)[kReduceChunk], long long t0,
-                                             long long t1, int kb, int k0, double ppf, double (&s)[kReduceCols],
-                                             double (&cc)[kReduceCols])
+__device__ __forceinline__ void reduce_chunk(const JobDesc& jb, long long t0, long long t1, int k, double ppf,
+                                             double& s, double& cc)
 {
-    for (long long t = t0; t < t1; t += kReduceItems) {
-        int n[kReduceItems][HALVES];
-        int reach = 0;
+    for (long long t = t0; t < t1; t += kReduceUnroll) {
+        double v[kReduceUnroll];
 #pragma unroll
-        for (int u = 0; u < kReduceItems; ++u) {
-            reduce_lengths<HALVES>(cnt, t + u, t0, t1, n[u]);
+        for (int u = 0; u < kReduceUnroll; ++u) v[u] = reduce_value<HALVES>(jb, t + u, t1, k);
 #pragma unroll
-            for (int h = 0; h < HALVES; ++h) reach = max(reach, n[u][h]);
-        }
-        if (reach <= kb) continue;
-        double v[kReduceItems][kReduceCols];
-#pragma unroll
-        for (int u = 0; u < kReduceItems; ++u) reduce_values<HALVES>(jb, t + u, t1, n[u], k0, v[u]);
-#pragma unroll
-        for (int u = 0; u < kReduceItems; ++u)
-#pragma unroll
-            for (int j = 0; j < kReduceCols; ++j) neumaier_add(s[j], cc[j], ppf * v[u][j]);
+        for (int u = 0; u < kReduceUnroll; ++u) neumaier_add(s, cc, ppf * v[u]);
     }
 }
 
@@ -1267,51 +1231,32 @@ __global__ void __launch_bounds__(256) cf_reduce_partial_kernel(JobDesc jb, long
                                                                 double* __restrict__ pcomp)
 {
     __shared__ int s_maxlen;
-    __shared__ int s_cnt[2][kReduceChunk];   // samples that count, per item of the chunk and half
-    const int kb = blockIdx.x * 256 * kReduceCols;   // the workgroup's first column
-    const int k0 = kb + threadIdx.x;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int c = blockIdx.y;
     const long long t0 = (long long)c * chunk;
     long long t1 = t0 + chunk;
     if (t1 > jb.ntraj) t1 = jb.ntraj;
-    // the counted length of every item of the chunk, and the longest: a workgroup whose columns
-    // all lie past it only writes zeros (what adding +0.0 for every item gives)
+    // the longest row of the chunk: a workgroup whose columns all lie past it only writes zeros
+    // (what adding +0.0 for every item gives)
     if (threadIdx.x == 0) s_maxlen = 0;
     __syncthreads();
     {
         int m = 0;
-        for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-            if (jb.halves == 2) {
-                const bool use = !dropped(jb.status[2 * t], jb.status[2 * t + 1]);
-                s_cnt[0][t - t0] = use ? jb.len[2 * t] : 0;
-                s_cnt[1][t - t0] = use ? jb.len[2 * t + 1] : 0;
-                m = max(m, max(s_cnt[0][t - t0], s_cnt[1][t - t0]));
-            } else {
-                const bool use = !dropped(jb.status[t], jb.status[t]);
-                s_cnt[0][t - t0] = use ? jb.len[t] : 0;
-                m = max(m, s_cnt[0][t - t0]);
-            }
-        }
+        for (long long t = t0 + threadIdx.x; t < t1; t += blockDim.x)
+            m = max(m, jb.halves == 2 ? max(jb.len[2 * t], jb.len[2 * t + 1]) : jb.len[t]);
         atomicMax(&s_maxlen, m);
     }
     __syncthreads();
-    double s[kReduceCols], cc[kReduceCols];
-#pragma unroll
-    for (int j = 0; j < kReduceCols; ++j) s[j] = cc[j] = 0.0;
-    if (kb < s_maxlen) {
+    if (k >= jb.L) return;
+    double s = 0.0, cc = 0.0;
+    if ((int)(blockIdx.x * blockDim.x) < s_maxlen) {
         if (jb.halves == 2)
-            reduce_chunk<2>(jb, s_cnt, t0, t1, kb, k0, ppf, s, cc);
+            reduce_chunk<2>(jb, t0, t1, k, ppf, s, cc);
         else
-            reduce_chunk<1>(jb, s_cnt, t0, t1, kb, k0, ppf, s, cc);
+            reduce_chunk<1>(jb, t0, t1, k, ppf, s, cc);
     }
-#pragma unroll
-    for (int j = 0; j < kReduceCols; ++j) {
-        const int k = k0 + 256 * j;
-        if (k < jb.L) {
-            psum[(long long)c * jb.L + k] = s[j];
-            pcomp[(long long)c * jb.L + k] = cc[j];
-        }
-    }
+    psum[(long long)c * jb.L + k] = s;
+    pcomp[(long long)c * jb.L + k] = cc;
 }
 
 __global__ void __launch_bounds__(256) cf_reduce_final_kernel(int L, int nchunks, long long ntraj,
@@ -1558,8 +1503,7 @@ hipError_t run_reduce_weighted(const JobDesc& jb, int ntemp, const double* wr, c
 hipError_t run_reduce(const JobDesc& jb, double ppf, double* psum, double* pcomp, long long chunk, int nchunks,
                       double* cf_out, hipStream_t st)
 {
-    if (chunk > kReduceChunk) return hipErrorInvalidValue;   // the partial kernel's LDS table
-    dim3 g1((jb.L + 256 * kReduceCols - 1) / (256 * kReduceCols), nchunks);
+    dim3 g1((jb.L + 255) / 256, nchunks);
     hipLaunchKernelGGL(cf_reduce_partial_kernel, g1, dim3(256), 0, st, jb, chunk, ppf, psum, pcomp);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -55,7 +55,7 @@ int pool_get(hh_engine* e, PoolSet& set, const hh_system& sys, bool mixed, LaneP
         return fail(HH_E_NOMEM, "cannot allocate a lane pool (%.1f MiB of slots and arena)",
                     (2.0 * sb + arena) / 1048576.0);
     HH_CHECK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
-    HH_CHECK(hipEventCreateWithFlags(&pl->slice_end, hipEventDisableTiming));
+    HH_CHECK(hipEventCreate(&pl->slice_end));
     HH_CHECK(run_slots_clear(pl->view(0), e->stream));
     const unsigned long long z = 0;
     HH_CHECK(hipMemcpyAsync(pl->next.p, &z, sizeof z, hipMemcpyHostToDevice, e->stream));
@@ -218,7 +218,10 @@ int pool_order_group(hh_engine* e, LanePool& pl)
     return HH_OK;
 }
 
-// Slices run as graphs captured below (the default; HH_GRAPHS=0 launches each slice directly).
+// The graph-captured slices below are opt-in (HH_GRAPHS=1); the default path launches each
+// slice directly.  A benchmark run with captured slices as the default (serving mix of
+// lane-pool sessions and M0 requests) ended in an illegal-address fault whose cause was not
+// found, so they stay off unless asked for.
 namespace {
 
 bool same_ctl(const PoolCtl& a, const PoolCtl& b)
@@ -372,6 +375,12 @@ int pool_step(hh_engine* e, PoolSet& set, double slice_ms, double* kernel_ms, do
         float ms = 0.0f;
         (void)hipEventElapsedTime(&ms, e->ev[0], e->ev[1]);
         if (kernel_ms) *kernel_ms += ms;
+        for (LanePool* pl : run) {
+            float pms = 0.0f;
+            (void)hipEventElapsedTime(&pms, e->ev[0], pl->slice_end);
+            pl->launch_ms += pms;
+            pl->launch_count++;
+        }
         if (trace) std::fprintf(stderr, "[hh pool] slice of %zu pools: %.2f ms\n", run.size(), ms);
     }
 

@@ -173,6 +173,10 @@ typedef struct {
                                lengths are the FP64 run's, nrhs counts both runs */
     long long nrhs_orbit;   /* right-hand sides of the FP64 runs of those items (part of nrhs) */
     long long nsamples_orbit; /* samples recorded by the FP64 runs (part of nsamples) */
+    int pool_launches;      /* batched engine: launches of the run's own pool (its system and
+                               precision; the FP64 orbit pool's launches are not counted) */
+    double pool_kernel_ms;  /* their summed durations, each from its slice's start to the end of
+                               that pool's launch on its stream (HIP events) */
 } hh_cf_stats;
 
 /* Run `njobs` correlation-function jobs of one system on the device.  ic_dev[j] holds

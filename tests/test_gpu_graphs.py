@@ -1,8 +1,8 @@
 """Graph-captured slices of the batched engine.
 
-The slices of a lane pool run as captured graphs (repack when the queue has drained, the pool
-kernel, copies of the counters into pinned memory), replayed many times per run with the host's
-reductions and bookkeeping between replays.  Every replay must give the same bits as the
+With HH_GRAPHS=1 the slices of a lane pool run as captured graphs (repack when the queue has
+drained, the pool kernel, copies of the counters into pinned memory), replayed many times per
+run with the host's reductions and bookkeeping between replays.  Every replay must give the same bits as the
 directly launched slices (an engine created with HH_GRAPHS=0), in FP64 and in mixed
 precision.  The second and later runs on one engine reuse its lane pool, whose device queue head
 the first run left past the end (pool_admit clamps it back before the new jobs).

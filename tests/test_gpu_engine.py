@@ -202,6 +202,24 @@ def test_batched_engine_reproduces_the_plain_engine_bit_for_bit(eng):
             eng.cf_run(H.co2_ar(), H.cf_params(512, precision=H.PREC_FP64, rtol=1e-10, block=block), xs)
 
 
+def test_pool_launch_timing_is_the_run_own_pool(eng):
+    """hh_cf_stats.pool_launches / pool_kernel_ms count and time the launches of the run's own
+    pool only: for a mixed run with hand-overs they are fewer than the launches of both pools,
+    and each launch, timed from its slice's start to its own end, lies within its slice, so
+    their sum is at most the slices' wall time."""
+    smp = H.Sampler(H.IC_R0_INJECT, 1, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
+    ic = H.sample_ics(H.co2_ar(), smp, 42, 2 * 8192)[0]
+    xs = [eng.to_device_soa(ic[j * 8192:(j + 1) * 8192]) for j in range(2)]
+    _, st = eng.cf_run(H.co2_ar(), H.cf_params(1024, rtol=1e-10, precision=H.PREC_MIXED), xs)
+    print(f"  launches {st.launches}, own pool {st.pool_launches} in {st.pool_kernel_ms:.2f} ms, "
+          f"slices {st.integrate_ms:.2f} ms, handed over {st.n_orbit}", flush=True)
+    assert st.n_orbit > 0
+    assert 0 < st.pool_launches < st.launches
+    assert 0.0 < st.pool_kernel_ms <= st.integrate_ms * (1.0 + 1e-6) + 1e-3
+    _, s64 = eng.cf_run(H.co2_ar(), H.cf_params(1024, rtol=1e-12, precision=H.PREC_FP64), xs)
+    assert s64.pool_launches == s64.launches and 0.0 < s64.pool_kernel_ms <= s64.integrate_ms * (1.0 + 1e-6) + 1e-3
+
+
 def test_mixed_precision_runs_reproducibly(eng):
     """Three runs of the mixed engine on 3 jobs of 8192 trajectories give the same bits and the
     same right-hand-side counts per item."""

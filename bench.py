@@ -428,6 +428,7 @@ def main():
     torch.cuda.synchronize()
     t_b64 = time.perf_counter() - t0
     b64_cf0 = outs64[0].cpu().numpy()
+    b64_cf1 = outs64[1].cpu().numpy() if K > 1 else None
     del outs64
     log(f"batched fp64: {t_b64 * 1e3:.1f} ms")
 
@@ -440,6 +441,7 @@ def main():
     torch.cuda.synchronize()
     t_mixed = time.perf_counter() - t0
     mixed_cf0 = outs[0].cpu().numpy()
+    mixed_cf1 = outs[1].cpu().numpy() if K > 1 else None
     mixed_items0 = eng.items(0, n, L, rows=True)
     mixed_len = [mixed_items0["len"]] + [eng.items(j, n, L)["len"] for j in range(1, K)]
     # per-trajectory rows of the plain engine for the first timed step (outside the timings)
@@ -599,6 +601,16 @@ def main():
         O.lib().orc_cf_ensemble(C.byref(O.co2ar()), C.byref(O.CFParams(1e-12, 1e-12, 200.0, L, 40.0, 1, 50000)),
                                 O.ptr(np.ascontiguousarray(ic_all[W * n:W * n + m])), m, cores, 1.0, O.ptr(cf), None)
         t_cpu = time.time() - t0
+        if K > 1 and m == 2 * n:
+            # the same comparison over the first two timed steps (131072 trajectories of the
+            # oracle's CPU-baseline run), each engine's CF the mean of its two per-step CFs
+            result["accuracy_vs_fp64_oracle"]["first_two_steps"] = {
+                "trajectories": m,
+                "cf_rel_l2_batched_mixed": rel(0.5 * (mixed_cf0 + mixed_cf1), cf),
+                "cf_rel_l2_batched_fp64": rel(0.5 * (b64_cf0 + b64_cf1), cf),
+                "cf_rel_l2_batched_mixed_vs_batched_fp64": rel(0.5 * (mixed_cf0 + mixed_cf1),
+                                                               0.5 * (b64_cf0 + b64_cf1))}
+            log("accuracy over the first two steps", result["accuracy_vs_fp64_oracle"]["first_two_steps"])
         model = "unknown"
         try:
             with open("/proc/cpuinfo") as f:

@@ -157,6 +157,34 @@ def hbm_section(H, eng, stm, mixed_len, plain_len, plain_reduce_ms, L, prmu_ntra
     return out
 
 
+def single_trajectory(H, eng, index, ntraj, L, rtol_mixed):
+    """One trajectory of the workload alone on the device, on each engine (a warm-up run, then a
+    timed one): wall time, right-hand sides and time per right-hand side.  The longest
+    trajectories of a batch run this way at its end, so this is the latency that bounds the
+    batch's tail."""
+    import torch
+    smp = H.Sampler(H.IC_R0_INJECT, H.PAIR_FREE, 200.0, 4.0, 40.0, H.PESMIN_CO2AR, 40.0)
+    ic = H.sample_ics(H.co2_ar(), smp, 42, ntraj)[0]
+    if not 0 <= index < ntraj:
+        raise SystemExit(f"--single: index {index} outside [0, {ntraj})")
+    x = eng.to_device_soa(ic[index:index + 1])
+    out = {"trajectory": index, "workload": f"CO2-Ar, T=200 K, R0=Rcut=40 bohr incoming, L={L}, dt=200 a.u."}
+    for name, p in (("plain_fp64", H.cf_params(L, rtol=1e-12, engine=H.ENGINE_PLAIN, precision=H.PREC_FP64)),
+                    ("batched_fp64", H.cf_params(L, rtol=1e-12, engine=H.ENGINE_BATCHED, precision=H.PREC_FP64)),
+                    ("batched_mixed", H.cf_params(L, rtol=rtol_mixed, engine=H.ENGINE_BATCHED,
+                                                  precision=H.PREC_MIXED))):
+        eng.cf_run(H.co2_ar(), p, [x])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st = eng.cf_run(H.co2_ar(), p, [x])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        out[name] = {"ms": ms, "nrhs": st.nrhs, "us_per_rhs": 1e3 * ms / max(st.nrhs, 1),
+                     "handed_over_to_fp64": st.n_orbit}
+        log(name, out[name])
+    print(json.dumps(out), flush=True)
+
+
 def serving_mix(H, eng, rate_hz, seconds, seed, log_every=25):
     """Poisson arrivals of CF requests (He-Ar or CO2-Ar, T ~ U[50, 500] K, ntraj ~ 2^U[10, 16])
     and M0 requests (10^U[6, 8] accepted points of the rejection sampler), 4:1; inputs staged in
@@ -360,6 +388,10 @@ def main():
     ap.add_argument("--n2ar-points", type=int, default=1 << 20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device-to-device copy first (reference kernel for HBM byte counters)")
+    ap.add_argument("--single", type=int, default=None, metavar="INDEX",
+                    help="latency of one trajectory alone on the device: trajectory INDEX of the first "
+                         "--ntraj of the workload's stream, on the plain FP64, batched FP64 and batched "
+                         "mixed engines (one JSON line; what bounds a batch's tail)")
     args = ap.parse_args()
     if args.traffic_from:
         r = traffic_from_counters(args.traffic_from[0], args.traffic_from[1],
@@ -377,6 +409,9 @@ def main():
 
     eng = H.Engine(0)
     log("device", eng.info())
+    if args.single is not None:
+        single_trajectory(H, eng, args.single, args.ntraj, args.L, args.rtol_mixed)
+        return
     if args.calibrate_copy:
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")   # 1 GiB
         b = torch.empty_like(a)

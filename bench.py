@@ -26,6 +26,10 @@ at 1e-12), the roofline of the pool kernel, a serving run with Poisson arrivals 
 request latency), the N2-Ar PIP-NN surfaces on the matrix cores (2^20 phase points at
 100-500 K; BF16 and split-BF16 errors against the FP64 oracle), and the CPU oracle timed on
 the host cores.  One JSON line on stdout.
+
+Other modes: `--single INDEX` times one trajectory of the workload alone on each engine (the
+latency that bounds a batch's tail); `--traffic-from FETCH_DIR WRITE_DIR` turns two rocprofv3
+counter runs into the pool kernel's bytes per launch (profiles/).
 """
 import argparse
 import json
